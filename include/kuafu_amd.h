@@ -1,0 +1,289 @@
+/*
+ * kuafu_amd.h — C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * The reference exposes a C++ API only (kuafu.hpp:31-93); the C++ facade in
+ * include/kuafu.hpp keeps that API.  This C ABI is the seam underneath it:
+ * plain pointers and sizes, no C++ or torch types, so that ctypes / cgo / JNI
+ * harnesses (and the parity tests) can drive the same code the facade drives.
+ *
+ * Every POD struct below reproduces a std430 / host layout of the reference
+ * byte for byte, so that a scene uploaded by the facade and a scene handed in
+ * through this ABI are the same bytes in HBM.
+ *
+ * Error convention: functions return 0 on success and a negative code on
+ * failure; kf_last_error() returns the message.  The C++ facade turns a
+ * negative code into std::runtime_error, the reference's convention
+ * (stdafx.hpp:61-67).
+ */
+#ifndef KUAFU_AMD_H
+#define KUAFU_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Layouts shared with the reference                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Vertex, 48 B: vertex.hpp:28-34; unpacked as 3 x vec4 in PathTrace.rchit:52-64. */
+typedef struct kf_vertex {
+    float pos[3];
+    float normal[3];
+    float color[3];
+    float uv[2];
+    float pad;
+} kf_vertex;
+
+/* NiceMaterialSSBO, 80 B: geometry.hpp:70-87, Geometry.glsl:1-19. */
+typedef struct kf_material {
+    float diffuse[4];   /* rgb + pad */
+    float emission[4];  /* rgb + strength */
+    float alpha, metallic, specular, roughness, ior, transmission;
+    int32_t diffuse_tex, metallic_tex, roughness_tex, transmission_tex;
+    int32_t pad0, pad1;
+} kf_material;
+
+/* GeometryInstanceSSBO, 80 B: geometry.hpp:91-99.  transform is a
+ * column-major glm::mat4 (object -> world). */
+typedef struct kf_instance {
+    float transform[16];
+    uint32_t geometry_index;
+    uint32_t pad[3];
+} kf_instance;
+
+/* DirectionalLightUBO, 32 B: light.hpp:41-44 (direction is normalized by the
+ * host, scene.cpp:254). */
+typedef struct kf_dir_light {
+    float direction[4]; /* xyz + softness */
+    float rgbs[4];      /* rgb + strength */
+} kf_dir_light;
+
+#define KF_MAX_POINT_LIGHTS 32
+#define KF_MAX_ACTIVE_LIGHTS 8
+
+/* PointLightsUBO, 1024 B: light.hpp:46-49. */
+typedef struct kf_point_lights {
+    float posr[KF_MAX_POINT_LIGHTS][4]; /* xyz + radius */
+    float rgbs[KF_MAX_POINT_LIGHTS][4]; /* rgb + strength (w > 0 => active) */
+} kf_point_lights;
+
+/* ActiveLightsUBO, 1536 B: light.hpp:51-58, filled as in scene.cpp:266-287. */
+typedef struct kf_active_lights {
+    float view[KF_MAX_ACTIVE_LIGHTS][16];
+    float proj[KF_MAX_ACTIVE_LIGHTS][16];
+    float front[KF_MAX_ACTIVE_LIGHTS][4];    /* xyz + in-use flag (w > 0) */
+    float rgbs[KF_MAX_ACTIVE_LIGHTS][4];     /* rgb + strength */
+    float position[KF_MAX_ACTIVE_LIGHTS][4];
+    float sftp[KF_MAX_ACTIVE_LIGHTS][4];     /* softness, fov, texID, pad */
+} kf_active_lights;
+
+/* CameraUBO, 320 B: camera.hpp:181-192, Camera.glsl:1-13. */
+typedef struct kf_camera_ubo {
+    float view[16], proj[16], view_inverse[16], proj_inverse[16];
+    float position[4]; /* xyz + aperture */
+    float front[4];    /* xyz + focal distance */
+    float pad1[4], pad2[4];
+} kf_camera_ubo;
+
+/* RtPushConstants, 48 B: rt.hpp:31-42, PushConstants.glsl:1-16. */
+typedef struct kf_push_constants {
+    float clear_color[4];
+    int32_t frame_count;
+    uint32_t spp;
+    uint32_t path_depth;
+    uint32_t use_env_map;
+    uint32_t russian_roulette;
+    uint32_t rr_min_bounces;
+    uint32_t nee;            /* pushed but unused by the shaders */
+    uint32_t nee_min_bounces;
+} kf_push_constants;
+
+/* RGBA8 image, row-major, 4 bytes per texel, sRGB-encoded (vkCore.hpp:585). */
+typedef struct kf_texture_desc {
+    uint32_t width, height;
+    const uint8_t* rgba8;
+} kf_texture_desc;
+
+enum {
+    KF_GEOM_OPAQUE = 1u,   /* Geometry::isOpaque (rt.cpp:101-102) */
+    KF_GEOM_HIDDEN = 2u    /* Geometry::hideRender -> empty BLAS (rt.cpp:156) */
+};
+
+enum {
+    KF_ENV_CLEAR_COLOR = 0, /* rmiss:21: clearColor.rgb * clearColor.a */
+    KF_ENV_CUBEMAP = 1,     /* rmiss:19: samplerCube lookup */
+    KF_ENV_GRADIENT = 2     /* procedural gradient (extension, not in the reference) */
+};
+
+/* Whole-scene description.  Geometries are concatenated: geometry g owns
+ * vertices [geom_vertex_offset[g], geom_vertex_offset[g+1]) and triangles
+ * [geom_tri_offset[g], geom_tri_offset[g+1]); indices are local to the
+ * geometry (Scene::uploadGeometries, scene.cpp:423-440). */
+typedef struct kf_scene_desc {
+    uint32_t num_geometries;
+    const uint32_t* geom_vertex_offset; /* num_geometries + 1 */
+    const uint32_t* geom_tri_offset;    /* num_geometries + 1 */
+    const uint32_t* geom_flags;         /* num_geometries, KF_GEOM_* */
+    const kf_vertex* vertices;
+    const uint32_t* indices;            /* 3 per triangle */
+    const uint32_t* mat_index;          /* 1 per triangle (global material index) */
+    uint32_t num_instances;
+    const kf_instance* instances;
+    uint32_t num_materials;
+    const kf_material* materials;
+    uint32_t num_textures;
+    const kf_texture_desc* textures;
+    const kf_dir_light* dir_light;
+    const kf_point_lights* point_lights;
+    const kf_active_lights* active_lights;
+    uint32_t env_kind;                  /* KF_ENV_* */
+    const kf_texture_desc* env_faces;   /* 6 faces +X,-X,+Y,-Y,+Z,-Z for KF_ENV_CUBEMAP */
+    float env_gradient[8];              /* bottom rgb, pad, top rgb, pad for KF_ENV_GRADIENT */
+} kf_scene_desc;
+
+/* A ray of a fixed batch: origin, tmin, direction, tmax (32 B). */
+typedef struct kf_ray {
+    float org[3], tmin;
+    float dir[3], tmax;
+} kf_ray;
+
+/* Closest-hit record.  inst = ~0u means miss.  b1/b2 are gl_HitAttributeEXT
+ * (weights of vertex 1 and 2); back = 1 for gl_HitKindBackFacingTriangleEXT. */
+typedef struct kf_hit {
+    float t;
+    uint32_t inst;
+    uint32_t prim;
+    float b1, b2;
+    uint32_t back;
+} kf_hit;
+
+/* Per-frame render statistics, counted on the device. */
+typedef struct kf_render_stats {
+    uint64_t primary_rays;
+    uint64_t extension_rays;
+    uint64_t shadow_rays;
+    uint64_t node_visits;     /* only when the counting traversal is enabled */
+    uint64_t tri_tests;
+    float trace_ms;           /* kernel wall time of the frame (HIP events) */
+    float extend_ms;          /* closest-hit traversal kernels */
+    float shadow_ms;          /* any-hit traversal kernels */
+    float other_ms;
+} kf_render_stats;
+
+/* Render options that the reference takes from Config / push constants. */
+typedef struct kf_render_opts {
+    uint32_t seed;            /* replaces clockARB() seeding (rgen:23, 32) */
+    uint32_t samples_per_batch; /* wavefront batch size in samples per pixel; 0 = auto */
+    uint32_t keep_samples;    /* keep per-sample radiance for kf_renderer_read_samples */
+    uint32_t count_traversal; /* count node visits / triangle tests */
+} kf_render_opts;
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces Context::init's device selection (context.cpp:88-296): picks the
+ * HIP device and creates the library's stream.  Idempotent. */
+int kf_init(int device);
+const char* kf_last_error(void);
+const char* kf_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host math used by the facade (camera.cpp, scene.cpp)                      */
+/* ------------------------------------------------------------------------ */
+
+/* CameraUBO contents for a pose + intrinsics, as Scene::uploadCameraBuffer
+ * builds them (scene.cpp:236-250, camera.cpp:92-128). */
+int kf_camera_ubo_compute(const float position[3], const float front[3], const float up[3],
+                          int width, int height, float fx, float fy, float cx, float cy, float skew,
+                          float aperture, float focal_length, kf_camera_ubo* out);
+
+/* World -> object 3x4 (row-major) of an instance transform; the matrix the
+ * traversal applies to rays (gl_WorldToObjectEXT). */
+int kf_world_to_object(const float transform[16], float out_rows[12]);
+
+/* Active-light UBO slot for a light, as scene.cpp:266-281 fills it. */
+int kf_active_light_fill(kf_active_lights* ubo, int slot, const float view[16], const float color[3],
+                         float fov, float strength, float softness, int tex_id);
+
+/* ------------------------------------------------------------------------ */
+/* Acceleration structures + traversal                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct kf_scene kf_scene;
+
+/* Upload the scene and build every BLAS plus the TLAS on the GPU
+ * (Context::update -> uploadGeometries, createBottomLevelAS, buildTlas:
+ * context.cpp:323-336, rt.cpp:142-495). */
+int kf_scene_create(const kf_scene_desc* desc, kf_scene** out);
+void kf_scene_destroy(kf_scene* s);
+
+/* Replace instance transforms and rebuild the TLAS on the GPU (the per-frame
+ * RayTracer::updateTlas, rt.cpp:372-404, context.cpp:337-341). */
+int kf_scene_update_instances(kf_scene* s, const kf_instance* inst, uint32_t n);
+
+/* Replace the lights (Scene::uploadLightBuffers, scene.cpp:252-292). */
+int kf_scene_set_lights(kf_scene* s, const kf_dir_light* d, const kf_point_lights* p,
+                        const kf_active_lights* a);
+
+/* Statistics of the built BVHs. */
+typedef struct kf_bvh_info {
+    uint32_t blas_nodes, tlas_nodes, triangles, instances;
+    uint32_t max_blas_depth, max_tlas_depth;
+    float build_ms;
+} kf_bvh_info;
+int kf_scene_bvh_info(kf_scene* s, kf_bvh_info* out);
+
+/* Copy the BLAS of one geometry back: nodes are 16 floats each (left box
+ * lo/hi xyz, right box lo/hi xyz as in the device layout, then child refs as
+ * int bits) and the triangle order.  For BVH invariant tests. */
+int kf_scene_read_blas(kf_scene* s, uint32_t geometry, float* nodes, uint32_t max_nodes,
+                       uint32_t* n_nodes, int32_t* root_ref, uint32_t* tri_order, uint32_t max_tris);
+
+/* Fixed ray batches (host pointers, synchronous): the traceRayEXT calls of
+ * PathTrace.rgen:97-107 (closest hit) and PathTrace.rchit:186-197
+ * (terminate-on-first-hit shadow rays). */
+int kf_trace_closest(kf_scene* s, const kf_ray* rays, uint32_t n, kf_hit* hits);
+int kf_trace_any(kf_scene* s, const kf_ray* rays, uint32_t n, uint8_t* occluded);
+
+/* Device-resident variants for measurement: rays/hits live in HBM. */
+int kf_trace_closest_device(kf_scene* s, const kf_ray* d_rays, uint32_t n, kf_hit* d_hits,
+                            float* kernel_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Renderer: raygen + closest hit + miss + shadow (PathTrace.*)              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct kf_renderer kf_renderer;
+
+/* Render targets rgba/albedo/normal RGBA32F (rt.cpp:497-512). */
+int kf_renderer_create(uint32_t width, uint32_t height, kf_renderer** out);
+void kf_renderer_destroy(kf_renderer* r);
+
+/* One frame: RayTracer::trace -> vkCmdTraceRaysKHR(W,H,1) (rt.cpp:637-666)
+ * with the push constants of context.cpp:605-614. */
+int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, const kf_push_constants* pc,
+              const kf_render_opts* opts, kf_render_stats* stats);
+
+/* Read a render target: 0 = rgba, 1 = albedo, 2 = normal; W*H*4 floats. */
+int kf_renderer_read(kf_renderer* r, int target, float* out);
+/* Overwrite the rgba target (running-mean state), W*H*4 floats. */
+int kf_renderer_write(kf_renderer* r, int target, const float* in);
+/* Per-sample radiance of the last frame: spp * W * H * 3 floats, sample-major. */
+int kf_renderer_read_samples(kf_renderer* r, float* out, uint32_t max_floats);
+/* Device pointer of a target (for the denoiser and readback kernels). */
+float* kf_renderer_device_target(kf_renderer* r, int target);
+
+/* downloadLatestFrame: RGBA32F -> B8G8R8A8_SRGB bytes (PostProcessing.frag:17,
+ * config.hpp:144, vkCore.hpp:3008-3111); W*H*4 bytes, order B,G,R,A. */
+int kf_renderer_download_bgra8(kf_renderer* r, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KUAFU_AMD_H */

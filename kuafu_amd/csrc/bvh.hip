@@ -1,0 +1,561 @@
+// bvh.hip — GPU LBVH builder for the BLASes and the instance TLAS (gfx950).
+//
+// Replaces the driver's acceleration-structure builds that the reference
+// requests in RayTracer::createBottomLevelAS / buildBlas (rt.cpp:142-370) and
+// RayTracer::buildTlas / updateTlas (rt.cpp:372-495).
+//
+// One batched build handles every geometry at once (segments): primitives of
+// segment s get 64-bit keys (s << 30 | morton30(centroid)), one radix sort
+// orders all of them, the Karras (2012) hierarchy is emitted per segment and
+// a bottom-up refit fills the child boxes.
+//   k_seg_bounds   centroid bounds per segment (ordered-int atomics)
+//   k_morton       30-bit Morton codes, key = seg << 30 | code
+//   radix sort     8-bit digits: per-tile histogram, one-block scan, stable
+//                  scatter with wave64 ballot multi-split ranks in LDS
+//   k_karras       internal nodes (ranges, splits, child refs, parents)
+//   k_refit        leaves walk up; the second arrival at a node unions it
+#include "kf_internal.h"
+#include <algorithm>
+#include <cmath>
+
+namespace kf {
+
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kDigitBits = 8;
+constexpr int kDigits = 1 << kDigitBits;
+
+__device__ __forceinline__ int f2ord(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float ord2f(int o) { return __int_as_float(o >= 0 ? o : o ^ 0x7fffffff); }
+
+__global__ void k_fill_int(int* p, int v, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_seg_bounds(uint32_t n, const float4* __restrict__ lo, const float4* __restrict__ hi,
+                             const uint32_t* __restrict__ seg, int* __restrict__ sb /* 6 per seg */) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 a = lo[i], b = hi[i];
+    if (!(a.x <= b.x)) return;  // empty box
+    uint32_t s = seg[i];
+    float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+    for (int k = 0; k < 3; ++k) {
+        atomicMin(&sb[6 * s + k], f2ord(c[k]));
+        atomicMax(&sb[6 * s + 3 + k], f2ord(c[k]));
+    }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void k_morton(uint32_t n, const float4* __restrict__ lo, const float4* __restrict__ hi,
+                         const uint32_t* __restrict__ seg, const int* __restrict__ sb,
+                         unsigned long long* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = seg[i];
+    float4 a = lo[i], b = hi[i];
+    uint32_t q[3] = {0, 0, 0};
+    if (a.x <= b.x) {
+        float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+        for (int k = 0; k < 3; ++k) {
+            float l = ord2f(sb[6 * s + k]), h = ord2f(sb[6 * s + 3 + k]);
+            float e = h - l;
+            float u = e > 0.0f ? (c[k] - l) / e : 0.5f;
+            u = fminf(fmaxf(u, 0.0f), 1.0f);
+            q[k] = min((uint32_t)(u * 1024.0f), 1023u);
+        }
+    }
+    uint32_t m = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    keys[i] = ((unsigned long long)s << 30) | m;
+    vals[i] = i;
+}
+
+// ---------------------------------------------------------------- radix sort
+__global__ __launch_bounds__(kSortThreads) void k_rs_hist(uint32_t n, const unsigned long long* __restrict__ keys,
+                                                          int shift, uint32_t n_tiles, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[kDigits];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (kDigits - 1)], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * n_tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of m entries with one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t m, uint32_t* __restrict__ h) {
+    __shared__ uint32_t part[1024];
+    uint32_t per = (m + 1023) / 1024;
+    uint32_t b = threadIdx.x * per, e = min(b + per, m);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += h[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= (uint32_t)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (uint32_t i = b; i < e; ++i) {
+        uint32_t v = h[i];
+        h[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_rs_scatter(uint32_t n, const unsigned long long* __restrict__ kin,
+                                                             const uint32_t* __restrict__ vin, int shift,
+                                                             uint32_t n_tiles, const uint32_t* __restrict__ hist,
+                                                             unsigned long long* __restrict__ kout,
+                                                             uint32_t* __restrict__ vout) {
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t gbase[kDigits];
+    __shared__ uint32_t run[kDigits];
+    __shared__ uint32_t wcnt[kWaves][kDigits];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    gbase[t] = hist[t * n_tiles + blockIdx.x];
+    run[t] = 0;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        for (int k = 0; k < kWaves; ++k) wcnt[k][t] = 0;
+        __syncthreads();
+        uint32_t i = base + r * kSortThreads + t;
+        bool valid = i < n;
+        unsigned long long key = valid ? kin[i] : 0ull;
+        uint32_t d = (uint32_t)(key >> shift) & (kDigits - 1);
+        unsigned long long mask = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kDigitBits; ++b) {
+            bool bit = (d >> b) & 1u;
+            unsigned long long m = __ballot(bit);
+            mask &= bit ? m : ~m;
+        }
+        uint32_t rank = __popcll(mask & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(mask);
+        __syncthreads();
+        {
+            uint32_t s = run[t];
+            for (int k = 0; k < kWaves; ++k) {
+                uint32_t c = wcnt[k][t];
+                wcnt[k][t] = s;
+                s += c;
+            }
+            run[t] = s;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = gbase[d] + wcnt[w][d] + rank;
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- Karras
+__device__ __forceinline__ int delta(const unsigned long long* k, int n, int a, int b) {
+    if (b < 0 || b >= n) return -1;
+    unsigned long long x = k[a] ^ k[b];
+    if (x) return __clzll(x);
+    return 64 + __clz((uint32_t)(a ^ b));
+}
+
+__global__ void k_karras(uint32_t n, const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ order,
+                         const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ node_off,
+                         Node* __restrict__ nodes, int* __restrict__ parent_node, int* __restrict__ parent_leaf,
+                         int leaf_is_slot) {
+    uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= n) return;
+    uint32_t s = (uint32_t)(keys[gi] >> 30);
+    uint32_t so = seg_off[s];
+    int ns = (int)(seg_off[s + 1] - so);
+    int i = (int)(gi - so);
+    if (i >= ns - 1) return;
+    const unsigned long long* k = keys + so;
+    int d = (delta(k, ns, i, i + 1) - delta(k, ns, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(k, ns, i, i - d);
+    int lmax = 2;
+    while (delta(k, ns, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(k, ns, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(k, ns, i, j);
+    int sp = 0;
+    int div = 2;
+    int t = (l + div - 1) / div;
+    while (true) {
+        if (delta(k, ns, i, i + (sp + t) * d) > dnode) sp += t;
+        if (t == 1) break;
+        div <<= 1;
+        t = (l + div - 1) / div;
+    }
+    int gamma = i + sp * d + min(d, 0);
+    int nb = (int)node_off[s];
+    int me = nb + i;
+    int lref, rref;
+    if (min(i, j) == gamma) {
+        int leaf = (int)so + gamma;
+        lref = ~(leaf_is_slot ? leaf : (int)order[leaf]);
+        parent_leaf[leaf] = me;
+    } else {
+        lref = nb + gamma;
+        parent_node[nb + gamma] = me;
+    }
+    if (max(i, j) == gamma + 1) {
+        int leaf = (int)so + gamma + 1;
+        rref = ~(leaf_is_slot ? leaf : (int)order[leaf]);
+        parent_leaf[leaf] = me;
+    } else {
+        rref = nb + gamma + 1;
+        parent_node[nb + gamma + 1] = me;
+    }
+    nodes[me].d = make_int4(lref, rref, 0, 0);
+    if (i == 0) parent_node[me] = -1;
+}
+
+__device__ __forceinline__ void write_child(Node* nd, bool left, const float lo[3], const float hi[3]) {
+    float* a = left ? &nd->a.x : &nd->b.x;
+    a[0] = lo[0];
+    a[1] = hi[0];
+    a[2] = lo[1];
+    a[3] = hi[1];
+    float* c = left ? &nd->c.x : &nd->c.z;
+    c[0] = lo[2];
+    c[1] = hi[2];
+}
+
+__global__ void k_refit(uint32_t n, const uint32_t* __restrict__ order, const float4* __restrict__ plo,
+                        const float4* __restrict__ phi, const int* __restrict__ parent_node,
+                        const int* __restrict__ parent_leaf, const unsigned long long* __restrict__ keys,
+                        const uint32_t* __restrict__ seg_off, Node* nodes, int* flags, int leaf_is_slot) {
+    uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= n) return;
+    uint32_t s = (uint32_t)(keys[gi] >> 30);
+    if (seg_off[s + 1] - seg_off[s] < 2) return;  // single-leaf segment: no internal node
+    uint32_t prim = order[gi];
+    float4 a = plo[prim], b = phi[prim];
+    float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+    int child = ~(leaf_is_slot ? (int)gi : (int)prim);
+    int p = parent_leaf[gi];
+    while (p >= 0) {
+        Node* nd = &nodes[p];
+        bool left = nd->d.x == child;
+        write_child(nd, left, lo, hi);
+        int old = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        float4 na = nd->a, nb = nd->b, nc = nd->c;
+        lo[0] = fminf(na.x, nb.x);
+        hi[0] = fmaxf(na.y, nb.y);
+        lo[1] = fminf(na.z, nb.z);
+        hi[1] = fmaxf(na.w, nb.w);
+        lo[2] = fminf(nc.x, nc.z);
+        hi[2] = fmaxf(nc.y, nc.w);
+        child = p;
+        p = parent_node[p];
+    }
+}
+
+__global__ void k_seg_root(uint32_t n_seg, const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ node_off,
+                           const uint32_t* __restrict__ order, int* __restrict__ root, int leaf_is_slot) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_seg) return;
+    uint32_t ns = seg_off[s + 1] - seg_off[s];
+    if (ns == 0) root[s] = kEmptyRef;
+    else if (ns == 1) root[s] = ~(leaf_is_slot ? (int)seg_off[s] : (int)order[seg_off[s]]);
+    else root[s] = (int)node_off[s];
+}
+
+template <class T> T* dalloc(size_t n) {
+    T* p = nullptr;
+    if (n) KF_CHECK(hipMallocAsync((void**)&p, n * sizeof(T), stream()));
+    return p;
+}
+template <class T> void dfree(T* p) {
+    if (p) KF_CHECK(hipFreeAsync(p, stream()));
+}
+
+inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+
+// ---------------------------------------------------------------- BLAS inputs
+__global__ void k_tri_geom(uint32_t n_tri, uint32_t n_geom, const uint32_t* __restrict__ toff,
+                           uint32_t* __restrict__ tri_geom) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_tri) return;
+    uint32_t lo = 0, hi = n_geom;  // largest g with toff[g] <= i
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (toff[mid] <= i) lo = mid; else hi = mid;
+    }
+    tri_geom[i] = lo;
+}
+
+__global__ void k_tri_bounds(uint32_t n_tri, const float4* __restrict__ verts, const uint32_t* __restrict__ idx,
+                             const uint32_t* __restrict__ voff, const uint32_t* __restrict__ tri_geom,
+                             float4* __restrict__ lo, float4* __restrict__ hi) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_tri) return;
+    uint32_t vb = voff[tri_geom[i]];
+    float3 p[3];
+    for (int k = 0; k < 3; ++k) {
+        float4 v = verts[3 * (size_t)(vb + idx[3 * (size_t)i + k])];
+        p[k] = make_float3(v.x, v.y, v.z);
+    }
+    lo[i] = make_float4(fminf(p[0].x, fminf(p[1].x, p[2].x)), fminf(p[0].y, fminf(p[1].y, p[2].y)),
+                        fminf(p[0].z, fminf(p[1].z, p[2].z)), 0.f);
+    hi[i] = make_float4(fmaxf(p[0].x, fmaxf(p[1].x, p[2].x)), fmaxf(p[0].y, fmaxf(p[1].y, p[2].y)),
+                        fmaxf(p[0].z, fmaxf(p[1].z, p[2].z)), 0.f);
+}
+
+// Triangle slots in BVH leaf order: (v0, local prim id bits), v1, v2.
+__global__ void k_tri_reorder(uint32_t n_tri, const uint32_t* __restrict__ order, const float4* __restrict__ verts,
+                              const uint32_t* __restrict__ idx, const uint32_t* __restrict__ voff,
+                              const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tri_geom,
+                              float4* __restrict__ tri) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_tri) return;
+    uint32_t i = order[s];
+    uint32_t g = tri_geom[i];
+    uint32_t vb = voff[g];
+    for (int k = 0; k < 3; ++k) {
+        float4 v = verts[3 * (size_t)(vb + idx[3 * (size_t)i + k])];
+        tri[3 * (size_t)s + k] = make_float4(v.x, v.y, v.z, k == 0 ? __uint_as_float(i - toff[g]) : 0.f);
+    }
+}
+
+// Object-space bounds of each BLAS (union of the root's children, or the
+// single triangle).
+__global__ void k_blas_box(uint32_t n_geom, const int* __restrict__ root, const Node* __restrict__ nodes,
+                           const float4* __restrict__ tri, float4* __restrict__ glo, float4* __restrict__ ghi) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_geom) return;
+    int r = root[g];
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0);
+    if (r == kEmptyRef) {
+    } else if (r < 0) {
+        size_t s = (size_t)(~r);
+        for (int k = 0; k < 3; ++k) {
+            float4 v = tri[3 * s + k];
+            lo.x = fminf(lo.x, v.x); lo.y = fminf(lo.y, v.y); lo.z = fminf(lo.z, v.z);
+            hi.x = fmaxf(hi.x, v.x); hi.y = fmaxf(hi.y, v.y); hi.z = fmaxf(hi.z, v.z);
+        }
+    } else {
+        Node n = nodes[r];
+        lo = make_float4(fminf(n.a.x, n.b.x), fminf(n.a.z, n.b.z), fminf(n.c.x, n.c.z), 0);
+        hi = make_float4(fmaxf(n.a.y, n.b.y), fmaxf(n.a.w, n.b.w), fmaxf(n.c.y, n.c.w), 0);
+    }
+    glo[g] = lo;
+    ghi[g] = hi;
+}
+
+// Instance records + padded world AABBs (corners transformed in double).
+__global__ void k_instances(uint32_t n, const kf_instance* __restrict__ src, const float* __restrict__ w2o,
+                            uint32_t n_geom, const uint32_t* __restrict__ gflags, const int* __restrict__ root,
+                            const float4* __restrict__ glo, const float4* __restrict__ ghi, Inst* __restrict__ out,
+                            float4* __restrict__ ilo, float4* __restrict__ ihi, uint32_t* __restrict__ iseg) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const kf_instance& in = src[i];
+    uint32_t g = in.geometry_index;
+    Inst rec;
+    for (int r = 0; r < 3; ++r) rec.w2o[r] = make_float4(w2o[12 * i + 4 * r], w2o[12 * i + 4 * r + 1],
+                                                         w2o[12 * i + 4 * r + 2], w2o[12 * i + 4 * r + 3]);
+    bool valid = g < n_geom && !(gflags[g] & KF_GEOM_HIDDEN) && root[g] != kEmptyRef;
+    int flags = (valid ? kInstValid : 0) | ((g < n_geom && (gflags[g] & KF_GEOM_OPAQUE)) ? kInstOpaque : 0);
+    rec.info = make_int4(valid ? root[g] : kEmptyRef, (int)g, flags, 0);
+    out[i] = rec;
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0);
+    if (valid) {
+        float4 a = glo[g], b = ghi[g];
+        double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+        const float* t = in.transform;
+        for (int c = 0; c < 8; ++c) {
+            double p[3] = {(c & 1) ? b.x : a.x, (c & 2) ? b.y : a.y, (c & 4) ? b.z : a.z};
+            for (int r = 0; r < 3; ++r) {
+                double v = (double)t[r] * p[0] + (double)t[4 + r] * p[1] + (double)t[8 + r] * p[2] + (double)t[12 + r];
+                l[r] = fmin(l[r], v);
+                h[r] = fmax(h[r], v);
+            }
+        }
+        double ext = fmax(h[0] - l[0], fmax(h[1] - l[1], h[2] - l[2]));
+        float fl[3], fh[3];
+        for (int r = 0; r < 3; ++r) {
+            double pad = 1e-5 * (ext + fabs(l[r]) + fabs(h[r])) + 1e-6;
+            fl[r] = nextafterf((float)(l[r] - pad), -INFINITY);
+            fh[r] = nextafterf((float)(h[r] + pad), INFINITY);
+        }
+        lo = make_float4(fl[0], fl[1], fl[2], 0);
+        hi = make_float4(fh[0], fh[1], fh[2], 0);
+    }
+    ilo[i] = lo;
+    ihi[i] = hi;
+    iseg[i] = 0;
+}
+
+}  // namespace
+
+void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_seg, uint32_t n_seg,
+                const uint32_t* d_seg_off, const uint32_t* d_node_off, Node* d_nodes, uint32_t* d_order, int* d_root,
+                bool leaf_is_slot, hipStream_t st) {
+    const uint32_t TB = 256;
+    if (n == 0) {
+        k_seg_root<<<blocks(n_seg, TB), TB, 0, st>>>(n_seg, d_seg_off, d_node_off, d_order, d_root, leaf_is_slot);
+        return;
+    }
+    int* sb = dalloc<int>(6 * (size_t)n_seg);
+    unsigned long long* k0 = dalloc<unsigned long long>(n);
+    unsigned long long* k1 = dalloc<unsigned long long>(n);
+    uint32_t* v0 = dalloc<uint32_t>(n);
+    uint32_t* v1 = dalloc<uint32_t>(n);
+    int* parent_node = dalloc<int>(std::max<uint32_t>(n, 1));
+    int* parent_leaf = dalloc<int>(n);
+    int* flags = dalloc<int>(std::max<uint32_t>(n, 1));
+    uint32_t n_tiles = blocks(n, kSortTile);
+    uint32_t* hist = dalloc<uint32_t>((size_t)n_tiles * kDigits);
+
+    // centroid bounds per segment: lo = +inf, hi = -inf in ordered-int space
+    for (uint32_t s = 0; s < n_seg; ++s) {
+    }
+    {
+        // fill 6 ints per segment: 3 x ord(+inf), 3 x ord(-inf)
+        std::vector<int> init(6 * (size_t)n_seg);
+        int pinf = 0x7f800000, ninf = (int)(0xff800000u ^ 0x7fffffffu);
+        for (uint32_t s = 0; s < n_seg; ++s)
+            for (int k = 0; k < 3; ++k) { init[6 * s + k] = pinf; init[6 * s + 3 + k] = ninf; }
+        KF_CHECK(hipMemcpyAsync(sb, init.data(), init.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        KF_CHECK(hipStreamSynchronize(st));
+    }
+    k_seg_bounds<<<blocks(n, TB), TB, 0, st>>>(n, d_lo, d_hi, d_seg, sb);
+    k_morton<<<blocks(n, TB), TB, 0, st>>>(n, d_lo, d_hi, d_seg, sb, k0, v0);
+
+    int seg_bits = 0;
+    while ((1u << seg_bits) < n_seg) ++seg_bits;
+    int total_bits = 30 + seg_bits;
+    for (int shift = 0; shift < total_bits; shift += kDigitBits) {
+        k_rs_hist<<<n_tiles, kSortThreads, 0, st>>>(n, k0, shift, n_tiles, hist);
+        k_rs_scan<<<1, 1024, 0, st>>>(n_tiles * kDigits, hist);
+        k_rs_scatter<<<n_tiles, kSortThreads, 0, st>>>(n, k0, v0, shift, n_tiles, hist, k1, v1);
+        std::swap(k0, k1);
+        std::swap(v0, v1);
+    }
+    KF_CHECK(hipMemcpyAsync(d_order, v0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    k_fill_int<<<blocks(n, TB), TB, 0, st>>>(flags, 0, n);
+    k_fill_int<<<blocks(n, TB), TB, 0, st>>>(parent_leaf, -1, n);
+    k_karras<<<blocks(n, TB), TB, 0, st>>>(n, k0, v0, d_seg_off, d_node_off, d_nodes, parent_node, parent_leaf,
+                                           leaf_is_slot);
+    k_refit<<<blocks(n, TB), TB, 0, st>>>(n, v0, d_lo, d_hi, parent_node, parent_leaf, k0, d_seg_off, d_nodes, flags,
+                                          leaf_is_slot);
+    k_seg_root<<<blocks(n_seg, TB), TB, 0, st>>>(n_seg, d_seg_off, d_node_off, v0, d_root, leaf_is_slot);
+    KF_CHECK(hipGetLastError());
+    dfree(sb); dfree(k0); dfree(k1); dfree(v0); dfree(v1);
+    dfree(parent_node); dfree(parent_leaf); dfree(flags); dfree(hist);
+}
+
+void build_blas(Scene& s) {
+    hipStream_t st = stream();
+    const uint32_t TB = 256;
+    uint32_t n = s.n_tri;
+    s.h_node_off.assign(s.n_geom, 0);
+    uint32_t acc = 0;
+    for (uint32_t g = 0; g < s.n_geom; ++g) {
+        s.h_node_off[g] = acc;
+        uint32_t ng = s.h_toff[g + 1] - s.h_toff[g];
+        acc += ng > 1 ? ng - 1 : 0;
+    }
+    s.n_blas_nodes = acc;
+    dfree(s.d_blas);
+    dfree(s.d_tri);
+    dfree(s.d_blas_root);
+    dfree(s.d_tri_geom);
+    s.d_blas = dalloc<Node>(std::max<uint32_t>(acc, 1));
+    s.d_tri = dalloc<float4>(3 * (size_t)std::max<uint32_t>(n, 1));
+    s.d_blas_root = dalloc<int>(std::max<uint32_t>(s.n_geom, 1));
+    s.d_tri_geom = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
+    uint32_t* d_node_off = dalloc<uint32_t>(std::max<uint32_t>(s.n_geom, 1));
+    if (s.n_geom)
+        KF_CHECK(hipMemcpyAsync(d_node_off, s.h_node_off.data(), s.n_geom * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    float4* lo = dalloc<float4>(std::max<uint32_t>(n, 1));
+    float4* hi = dalloc<float4>(std::max<uint32_t>(n, 1));
+    uint32_t* order = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
+    if (n) {
+        k_tri_geom<<<blocks(n, TB), TB, 0, st>>>(n, s.n_geom, s.d_toff, s.d_tri_geom);
+        k_tri_bounds<<<blocks(n, TB), TB, 0, st>>>(n, s.d_verts, s.d_idx, s.d_voff, s.d_tri_geom, lo, hi);
+    }
+    lbvh_build(n, lo, hi, s.d_tri_geom, s.n_geom, s.d_toff, d_node_off, s.d_blas, order, s.d_blas_root, true, st);
+    if (n)
+        k_tri_reorder<<<blocks(n, TB), TB, 0, st>>>(n, order, s.d_verts, s.d_idx, s.d_voff, s.d_toff, s.d_tri_geom,
+                                                    s.d_tri);
+    KF_CHECK(hipGetLastError());
+    s.h_blas_root.resize(s.n_geom);
+    if (s.n_geom)
+        KF_CHECK(hipMemcpyAsync(s.h_blas_root.data(), s.d_blas_root, s.n_geom * sizeof(int), hipMemcpyDeviceToHost, st));
+    dfree(d_node_off); dfree(lo); dfree(hi); dfree(order);
+    KF_CHECK(hipStreamSynchronize(st));
+}
+
+void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
+    hipStream_t st = stream();
+    const uint32_t TB = 256;
+    if (n != s.n_inst || !s.d_inst) {
+        dfree(s.d_inst); dfree(s.d_tlas); dfree(s.d_inst_src);
+        s.d_inst = dalloc<Inst>(std::max<uint32_t>(n, 1));
+        s.d_tlas = dalloc<Node>(std::max<uint32_t>(n, 1));
+        s.d_inst_src = dalloc<kf_instance>(std::max<uint32_t>(n, 1));
+        s.n_inst = n;
+    }
+    std::vector<float> w2o(12 * (size_t)std::max<uint32_t>(n, 1));
+    for (uint32_t i = 0; i < n; ++i) kf_world_to_object(h_inst[i].transform, &w2o[12 * i]);
+    float* d_w2o = dalloc<float>(w2o.size());
+    KF_CHECK(hipMemcpyAsync(d_w2o, w2o.data(), w2o.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    if (n) KF_CHECK(hipMemcpyAsync(s.d_inst_src, h_inst, n * sizeof(kf_instance), hipMemcpyHostToDevice, st));
+    uint32_t* d_gflags = dalloc<uint32_t>(std::max<uint32_t>(s.n_geom, 1));
+    if (s.n_geom)
+        KF_CHECK(hipMemcpyAsync(d_gflags, s.h_gflags.data(), s.n_geom * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    float4* glo = dalloc<float4>(std::max<uint32_t>(s.n_geom, 1));
+    float4* ghi = dalloc<float4>(std::max<uint32_t>(s.n_geom, 1));
+    if (s.n_geom) k_blas_box<<<blocks(s.n_geom, TB), TB, 0, st>>>(s.n_geom, s.d_blas_root, s.d_blas, s.d_tri, glo, ghi);
+    float4* ilo = dalloc<float4>(std::max<uint32_t>(n, 1));
+    float4* ihi = dalloc<float4>(std::max<uint32_t>(n, 1));
+    uint32_t* iseg = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
+    if (n)
+        k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, d_gflags, s.d_blas_root, glo, ghi,
+                                                  s.d_inst, ilo, ihi, iseg);
+    uint32_t seg_off_h[2] = {0, n}, node_off_h[1] = {0};
+    uint32_t* d_seg_off = dalloc<uint32_t>(2);
+    uint32_t* d_node_off = dalloc<uint32_t>(1);
+    KF_CHECK(hipMemcpyAsync(d_seg_off, seg_off_h, sizeof(seg_off_h), hipMemcpyHostToDevice, st));
+    KF_CHECK(hipMemcpyAsync(d_node_off, node_off_h, sizeof(node_off_h), hipMemcpyHostToDevice, st));
+    uint32_t* order = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
+    int* d_root = dalloc<int>(1);
+    lbvh_build(n, ilo, ihi, iseg, 1, d_seg_off, d_node_off, s.d_tlas, order, d_root, false, st);
+    int root = kEmptyRef;
+    KF_CHECK(hipMemcpyAsync(&root, d_root, sizeof(int), hipMemcpyDeviceToHost, st));
+    KF_CHECK(hipStreamSynchronize(st));
+    s.tlas_root = root;
+    dfree(d_w2o); dfree(d_gflags); dfree(glo); dfree(ghi); dfree(ilo); dfree(ihi); dfree(iseg);
+    dfree(d_seg_off); dfree(d_node_off); dfree(order); dfree(d_root);
+    KF_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace kf

@@ -1,0 +1,169 @@
+// kf_internal.h — device data layout and host-side objects of libkuafu_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/kuafu_amd.h"
+
+#define KF_CHECK(expr)                                                                    \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) throw kf::Error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+namespace kf {
+
+struct Error : std::exception {
+    std::string msg;
+    explicit Error(std::string m) : msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+void set_error(const std::string& m);
+hipStream_t stream();
+
+// tea (Random.glsl:7-21) on the host, for the frame seed.
+inline uint32_t tea_host(uint32_t v0, uint32_t v1) {
+    uint32_t s0 = 0;
+    for (int n = 0; n < 16; ++n) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+
+// ------------------------------------------------------------------ BVH layout
+// BVH2 node, 64 B, both child boxes stored in the parent so one 64-B fetch
+// tests two children:
+//   a = (L.lo.x, L.hi.x, L.lo.y, L.hi.y)
+//   b = (R.lo.x, R.hi.x, R.lo.y, R.hi.y)
+//   c = (L.lo.z, L.hi.z, R.lo.z, R.hi.z)
+//   d = (left_ref, right_ref, -, -)
+// A child ref >= 0 is an internal node index (global over all BLAS / TLAS
+// arrays); < 0 is a leaf: ~leaf where leaf is a triangle slot (BLAS) or an
+// instance index (TLAS).
+struct alignas(16) Node {
+    float4 a, b, c;
+    int4 d;
+};
+
+// Instance record read by the traversal, 64 B: world->object rows and
+// (root_ref, geometry, flags, 0).  flags bit0 = opaque, bit1 = valid.
+struct alignas(16) Inst {
+    float4 w2o[3];
+    int4 info;
+};
+
+enum : int { kInstOpaque = 1, kInstValid = 2 };
+constexpr int kEmptyRef = 0x7fffffff;  // root ref of a BLAS without triangles
+
+// Everything the traversal and shading kernels read, by value.
+struct SceneView {
+    const Node* blas;
+    const float4* tri;        // 3 float4 per triangle slot; tri[3s].w = local prim id bits
+    const Node* tlas;
+    int tlas_root;
+    const Inst* inst;
+    uint32_t n_inst;
+    // shading data (reference SSBO layouts)
+    const float4* verts;      // kf_vertex as 3 x float4
+    const uint32_t* idx;      // 3 per triangle (local)
+    const uint32_t* matidx;   // per triangle
+    const uint32_t* voff;     // per geometry
+    const uint32_t* toff;     // per geometry
+    const kf_material* mats;
+    uint32_t n_mats;
+    const uint8_t* tex_data;  // all RGBA8 textures
+    const uint4* tex_info;    // (byte offset, width, height, 0)
+    uint32_t n_tex;
+    uint32_t env_kind;
+    const uint4* env_info;    // 6 faces, same encoding as tex_info
+    float env_grad[8];
+    const float* srgb_lut;    // 256 floats: sRGB byte -> linear
+};
+
+// ------------------------------------------------------------------ scene
+struct Scene {
+    uint32_t n_geom = 0, n_tri = 0, n_vert = 0, n_inst = 0, n_mat = 0;
+    std::vector<uint32_t> h_voff, h_toff, h_gflags, h_node_off;
+    std::vector<int> h_blas_root;
+    // device buffers
+    float4* d_verts = nullptr;
+    uint32_t *d_idx = nullptr, *d_matidx = nullptr, *d_voff = nullptr, *d_toff = nullptr;
+    uint32_t* d_tri_geom = nullptr;
+    kf_material* d_mats = nullptr;
+    Node* d_blas = nullptr;
+    float4* d_tri = nullptr;
+    int* d_blas_root = nullptr;
+    uint32_t n_blas_nodes = 0;
+    Node* d_tlas = nullptr;
+    Inst* d_inst = nullptr;
+    kf_instance* d_inst_src = nullptr;
+    int tlas_root = kEmptyRef;
+    uint8_t* d_tex = nullptr;
+    uint4* d_tex_info = nullptr;
+    uint32_t n_tex = 0;
+    uint4* d_env_info = nullptr;
+    uint8_t* d_env = nullptr;
+    uint32_t env_kind = 0;
+    float env_grad[8] = {0};
+    float* d_srgb = nullptr;
+    kf_dir_light dl{};
+    kf_point_lights pl{};
+    kf_active_lights al{};
+    float build_ms = 0.f;
+    std::vector<hipEvent_t> events;
+
+    SceneView view() const;
+    ~Scene();
+};
+
+// ------------------------------------------------------------------ builders (bvh.hip)
+// Batched LBVH over n prims in n_seg segments (prims of segment s occupy
+// [seg_off[s], seg_off[s+1])).  Writes nodes (internal nodes of segment s at
+// node_off[s] ...), the sorted prim order, and each segment's root ref.
+// leaf_is_slot: leaf refs are ~slot (BLAS) instead of ~prim (TLAS).
+void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_seg, uint32_t n_seg,
+                const uint32_t* d_seg_off, const uint32_t* d_node_off, Node* d_nodes, uint32_t* d_order,
+                int* d_root, bool leaf_is_slot, hipStream_t st);
+void build_blas(Scene& s);
+void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n);
+
+// ------------------------------------------------------------------ renderer
+struct Renderer {
+    uint32_t W = 0, H = 0;
+    float4 *d_rgba = nullptr, *d_albedo = nullptr, *d_normal = nullptr;
+    uint8_t* d_bgra = nullptr;
+    // wavefront state, sized for `cap` paths and `slots` NEE slots per path
+    size_t cap = 0;
+    uint32_t slots = 0;
+    float4 *ray_o = nullptr, *ray_d = nullptr, *thr = nullptr, *col = nullptr, *rw = nullptr, *em = nullptr;
+    float4* hit = nullptr;
+    uint32_t* hit_prim = nullptr;
+    uint32_t* seed = nullptr;
+    uint32_t *queue_a = nullptr, *queue_b = nullptr;
+    float4 *sh_o = nullptr, *sh_d = nullptr;   // shadow queue (o.xyz, slot bits), (d.xyz, tmax)
+    float4 *nee_a = nullptr, *nee_b = nullptr; // per slot (c_a.xyz, theta), (c_b.xyz, mode|group bits)
+    uint8_t* vis = nullptr;
+    float* samples = nullptr;                  // spp * W * H * 3
+    size_t samples_cap = 0;
+    uint32_t samples_spp = 0;
+    unsigned long long* counters = nullptr;    // device counters
+    void* d_lights = nullptr;                  // DevLights of the frame
+    uint2* d_jump = nullptr;                   // LCG jump-ahead per sample
+    uint32_t jump_cap = 0;
+    float4* d_acc = nullptr;                   // per-pixel running sample sum
+    float4* d_out = nullptr;                   // per-path final radiance of a batch
+    size_t out_cap = 0;
+    std::vector<hipEvent_t> ev;
+    ~Renderer();
+    void ensure(size_t paths, uint32_t nslots, uint32_t spp);
+};
+
+}  // namespace kf
+
+struct kf_scene : kf::Scene {};
+struct kf_renderer : kf::Renderer {};

@@ -1,0 +1,854 @@
+// pathtrace.hip — wavefront path tracer for gfx950.
+//
+// Replaces RayTracer::trace -> vkCmdTraceRaysKHR(W,H,1) (rt.cpp:637-666) and
+// the ray-tracing programs it runs (PathTrace.rgen/.rchit/.rahit/.rmiss,
+// PathTraceShadow.rmiss).  The per-pixel megakernel of the reference becomes
+// a wavefront over (pixel, sample) paths of one batch:
+//   k_raygen   camera rays, seeds                          (rgen:22-66)
+//   k_extend   closest-hit traversal of the active queue   (rgen:97-107)
+//   k_shade    closest hit / miss: BSDF sample, emission,
+//              NEE shadow-ray generation                   (rchit:322-469, rmiss)
+//   k_shadow   any-hit traversal of the shadow queue       (rchit:175-200)
+//   k_connect  light contributions (in slot order), path
+//              accumulation, Russian roulette, compaction  (rgen:109-138)
+//   k_resolve  per-pixel sample sum, running mean, AOVs    (rgen:140-166)
+// Every path consumes its random numbers in the reference's order, so the
+// per-sample radiance equals the FP32 oracle's.  When a light needs random
+// perturbation (softness / radius != 0) the draws of one light depend on the
+// visibility of the previous one, so k_shade then traces its shadow rays
+// inline (NEE_INLINE) instead of deferring them to k_shadow.
+#include "kf_internal.h"
+#include "shading.h"
+#include "traverse.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace kf {
+
+namespace {
+
+constexpr int TB = 256;
+
+// Device counters of one frame: per bounce b: [0] active paths, [1] shadow
+// rays, [2] next queue.  Global: primary / extension / shadow / nodes / tris
+// / stack overflow at the end.
+constexpr int kMaxBounces = 64;
+constexpr int kCntPerBounce = 3;
+constexpr int kCntGlobal = kMaxBounces * kCntPerBounce;
+constexpr int kCntSize = kCntGlobal + 8;
+
+struct Frame {
+    kf_camera_ubo cam;
+    kf_push_constants pc;
+    uint32_t W, H, WH;
+    uint32_t s0, nsamp;   // samples [s0, s0 + nsamp) in this batch
+    uint32_t S0;          // frame seed
+    uint32_t slots;       // NEE slots per path
+    int inline_nee;
+    int count;            // count traversal steps
+};
+
+KF_DEV void wave_add(unsigned long long* p, uint32_t v) {
+    // wave64 reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(p, (unsigned long long)v);
+}
+
+// Wave-aggregated queue append; returns the slot index for lanes with pred.
+KF_DEV uint32_t wave_append(unsigned long long* counter, bool pred) {
+    unsigned long long m = __ballot(pred);
+    uint32_t lane = threadIdx.x & 63;
+    uint32_t leader = m ? (uint32_t)__ffsll((long long)m) - 1 : 0;
+    uint32_t base = 0;
+    if (m && lane == leader) base = (uint32_t)atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    return base + (uint32_t)__popcll(m & lt);
+}
+
+__global__ void k_zero(unsigned long long* c, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = 0;
+}
+
+__global__ __launch_bounds__(TB) void k_raygen(Frame F, const uint2* __restrict__ jump, float4* __restrict__ ray_o,
+                                               float4* __restrict__ ray_d, float4* __restrict__ thr,
+                                               float4* __restrict__ col, float4* __restrict__ rw,
+                                               uint32_t* __restrict__ seed, uint32_t* __restrict__ queue,
+                                               unsigned long long* __restrict__ cnt) {
+    uint32_t n = F.WH * F.nsamp;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t pixel = p % F.WH;
+        uint32_t i = F.s0 + p / F.WH;
+        uint32_t x = pixel % F.W, y = pixel / F.W;
+        // jitter stream: tea(pixel, S0) advanced by 2i draws (rgen:23, 35)
+        uint32_t js = tea(pixel, F.S0);
+        uint2 jm = jump[i];
+        js = jm.x * js + jm.y;
+        float r0 = rnd(js);
+        float r1 = rnd(js);
+        uint32_t rs = tea(pixel, F.S0 + 1u + i);
+        float posx = float(x) + r0, posy = float(y) + r1;
+        float npx = posx / float(F.W), npy = posy / float(F.H);
+        float dx = npx * 2.0f - 1.0f, dy = npy * 2.0f - 1.0f;
+        float aperture = F.cam.position[3];
+        float focus = F.cam.front[3];
+        float ox, oy;
+        do {  // diskSampling (Sampling.glsl:124-134), always drawn (rgen:42)
+            float a = rnd(rs), b = rnd(rs);
+            ox = 2.0f * a - 1.0f;
+            oy = 2.0f * b - 1.0f;
+        } while (ox * ox + oy * oy >= 1.0f);
+        float half = aperture / 2.0f;
+        ox = half * ox;
+        oy = half * oy;
+        float tv[4] = {dx, dy, 1.0f, 1.0f};
+        float target[4];
+        mat4_mul(F.cam.proj_inverse, tv, target);
+        float org[4], dir4[4];
+        if (aperture > 0.0f) {
+            float o4[4] = {ox, oy, 0.0f, 1.0f};
+            mat4_mul(F.cam.view_inverse, o4, org);
+            v3 nd = normalize(mk(target[0], target[1], target[2]) * focus - mk(ox, oy, 0.0f));
+            float d4[4] = {nd.x, nd.y, nd.z, 0.0f};
+            mat4_mul(F.cam.view_inverse, d4, dir4);
+        } else {
+            float o4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+            mat4_mul(F.cam.view_inverse, o4, org);
+            v3 nd = normalize(mk(target[0], target[1], target[2]));
+            float d4[4] = {nd.x, nd.y, nd.z, 0.0f};
+            mat4_mul(F.cam.view_inverse, d4, dir4);
+        }
+        ray_o[p] = make_float4(org[0], org[1], org[2], 0.0f);
+        ray_d[p] = make_float4(dir4[0], dir4[1], dir4[2], 0.0f);
+        thr[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+        col[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rw[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        seed[p] = rs;
+        queue[p] = p;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt[0 * kCntPerBounce + 0] = n;  // active paths at bounce 0
+        atomicAdd(&cnt[kCntGlobal + 0], (unsigned long long)n);
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, const uint32_t* __restrict__ queue,
+                                               const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
+                                               const uint32_t* __restrict__ seed, float4* __restrict__ hit,
+                                               uint32_t* __restrict__ hit_prim, unsigned long long* __restrict__ cnt) {
+    uint32_t n = (uint32_t)cnt[bounce * kCntPerBounce + 0];
+    uint32_t nodes = 0, tris = 0, ovf = 0;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        uint32_t p = queue[q];
+        float4 o = ray_o[p], d = ray_d[p];
+        THit h;
+        TCount c{0, 0};
+        traverse<false, COUNT>(S, xyz(o), xyz(d), 0.001f, 10000.0f, true, seed[p], h, c, ovf);
+        hit[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.inst));
+        hit_prim[p] = h.prim | (h.back << 31);
+        nodes += c.nodes;
+        tris += c.tris;
+    }
+    if (COUNT) {
+        wave_add(&cnt[kCntGlobal + 3], nodes);
+        wave_add(&cnt[kCntGlobal + 4], tris);
+    }
+    if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
+}
+
+// Closest-hit / miss for one path; writes next ray, emission, BSDF weight and
+// NEE slots (deferred) or the NEE sum (inline).
+template <bool NEE_INLINE>
+__global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLights* __restrict__ LT, int bounce,
+                                              const uint32_t* __restrict__ queue, float4* __restrict__ ray_o,
+                                              float4* __restrict__ ray_d, const float4* __restrict__ thr,
+                                              const float4* __restrict__ hit, const uint32_t* __restrict__ hit_prim,
+                                              float4* __restrict__ rw, float4* __restrict__ em,
+                                              uint32_t* __restrict__ seed, float4* __restrict__ nee_a,
+                                              float4* __restrict__ nee_b, uint8_t* __restrict__ vis,
+                                              float4* __restrict__ sh_o, float4* __restrict__ sh_d,
+                                              float4* __restrict__ albedo_out, float4* __restrict__ normal_out,
+                                              unsigned long long* __restrict__ cnt) {
+    uint32_t n = (uint32_t)cnt[bounce * kCntPerBounce + 0];
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t iters = (n + stride - 1) / stride;
+    const uint32_t K = F.slots;
+    uint32_t inline_shadow = 0, ovf = 0;
+    for (uint32_t it = 0; it < iters; ++it) {
+        uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool active = q < n;
+        uint32_t p = active ? queue[q] : 0;
+        uint32_t depth = active ? __float_as_uint(thr[p].w) : 0;
+        bool aov = active && depth == 0 && F.s0 == 0 && p < F.WH;  // sample 0, first segment
+        if (active) {
+            float4 h = hit[p];
+            float4 ro = ray_o[p], rd = ray_d[p];
+            v3 rdir = xyz(rd);
+            uint32_t rs = seed[p];
+            uint32_t inst = __float_as_uint(h.w);
+            for (uint32_t j = 0; j < K; ++j) vis[(size_t)p * K + j] = 0;
+            if (inst == ~0u) {
+                // miss (rmiss:12-31)
+                v3 e = miss_emission(S, F.pc, rdir);
+                em[p] = f4(e, __uint_as_float(1u));
+                if (aov) {
+                    albedo_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+                    normal_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+                }
+                if (NEE_INLINE) nee_a[(size_t)p * K] = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                uint32_t hp = hit_prim[p];
+                uint32_t prim = hp & 0x7fffffffu;
+                bool inside = (hp >> 31) != 0;
+                Inst I = S.inst[inst];
+                uint32_t g = (uint32_t)I.info.y;
+                size_t tri = (size_t)S.toff[g] + prim;
+                uint32_t vb = S.voff[g];
+                uint32_t i0 = S.idx[3 * tri + 0], i1 = S.idx[3 * tri + 1], i2 = S.idx[3 * tri + 2];
+                const float4* V0 = S.verts + 3 * (size_t)(vb + i0);
+                const float4* V1 = S.verts + 3 * (size_t)(vb + i1);
+                const float4* V2 = S.verts + 3 * (size_t)(vb + i2);
+                float4 a0 = V0[0], a1 = V0[1], a2v = V0[2];
+                float4 b0 = V1[0], b1 = V1[1], b2v = V1[2];
+                float4 c0 = V2[0], c1 = V2[1], c2v = V2[2];
+                float bx = 1.0f - h.y - h.z, by = h.y, bz = h.z;
+                v3 n0 = mk(a0.w, a1.x, a1.y), n1 = mk(b0.w, b1.x, b1.y), n2 = mk(c0.w, c1.x, c1.y);
+                v3 ln = n0 * bx + n1 * by + n2 * bz;
+                v3 wn = mk(ln.x * I.w2o[0].x + ln.y * I.w2o[1].x + ln.z * I.w2o[2].x,
+                           ln.x * I.w2o[0].y + ln.y * I.w2o[1].y + ln.z * I.w2o[2].y,
+                           ln.x * I.w2o[0].z + ln.y * I.w2o[1].z + ln.z * I.w2o[2].z);
+                v3 N = normalize(wn);
+                v3 worldPos = xyz(ro) + rdir * h.x;
+                float uvx = a2v.y * bx + b2v.y * by + c2v.y * bz;
+                float uvy = a2v.z * bx + b2v.z * by + c2v.z * bz;
+                const kf_material& m = S.mats[S.matidx[tri]];
+                v3 emission = mk(m.emission[0], m.emission[1], m.emission[2]) * m.emission[3];
+                if (!all_zero(emission)) {
+                    em[p] = f4(emission, __uint_as_float(1u));
+                    if (aov) {
+                        albedo_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+                        normal_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+                    }
+                    if (NEE_INLINE) nee_a[(size_t)p * K] = make_float4(0.f, 0.f, 0.f, 0.f);
+                } else {
+                    auto tex = [&](int id) -> float4 {
+                        return tex_sample(S, S.tex_data, S.tex_info[id], uvx, uvy, true);
+                    };
+                    auto valid_tex = [&](int id) { return id >= 0 && (uint32_t)id < S.n_tex; };
+                    v3 base = mk(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+                    if (valid_tex(m.diffuse_tex)) base = xyz(tex(m.diffuse_tex));
+                    base = base / kPi;
+                    float metallic = m.metallic;
+                    if (valid_tex(m.metallic_tex)) metallic = tex(m.metallic_tex).x;
+                    float roughness = m.roughness;
+                    if (valid_tex(m.roughness_tex)) roughness = tex(m.roughness_tex).x;
+                    float a2 = roughness * roughness;
+                    float transmission = m.transmission;
+                    if (valid_tex(m.transmission_tex)) transmission = tex(m.transmission_tex).x;
+                    float f = gmax(m.ior, 1e-5f);
+                    float diffuse_weight = (1.0f - gclamp(metallic, 0.0f, 1.0f)) * (1.0f - gclamp(transmission, 0.0f, 1.0f));
+                    float final_transmission = gclamp(transmission, 0.0f, 1.0f) * (1.0f - gclamp(metallic, 0.0f, 1.0f));
+                    float specular_weight = 1.0f - final_transmission;
+                    v3 weight = splat(0.0f);
+                    v3 V = normalize(-rdir);
+                    N = inside ? -N : N;
+                    v3 L = splat(0.0f);
+                    float NdotV = dot(N, V);
+                    v3 diffuseColor = diffuse_weight * base;
+                    v3 specularColor = specular_weight * (base * metallic + splat(m.specular * 0.08f) * (1.0f - metallic));
+                    v3 transmissionColor = transmission * base;
+                    float dl = length(diffuseColor), sl = length(specularColor);
+                    float pD = dl / (dl + sl);
+                    if (dl == 0.0f && sl == 0.0f) {
+                        if (all_zero(base)) {
+                            if (diffuse_weight == 1.0f) pD = 1.0f;
+                            else if (diffuse_weight == 0.0f) pD = 0.0f;
+                            else pD = 0.5f;
+                        } else {
+                            pD = 0.0f;
+                        }
+                    } else {
+                        pD *= specular_weight;
+                    }
+                    bool chooseDiffuse = rnd(rs) < pD;
+                    if (chooseDiffuse) {
+                        L = cosine_hemisphere(rs, N);
+                        float NdotL = gclamp(dot(N, L), 0.0f, 1.0f);
+                        weight = kPi * diffuseColor * NdotL / pD;
+                    } else if (final_transmission == 0.0f) {
+                        v3 H = sample_ggx(rs, a2, N);
+                        float HdotV = dot(H, V);
+                        L = 2.0f * HdotV * H - V;
+                        float NoV = gmax(dot(N, V), 1e-7f);
+                        float NoL = dot(N, L);
+                        float NoH = gmax(dot(N, H), 1e-7f);
+                        float VoH = gmax(dot(V, H), 1e-7f);
+                        if (NoL >= 0.0f) {
+                            float G = geom_shadow(NoV, NoL, a2);
+                            v3 F = schlick_fresnel(specularColor, VoH);
+                            weight = kPi * F * G * VoH / (NoH * NoV * (1.0f - pD));
+                        } else {
+                            weight = splat(0.0f);
+                        }
+                    } else {
+                        float ior = inside ? 1.0f / f : f;
+                        float dd = inside ? NdotV * ior : NdotV;
+                        v3 refr = refract(-V, N, 1.0f / f);
+                        float reflectProb = !all_zero(refr) ? schlick(dd, f) : 1.0f;
+                        if (rnd(rs) >= reflectProb) {
+                            L = refr;
+                            weight = kPi * transmissionColor / (1.0f - pD);
+                        } else {
+                            L = reflect(-V, N);
+                            weight = kPi * transmissionColor / (1.0f - pD);
+                        }
+                    }
+                    ShadeCtx sh;
+                    sh.N = N; sh.V = V; sh.worldPos = worldPos; sh.f = f; sh.a2 = a2;
+                    sh.diffuseColor = diffuseColor; sh.specularColor = specularColor;
+                    sh.transmissionColor = transmissionColor; sh.inside = inside;
+                    // NEE slots in evaluation order (rchit:457-460)
+                    v3 sum_d = splat(0.0f), sum_p = splat(0.0f), sum_a = splat(0.0f);
+                    for (uint32_t j = 0; j < K; ++j) {
+                        int kind = LT->kind[j], li = LT->index[j];
+                        v3 Ld, lem;
+                        float maxd;
+                        bool ok = true;
+                        if (kind == 0) {  // traceDirectionalLight (rchit:204-223)
+                            lem = mk(LT->dl.rgbs[0], LT->dl.rgbs[1], LT->dl.rgbs[2]) * LT->dl.rgbs[3];
+                            Ld = -mk(LT->dl.direction[0], LT->dl.direction[1], LT->dl.direction[2]);
+                            if (NEE_INLINE && LT->dl.direction[3] != 0.0f) {
+                                float a = rnd(rs), b = rnd(rs), c = rnd(rs);
+                                Ld = normalize(Ld + LT->dl.direction[3] * mk(a, b, c));
+                            }
+                            maxd = 1e6f;
+                        } else if (kind == 1) {  // tracePointLights (rchit:227-258)
+                            v3 rgb = mk(LT->pl.rgbs[li][0], LT->pl.rgbs[li][1], LT->pl.rgbs[li][2]);
+                            v3 lpos = mk(LT->pl.posr[li][0], LT->pl.posr[li][1], LT->pl.posr[li][2]);
+                            if (NEE_INLINE && LT->pl.posr[li][3] != 0.0f) {
+                                v3 pert = uniform_sphere(rs);
+                                lpos = lpos + LT->pl.posr[li][3] * normalize(pert);
+                            }
+                            v3 Lv = lpos - worldPos;
+                            float dd2 = length(Lv);
+                            Ld = normalize(Lv);
+                            lem = rgb * LT->pl.rgbs[li][3] / dd2 / dd2;
+                            maxd = dd2;
+                        } else {  // traceActiveLights (rchit:262-316)
+                            v3 rgb = mk(LT->al.rgbs[li][0], LT->al.rgbs[li][1], LT->al.rgbs[li][2]);
+                            float softness = LT->al.sftp[li][0];
+                            v3 lpos = mk(LT->al.position[li][0], LT->al.position[li][1], LT->al.position[li][2]);
+                            if (NEE_INLINE && softness != 0.0f) {
+                                v3 pert = uniform_sphere(rs);
+                                lpos = lpos + softness * normalize(pert);
+                            }
+                            v3 Lv = lpos - worldPos;
+                            float dd2 = length(Lv);
+                            Ld = normalize(Lv);
+                            float fov = LT->al.sftp[li][1];
+                            v3 ldir = normalize(mk(LT->al.front[li][0], LT->al.front[li][1], LT->al.front[li][2]));
+                            float halfA = gclamp(fov, 0.0f, kPi) / 2.0f;
+                            float cs = dot(ldir, -Ld);
+                            ok = cs > cos_k(halfA);
+                            v3 color = rgb;
+                            int texID = (int)LT->al.sftp[li][2];
+                            if (ok && texID >= 0 && (uint32_t)texID < S.n_tex) {
+                                float v4[4] = {worldPos.x, worldPos.y, worldPos.z, 1.0f};
+                                float tc[4];
+                                mat4_mul(LT->pv[li], v4, tc);
+                                float u = (tc[0] / tc[3]) * 0.5f + 0.5f;
+                                float v = (tc[1] / tc[3]) * 0.5f + 0.5f;
+                                float4 tx = tex_sample(S, S.tex_data, S.tex_info[texID], u, v, true);
+                                color = color * xyz(tx);
+                            }
+                            lem = color * LT->al.rgbs[li][3] / dd2 / dd2;
+                            maxd = dd2;
+                        }
+                        if (!ok) continue;
+                        float NdotL = dot(N, Ld);
+                        if (!(NdotL > 0.0f)) continue;
+                        if (NEE_INLINE) {
+                            THit hh;
+                            TCount cc{0, 0};
+                            inline_shadow++;
+                            bool occ = traverse<true, false>(S, worldPos, Ld, 0.001f, maxd, false, 0u, hh, cc, ovf);
+                            if (!occ) {
+                                v3 ca, cb;
+                                float th;
+                                int mode;
+                                direct_split(sh, Ld, lem, ca, cb, th, mode);
+                                v3 cj = direct_resolve(ca, cb, th, mode, rs);
+                                if (kind == 0) sum_d = sum_d + cj;
+                                else if (kind == 1) sum_p = sum_p + cj;
+                                else sum_a = sum_a + cj;
+                            }
+                        } else {
+                            v3 ca, cb;
+                            float th;
+                            int mode;
+                            direct_split(sh, Ld, lem, ca, cb, th, mode);
+                            size_t slot = (size_t)p * K + j;
+                            nee_a[slot] = f4(ca, th);
+                            nee_b[slot] = f4(cb, __int_as_float(mode | (kind << 4)));
+                            uint32_t at = (uint32_t)atomicAdd(&cnt[bounce * kCntPerBounce + 1], 1ull);
+                            sh_o[at] = f4(worldPos, __uint_as_float((uint32_t)slot));
+                            sh_d[at] = f4(Ld, maxd);
+                        }
+                    }
+                    if (NEE_INLINE) nee_a[(size_t)p * K] = f4((sum_d + sum_p) + sum_a, 0.0f);
+                    ray_o[p] = f4(worldPos, 0.0f);
+                    ray_d[p] = f4(L, 0.0f);
+                    em[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
+                    rw[p] = f4(weight, 0.0f);
+                    if (aov) {
+                        albedo_out[p] = f4(base, 1.0f);
+                        normal_out[p] = f4(N, 1.0f);
+                    }
+                }
+            }
+            seed[p] = rs;
+        }
+    }
+    if (NEE_INLINE) {
+        wave_add(&cnt[kCntGlobal + 2], inline_shadow);
+        if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, const float4* __restrict__ sh_o,
+                                               const float4* __restrict__ sh_d, uint8_t* __restrict__ vis,
+                                               unsigned long long* __restrict__ cnt) {
+    uint32_t n = (uint32_t)cnt[bounce * kCntPerBounce + 1];
+    uint32_t ovf = 0;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        float4 o = sh_o[q], d = sh_d[q];
+        THit h;
+        TCount c{0, 0};
+        bool occ = traverse<true, false>(S, xyz(o), xyz(d), 0.001f, d.w, false, 0u, h, c, ovf);
+        if (!occ) vis[__float_as_uint(o.w)] = 1;
+    }
+    if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
+}
+
+template <bool NEE_INLINE>
+__global__ __launch_bounds__(TB) void k_connect(Frame F, int bounce, const uint32_t* __restrict__ queue,
+                                                uint32_t* __restrict__ next, float4* __restrict__ thr,
+                                                float4* __restrict__ col, const float4* __restrict__ rw,
+                                                const float4* __restrict__ em, uint32_t* __restrict__ seed,
+                                                const float4* __restrict__ nee_a, const float4* __restrict__ nee_b,
+                                                const uint8_t* __restrict__ vis, float4* __restrict__ out,
+                                                unsigned long long* __restrict__ cnt) {
+    uint32_t n = (uint32_t)cnt[bounce * kCntPerBounce + 0];
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t iters = (n + stride - 1) / stride;
+    const uint32_t K = F.slots;
+    const uint32_t maxDepth = F.pc.path_depth;
+    for (uint32_t it = 0; it < iters; ++it) {
+        uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool cont = false;
+        uint32_t p = 0;
+        if (q < n) {
+            p = queue[q];
+            float4 t4 = thr[p];
+            v3 weight = xyz(t4);
+            uint32_t depth = __float_as_uint(t4.w);
+            v3 color = xyz(col[p]);
+            float4 e4 = em[p];
+            bool term = __float_as_uint(e4.w) != 0u;
+            uint32_t rs = seed[p];
+            v3 shadow = splat(0.0f);
+            if (!term) {
+                if (NEE_INLINE) {
+                    shadow = xyz(nee_a[(size_t)p * K]);
+                } else {
+                    v3 sd = splat(0.0f), sp = splat(0.0f), sa = splat(0.0f);
+                    for (uint32_t j = 0; j < K; ++j) {
+                        size_t slot = (size_t)p * K + j;
+                        if (!vis[slot]) continue;
+                        float4 a = nee_a[slot], b = nee_b[slot];
+                        int mb = __float_as_int(b.w);
+                        v3 cj = direct_resolve(xyz(a), xyz(b), a.w, mb & 15, rs);
+                        int kind = mb >> 4;
+                        if (kind == 0) sd = sd + cj;
+                        else if (kind == 1) sp = sp + cj;
+                        else sa = sa + cj;
+                    }
+                    shadow = (sd + sp) + sa;
+                }
+            }
+            // rgen:109-138
+            color = color + xyz(e4) * weight;
+            weight = weight * xyz(rw[p]);
+            color = color + shadow * weight;
+            uint32_t d2 = term ? maxDepth + 1 : depth;
+            cont = true;
+            if (all_zero(weight)) {
+                cont = false;
+            } else if (F.pc.russian_roulette && d2 >= F.pc.rr_min_bounces) {
+                float pr = gmax(weight.x, gmax(weight.y, weight.z));
+                float rr = rnd(rs);
+                if (rr > pr) cont = false;
+                else weight = weight * (1.0f / pr);
+            }
+            uint32_t nd = d2 + 1;
+            if (nd > maxDepth) cont = false;
+            if (cont) {
+                thr[p] = f4(weight, __uint_as_float(nd));
+                col[p] = f4(color, 0.0f);
+                seed[p] = rs;
+            } else {
+                out[p] = f4(color, 0.0f);
+            }
+        }
+        uint32_t at = wave_append(&cnt[bounce * kCntPerBounce + 2], cont);
+        if (cont) next[at] = p;
+    }
+}
+
+__global__ void k_next_count(int bounce, unsigned long long* cnt) {
+    unsigned long long nx = cnt[bounce * kCntPerBounce + 2];
+    cnt[(bounce + 1) * kCntPerBounce + 0] = nx;
+    cnt[kCntGlobal + 1] += nx;
+    cnt[kCntGlobal + 2] += cnt[bounce * kCntPerBounce + 1];
+}
+
+// Sum the batch's samples into acc in sample order (rgen:140), then on the
+// last batch: average, running mean over frames (rgen:153-163).
+__global__ __launch_bounds__(TB) void k_resolve(Frame F, const float4* __restrict__ out, float4* __restrict__ acc,
+                                                float4* __restrict__ rgba, float* __restrict__ keep, int last) {
+    uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= F.WH) return;
+    v3 c = F.s0 == 0 ? splat(0.0f) : xyz(acc[px]);
+    for (uint32_t i = 0; i < F.nsamp; ++i) {
+        v3 s = xyz(out[(size_t)i * F.WH + px]);
+        c = c + s;
+        if (keep) {
+            float* k = keep + 3 * ((size_t)(F.s0 + i) * F.WH + px);
+            k[0] = s.x; k[1] = s.y; k[2] = s.z;
+        }
+    }
+    if (!last) {
+        acc[px] = f4(c, 0.0f);
+        return;
+    }
+    v3 fc = c / float(F.pc.spp);
+    if (F.pc.frame_count <= 0) {
+        rgba[px] = f4(fc, 1.0f);
+    } else {
+        float a = 1.0f / float(F.pc.frame_count + 1);
+        v3 old = xyz(rgba[px]);
+        v3 m = old * (1.0f - a) + fc * a;
+        rgba[px] = f4(m, 1.0f);
+    }
+}
+
+// B8G8R8A8_SRGB bytes of the rgba target (PostProcessing.frag:17 samples the
+// texel; the attachment's sRGB encode clamps, encodes and rounds).
+__global__ void k_to_bgra8(uint32_t n, const float4* __restrict__ rgba, const double* __restrict__ thr,
+                           uchar4* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 c = rgba[i];
+    auto enc = [&](float x) -> unsigned char {
+        if (!(x > 0.0f)) return 0;
+        int lo = 0, hi = 255;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if ((double)x >= thr[mid]) lo = mid + 1; else hi = mid;
+        }
+        return (unsigned char)lo;
+    };
+    unsigned char a;
+    if (!(c.w > 0.0f)) a = 0;
+    else if (c.w >= 1.0f) a = 255;
+    else a = (unsigned char)(int)rintf(c.w * 255.0f);
+    out[i] = make_uchar4(enc(c.z), enc(c.y), enc(c.x), a);
+}
+
+template <class T> T* dmalloc(size_t n) {
+    T* p = nullptr;
+    if (n) KF_CHECK(hipMalloc((void**)&p, n * sizeof(T)));
+    return p;
+}
+
+}  // namespace
+
+Renderer::~Renderer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    void* ptrs[] = {d_rgba, d_albedo, d_normal, d_bgra, ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed,
+                    queue_a, queue_b, sh_o, sh_d, nee_a, nee_b, vis, samples, counters, d_lights, d_jump, d_acc, d_out};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+void Renderer::ensure(size_t paths, uint32_t nslots, uint32_t spp) {
+    if (paths > cap || nslots > slots) {
+        void* ptrs[] = {ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed, queue_a, queue_b,
+                        sh_o, sh_d, nee_a, nee_b, vis};
+        for (void* p : ptrs)
+            if (p) KF_CHECK(hipFree(p));
+        cap = std::max(cap, paths);
+        slots = std::max(slots, std::max<uint32_t>(nslots, 1));
+        ray_o = dmalloc<float4>(cap);
+        ray_d = dmalloc<float4>(cap);
+        thr = dmalloc<float4>(cap);
+        col = dmalloc<float4>(cap);
+        rw = dmalloc<float4>(cap);
+        em = dmalloc<float4>(cap);
+        hit = dmalloc<float4>(cap);
+        hit_prim = dmalloc<uint32_t>(cap);
+        seed = dmalloc<uint32_t>(cap);
+        queue_a = dmalloc<uint32_t>(cap);
+        queue_b = dmalloc<uint32_t>(cap);
+        sh_o = dmalloc<float4>(cap * slots);
+        sh_d = dmalloc<float4>(cap * slots);
+        nee_a = dmalloc<float4>(cap * slots);
+        nee_b = dmalloc<float4>(cap * slots);
+        vis = dmalloc<uint8_t>(cap * slots);
+    }
+    size_t sneed = (size_t)spp * W * H * 3;
+    if (spp && sneed > samples_cap) {
+        if (samples) KF_CHECK(hipFree(samples));
+        samples = dmalloc<float>(sneed);
+        samples_cap = sneed;
+    }
+    if (!counters) counters = dmalloc<unsigned long long>(kCntSize);
+}
+
+// ------------------------------------------------------------------ frame
+struct FrameOut {
+    float trace_ms, extend_ms, shadow_ms;
+};
+
+void build_light_list(const Scene& s, DevLights& L, bool& inline_nee) {
+    L.dl = s.dl;
+    L.pl = s.pl;
+    L.al = s.al;
+    inline_nee = false;
+    int n = 0;
+    // traceDirectionalLight returns early when rgb * strength == 0
+    float em[3] = {s.dl.rgbs[0] * s.dl.rgbs[3], s.dl.rgbs[1] * s.dl.rgbs[3], s.dl.rgbs[2] * s.dl.rgbs[3]};
+    if (em[0] != 0.f || em[1] != 0.f || em[2] != 0.f) {
+        L.kind[n] = 0; L.index[n] = 0; ++n;
+        if (s.dl.direction[3] != 0.f) inline_nee = true;
+    }
+    for (int i = 0; i < KF_MAX_POINT_LIGHTS; ++i) {
+        if (!(s.pl.rgbs[i][3] > 0.f)) continue;
+        float r = s.pl.rgbs[i][0] * s.pl.rgbs[i][3], g = s.pl.rgbs[i][1] * s.pl.rgbs[i][3],
+              b = s.pl.rgbs[i][2] * s.pl.rgbs[i][3];
+        if (sqrtf(r * r + g * g + b * b) == 0.f) continue;
+        L.kind[n] = 1; L.index[n] = i; ++n;
+        if (s.pl.posr[i][3] != 0.f) inline_nee = true;
+    }
+    for (int i = 0; i < KF_MAX_ACTIVE_LIGHTS; ++i) {
+        if (!(s.al.front[i][3] > 0.f)) continue;
+        float r = s.al.rgbs[i][0] * s.al.rgbs[i][3], g = s.al.rgbs[i][1] * s.al.rgbs[i][3],
+              b = s.al.rgbs[i][2] * s.al.rgbs[i][3];
+        if (sqrtf(r * r + g * g + b * b) == 0.f) continue;
+        L.kind[n] = 2; L.index[n] = i; ++n;
+        if (s.al.sftp[i][0] != 0.f) inline_nee = true;
+        // proj * view, column order (rchit:303)
+        for (int c = 0; c < 4; ++c)
+            for (int r2 = 0; r2 < 4; ++r2)
+                L.pv[i][c * 4 + r2] = s.al.proj[i][0 * 4 + r2] * s.al.view[i][c * 4 + 0] +
+                                      s.al.proj[i][1 * 4 + r2] * s.al.view[i][c * 4 + 1] +
+                                      s.al.proj[i][2 * 4 + r2] * s.al.view[i][c * 4 + 2] +
+                                      s.al.proj[i][3 * 4 + r2] * s.al.view[i][c * 4 + 3];
+    }
+    L.n_slots = n;
+}
+
+}  // namespace kf
+
+using namespace kf;
+
+extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, const kf_push_constants* pc,
+                         const kf_render_opts* opts, kf_render_stats* stats) {
+    try {
+        if (!r || !s || !cam || !pc) throw Error("kf_render: null argument");
+        if (pc->spp == 0) throw Error("kf_render: spp must be > 0");
+        if (pc->path_depth + 2 >= (uint32_t)kMaxBounces) throw Error("kf_render: path depth too large");
+        hipStream_t st = stream();
+        kf_render_opts o{};
+        if (opts) o = *opts;
+        DevLights L;
+        memset(&L, 0, sizeof(L));
+        bool inline_nee = false;
+        build_light_list(*s, L, inline_nee);
+        uint32_t WH = r->W * r->H;
+        uint32_t spb = o.samples_per_batch ? o.samples_per_batch : 0;
+        if (spb == 0) {
+            // default: about 8M paths per batch
+            spb = std::max<uint32_t>(1, std::min<uint32_t>(pc->spp, (8u << 20) / std::max<uint32_t>(WH, 1)));
+        }
+        spb = std::min(spb, pc->spp);
+        uint32_t K = inline_nee ? 1u : (uint32_t)std::max(L.n_slots, 1);
+        r->ensure((size_t)WH * spb, K, o.keep_samples ? pc->spp : 0);
+        r->samples_spp = o.keep_samples ? pc->spp : 0;
+        // device light list (per frame, like the UBO upload)
+        if (!r->d_lights) KF_CHECK(hipMalloc(&r->d_lights, sizeof(DevLights)));
+        DevLights* d_lights = (DevLights*)r->d_lights;
+        KF_CHECK(hipMemcpyAsync(d_lights, &L, sizeof(DevLights), hipMemcpyHostToDevice, st));
+        // LCG jump-ahead for the jitter stream: sample i starts after 2i draws
+        std::vector<uint2> jump(pc->spp);
+        {
+            uint32_t a = 1, c = 0;  // identity
+            for (uint32_t i = 0; i < pc->spp; ++i) {
+                jump[i] = make_uint2(a, c);
+                for (int k = 0; k < 2; ++k) {  // compose with one LCG step
+                    a = 1664525u * a;
+                    c = 1664525u * c + 1013904223u;
+                }
+            }
+        }
+        if (pc->spp > r->jump_cap) {
+            if (r->d_jump) KF_CHECK(hipFree(r->d_jump));
+            KF_CHECK(hipMalloc((void**)&r->d_jump, pc->spp * sizeof(uint2)));
+            r->jump_cap = pc->spp;
+        }
+        uint2* d_jump = r->d_jump;
+        KF_CHECK(hipMemcpyAsync(d_jump, jump.data(), pc->spp * sizeof(uint2), hipMemcpyHostToDevice, st));
+        if (!r->d_acc) KF_CHECK(hipMalloc((void**)&r->d_acc, (size_t)WH * sizeof(float4)));
+        float4* d_acc = r->d_acc;
+        if ((size_t)WH * spb > r->out_cap) {
+            if (r->d_out) KF_CHECK(hipFree(r->d_out));
+            KF_CHECK(hipMalloc((void**)&r->d_out, (size_t)WH * spb * sizeof(float4)));
+            r->out_cap = (size_t)WH * spb;
+        }
+        float4* d_out = r->d_out;
+
+        SceneView S = s->view();
+        Frame F;
+        F.cam = *cam;
+        F.pc = *pc;
+        F.W = r->W;
+        F.H = r->H;
+        F.WH = WH;
+        F.S0 = tea_host(o.seed, (uint32_t)pc->frame_count);
+        F.slots = K;
+        F.inline_nee = inline_nee;
+        F.count = o.count_traversal;
+
+        if (r->ev.empty()) {
+            r->ev.resize(2 + 4 * kMaxBounces);
+            for (auto& e : r->ev) KF_CHECK(hipEventCreate(&e));
+        }
+        int dev = 0;
+        KF_CHECK(hipGetDevice(&dev));
+        int n_cu = 256;
+        KF_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        const uint32_t grid = (uint32_t)n_cu * 8;
+
+        unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
+        float ms_total = 0.f, ms_ext = 0.f, ms_sh = 0.f;
+        for (uint32_t s0 = 0; s0 < pc->spp; s0 += spb) {
+            F.s0 = s0;
+            F.nsamp = std::min(spb, pc->spp - s0);
+            uint32_t B = WH * F.nsamp;
+            k_zero<<<(kCntSize + TB - 1) / TB, TB, 0, st>>>(r->counters, kCntSize);
+            KF_CHECK(hipEventRecord(r->ev[0], st));
+            k_raygen<<<std::min<uint32_t>((B + TB - 1) / TB, 65535u), TB, 0, st>>>(
+                F, d_jump, r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->seed, r->queue_a, r->counters);
+            uint32_t* qa = r->queue_a;
+            uint32_t* qb = r->queue_b;
+            uint32_t nb = pc->path_depth + 1;
+            for (uint32_t b = 0; b < nb; ++b) {
+                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
+                if (o.count_traversal)
+                    k_extend<true><<<grid, TB, 0, st>>>(S, (int)b, qa, r->ray_o, r->ray_d, r->seed, r->hit,
+                                                         r->hit_prim, r->counters);
+                else
+                    k_extend<false><<<grid, TB, 0, st>>>(S, (int)b, qa, r->ray_o, r->ray_d, r->seed, r->hit,
+                                                          r->hit_prim, r->counters);
+                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 1], st));
+                if (inline_nee)
+                    k_shade<true><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->hit,
+                                                       r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis,
+                                                       r->sh_o, r->sh_d, r->d_albedo, r->d_normal, r->counters);
+                else
+                    k_shade<false><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr,
+                                                        r->hit, r->hit_prim, r->rw, r->em, r->seed, r->nee_a,
+                                                        r->nee_b, r->vis, r->sh_o, r->sh_d, r->d_albedo, r->d_normal,
+                                                        r->counters);
+                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], st));
+                if (!inline_nee)
+                    k_shadow<<<grid, TB, 0, st>>>(S, (int)b, r->sh_o, r->sh_d, r->vis, r->counters);
+                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], st));
+                if (inline_nee)
+                    k_connect<true><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
+                                                         r->nee_a, r->nee_b, r->vis, d_out, r->counters);
+                else
+                    k_connect<false><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
+                                                          r->nee_a, r->nee_b, r->vis, d_out, r->counters);
+                k_next_count<<<1, 1, 0, st>>>((int)b, r->counters);
+                std::swap(qa, qb);
+            }
+            bool last = s0 + F.nsamp >= pc->spp;
+            k_resolve<<<(WH + TB - 1) / TB, TB, 0, st>>>(F, d_out, d_acc, r->d_rgba,
+                                                         o.keep_samples ? r->samples : nullptr, last ? 1 : 0);
+            KF_CHECK(hipEventRecord(r->ev[1], st));
+            KF_CHECK(hipGetLastError());
+            unsigned long long h[kCntSize];
+            KF_CHECK(hipMemcpyAsync(h, r->counters, sizeof(h), hipMemcpyDeviceToHost, st));
+            KF_CHECK(hipStreamSynchronize(st));
+            for (int k = 0; k < 6; ++k) totals[k] += h[kCntGlobal + k];
+            float ms = 0.f;
+            KF_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
+            ms_total += ms;
+            for (uint32_t b = 0; b < nb; ++b) {
+                float a = 0.f, c = 0.f;
+                KF_CHECK(hipEventElapsedTime(&a, r->ev[2 + 4 * b + 0], r->ev[2 + 4 * b + 1]));
+                KF_CHECK(hipEventElapsedTime(&c, r->ev[2 + 4 * b + 2], r->ev[2 + 4 * b + 3]));
+                ms_ext += a;
+                ms_sh += c;
+            }
+        }
+        if (totals[5]) throw Error("kf_render: traversal stack overflow (BVH deeper than the stack)");
+        if (stats) {
+            stats->primary_rays = totals[0];
+            stats->extension_rays = totals[1];
+            stats->shadow_rays = totals[2];
+            stats->node_visits = totals[3];
+            stats->tri_tests = totals[4];
+            stats->trace_ms = ms_total;
+            stats->extend_ms = ms_ext;
+            stats->shadow_ms = ms_sh;
+            stats->other_ms = ms_total - ms_ext - ms_sh;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+extern "C" int kf_renderer_download_bgra8(kf_renderer* r, uint8_t* out) {
+    try {
+        if (!r || !out) throw Error("kf_renderer_download_bgra8: null argument");
+        hipStream_t st = stream();
+        uint32_t n = r->W * r->H;
+        static double* d_thr = nullptr;
+        if (!d_thr) {
+            double thr[255];
+            for (int k = 0; k < 255; ++k) {
+                double sv = (k + 0.5) / 255.0;
+                thr[k] = sv <= 0.04045 ? sv / 12.92 : std::pow((sv + 0.055) / 1.055, 2.4);
+            }
+            KF_CHECK(hipMalloc((void**)&d_thr, sizeof(thr)));
+            KF_CHECK(hipMemcpy(d_thr, thr, sizeof(thr), hipMemcpyHostToDevice));
+        }
+        if (!r->d_bgra) KF_CHECK(hipMalloc((void**)&r->d_bgra, 4 * (size_t)n));
+        k_to_bgra8<<<(n + TB - 1) / TB, TB, 0, st>>>(n, r->d_rgba, d_thr, (uchar4*)r->d_bgra);
+        KF_CHECK(hipGetLastError());
+        KF_CHECK(hipMemcpyAsync(out, r->d_bgra, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}

@@ -1,0 +1,150 @@
+// trace.hip — closest-hit / any-hit queries on fixed ray batches.
+//
+// The traceRayEXT calls of PathTrace.rgen:97-107 (path rays) and
+// PathTrace.rchit:186-197 (shadow rays), exposed on their own so that the
+// traversal can be checked and measured on fixed ray batches.
+#include "kf_internal.h"
+#include "traverse.h"
+
+namespace kf {
+namespace {
+
+constexpr int TB = 256;
+
+template <bool COUNT>
+__global__ __launch_bounds__(TB) void k_closest(SceneView S, const kf_ray* __restrict__ rays, uint32_t n,
+                                                kf_hit* __restrict__ hits, unsigned long long* __restrict__ cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t ovf = 0;
+    if (i < n) {
+        kf_ray r = rays[i];
+        THit h;
+        TCount c{0, 0};
+        traverse<false, COUNT>(S, mk(r.org[0], r.org[1], r.org[2]), mk(r.dir[0], r.dir[1], r.dir[2]), r.tmin, r.tmax,
+                               false, 0u, h, c, ovf);
+        kf_hit o;
+        o.t = h.t;
+        o.inst = h.inst;
+        o.prim = h.prim;
+        o.b1 = h.b1;
+        o.b2 = h.b2;
+        o.back = h.back;
+        hits[i] = o;
+        if (COUNT) {
+            atomicAdd(&cnt[0], (unsigned long long)c.nodes);
+            atomicAdd(&cnt[1], (unsigned long long)c.tris);
+        }
+    }
+    if (ovf) atomicAdd(&cnt[2], 1ull);
+}
+
+__global__ __launch_bounds__(TB) void k_any(SceneView S, const kf_ray* __restrict__ rays, uint32_t n,
+                                            uint8_t* __restrict__ occ, unsigned long long* __restrict__ cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t ovf = 0;
+    if (i < n) {
+        kf_ray r = rays[i];
+        THit h;
+        TCount c{0, 0};
+        occ[i] = traverse<true, false>(S, mk(r.org[0], r.org[1], r.org[2]), mk(r.dir[0], r.dir[1], r.dir[2]), r.tmin,
+                                       r.tmax, false, 0u, h, c, ovf) ? 1 : 0;
+    }
+    if (ovf) atomicAdd(&cnt[2], 1ull);
+}
+
+}  // namespace
+}  // namespace kf
+
+using namespace kf;
+
+namespace {
+unsigned long long* trace_counters() {
+    static unsigned long long* c = nullptr;
+    if (!c) KF_CHECK(hipMalloc((void**)&c, 4 * sizeof(unsigned long long)));
+    return c;
+}
+}  // namespace
+
+extern "C" int kf_trace_closest(kf_scene* s, const kf_ray* rays, uint32_t n, kf_hit* hits) {
+    try {
+        if (!s || (n && (!rays || !hits))) throw Error("kf_trace_closest: null argument");
+        if (n == 0) return 0;
+        hipStream_t st = stream();
+        kf_ray* d_r = nullptr;
+        kf_hit* d_h = nullptr;
+        KF_CHECK(hipMalloc((void**)&d_r, n * sizeof(kf_ray)));
+        KF_CHECK(hipMalloc((void**)&d_h, n * sizeof(kf_hit)));
+        unsigned long long* c = trace_counters();
+        KF_CHECK(hipMemsetAsync(c, 0, 4 * sizeof(unsigned long long), st));
+        KF_CHECK(hipMemcpyAsync(d_r, rays, n * sizeof(kf_ray), hipMemcpyHostToDevice, st));
+        k_closest<false><<<(n + TB - 1) / TB, TB, 0, st>>>(s->view(), d_r, n, d_h, c);
+        KF_CHECK(hipGetLastError());
+        unsigned long long h[4];
+        KF_CHECK(hipMemcpyAsync(hits, d_h, n * sizeof(kf_hit), hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipMemcpyAsync(h, c, sizeof(h), hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        KF_CHECK(hipFree(d_r));
+        KF_CHECK(hipFree(d_h));
+        if (h[2]) throw Error("kf_trace_closest: traversal stack overflow");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+extern "C" int kf_trace_any(kf_scene* s, const kf_ray* rays, uint32_t n, uint8_t* occluded) {
+    try {
+        if (!s || (n && (!rays || !occluded))) throw Error("kf_trace_any: null argument");
+        if (n == 0) return 0;
+        hipStream_t st = stream();
+        kf_ray* d_r = nullptr;
+        uint8_t* d_o = nullptr;
+        KF_CHECK(hipMalloc((void**)&d_r, n * sizeof(kf_ray)));
+        KF_CHECK(hipMalloc((void**)&d_o, n));
+        unsigned long long* c = trace_counters();
+        KF_CHECK(hipMemsetAsync(c, 0, 4 * sizeof(unsigned long long), st));
+        KF_CHECK(hipMemcpyAsync(d_r, rays, n * sizeof(kf_ray), hipMemcpyHostToDevice, st));
+        k_any<<<(n + TB - 1) / TB, TB, 0, st>>>(s->view(), d_r, n, d_o, c);
+        KF_CHECK(hipGetLastError());
+        unsigned long long h[4];
+        KF_CHECK(hipMemcpyAsync(occluded, d_o, n, hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipMemcpyAsync(h, c, sizeof(h), hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        KF_CHECK(hipFree(d_r));
+        KF_CHECK(hipFree(d_o));
+        if (h[2]) throw Error("kf_trace_any: traversal stack overflow");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+extern "C" int kf_trace_closest_device(kf_scene* s, const kf_ray* d_rays, uint32_t n, kf_hit* d_hits,
+                                       float* kernel_ms) {
+    try {
+        if (!s) throw Error("kf_trace_closest_device: null scene");
+        if (n == 0) return 0;
+        hipStream_t st = stream();
+        unsigned long long* c = trace_counters();
+        KF_CHECK(hipMemsetAsync(c, 0, 4 * sizeof(unsigned long long), st));
+        hipEvent_t e0, e1;
+        KF_CHECK(hipEventCreate(&e0));
+        KF_CHECK(hipEventCreate(&e1));
+        KF_CHECK(hipEventRecord(e0, st));
+        k_closest<false><<<(n + TB - 1) / TB, TB, 0, st>>>(s->view(), d_rays, n, d_hits, c);
+        KF_CHECK(hipEventRecord(e1, st));
+        KF_CHECK(hipGetLastError());
+        KF_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        KF_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (kernel_ms) *kernel_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
