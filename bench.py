@@ -1,0 +1,130 @@
+"""Benchmark: path-traced frames of the 1080p icosphere scene on one MI355X.
+
+One step = one frame of the hot path (RayTracer::trace equivalent): 1920x1080,
+16 spp, path depth 8, 256 icosphere instances (1.31 M triangles), area + spot
+light, gradient environment.  Scene, BVHs and render targets are resident in
+HBM before timing starts.
+
+value = rays traced (primary + extension + shadow, counted on the device) /
+wall time of the K timed frames / 1e6  [Mrays/s].
+roofline = the closest-hit traversal kernel (k_extend): algorithmic bytes per
+ray = 32 (ray) + 20 (hit) + 64 * node visits + 48 * triangle tests, counted by
+the instrumented traversal on the same frame, / the kernel's HIP-event time.
+cpu_baseline = the FP32 CPU oracle on a bounded band of rows of the same frame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mrays/s at 1080p; time-to-relMSE 1e-2, denoised vs plain spp, 1 GPU"
+
+
+def cpu_baseline(workload, budget_s=15.0):
+    """Time the FP32 oracle on a band of rows (bounded CPU sample)."""
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    sc = oracle.Scene(workload.data)
+    W, H = workload.width, workload.height
+    y0 = H // 2
+    # calibrate on 2 rows, then size the band to about budget_s
+    t = time.perf_counter()
+    r = sc.render(workload.cam, workload.pc, W, H, rows=(y0, y0 + 2), threads=threads)
+    dt = time.perf_counter() - t
+    rows = int(max(2, min(H // 2, 2 * budget_s / max(dt, 1e-3))))
+    t = time.perf_counter()
+    r = sc.render(workload.cam, workload.pc, W, H, rows=(y0, y0 + rows), threads=threads)
+    dt = time.perf_counter() - t
+    rays = int(r["counts"].sum())
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "FP32 CPU oracle (restatement of the reference shaders), rows %d-%d of the %dx%d frame, "
+                      "%d spp, %d rays in %.1f s" % (y0, y0 + rows, W, H, workload.pc.spp, rays, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--samples-per-batch", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this path runs on one GPU; --gpus %d is not supported (see DESIGN.md for sharding)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from kuafu_amd import GpuScene, Renderer, lib, scenes
+    from kuafu_amd.abi import check
+
+    check(lib().kf_init(0))
+    w = scenes.config2(width=args.width, height=args.height)
+    print("bench: building scene (%d triangles)" % w.data.n_triangles(), file=sys.stderr, flush=True)
+    gs = GpuScene(w.data)
+    info = gs.bvh_info()
+    rd = Renderer(w.width, w.height)
+    pc = w.pc
+    pc.frame_count = 0
+
+    # instrumented frame (untimed): node visits / triangle tests of k_extend
+    cst = rd.render(gs, w.cam, pc, seed=1, samples_per_batch=args.samples_per_batch, count_traversal=True)
+    for _ in range(args.warmup):
+        rd.render(gs, w.cam, pc, seed=0, samples_per_batch=args.samples_per_batch)
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    rays = 0
+    ext_rays = 0
+    ext_ms = 0.0
+    trace_ms = 0.0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = rd.render(gs, w.cam, pc, seed=100 + k, samples_per_batch=args.samples_per_batch)
+        rays += st.rays()
+        ext_rays += st.primary_rays + st.extension_rays
+        ext_ms += st.extend_ms
+        trace_ms += st.trace_ms
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    value = rays / wall / 1e6
+
+    closest = cst.primary_rays + cst.extension_rays
+    nodes_per_ray = cst.node_visits / max(closest, 1)
+    tris_per_ray = cst.tri_tests / max(closest, 1)
+    bytes_per_ray = 32 + 20 + 64 * nodes_per_ray + 48 * tris_per_ray
+    launches = args.steps * (pc.path_depth + 1)
+    achieved = bytes_per_ray * ext_rays / (ext_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "extend_traffic_config2.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_extend (closest-hit traversal)",
+            "bytes_per_ray": round(bytes_per_ray, 1), "node_visits_per_ray": round(nodes_per_ray, 2),
+            "tri_tests_per_ray": round(tris_per_ray, 2),
+            "avg_launch_ms": round(ext_ms / launches, 4)}
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name, "width": w.width, "height": w.height, "spp": pc.spp,
+                   "path_depth": pc.path_depth, "triangles": int(info.triangles), "instances": int(info.instances),
+                   "rays_per_frame": rays // args.steps,
+                   "kernel_mrays_s": round(rays / (trace_ms * 1e-3) / 1e6, 2)},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

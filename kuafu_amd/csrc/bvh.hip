@@ -166,8 +166,10 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_scatter(uint32_t n, const u
         __syncthreads();
         if (valid) {
             uint32_t pos = gbase[d] + wcnt[w][d] + rank;
-            kout[pos] = key;
-            vout[pos] = vin[i];
+            if (pos < n) {
+                kout[pos] = key;
+                vout[pos] = vin[i];
+            }
         }
         __syncthreads();
     }
@@ -246,7 +248,7 @@ __device__ __forceinline__ void write_child(Node* nd, bool left, const float lo[
     c[1] = hi[2];
 }
 
-__global__ void k_refit(uint32_t n, const uint32_t* __restrict__ order, const float4* __restrict__ plo,
+__global__ void k_refit(uint32_t n, uint32_t n_nodes, const uint32_t* __restrict__ order, const float4* __restrict__ plo,
                         const float4* __restrict__ phi, const int* __restrict__ parent_node,
                         const int* __restrict__ parent_leaf, const unsigned long long* __restrict__ keys,
                         const uint32_t* __restrict__ seg_off, Node* nodes, int* flags, int leaf_is_slot) {
@@ -259,7 +261,7 @@ __global__ void k_refit(uint32_t n, const uint32_t* __restrict__ order, const fl
     float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
     int child = ~(leaf_is_slot ? (int)gi : (int)prim);
     int p = parent_leaf[gi];
-    while (p >= 0) {
+    for (int step = 0; p >= 0 && (uint32_t)p < n_nodes && step < 4096; ++step) {
         Node* nd = &nodes[p];
         bool left = nd->d.x == child;
         write_child(nd, left, lo, hi);
@@ -435,8 +437,6 @@ void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32
     uint32_t* hist = dalloc<uint32_t>((size_t)n_tiles * kDigits);
 
     // centroid bounds per segment: lo = +inf, hi = -inf in ordered-int space
-    for (uint32_t s = 0; s < n_seg; ++s) {
-    }
     {
         // fill 6 ints per segment: 3 x ord(+inf), 3 x ord(-inf)
         std::vector<int> init(6 * (size_t)n_seg);
@@ -462,9 +462,10 @@ void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32
     KF_CHECK(hipMemcpyAsync(d_order, v0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
     k_fill_int<<<blocks(n, TB), TB, 0, st>>>(flags, 0, n);
     k_fill_int<<<blocks(n, TB), TB, 0, st>>>(parent_leaf, -1, n);
+    k_fill_int<<<blocks(n, TB), TB, 0, st>>>(parent_node, -1, n);
     k_karras<<<blocks(n, TB), TB, 0, st>>>(n, k0, v0, d_seg_off, d_node_off, d_nodes, parent_node, parent_leaf,
                                            leaf_is_slot);
-    k_refit<<<blocks(n, TB), TB, 0, st>>>(n, v0, d_lo, d_hi, parent_node, parent_leaf, k0, d_seg_off, d_nodes, flags,
+    k_refit<<<blocks(n, TB), TB, 0, st>>>(n, n, v0, d_lo, d_hi, parent_node, parent_leaf, k0, d_seg_off, d_nodes, flags,
                                           leaf_is_slot);
     k_seg_root<<<blocks(n_seg, TB), TB, 0, st>>>(n_seg, d_seg_off, d_node_off, v0, d_root, leaf_is_slot);
     KF_CHECK(hipGetLastError());

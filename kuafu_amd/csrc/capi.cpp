@@ -56,6 +56,9 @@ SceneView Scene::view() const {
     v.tri = d_tri;
     v.tlas = d_tlas;
     v.tlas_root = n_inst ? tlas_root : kEmptyRef;
+    v.n_blas_nodes = n_blas_nodes;
+    v.n_tri = n_tri;
+    v.n_tlas_nodes = n_inst > 1 ? n_inst - 1 : 0;
     v.inst = d_inst;
     v.n_inst = n_inst;
     v.verts = d_verts;
@@ -76,7 +79,7 @@ SceneView Scene::view() const {
 }
 
 Scene::~Scene() {
-    (void)hipStreamSynchronize(stream());
+    if (g_stream) (void)hipStreamSynchronize(g_stream);
     release(d_verts); release(d_idx); release(d_matidx); release(d_voff); release(d_toff); release(d_tri_geom);
     release(d_mats); release(d_blas); release(d_tri); release(d_blas_root); release(d_tlas); release(d_inst);
     release(d_inst_src); release(d_tex); release(d_tex_info); release(d_env_info); release(d_srgb);
@@ -293,7 +296,6 @@ int kf_scene_create(const kf_scene_desc* d, kf_scene** out) {
     kf_scene* s = nullptr;
     try {
         if (!d || !out) throw Error("kf_scene_create: null argument");
-        hipStream_t st = stream();
         s = new kf_scene();
         s->n_geom = d->num_geometries;
         if (s->n_geom && (!d->geom_vertex_offset || !d->geom_tri_offset || !d->geom_flags))
@@ -321,6 +323,7 @@ int kf_scene_create(const kf_scene_desc* d, kf_scene** out) {
         for (uint32_t i = 0; i < d->num_instances; ++i)
             if (d->instances[i].geometry_index >= s->n_geom)
                 throw Error("Geometry index is out of bounds.");  // rt.cpp:118-121
+        hipStream_t st = stream();
         // vertices as 3 x float4 (the GLSL unpack of rchit:52-64)
         s->d_verts = upload((const float4*)d->vertices, 3 * (size_t)s->n_vert);
         s->d_idx = upload(d->indices, 3 * (size_t)s->n_tri);
@@ -449,11 +452,19 @@ int kf_scene_read_blas(kf_scene* s, uint32_t g, float* nodes, uint32_t max_nodes
         uint32_t nt = s->h_toff[g + 1] - s->h_toff[g];
         uint32_t nn = nt > 1 ? nt - 1 : 0;
         if (n_nodes) *n_nodes = nn;
-        if (root_ref) *root_ref = s->h_blas_root[g];
+        // refs are returned local to the geometry: internal nodes from 0, leaves ~(slot - first slot)
+        int node0 = (int)s->h_node_off[g], slot0 = (int)s->h_toff[g];
+        auto local = [&](int r) { return r == kEmptyRef ? r : (r >= 0 ? r - node0 : ~(~r - slot0)); };
+        if (root_ref) *root_ref = local(s->h_blas_root[g]);
         if (nodes) {
             if (max_nodes < nn) throw Error("kf_scene_read_blas: node buffer too small");
-            if (nn)
-                KF_CHECK(hipMemcpy(nodes, s->d_blas + s->h_node_off[g], nn * sizeof(Node), hipMemcpyDeviceToHost));
+            std::vector<Node> h(nn);
+            if (nn) KF_CHECK(hipMemcpy(h.data(), s->d_blas + node0, nn * sizeof(Node), hipMemcpyDeviceToHost));
+            for (auto& n : h) {
+                n.d.x = local(n.d.x);
+                n.d.y = local(n.d.y);
+            }
+            if (nn) memcpy(nodes, h.data(), nn * sizeof(Node));
         }
         if (tri_order) {
             if (max_tris < nt) throw Error("kf_scene_read_blas: order buffer too small");
@@ -500,7 +511,7 @@ int kf_renderer_create(uint32_t w, uint32_t h, kf_renderer** out) {
 }
 
 void kf_renderer_destroy(kf_renderer* r) {
-    if (r) (void)hipStreamSynchronize(stream());
+    if (r && g_stream) (void)hipStreamSynchronize(g_stream);
     delete r;
 }
 

@@ -66,6 +66,7 @@ struct SceneView {
     const float4* tri;        // 3 float4 per triangle slot; tri[3s].w = local prim id bits
     const Node* tlas;
     int tlas_root;
+    uint32_t n_blas_nodes, n_tri, n_tlas_nodes;  // bounds every traversal index is checked against
     const Inst* inst;
     uint32_t n_inst;
     // shading data (reference SSBO layouts)

@@ -136,11 +136,11 @@ __global__ __launch_bounds__(TB) void k_raygen(Frame F, const uint2* __restrict_
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, const uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t cap, const uint32_t* __restrict__ queue,
                                                const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
                                                const uint32_t* __restrict__ seed, float4* __restrict__ hit,
                                                uint32_t* __restrict__ hit_prim, unsigned long long* __restrict__ cnt) {
-    uint32_t n = (uint32_t)cnt[bounce * kCntPerBounce + 0];
+    uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)cap);
     uint32_t nodes = 0, tris = 0, ovf = 0;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         uint32_t p = queue[q];
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                                               float4* __restrict__ sh_o, float4* __restrict__ sh_d,
                                               float4* __restrict__ albedo_out, float4* __restrict__ normal_out,
                                               unsigned long long* __restrict__ cnt) {
-    uint32_t n = (uint32_t)cnt[bounce * kCntPerBounce + 0];
+    uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t iters = (n + stride - 1) / stride;
     const uint32_t K = F.slots;
@@ -313,7 +313,8 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                     sh.transmissionColor = transmissionColor; sh.inside = inside;
                     // NEE slots in evaluation order (rchit:457-460)
                     v3 sum_d = splat(0.0f), sum_p = splat(0.0f), sum_a = splat(0.0f);
-                    for (uint32_t j = 0; j < K; ++j) {
+                    const int nl = LT->n_slots;
+                    for (int j = 0; j < nl; ++j) {
                         int kind = LT->kind[j], li = LT->index[j];
                         v3 Ld, lem;
                         float maxd;
@@ -419,10 +420,10 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
     }
 }
 
-__global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, const float4* __restrict__ sh_o,
+__global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t cap, const float4* __restrict__ sh_o,
                                                const float4* __restrict__ sh_d, uint8_t* __restrict__ vis,
                                                unsigned long long* __restrict__ cnt) {
-    uint32_t n = (uint32_t)cnt[bounce * kCntPerBounce + 1];
+    uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 1], (unsigned long long)cap);
     uint32_t ovf = 0;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         float4 o = sh_o[q], d = sh_d[q];
@@ -442,7 +443,7 @@ __global__ __launch_bounds__(TB) void k_connect(Frame F, int bounce, const uint3
                                                 const float4* __restrict__ nee_a, const float4* __restrict__ nee_b,
                                                 const uint8_t* __restrict__ vis, float4* __restrict__ out,
                                                 unsigned long long* __restrict__ cnt) {
-    uint32_t n = (uint32_t)cnt[bounce * kCntPerBounce + 0];
+    uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t iters = (n + stride - 1) / stride;
     const uint32_t K = F.slots;
@@ -759,10 +760,10 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
             for (uint32_t b = 0; b < nb; ++b) {
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
                 if (o.count_traversal)
-                    k_extend<true><<<grid, TB, 0, st>>>(S, (int)b, qa, r->ray_o, r->ray_d, r->seed, r->hit,
+                    k_extend<true><<<grid, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
                                                          r->hit_prim, r->counters);
                 else
-                    k_extend<false><<<grid, TB, 0, st>>>(S, (int)b, qa, r->ray_o, r->ray_d, r->seed, r->hit,
+                    k_extend<false><<<grid, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
                                                           r->hit_prim, r->counters);
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 1], st));
                 if (inline_nee)
@@ -776,7 +777,7 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
                                                         r->counters);
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], st));
                 if (!inline_nee)
-                    k_shadow<<<grid, TB, 0, st>>>(S, (int)b, r->sh_o, r->sh_d, r->vis, r->counters);
+                    k_shadow<<<grid, TB, 0, st>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->vis, r->counters);
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], st));
                 if (inline_nee)
                     k_connect<true><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,

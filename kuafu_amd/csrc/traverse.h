@@ -19,6 +19,7 @@
 namespace kf {
 
 constexpr int kStack = 64;
+constexpr uint32_t kMaxSteps = 1u << 22;  // bound on loop iterations (malformed-tree guard)
 constexpr int kPopRef = (int)0x80000000;      // "pop the stack"
 constexpr int kExitBlasRef = (int)0x80000001; // marker: leave the current instance
 constexpr float kGamma3 = 3.0f * 5.96046448e-8f / (1.0f - 3.0f * 5.96046448e-8f);
@@ -165,6 +166,10 @@ KF_DEV bool traverse(const SceneView& S, v3 o, v3 d, float tmin, float tmax, boo
 
     auto test_tri = [&](int slot) -> bool {
         if (COUNT) cnt.tris++;
+        if ((uint32_t)slot >= S.n_tri) {
+            overflow = 1;
+            return false;
+        }
         const float4* t = S.tri + 3 * (size_t)slot;
         float4 v0 = t[0], v1 = t[1], v2 = t[2];
         TriHit h;
@@ -186,7 +191,12 @@ KF_DEV bool traverse(const SceneView& S, v3 o, v3 d, float tmin, float tmax, boo
         return true;
     };
 
+    uint32_t steps = 0;
     while (true) {
+        if (++steps > kMaxSteps) {
+            overflow = 1;
+            break;
+        }
         if (ref == kPopRef) {
             if (sp == 0) break;
             ref = stack[--sp];
@@ -200,6 +210,11 @@ KF_DEV bool traverse(const SceneView& S, v3 o, v3 d, float tmin, float tmax, boo
         }
         if (ref >= 0) {
             if (COUNT) cnt.nodes++;
+            if ((uint32_t)ref >= (in_blas ? S.n_blas_nodes : S.n_tlas_nodes)) {
+                overflow = 1;
+                ref = kPopRef;
+                continue;
+            }
             Node n = in_blas ? S.blas[ref] : S.tlas[ref];
             bool hl, hr;
             float tl, tr;
@@ -238,8 +253,12 @@ KF_DEV bool traverse(const SceneView& S, v3 o, v3 d, float tmin, float tmax, boo
         // leaf
         if (!in_blas) {
             uint32_t ii = (uint32_t)(~ref);
-            Inst I = S.inst[ii];
             ref = kPopRef;
+            if (ii >= S.n_inst) {
+                overflow = 1;
+                continue;
+            }
+            Inst I = S.inst[ii];
             if (!(I.info.z & kInstValid)) continue;
             v3 oo = xform_point(I, o);
             v3 od = xform_dir(I, d);

@@ -80,10 +80,10 @@ def exp2(x):
 
 
 def intersect_triangle(org, d, tmin, tmax, v0, v1, v2):
-    f = lambda v: np.ascontiguousarray(v, np.float32)
+    arrs = [np.ascontiguousarray(v, np.float32) for v in (org, d, v0, v1, v2)]  # kept alive for the call
     t, b1, b2 = C.c_float(), C.c_float(), C.c_float()
     back = C.c_uint32()
-    h = lib().ko_intersect_triangle(_p(f(org)), _p(f(d)), tmin, tmax, _p(f(v0)), _p(f(v1)), _p(f(v2)),
+    h = lib().ko_intersect_triangle(_p(arrs[0]), _p(arrs[1]), tmin, tmax, _p(arrs[2]), _p(arrs[3]), _p(arrs[4]),
                                     C.byref(t), C.byref(b1), C.byref(b2), C.byref(back))
     return (bool(h), t.value, b1.value, b2.value, back.value)
 
