@@ -282,6 +282,54 @@ float* kf_renderer_device_target(kf_renderer* r, int target);
  * config.hpp:144, vkCore.hpp:3008-3111); W*H*4 bytes, order B,G,R,A. */
 int kf_renderer_download_bgra8(kf_renderer* r, uint8_t* out);
 
+/* ------------------------------------------------------------------------ */
+/* Denoise stage (replaces DenoiserOptix, denoiser.cpp:24-344)               */
+/* ------------------------------------------------------------------------ */
+
+/* U-Net, 5 levels with 32/64/96/128/160 channels, 3x3 convolutions, BF16
+ * weights and activations with FP32 accumulation (MFMA).  Input: noisy RGB,
+ * albedo and normal targets; output: denoised RGB. */
+typedef struct kf_denoiser kf_denoiser;
+
+/* weights_path: weight file written by kf_denoiser_save; NULL loads the
+ * default file (kuafu_amd/weights/unet.kfw next to the library) or, if it is
+ * missing, starts from seeded random weights. */
+int kf_denoiser_create(const char* weights_path, kf_denoiser** out);
+void kf_denoiser_destroy(kf_denoiser* d);
+int kf_denoiser_init_random(kf_denoiser* d, uint32_t seed);
+int kf_denoiser_save(kf_denoiser* d, const char* path);
+int kf_denoiser_load(kf_denoiser* d, const char* path);
+/* Number of parameters (FP32 master copy) and host access to them. */
+int kf_denoiser_num_params(kf_denoiser* d, uint64_t* n);
+int kf_denoiser_get_params(kf_denoiser* d, float* out);
+int kf_denoiser_set_params(kf_denoiser* d, const float* in);
+
+/* The denoise stage of a frame: the renderer's rgba target is replaced by the
+ * denoised image (context.cpp:650-670 writes the OptiX output back to rgba). */
+int kf_denoiser_apply(kf_denoiser* d, kf_renderer* r);
+/* Device pointers: rgba/albedo/normal W*H float4 in HBM -> out W*H float4. */
+int kf_denoiser_infer_device(kf_denoiser* d, const float* d_rgba, const float* d_albedo, const float* d_normal,
+                             uint32_t width, uint32_t height, float* d_out, float* kernel_ms);
+
+/* Training on host batches: inputs n*h*w*9 (RGB, albedo, normal), targets
+ * n*h*w*3; one forward + backward + Adam step; h and w multiples of 16. */
+int kf_denoiser_train_step(kf_denoiser* d, const float* inputs, const float* targets, uint32_t n, uint32_t h,
+                           uint32_t w, float lr, float* loss);
+/* Forward + backward without the update: loss and the gradient of every
+ * parameter (same order as kf_denoiser_get_params), for gradient checks. */
+int kf_denoiser_gradients(kf_denoiser* d, const float* inputs, const float* targets, uint32_t n, uint32_t h,
+                          uint32_t w, float* loss, float* grads);
+/* Forward only: denoised linear RGB, n*h*w*3. */
+int kf_denoiser_forward(kf_denoiser* d, const float* inputs, uint32_t n, uint32_t h, uint32_t w, float* out);
+
+/* Single 3x3 convolution layer on host arrays (BF16 bit patterns), NHWC:
+ * forward y = act(conv(x, w) + b); backward dx (masked by x > 0 when
+ * mask_by_x), dw and db.  For layer-level gradient checks. */
+int kf_conv3x3_forward(const uint16_t* x, uint32_t cin, const uint16_t* w, const float* bias, uint32_t cout,
+                       uint32_t n, uint32_t h, uint32_t wd, int relu, uint16_t* y);
+int kf_conv3x3_backward(const uint16_t* x, uint32_t cin, const uint16_t* w, uint32_t cout, const uint16_t* dy,
+                        uint32_t n, uint32_t h, uint32_t wd, int mask_by_x, uint16_t* dx, float* dw, float* db);
+
 #ifdef __cplusplus
 }
 #endif

@@ -138,6 +138,26 @@ EXPORTS = [
     ("kf_renderer_read_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     ("kf_renderer_device_target", C.c_void_p, [C.c_void_p, C.c_int]),
     ("kf_renderer_download_bgra8", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("kf_denoiser_create", C.c_int, [C.c_char_p, C.c_void_p]),
+    ("kf_denoiser_destroy", None, [C.c_void_p]),
+    ("kf_denoiser_init_random", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("kf_denoiser_save", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("kf_denoiser_load", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("kf_denoiser_num_params", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("kf_denoiser_get_params", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("kf_denoiser_set_params", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("kf_denoiser_apply", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("kf_denoiser_infer_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                           C.c_void_p, C.c_void_p]),
+    ("kf_denoiser_train_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_float, C.c_void_p]),
+    ("kf_denoiser_gradients", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_void_p, C.c_void_p]),
+    ("kf_denoiser_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    ("kf_conv3x3_forward", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                     C.c_uint32, C.c_uint32, C.c_int, C.c_void_p]),
+    ("kf_conv3x3_backward", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                      C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 _lib = None

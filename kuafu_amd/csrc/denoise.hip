@@ -1,0 +1,1080 @@
+// denoise.hip — the denoise stage: a BF16 U-Net on MFMA (gfx950).
+//
+// Replaces DenoiserOptix (denoiser.cpp:24-344): the reference hands rgba,
+// albedo and normal to the closed OptiX LDR denoiser and writes the result
+// back into rgba (context.cpp:650-670).  Here the same three targets feed a
+// U-Net trained on this tracer's own images:
+//   5 levels, 32/64/96/128/160 channels, two 3x3 convs + ReLU per level,
+//   2x2 average pooling, nearest 2x upsampling, skip concatenation,
+//   final 3x3 conv to RGB, residual on log1p(noisy RGB).
+// Tensors are NHWC BF16 with FP32 accumulation.  Convolutions are implicit
+// GEMMs on v_mfma_f32_16x16x32_bf16:
+//   k_conv   forward (bias + ReLU fused) and data gradient (the same kernel on
+//            flipped, transposed weights, ReLU mask of the forward input fused)
+//   k_wgrad  weight gradient, pixels as the K dimension, split-K with FP32
+//            atomics
+// Concatenations cost no copies: producers write into channel slices of the
+// concatenated tensor (row stride "ld").
+#include "kf_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+namespace kf {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TB = 256;
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                       // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// ---------------------------------------------------------------- convolution
+// y[p, co] = sum_{tap, ci} x[p + off(tap), ci] * w[tap, co, ci]   (zero padding)
+// Block tile 64 pixels x 64 output channels, K step 32 (one tap, 32 input
+// channels).  Wave w owns pixels 16w..16w+15 and all 64 channels (4 MFMA
+// tiles).  Cin must be a multiple of 32; ldx, ldy multiples of 8.
+constexpr int CBM = 64, CBN = 64, CBK = 32, CLD = CBK + 8;
+
+__global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int ldx, int Cin,
+                                             const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
+                                             int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
+                                             float* __restrict__ Yf, int ldyf, int cstore,
+                                             const uint16_t* __restrict__ M, int ldm, int relu) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[CBM * CLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[CBN * CLD];
+    const int P = N * H * W;
+    const int p0 = blockIdx.x * CBM, co0 = blockIdx.y * CBN;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    f32x4 acc[4];
+    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int lr = t >> 2, lk = (t & 3) * 8;
+    const int p = p0 + lr;
+    const bool pv = p < P;
+    int n_ = 0, y = 0, x = 0;
+    if (pv) {
+        n_ = p / (H * W);
+        int rem = p - n_ * H * W;
+        y = rem / W;
+        x = rem - y * W;
+    }
+    const int co = co0 + lr;
+    const bool cv = co < Cout;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    const uint4 zero = make_uint4(0, 0, 0, 0);
+    for (int tap = 0; tap < 9; ++tap) {
+        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+        const bool in = pv && yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const uint16_t* xrow = X + (size_t)((n_ * H + (in ? yy : 0)) * W + (in ? xx : 0)) * ldx;
+        const uint16_t* wrow = Wt + ((size_t)tap * Cout + (cv ? co : 0)) * Cin;
+        for (int ci0 = 0; ci0 < Cin; ci0 += CBK) {
+            uint4 av = in ? *(const uint4*)(xrow + ci0 + lk) : zero;
+            uint4 bv = cv ? *(const uint4*)(wrow + ci0 + lk) : zero;
+            __syncthreads();
+            *(uint4*)&As[lr * CLD + lk] = av;
+            *(uint4*)&Bs[lr * CLD + lk] = bv;
+            __syncthreads();
+            bf16x8 a = __builtin_bit_cast(bf16x8, *(const uint4*)&As[(wv * 16 + fr) * CLD + fk]);
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                bf16x8 b = __builtin_bit_cast(bf16x8, *(const uint4*)&Bs[(nb * 16 + fr) * CLD + fk]);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int pp = p0 + wv * 16 + (lane >> 4) * 4 + i;
+            const int c = co0 + nb * 16 + (lane & 15);
+            if (pp >= P || c >= cstore) continue;
+            float v = acc[nb][i] + (bias ? bias[c] : 0.0f);
+            if (relu) v = fmaxf(v, 0.0f);
+            if (M && !(bf2f(M[(size_t)pp * ldm + c]) > 0.0f)) v = 0.0f;
+            if (Yf) Yf[(size_t)pp * ldyf + c] = v;
+            else Y[(size_t)pp * ldy + c] = f2bf(v);
+        }
+}
+
+// dW[tap, co, ci] += sum_p dY[p, co] * X[p + off(tap), ci] over one pixel chunk.
+// Tile 64 co x 64 ci, K step 32 pixels; LDS tiles are stored channel-major
+// (pixels contiguous) so each lane's 8 K values are one 16-byte read.
+__global__ __launch_bounds__(TB) void k_wgrad(const uint16_t* __restrict__ X, int ldx, int Cin,
+                                              const uint16_t* __restrict__ dY, int ldy, int Cout, int N, int H,
+                                              int W, int chunk, float* __restrict__ dW) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[64 * CLD];  // [co][px]
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[64 * CLD];  // [ci][px]
+    const int P = N * H * W;
+    const int tap = blockIdx.y;
+    const int n_ci_t = (Cin + 63) / 64;
+    const int co0 = (blockIdx.z / n_ci_t) * 64, ci0 = (blockIdx.z % n_ci_t) * 64;
+    const int pbeg = blockIdx.x * chunk, pend = min(P, pbeg + chunk);
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    f32x4 acc[4];
+    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int lr = t >> 3, lc = (t & 7) * 8;  // pixel row 0..31, channel chunk
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    for (int pk = pbeg; pk < pend; pk += 32) {
+        const int p = pk + lr;
+        uint16_t av[8], bv[8];
+        bool pv = p < pend;
+        uint4 a4 = make_uint4(0, 0, 0, 0), b4 = make_uint4(0, 0, 0, 0);
+        if (pv && co0 + lc < Cout) a4 = *(const uint4*)(dY + (size_t)p * ldy + co0 + lc);
+        if (pv && ci0 + lc < Cin) {
+            int n_ = p / (H * W), rem = p - n_ * H * W, y = rem / W, x = rem - y * W;
+            int yy = y + dy, xx = x + dx;
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                b4 = *(const uint4*)(X + (size_t)((n_ * H + yy) * W + xx) * ldx + ci0 + lc);
+        }
+        memcpy(av, &a4, 16);
+        memcpy(bv, &b4, 16);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            As[(lc + j) * CLD + lr] = av[j];
+            Bs[(lc + j) * CLD + lr] = bv[j];
+        }
+        __syncthreads();
+        bf16x8 a = __builtin_bit_cast(bf16x8, *(const uint4*)&As[(wv * 16 + fr) * CLD + fk]);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            bf16x8 b = __builtin_bit_cast(bf16x8, *(const uint4*)&Bs[(nb * 16 + fr) * CLD + fk]);
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int co = co0 + wv * 16 + (lane >> 4) * 4 + i;
+            const int ci = ci0 + nb * 16 + (lane & 15);
+            if (co < Cout && ci < Cin) atomicAdd(&dW[((size_t)tap * Cout + co) * Cin + ci], acc[nb][i]);
+        }
+}
+
+__global__ void k_bias_grad(const uint16_t* __restrict__ dY, int ldy, int Cout, int P, int chunk, float* db) {
+    int pbeg = blockIdx.x * chunk, pend = min(P, pbeg + chunk);
+    for (int c = threadIdx.x; c < Cout; c += blockDim.x) {
+        float s = 0.0f;
+        for (int p = pbeg; p < pend; ++p) s += bf2f(dY[(size_t)p * ldy + c]);
+        atomicAdd(&db[c], s);
+    }
+}
+
+// ---------------------------------------------------------------- glue kernels
+// 2x2 average pooling: Y[n, y, x, c] = mean of X[n, 2y..2y+1, 2x..2x+1, c].
+__global__ void k_pool(const uint16_t* __restrict__ X, int ldx, int C, int N, int H, int W, uint16_t* __restrict__ Y) {
+    int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * Ho * Wo * C8) return;
+    int c8 = (int)(i % C8);
+    size_t q = i / C8;
+    int x = (int)(q % Wo), y = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        size_t src = ((size_t)(n * H + 2 * y + (k >> 1)) * W + 2 * x + (k & 1)) * ldx + c8 * 8;
+        uint4 v = *(const uint4*)(X + src);
+        uint16_t e[8];
+        memcpy(e, &v, 16);
+        for (int j = 0; j < 8; ++j) s[j] += bf2f(e[j]);
+    }
+    uint16_t o[8];
+    for (int j = 0; j < 8; ++j) o[j] = f2bf(0.25f * s[j]);
+    uint4 ov;
+    memcpy(&ov, o, 16);
+    *(uint4*)(Y + q * C + c8 * 8) = ov;
+}
+
+// dS[hi] += (S[hi] > 0) * dP[lo] / 4   (gradient of the pooled skip tensor)
+__global__ void k_pool_bwd(const uint16_t* __restrict__ dPl, int C, const uint16_t* __restrict__ S, int lds,
+                           uint16_t* __restrict__ dS, int ldd, int N, int H, int W) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int C8 = C / 8;
+    if (i >= (size_t)N * H * W * C8) return;
+    int c8 = (int)(i % C8);
+    size_t p = i / C8;
+    int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
+    size_t lo = ((size_t)(n * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c8 * 8;
+    uint4 gv = *(const uint4*)(dPl + lo), sv = *(const uint4*)(S + p * lds + c8 * 8),
+          dv = *(const uint4*)(dS + p * ldd + c8 * 8);
+    uint16_t g[8], s[8], d[8];
+    memcpy(g, &gv, 16);
+    memcpy(s, &sv, 16);
+    memcpy(d, &dv, 16);
+    for (int j = 0; j < 8; ++j)
+        if (bf2f(s[j]) > 0.0f) d[j] = f2bf(bf2f(d[j]) + 0.25f * bf2f(g[j]));
+    memcpy(&dv, d, 16);
+    *(uint4*)(dS + p * ldd + c8 * 8) = dv;
+}
+
+// nearest 2x upsampling into a channel slice: Y[n, y, x, c] = X[n, y/2, x/2, c]
+__global__ void k_up(const uint16_t* __restrict__ X, int C, int N, int H, int W, uint16_t* __restrict__ Y, int ldy) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int C8 = C / 8, Hh = 2 * H, Wh = 2 * W;
+    if (i >= (size_t)N * Hh * Wh * C8) return;
+    int c8 = (int)(i % C8);
+    size_t p = i / C8;
+    int x = (int)(p % Wh), y = (int)((p / Wh) % Hh), n = (int)(p / ((size_t)Wh * Hh));
+    uint4 v = *(const uint4*)(X + ((size_t)(n * H + y / 2) * W + x / 2) * C + c8 * 8);
+    *(uint4*)(Y + p * ldy + c8 * 8) = v;
+}
+
+// dX[lo] = (X[lo] > 0) * sum of the 4 children of dY (upsampling backward + ReLU mask)
+__global__ void k_up_bwd(const uint16_t* __restrict__ dY, int ldy, int C, const uint16_t* __restrict__ Xf,
+                         int N, int H, int W, uint16_t* __restrict__ dX) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int C8 = C / 8;
+    if (i >= (size_t)N * H * W * C8) return;
+    int c8 = (int)(i % C8);
+    size_t p = i / C8;
+    int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        size_t hi = ((size_t)(n * 2 * H + 2 * y + (k >> 1)) * (2 * W) + 2 * x + (k & 1)) * ldy + c8 * 8;
+        uint4 v = *(const uint4*)(dY + hi);
+        uint16_t e[8];
+        memcpy(e, &v, 16);
+        for (int j = 0; j < 8; ++j) s[j] += bf2f(e[j]);
+    }
+    uint4 xv = *(const uint4*)(Xf + p * C + c8 * 8);
+    uint16_t xe[8], o[8];
+    memcpy(xe, &xv, 16);
+    for (int j = 0; j < 8; ++j) o[j] = bf2f(xe[j]) > 0.0f ? f2bf(s[j]) : (uint16_t)0;
+    uint4 ov;
+    memcpy(&ov, o, 16);
+    *(uint4*)(dX + p * C + c8 * 8) = ov;
+}
+
+__device__ inline float sane(float v) { return (v == v) ? fminf(fmaxf(v, -1e4f), 1e4f) : 0.0f; }
+
+// Network input from float4 render targets (one image, edge-replicated to
+// Hp x Wp): ch 0-2 log1p(rgb), 3-5 albedo * pi, 6-8 normal, 9-31 zero; lin
+// keeps log1p(rgb) for the residual.
+__global__ void k_pack_targets(const float4* __restrict__ rgba, const float4* __restrict__ alb,
+                               const float4* __restrict__ nrm, int H, int W, int Hp, int Wp, uint16_t* __restrict__ X,
+                               float* __restrict__ lin) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Hp * Wp) return;
+    int x = min(i % Wp, W - 1), y = min(i / Wp, H - 1);
+    int s = y * W + x;
+    float4 c = rgba[s], a = alb[s], n = nrm[s];
+    float v[9] = {log1pf(fmaxf(sane(c.x), 0.f)), log1pf(fmaxf(sane(c.y), 0.f)), log1pf(fmaxf(sane(c.z), 0.f)),
+                  sane(a.x) * 3.141592f, sane(a.y) * 3.141592f, sane(a.z) * 3.141592f, sane(n.x), sane(n.y),
+                  sane(n.z)};
+    uint16_t o[32];
+    for (int k = 0; k < 32; ++k) o[k] = k < 9 ? f2bf(v[k]) : (uint16_t)0;
+    for (int k = 0; k < 4; ++k) *(uint4*)(X + (size_t)i * 32 + 8 * k) = *(const uint4*)&o[8 * k];
+    lin[(size_t)i * 4 + 0] = v[0];
+    lin[(size_t)i * 4 + 1] = v[1];
+    lin[(size_t)i * 4 + 2] = v[2];
+    lin[(size_t)i * 4 + 3] = 0.f;
+}
+
+// Network input from packed records (n, h, w, 9): rgb radiance, albedo (as
+// stored in the albedo target, baseColor / pi), normal.
+__global__ void k_pack_records(const float* __restrict__ in, int P, uint16_t* __restrict__ X, float* __restrict__ lin) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const float* r = in + (size_t)i * 9;
+    float v[9] = {log1pf(fmaxf(sane(r[0]), 0.f)), log1pf(fmaxf(sane(r[1]), 0.f)), log1pf(fmaxf(sane(r[2]), 0.f)),
+                  sane(r[3]) * 3.141592f, sane(r[4]) * 3.141592f, sane(r[5]) * 3.141592f, sane(r[6]), sane(r[7]),
+                  sane(r[8])};
+    uint16_t o[32];
+    for (int k = 0; k < 32; ++k) o[k] = k < 9 ? f2bf(v[k]) : (uint16_t)0;
+    for (int k = 0; k < 4; ++k) *(uint4*)(X + (size_t)i * 32 + 8 * k) = *(const uint4*)&o[8 * k];
+    for (int k = 0; k < 3; ++k) lin[(size_t)i * 4 + k] = v[k];
+    lin[(size_t)i * 4 + 3] = 0.f;
+}
+
+constexpr float kLogClamp = 10.0f;
+
+// denoised linear RGB = expm1(min(y + log1p(noisy), 10)), cropped to H x W
+__global__ void k_unpack(const float* __restrict__ y, const float* __restrict__ lin, int H, int W, int Hp, int Wp,
+                         float4* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H * W) return;
+    int x = i % W, yy = i / W;
+    size_t s = (size_t)yy * Wp + x;
+    float r[3];
+    for (int k = 0; k < 3; ++k) r[k] = expm1f(fminf(y[s * 4 + k] + lin[s * 4 + k], kLogClamp));
+    out[i] = make_float4(r[0], r[1], r[2], 1.0f);
+}
+
+__global__ void k_unpack_records(const float* __restrict__ y, const float* __restrict__ lin, int P, float* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    for (int k = 0; k < 3; ++k) out[(size_t)i * 3 + k] = expm1f(fminf(y[(size_t)i * 4 + k] + lin[(size_t)i * 4 + k], kLogClamp));
+}
+
+// L = mean |yhat - log1p(t)| + mean (o - t)^2 / (t^2 + 0.01), o = expm1(yhat),
+// yhat = y + log1p(noisy).  Writes dL/dy (BF16, channels 0-2 of a 32-wide row).
+__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ lin, const float* __restrict__ target,
+                       int P, uint16_t* __restrict__ dy, float* __restrict__ loss) {
+    __shared__ float red[TB];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (i < P) {
+        uint16_t g[32];
+        for (int k = 0; k < 32; ++k) g[k] = 0;
+        const float inv = 1.0f / (3.0f * (float)P);
+        for (int k = 0; k < 3; ++k) {
+            float yh = y[(size_t)i * 4 + k] + lin[(size_t)i * 4 + k];
+            float t = fmaxf(sane(target[(size_t)i * 3 + k]), 0.0f);
+            float lt = log1pf(t);
+            bool clamped = yh > kLogClamp;
+            float yc = fminf(yh, kLogClamp);
+            float o = expm1f(yc);
+            float den = t * t + 0.01f;
+            l += fabsf(yh - lt) + (o - t) * (o - t) / den;
+            float d = (yh > lt ? 1.0f : (yh < lt ? -1.0f : 0.0f)) + (clamped ? 0.0f : 2.0f * (o - t) / den * (o + 1.0f));
+            g[k] = f2bf(d * inv);
+        }
+        for (int k = 0; k < 4; ++k) *(uint4*)(dy + (size_t)i * 32 + 8 * k) = *(const uint4*)&g[8 * k];
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = TB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] / (3.0f * (float)P));
+}
+
+__global__ void k_zero_f32(float* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// Adam on the FP32 master weights.
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       size_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    if (!(gi == gi)) gi = 0.0f;
+    float mi = b1 * m[i] + (1.0f - b1) * gi;
+    float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// BF16 compute copies of one conv: Wb[tap][co][ci] and the data-gradient
+// weights Wd[tap][ci][co] = W[8 - tap][co][ci].
+__global__ void k_weights_bf16(const float* __restrict__ w32, int Cin, int Cout, uint16_t* __restrict__ Wb,
+                               uint16_t* __restrict__ Wd) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = 9 * Cin * Cout;
+    if (i >= n) return;
+    int ci = i % Cin, co = (i / Cin) % Cout, tap = i / (Cin * Cout);
+    uint16_t b = f2bf(w32[i]);
+    Wb[i] = b;
+    Wd[((size_t)(8 - tap) * Cin + ci) * Cout + co] = b;
+}
+
+inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+template <class T> T* dalloc(size_t n) {
+    T* p = nullptr;
+    KF_CHECK(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return p;
+}
+
+// ---------------------------------------------------------------- launchers
+void conv_fwd(const uint16_t* X, int ldx, int Cin, const uint16_t* Wb, const float* bias, int Cout, int N, int H,
+              int W, uint16_t* Y, int ldy, bool relu, hipStream_t st, float* Yf = nullptr, int ldyf = 0,
+              int cstore = -1) {
+    dim3 grid(blocks((size_t)N * H * W, CBM), blocks(Cout, CBN));
+    k_conv<<<grid, TB, 0, st>>>(X, ldx, Cin, Wb, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore < 0 ? Cout : cstore,
+                                nullptr, 0, relu ? 1 : 0);
+}
+// dX = conv(dY, Wd) masked by (Xf > 0) when Xf is given
+void conv_dgrad(const uint16_t* dY, int ldy, int Cout, const uint16_t* Wd, int Cin, int N, int H, int W, uint16_t* dX,
+                int ldx, const uint16_t* Xf, int ldxf, hipStream_t st) {
+    dim3 grid(blocks((size_t)N * H * W, CBM), blocks(Cin, CBN));
+    k_conv<<<grid, TB, 0, st>>>(dY, ldy, Cout, Wd, nullptr, Cin, N, H, W, dX, ldx, nullptr, 0, Cin, Xf, ldxf, 0);
+}
+void conv_wgrad(const uint16_t* X, int ldx, int Cin, const uint16_t* dY, int ldy, int Cout, int N, int H, int W,
+                float* dW, float* db, hipStream_t st) {
+    const int P = N * H * W;
+    int chunk = 2048;
+    while ((size_t)blocks(P, chunk) * 9 * blocks(Cout, 64) * blocks(Cin, 64) < 2048 && chunk > 64) chunk /= 2;
+    dim3 grid(blocks(P, chunk), 9, blocks(Cout, 64) * blocks(Cin, 64));
+    k_wgrad<<<grid, TB, 0, st>>>(X, ldx, Cin, dY, ldy, Cout, N, H, W, chunk, dW);
+    int bchunk = 1024;
+    k_bias_grad<<<blocks(P, bchunk), TB, 0, st>>>(dY, ldy, Cout, P, bchunk, db);
+}
+
+// ---------------------------------------------------------------- network
+constexpr int kLevels = 5;
+constexpr int kCh[kLevels] = {32, 64, 96, 128, 160};
+constexpr int kInCh = 32;  // 9 used
+
+struct ConvDef {
+    int cin, cout;
+};
+
+// conv order: e0a e0b e1a e1b e2a e2b e3a e3b ba bb d3a d3b d2a d2b d1a d1b d0a d0b out
+std::vector<ConvDef> unet_convs() {
+    std::vector<ConvDef> c;
+    int prev = kInCh;
+    for (int l = 0; l < kLevels; ++l) {
+        c.push_back({prev, kCh[l]});
+        c.push_back({kCh[l], kCh[l]});
+        prev = kCh[l];
+    }
+    for (int l = kLevels - 2; l >= 0; --l) {
+        c.push_back({kCh[l] + kCh[l + 1], kCh[l]});
+        c.push_back({kCh[l], kCh[l]});
+    }
+    c.push_back({kCh[0], 32});  // output conv, channels 0-2 used
+    return c;
+}
+
+struct Acts {
+    int N = 0, H = 0, W = 0;
+    uint16_t* x0 = nullptr;
+    float* lin = nullptr;
+    uint16_t *t[kLevels] = {}, *cat[kLevels - 1] = {}, *pl[kLevels - 1] = {}, *bt = nullptr;
+    uint16_t *da[kLevels - 1] = {}, *u[kLevels - 1] = {};
+    float* y = nullptr;
+    // gradients
+    uint16_t* dy = nullptr;
+    uint16_t *du[kLevels - 1] = {}, *dda[kLevels - 1] = {}, *dcat[kLevels - 1] = {}, *dt[kLevels] = {};
+    uint16_t *dpl[kLevels - 1] = {}, *dbt = nullptr;
+    float* target = nullptr;
+    float* inrec = nullptr;
+    float* outrec = nullptr;
+    bool grads = false;
+    std::vector<void*> all;
+
+    template <class T> T* A(size_t n) {
+        T* p = dalloc<T>(n);
+        all.push_back(p);
+        return p;
+    }
+    void release() {
+        for (void* p : all) (void)hipFree(p);
+        all.clear();
+        N = H = W = 0;
+    }
+    void ensure(int n, int h, int w, bool need_grads) {
+        if (n == N && h == H && w == W && (grads || !need_grads)) return;
+        release();
+        N = n, H = h, W = w;
+        grads = need_grads;
+        auto P = [&](int l) { return (size_t)n * (h >> l) * (w >> l); };
+        x0 = A<uint16_t>(P(0) * kInCh);
+        lin = A<float>(P(0) * 4);
+        y = A<float>(P(0) * 4);
+        for (int l = 0; l < kLevels; ++l) t[l] = A<uint16_t>(P(l) * kCh[l]);
+        for (int l = 0; l < kLevels - 1; ++l) {
+            cat[l] = A<uint16_t>(P(l) * (kCh[l] + kCh[l + 1]));
+            pl[l] = A<uint16_t>(P(l + 1) * kCh[l]);
+            da[l] = A<uint16_t>(P(l) * kCh[l]);
+            u[l] = A<uint16_t>(P(l) * kCh[l]);
+        }
+        bt = A<uint16_t>(P(kLevels - 1) * kCh[kLevels - 1]);
+        if (need_grads) {
+            dy = A<uint16_t>(P(0) * 32);
+            for (int l = 0; l < kLevels; ++l) dt[l] = A<uint16_t>(P(l) * kCh[l]);
+            for (int l = 0; l < kLevels - 1; ++l) {
+                du[l] = A<uint16_t>(P(l) * kCh[l]);
+                dda[l] = A<uint16_t>(P(l) * kCh[l]);
+                dcat[l] = A<uint16_t>(P(l) * (kCh[l] + kCh[l + 1]));
+                dpl[l] = A<uint16_t>(P(l + 1) * kCh[l]);
+            }
+            dbt = A<uint16_t>(P(kLevels - 1) * kCh[kLevels - 1]);
+            target = A<float>(P(0) * 3);
+        }
+    }
+    ~Acts() { release(); }
+};
+
+}  // namespace
+
+struct Unet {
+    std::vector<ConvDef> defs;
+    std::vector<size_t> woff, boff;  // offsets into the flat parameter vector
+    size_t nparams = 0;
+    float *w32 = nullptr, *g32 = nullptr, *m32 = nullptr, *v32 = nullptr;
+    std::vector<uint16_t*> wb, wd;
+    float* d_loss = nullptr;
+    int step = 0;
+    Acts act;
+    float* rec_in = nullptr;
+    size_t rec_in_cap = 0;
+    float* rec_tg = nullptr;
+    size_t rec_tg_cap = 0;
+    float* rec_out = nullptr;
+    size_t rec_out_cap = 0;
+
+    Unet() {
+        defs = unet_convs();
+        for (auto& d : defs) {
+            woff.push_back(nparams);
+            nparams += (size_t)9 * d.cin * d.cout;
+            boff.push_back(nparams);
+            nparams += d.cout;
+        }
+        w32 = dalloc<float>(nparams);
+        g32 = dalloc<float>(nparams);
+        m32 = dalloc<float>(nparams);
+        v32 = dalloc<float>(nparams);
+        for (auto& d : defs) {
+            wb.push_back(dalloc<uint16_t>((size_t)9 * d.cin * d.cout));
+            wd.push_back(dalloc<uint16_t>((size_t)9 * d.cin * d.cout));
+        }
+        d_loss = dalloc<float>(1);
+    }
+    ~Unet() {
+        act.release();
+        for (void* p : {(void*)w32, (void*)g32, (void*)m32, (void*)v32, (void*)d_loss, (void*)rec_in, (void*)rec_tg,
+                        (void*)rec_out})
+            if (p) (void)hipFree(p);
+        for (auto p : wb) (void)hipFree(p);
+        for (auto p : wd) (void)hipFree(p);
+    }
+
+    void sync_bf16(hipStream_t st) {
+        for (size_t k = 0; k < defs.size(); ++k) {
+            int n = 9 * defs[k].cin * defs[k].cout;
+            k_weights_bf16<<<blocks(n, TB), TB, 0, st>>>(w32 + woff[k], defs[k].cin, defs[k].cout, wb[k], wd[k]);
+        }
+    }
+
+    void reset_optimizer(hipStream_t st) {
+        k_zero_f32<<<blocks(nparams, TB), TB, 0, st>>>(m32, nparams);
+        k_zero_f32<<<blocks(nparams, TB), TB, 0, st>>>(v32, nparams);
+        step = 0;
+    }
+
+    void set_params_host(const std::vector<float>& h) {
+        hipStream_t st = stream();
+        KF_CHECK(hipMemcpyAsync(w32, h.data(), nparams * sizeof(float), hipMemcpyHostToDevice, st));
+        sync_bf16(st);
+        reset_optimizer(st);
+        KF_CHECK(hipStreamSynchronize(st));
+    }
+
+    void init_random(uint32_t seed) {
+        std::vector<float> h(nparams, 0.0f);
+        uint64_t s = 0x9E3779B97F4A7C15ull ^ seed;
+        auto next = [&]() {  // splitmix64
+            s += 0x9E3779B97F4A7C15ull;
+            uint64_t z = s;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        auto unif = [&]() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+        for (size_t k = 0; k < defs.size(); ++k) {
+            bool last = k + 1 == defs.size();
+            int cin_eff = k == 0 ? 9 : defs[k].cin;
+            double std = last ? 1e-3 : std::sqrt(2.0 / (9.0 * cin_eff));
+            for (int tap = 0; tap < 9; ++tap)
+                for (int co = 0; co < defs[k].cout; ++co)
+                    for (int ci = 0; ci < defs[k].cin; ++ci) {
+                        double u1 = unif(), u2 = unif();
+                        double g = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+                        bool used = (k != 0 || ci < 9) && (!last || co < 3);
+                        h[woff[k] + ((size_t)tap * defs[k].cout + co) * defs[k].cin + ci] = used ? (float)(g * std) : 0.f;
+                    }
+        }
+        set_params_host(h);
+    }
+
+    // forward pass on act.x0 / act.lin -> act.y
+    void forward(hipStream_t st) {
+        const int N = act.N;
+        auto H = [&](int l) { return act.H >> l; };
+        auto W = [&](int l) { return act.W >> l; };
+        int k = 0;
+        const uint16_t* in = act.x0;
+        int ldin = kInCh, cin = kInCh;
+        for (int l = 0; l < kLevels; ++l) {
+            conv_fwd(in, ldin, cin, wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.t[l], kCh[l], true, st);
+            ++k;
+            if (l < kLevels - 1) {
+                int ldc = kCh[l] + kCh[l + 1];
+                conv_fwd(act.t[l], kCh[l], kCh[l], wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.cat[l], ldc,
+                         true, st);
+                ++k;
+                size_t n8 = (size_t)N * H(l + 1) * W(l + 1) * kCh[l] / 8;
+                k_pool<<<blocks(n8, TB), TB, 0, st>>>(act.cat[l], ldc, kCh[l], N, H(l), W(l), act.pl[l]);
+                in = act.pl[l];
+                ldin = cin = kCh[l];
+            } else {
+                conv_fwd(act.t[l], kCh[l], kCh[l], wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.bt, kCh[l], true,
+                         st);
+                ++k;
+            }
+        }
+        const uint16_t* low = act.bt;
+        for (int l = kLevels - 2; l >= 0; --l) {
+            int ldc = kCh[l] + kCh[l + 1];
+            size_t n8 = (size_t)N * H(l) * W(l) * kCh[l + 1] / 8;
+            k_up<<<blocks(n8, TB), TB, 0, st>>>(low, kCh[l + 1], N, H(l + 1), W(l + 1), act.cat[l] + kCh[l], ldc);
+            conv_fwd(act.cat[l], ldc, ldc, wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.da[l], kCh[l], true, st);
+            ++k;
+            conv_fwd(act.da[l], kCh[l], kCh[l], wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.u[l], kCh[l], true,
+                     st);
+            ++k;
+            low = act.u[l];
+        }
+        conv_fwd(act.u[0], kCh[0], kCh[0], wb[k], w32 + boff[k], 32, N, H(0), W(0), nullptr, 0, false, st, act.y, 4, 3);
+    }
+
+    // backward from act.dy; accumulates into g32 (zeroed here)
+    void backward(hipStream_t st) {
+        const int N = act.N;
+        auto H = [&](int l) { return act.H >> l; };
+        auto W = [&](int l) { return act.W >> l; };
+        k_zero_f32<<<blocks(nparams, TB), TB, 0, st>>>(g32, nparams);
+        int k = (int)defs.size() - 1;
+        // output conv
+        conv_wgrad(act.u[0], kCh[0], kCh[0], act.dy, 32, 32, N, H(0), W(0), g32 + woff[k], g32 + boff[k], st);
+        conv_dgrad(act.dy, 32, 32, wd[k], kCh[0], N, H(0), W(0), act.du[0], kCh[0], act.u[0], kCh[0], st);
+        --k;
+        for (int l = 0; l < kLevels - 1; ++l) {
+            int ldc = kCh[l] + kCh[l + 1];
+            // d_lb: input da[l], grad du[l]
+            conv_wgrad(act.da[l], kCh[l], kCh[l], act.du[l], kCh[l], kCh[l], N, H(l), W(l), g32 + woff[k],
+                       g32 + boff[k], st);
+            conv_dgrad(act.du[l], kCh[l], kCh[l], wd[k], kCh[l], N, H(l), W(l), act.dda[l], kCh[l], act.da[l], kCh[l],
+                       st);
+            --k;
+            // d_la: input cat[l], grad dda[l]
+            conv_wgrad(act.cat[l], ldc, ldc, act.dda[l], kCh[l], kCh[l], N, H(l), W(l), g32 + woff[k], g32 + boff[k],
+                       st);
+            conv_dgrad(act.dda[l], kCh[l], kCh[l], wd[k], ldc, N, H(l), W(l), act.dcat[l], ldc, act.cat[l], ldc, st);
+            --k;
+            // upsampled half of the concat -> gradient of the level below's output
+            const uint16_t* lowf = (l + 1 < kLevels - 1) ? act.u[l + 1] : act.bt;
+            uint16_t* dlow = (l + 1 < kLevels - 1) ? act.du[l + 1] : act.dbt;
+            size_t n8 = (size_t)N * H(l + 1) * W(l + 1) * kCh[l + 1] / 8;
+            k_up_bwd<<<blocks(n8, TB), TB, 0, st>>>(act.dcat[l] + kCh[l], ldc, kCh[l + 1], lowf, N, H(l + 1), W(l + 1),
+                                                   dlow);
+        }
+        // bottleneck bb, ba
+        int L = kLevels - 1;
+        conv_wgrad(act.t[L], kCh[L], kCh[L], act.dbt, kCh[L], kCh[L], N, H(L), W(L), g32 + woff[k], g32 + boff[k], st);
+        conv_dgrad(act.dbt, kCh[L], kCh[L], wd[k], kCh[L], N, H(L), W(L), act.dt[L], kCh[L], act.t[L], kCh[L], st);
+        --k;
+        conv_wgrad(act.pl[L - 1], kCh[L - 1], kCh[L - 1], act.dt[L], kCh[L], kCh[L], N, H(L), W(L), g32 + woff[k],
+                   g32 + boff[k], st);
+        conv_dgrad(act.dt[L], kCh[L], kCh[L], wd[k], kCh[L - 1], N, H(L), W(L), act.dpl[L - 1], kCh[L - 1], nullptr, 0,
+                   st);
+        --k;
+        for (int l = kLevels - 2; l >= 0; --l) {
+            int ldc = kCh[l] + kCh[l + 1];
+            // pooling backward into the skip half of dcat (masked by the skip values)
+            size_t n8 = (size_t)N * H(l) * W(l) * kCh[l] / 8;
+            k_pool_bwd<<<blocks(n8, TB), TB, 0, st>>>(act.dpl[l], kCh[l], act.cat[l], ldc, act.dcat[l], ldc, N, H(l),
+                                                     W(l));
+            // e_lb: input t[l], grad dcat[l][:, 0:C]
+            conv_wgrad(act.t[l], kCh[l], kCh[l], act.dcat[l], ldc, kCh[l], N, H(l), W(l), g32 + woff[k], g32 + boff[k],
+                       st);
+            conv_dgrad(act.dcat[l], ldc, kCh[l], wd[k], kCh[l], N, H(l), W(l), act.dt[l], kCh[l], act.t[l], kCh[l], st);
+            --k;
+            // e_la: input pl[l-1] (or x0), grad dt[l]
+            const uint16_t* in = l > 0 ? act.pl[l - 1] : act.x0;
+            int cin = l > 0 ? kCh[l - 1] : kInCh;
+            conv_wgrad(in, cin, cin, act.dt[l], kCh[l], kCh[l], N, H(l), W(l), g32 + woff[k], g32 + boff[k], st);
+            if (l > 0)
+                conv_dgrad(act.dt[l], kCh[l], kCh[l], wd[k], cin, N, H(l), W(l), act.dpl[l - 1], cin, nullptr, 0, st);
+            --k;
+        }
+    }
+
+    float loss_and_grad(hipStream_t st) {
+        k_zero_f32<<<1, 32, 0, st>>>(d_loss, 1);
+        int P = act.N * act.H * act.W;
+        k_loss<<<blocks(P, TB), TB, 0, st>>>(act.y, act.lin, act.target, P, act.dy, d_loss);
+        backward(st);
+        float l = 0;
+        KF_CHECK(hipMemcpyAsync(&l, d_loss, sizeof(float), hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        return l;
+    }
+
+    void upload_records(const float* in, const float* tg, int n, int h, int w, bool grads) {
+        hipStream_t st = stream();
+        act.ensure(n, h, w, grads);
+        size_t P = (size_t)n * h * w;
+        if (P * 9 > rec_in_cap) {
+            if (rec_in) KF_CHECK(hipFree(rec_in));
+            rec_in = dalloc<float>(P * 9);
+            rec_in_cap = P * 9;
+        }
+        KF_CHECK(hipMemcpyAsync(rec_in, in, P * 9 * sizeof(float), hipMemcpyHostToDevice, st));
+        k_pack_records<<<blocks(P, TB), TB, 0, st>>>(rec_in, (int)P, act.x0, act.lin);
+        if (tg) KF_CHECK(hipMemcpyAsync(act.target, tg, P * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+};
+
+}  // namespace kf
+
+struct kf_denoiser : kf::Unet {};
+
+using namespace kf;
+
+namespace {
+
+constexpr char kMagic[8] = {'K', 'F', 'U', 'N', 'E', 'T', '0', '1'};
+
+std::string default_weights_path() {
+    const char* env = getenv("KUAFU_AMD_WEIGHTS");
+    if (env && *env) return env;
+    // the weight file ships next to the library: <pkg>/weights/unet.kfw
+    std::string so;
+    {
+        std::ifstream maps("/proc/self/maps");
+        std::string line;
+        while (std::getline(maps, line)) {
+            size_t pos = line.find("libkuafu_amd.so");
+            if (pos != std::string::npos) {
+                size_t s = line.find('/');
+                so = line.substr(s, pos + strlen("libkuafu_amd.so") - s);
+                break;
+            }
+        }
+    }
+    if (so.empty()) return "";
+    return so.substr(0, so.rfind('/')) + "/weights/unet.kfw";
+}
+
+bool read_weights(const std::string& path, const Unet& u, std::vector<float>& out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    char magic[8];
+    f.read(magic, 8);
+    if (!f || memcmp(magic, kMagic, 8) != 0) throw Error("bad denoiser weight file: " + path);
+    uint32_t nconv = 0;
+    f.read((char*)&nconv, 4);
+    if (nconv != u.defs.size()) throw Error("denoiser weight file has a different network: " + path);
+    out.assign(u.nparams, 0.0f);
+    for (size_t k = 0; k < u.defs.size(); ++k) {
+        uint32_t ci = 0, co = 0;
+        f.read((char*)&ci, 4);
+        f.read((char*)&co, 4);
+        if ((int)ci != u.defs[k].cin || (int)co != u.defs[k].cout) throw Error("denoiser weight shape mismatch");
+        size_t nw = (size_t)9 * ci * co;
+        std::vector<uint16_t> wb(nw);
+        f.read((char*)wb.data(), nw * 2);
+        for (size_t i = 0; i < nw; ++i) out[u.woff[k] + i] = bf2f(wb[i]);
+        f.read((char*)&out[u.boff[k]], co * sizeof(float));
+    }
+    if (!f) throw Error("truncated denoiser weight file: " + path);
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kf_denoiser_create(const char* weights_path, kf_denoiser** out) {
+    kf_denoiser* d = nullptr;
+    try {
+        if (!out) throw Error("kf_denoiser_create: null argument");
+        d = new kf_denoiser();
+        std::vector<float> h;
+        std::string path = weights_path ? weights_path : default_weights_path();
+        if (!path.empty() && read_weights(path, *d, h)) {
+            d->set_params_host(h);
+        } else {
+            if (weights_path) throw Error("cannot open denoiser weights: " + path);
+            d->init_random(0);
+        }
+        *out = d;
+        return 0;
+    } catch (const std::exception& e) {
+        delete d;
+        set_error(e.what());
+        return -1;
+    }
+}
+
+void kf_denoiser_destroy(kf_denoiser* d) {
+    if (d) (void)hipStreamSynchronize(stream());
+    delete d;
+}
+
+int kf_denoiser_init_random(kf_denoiser* d, uint32_t seed) {
+    try {
+        if (!d) throw Error("null denoiser");
+        d->init_random(seed);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_denoiser_num_params(kf_denoiser* d, uint64_t* n) {
+    if (!d || !n) {
+        set_error("null argument");
+        return -1;
+    }
+    *n = d->nparams;
+    return 0;
+}
+
+int kf_denoiser_get_params(kf_denoiser* d, float* out) {
+    try {
+        if (!d || !out) throw Error("null argument");
+        KF_CHECK(hipMemcpyAsync(out, d->w32, d->nparams * sizeof(float), hipMemcpyDeviceToHost, stream()));
+        KF_CHECK(hipStreamSynchronize(stream()));
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_denoiser_set_params(kf_denoiser* d, const float* in) {
+    try {
+        if (!d || !in) throw Error("null argument");
+        d->set_params_host(std::vector<float>(in, in + d->nparams));
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_denoiser_save(kf_denoiser* d, const char* path) {
+    try {
+        if (!d || !path) throw Error("null argument");
+        std::vector<float> h(d->nparams);
+        KF_CHECK(hipMemcpy(h.data(), d->w32, d->nparams * sizeof(float), hipMemcpyDeviceToHost));
+        std::ofstream f(path, std::ios::binary);
+        if (!f) throw Error(std::string("cannot write ") + path);
+        f.write(kMagic, 8);
+        uint32_t nconv = (uint32_t)d->defs.size();
+        f.write((const char*)&nconv, 4);
+        for (size_t k = 0; k < d->defs.size(); ++k) {
+            uint32_t ci = d->defs[k].cin, co = d->defs[k].cout;
+            f.write((const char*)&ci, 4);
+            f.write((const char*)&co, 4);
+            size_t nw = (size_t)9 * ci * co;
+            std::vector<uint16_t> wb(nw);
+            for (size_t i = 0; i < nw; ++i) wb[i] = f2bf(h[d->woff[k] + i]);
+            f.write((const char*)wb.data(), nw * 2);
+            f.write((const char*)&h[d->boff[k]], co * sizeof(float));
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_denoiser_load(kf_denoiser* d, const char* path) {
+    try {
+        if (!d || !path) throw Error("null argument");
+        std::vector<float> h;
+        if (!read_weights(path, *d, h)) throw Error(std::string("cannot open ") + path);
+        d->set_params_host(h);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_denoiser_infer_device(kf_denoiser* d, const float* d_rgba, const float* d_albedo, const float* d_normal,
+                             uint32_t width, uint32_t height, float* d_out, float* kernel_ms) {
+    try {
+        if (!d || !d_rgba || !d_albedo || !d_normal || !d_out || !width || !height) throw Error("bad argument");
+        hipStream_t st = stream();
+        int Hp = (int)((height + 15) / 16 * 16), Wp = (int)((width + 15) / 16 * 16);
+        d->act.ensure(1, Hp, Wp, false);
+        hipEvent_t e0, e1;
+        KF_CHECK(hipEventCreate(&e0));
+        KF_CHECK(hipEventCreate(&e1));
+        KF_CHECK(hipEventRecord(e0, st));
+        k_pack_targets<<<blocks((size_t)Hp * Wp, TB), TB, 0, st>>>((const float4*)d_rgba, (const float4*)d_albedo,
+                                                                   (const float4*)d_normal, (int)height, (int)width,
+                                                                   Hp, Wp, d->act.x0, d->act.lin);
+        d->forward(st);
+        k_unpack<<<blocks((size_t)width * height, TB), TB, 0, st>>>(d->act.y, d->act.lin, (int)height, (int)width, Hp,
+                                                                    Wp, (float4*)d_out);
+        KF_CHECK(hipEventRecord(e1, st));
+        KF_CHECK(hipGetLastError());
+        KF_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        KF_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (kernel_ms) *kernel_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_denoiser_apply(kf_denoiser* d, kf_renderer* r) {
+    if (!d || !r) {
+        set_error("kf_denoiser_apply: null argument");
+        return -1;
+    }
+    // the output overwrites rgba (denoiser.cpp:225-251, context.cpp:669-670)
+    return kf_denoiser_infer_device(d, (const float*)r->d_rgba, (const float*)r->d_albedo, (const float*)r->d_normal,
+                                    r->W, r->H, (float*)r->d_rgba, nullptr);
+}
+
+int kf_denoiser_forward(kf_denoiser* d, const float* inputs, uint32_t n, uint32_t h, uint32_t w, float* out) {
+    try {
+        if (!d || !inputs || !out || h % 16 || w % 16 || !n) throw Error("kf_denoiser_forward: bad argument");
+        hipStream_t st = stream();
+        d->upload_records(inputs, nullptr, (int)n, (int)h, (int)w, d->act.grads);
+        d->forward(st);
+        size_t P = (size_t)n * h * w;
+        if (P * 3 > d->rec_out_cap) {
+            if (d->rec_out) KF_CHECK(hipFree(d->rec_out));
+            d->rec_out = dalloc<float>(P * 3);
+            d->rec_out_cap = P * 3;
+        }
+        k_unpack_records<<<blocks(P, TB), TB, 0, st>>>(d->act.y, d->act.lin, (int)P, d->rec_out);
+        KF_CHECK(hipMemcpyAsync(out, d->rec_out, P * 3 * sizeof(float), hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_denoiser_gradients(kf_denoiser* d, const float* inputs, const float* targets, uint32_t n, uint32_t h,
+                          uint32_t w, float* loss, float* grads) {
+    try {
+        if (!d || !inputs || !targets || h % 16 || w % 16 || !n) throw Error("kf_denoiser_gradients: bad argument");
+        hipStream_t st = stream();
+        d->upload_records(inputs, targets, (int)n, (int)h, (int)w, true);
+        d->forward(st);
+        float l = d->loss_and_grad(st);
+        if (loss) *loss = l;
+        if (grads) {
+            KF_CHECK(hipMemcpyAsync(grads, d->g32, d->nparams * sizeof(float), hipMemcpyDeviceToHost, st));
+            KF_CHECK(hipStreamSynchronize(st));
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_denoiser_train_step(kf_denoiser* d, const float* inputs, const float* targets, uint32_t n, uint32_t h,
+                           uint32_t w, float lr, float* loss) {
+    try {
+        if (!d || !inputs || !targets || h % 16 || w % 16 || !n) throw Error("kf_denoiser_train_step: bad argument");
+        hipStream_t st = stream();
+        d->upload_records(inputs, targets, (int)n, (int)h, (int)w, true);
+        d->forward(st);
+        float l = d->loss_and_grad(st);
+        d->step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        float c1 = 1.0f - std::pow(b1, (float)d->step), c2 = 1.0f - std::pow(b2, (float)d->step);
+        k_adam<<<blocks(d->nparams, TB), TB, 0, st>>>(d->w32, d->g32, d->m32, d->v32, d->nparams, lr, b1, b2, 1e-8f,
+                                                      c1, c2);
+        d->sync_bf16(st);
+        KF_CHECK(hipGetLastError());
+        if (loss) *loss = l;
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_conv3x3_forward(const uint16_t* x, uint32_t cin, const uint16_t* w, const float* bias, uint32_t cout,
+                       uint32_t n, uint32_t h, uint32_t wd, int relu, uint16_t* y) {
+    try {
+        if (!x || !w || !y || cin % 32 || !cout || !n || !h || !wd) throw Error("kf_conv3x3_forward: bad argument");
+        hipStream_t st = stream();
+        size_t P = (size_t)n * h * wd;
+        uint32_t ldy = (cout + 7) / 8 * 8;
+        uint16_t* dx = dalloc<uint16_t>(P * cin);
+        uint16_t* dw = dalloc<uint16_t>((size_t)9 * cin * cout);
+        float* db = dalloc<float>(cout);
+        uint16_t* dy = dalloc<uint16_t>(P * ldy);
+        KF_CHECK(hipMemcpyAsync(dx, x, P * cin * 2, hipMemcpyHostToDevice, st));
+        KF_CHECK(hipMemcpyAsync(dw, w, (size_t)9 * cin * cout * 2, hipMemcpyHostToDevice, st));
+        if (bias) KF_CHECK(hipMemcpyAsync(db, bias, cout * 4, hipMemcpyHostToDevice, st));
+        conv_fwd(dx, cin, cin, dw, bias ? db : nullptr, cout, n, h, wd, dy, ldy, relu != 0, st);
+        KF_CHECK(hipGetLastError());
+        std::vector<uint16_t> tmp(P * ldy);
+        KF_CHECK(hipMemcpyAsync(tmp.data(), dy, P * ldy * 2, hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        for (size_t p = 0; p < P; ++p) memcpy(y + p * cout, &tmp[p * ldy], cout * 2);
+        for (void* q : {(void*)dx, (void*)dw, (void*)db, (void*)dy}) KF_CHECK(hipFree(q));
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_conv3x3_backward(const uint16_t* x, uint32_t cin, const uint16_t* w, uint32_t cout, const uint16_t* dy,
+                        uint32_t n, uint32_t h, uint32_t wd, int mask_by_x, uint16_t* dx, float* dw, float* db) {
+    try {
+        if (!x || !w || !dy || cin % 32 || cout % 32 || !n || !h || !wd) throw Error("kf_conv3x3_backward: bad argument");
+        hipStream_t st = stream();
+        size_t P = (size_t)n * h * wd, NW = (size_t)9 * cin * cout;
+        uint16_t* d_x = dalloc<uint16_t>(P * cin);
+        uint16_t* d_dy = dalloc<uint16_t>(P * cout);
+        uint16_t* d_dx = dalloc<uint16_t>(P * cin);
+        float* d_w32 = dalloc<float>(NW);
+        uint16_t* d_wb = dalloc<uint16_t>(NW);
+        uint16_t* d_wd = dalloc<uint16_t>(NW);
+        float* d_dw = dalloc<float>(NW);
+        float* d_db = dalloc<float>(cout);
+        std::vector<float> w32(NW);
+        for (size_t i = 0; i < NW; ++i) w32[i] = bf2f(w[i]);
+        KF_CHECK(hipMemcpyAsync(d_x, x, P * cin * 2, hipMemcpyHostToDevice, st));
+        KF_CHECK(hipMemcpyAsync(d_dy, dy, P * cout * 2, hipMemcpyHostToDevice, st));
+        KF_CHECK(hipMemcpyAsync(d_w32, w32.data(), NW * 4, hipMemcpyHostToDevice, st));
+        k_weights_bf16<<<blocks(NW, TB), TB, 0, st>>>(d_w32, (int)cin, (int)cout, d_wb, d_wd);
+        k_zero_f32<<<blocks(NW, TB), TB, 0, st>>>(d_dw, NW);
+        k_zero_f32<<<blocks(cout, TB), TB, 0, st>>>(d_db, cout);
+        conv_dgrad(d_dy, cout, cout, d_wd, cin, n, h, wd, d_dx, cin, mask_by_x ? d_x : nullptr, cin, st);
+        conv_wgrad(d_x, cin, cin, d_dy, cout, cout, n, h, wd, d_dw, d_db, st);
+        KF_CHECK(hipGetLastError());
+        if (dx) KF_CHECK(hipMemcpyAsync(dx, d_dx, P * cin * 2, hipMemcpyDeviceToHost, st));
+        if (dw) KF_CHECK(hipMemcpyAsync(dw, d_dw, NW * 4, hipMemcpyDeviceToHost, st));
+        if (db) KF_CHECK(hipMemcpyAsync(db, d_db, cout * 4, hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        for (void* q : {(void*)d_x, (void*)d_dy, (void*)d_dx, (void*)d_w32, (void*)d_wb, (void*)d_wd, (void*)d_dw,
+                        (void*)d_db})
+            KF_CHECK(hipFree(q));
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+}  // extern "C"

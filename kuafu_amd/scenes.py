@@ -280,3 +280,67 @@ def tiny_scene(seed=0, n_spheres=3, with_lights=True, soft=False):
         view = look_at((2.0, 3.0, 2.0), (0.0, 0.0, 0.0), (0, 1, 0))
         s.add_active_light(view, (0.9, 0.9, 1.0), fov=math.radians(50), strength=20.0, softness=0.2 if soft else 0.0)
     return s
+
+
+def training_scene(seed, width=256, height=256):
+    """Randomised scene for denoiser training: the config-1 generator (ground
+    plane, UV spheres of the five material kinds, directional + point light,
+    clear colour) with random counts, materials, lights and camera; one in four
+    is a config-2-style cluster of icospheres under a gradient environment."""
+    rng = np.random.default_rng(1000003 * seed + 17)
+    s = SceneData()
+    kinds = ["diffuse", "specular", "metal", "glass", "emissive", "mixed"]
+
+    def rand_material(kind):
+        col = tuple(rng.uniform(0.05, 0.95, 3))
+        if kind == "diffuse":
+            return material(diffuse=col, roughness=rng.uniform(0.3, 1.0))
+        if kind == "specular":
+            return material(diffuse=col, specular=rng.uniform(0.5, 1.0), roughness=rng.uniform(0.02, 0.3))
+        if kind == "metal":
+            return material(diffuse=col, metallic=1.0, roughness=rng.uniform(0.05, 0.8))
+        if kind == "glass":
+            return material(diffuse=tuple(rng.uniform(0.7, 1.0, 3)), transmission=1.0, ior=rng.uniform(1.3, 1.7))
+        if kind == "emissive":
+            return material(emission=tuple(rng.uniform(0.5, 1.0, 3)), emission_strength=rng.uniform(1.0, 6.0))
+        return material(diffuse=col, metallic=rng.uniform(0, 1), roughness=rng.uniform(0.05, 1.0),
+                        transmission=float(rng.uniform() < 0.2), ior=rng.uniform(1.3, 1.7))
+
+    if rng.uniform() < 0.75:
+        gv, gi = quad(rng.uniform(6.0, 14.0))
+        g = s.add_geometry(gv, gi, material(diffuse=tuple(rng.uniform(0.2, 0.9, 3)), roughness=rng.uniform(0.2, 1.0),
+                                            metallic=float(rng.uniform() < 0.1)))
+        s.add_instance(g, mat_rotate(-math.pi / 2, (1, 0, 0)))
+        sv, si = uv_sphere(32, 16, 0.5)
+        for k in range(int(rng.integers(3, 9))):
+            m = s.add_geometry(sv, si, rand_material(kinds[int(rng.integers(0, len(kinds)))]))
+            r = rng.uniform(0.4, 1.6)
+            s.add_instance(m, mat_scale(r, mat_translate((rng.uniform(-2.5, 2.5), 0.5 * r, rng.uniform(-2.5, 1.5)))))
+        s.set_directional_light(tuple(rng.uniform(-1, 1, 3) * (1, 0, 1) + (0, -1, 0)), tuple(rng.uniform(0.7, 1, 3)),
+                                rng.uniform(0.5, 4.0), 0.0)
+        if rng.uniform() < 0.7:
+            s.add_point_light((rng.uniform(-2, 2), rng.uniform(1.5, 4), rng.uniform(-2, 2)), tuple(rng.uniform(0.7, 1, 3)),
+                              rng.uniform(2, 15), radius=0.0)
+        eye = np.array([rng.uniform(-3, 3), rng.uniform(0.8, 3.5), rng.uniform(4, 8)])
+        tgt = np.array([rng.uniform(-0.5, 0.5), rng.uniform(0.2, 0.8), 0.0])
+        up = (0, 1, 0)
+        cc = tuple(rng.uniform(0.3, 1.0, 3)) + (rng.uniform(0.02, 0.6),)
+        pc = push_constants(spp=1, path_depth=int(rng.integers(4, 9)), clear_color=cc)
+    else:
+        iv, ii = icosphere(3)
+        for k in range(int(rng.integers(12, 40))):
+            m = s.add_geometry(iv, ii, rand_material(kinds[int(rng.integers(0, len(kinds)))]))
+            R = random_rotation(rng) * rng.uniform(0.2, 1.2)
+            s.add_instance(m, mat_from_rows(R, rng.uniform(-4, 4, 3)))
+        view = look_at(tuple(rng.uniform(-8, 8, 3)), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+        s.add_active_light(view, (1.0, 0.95, 0.9), fov=math.radians(rng.uniform(20, 60)), strength=rng.uniform(50, 400))
+        s.set_gradient_environment(tuple(rng.uniform(0.0, 0.1, 3)), tuple(rng.uniform(0.2, 0.8, 3)))
+        d = rng.normal(size=3)
+        eye = 12.0 * d / np.linalg.norm(d)
+        tgt = np.zeros(3)
+        up = (0, 0, 1)
+        pc = push_constants(spp=1, path_depth=8, use_env_map=True)
+    front = (tgt - eye) / np.linalg.norm(tgt - eye)
+    f = width * rng.uniform(0.7, 1.4)
+    cam = camera_ubo(eye, front, up, width, height, fx=f, fy=f)
+    return Workload("train%d" % seed, s, cam, pc, width, height, seed)

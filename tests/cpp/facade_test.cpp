@@ -1,0 +1,270 @@
+// Test program of the kuafu:: C++ facade (include/kuafu.hpp).
+//   facade_test cpu          API semantics that need no GPU; prints OK
+//   facade_test gpu <dir>    renders through Kuafu::run() and writes
+//                            <dir>/scene.json (the scene, read back through
+//                            public fields only), rgba.bin, bgra.bin
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+
+#include "kuafu.hpp"
+
+using namespace kuafu;
+
+#define CHECK(c)                                                                      \
+    do {                                                                              \
+        if (!(c)) {                                                                   \
+            std::fprintf(stderr, "CHECK failed %s:%d: %s\n", __FILE__, __LINE__, #c); \
+            std::exit(1);                                                             \
+        }                                                                             \
+    } while (0)
+
+template <class F> static bool throws(F f) {
+    try {
+        f();
+    } catch (const std::exception&) {
+        return true;
+    }
+    return false;
+}
+
+static bool outward(const Geometry& g, glm::vec3 center) {
+    for (size_t t = 0; t + 2 < g.indices.size(); t += 3) {
+        glm::vec3 a = g.vertices[g.indices[t]].pos, b = g.vertices[g.indices[t + 1]].pos,
+                  c = g.vertices[g.indices[t + 2]].pos;
+        glm::vec3 n = glm::cross(b - a, c - a);
+        if (glm::dot(n, (a + b + c) * (1.0f / 3.0f) - center) <= 0) return false;
+    }
+    return true;
+}
+
+static int cpu_checks() {
+    Config cfg;
+    CHECK(cfg.getPathDepth() == 8 && cfg.getPerPixelSampleRate() == 32 && cfg.getRussianRoulette());
+    CHECK(cfg.getRussianRouletteMinBounces() == 4 && !cfg.isUsingDenoiser() && cfg.isAccumulatingFrames());
+    cfg.setPathDepth(20);  // the reference keeps the unclamped value (config.cpp:31-41)
+    CHECK(cfg.getPathDepth() == 20);
+    cfg.setPresent(false);
+    auto pc = std::make_shared<Config>(cfg);
+    Kuafu k(pc);  // no device work until run()
+    Scene* s = k.getScene();
+    CHECK(s && s->getCamera());
+    CHECK(k.getConfig().getPathDepth() == 20);
+
+    // generators: counts, outward winding, one material each
+    size_t m0 = global::materials.size();
+    auto plane = createYZPlane(true);
+    auto cube = createCube(true);
+    auto sphere = createSphere(true);
+    auto caps = createCapsule(1.0f, 0.5f, true);
+    CHECK(plane->indices.size() == 6 && cube->indices.size() == 36);
+    CHECK(sphere->indices.size() / 3 == 4900);
+    CHECK(global::materials.size() == m0 + 4);
+    CHECK(outward(*cube, {0, 0, 0}) && outward(*sphere, {0, 0, 0}));
+    glm::vec3 n = glm::cross(plane->vertices[plane->indices[1]].pos - plane->vertices[plane->indices[0]].pos,
+                             plane->vertices[plane->indices[2]].pos - plane->vertices[plane->indices[0]].pos);
+    CHECK(n.x > 0 && n.y == 0 && n.z == 0);  // YZ plane faces +X like its vertex normals
+    for (auto& v : caps->vertices) CHECK(std::fabs(glm::length(v.normal) - 1.0f) < 1e-4f);
+    for (auto& v : sphere->vertices) CHECK(std::fabs(glm::length(v.pos) - 1.0f) < 1e-5f);
+
+    // scene bookkeeping and limits (scene.cpp:58-120)
+    s->submitGeometry(cube);
+    auto gi = instance(cube, glm::translate(glm::mat4(1.0f), {1, 2, 3}));
+    s->submitGeometryInstance(gi);
+    CHECK(gi->geometryIndex == 0 && s->getGeometryInstances().size() == 1);
+    CHECK(throws([&] { s->submitGeometryInstance(instance(sphere)); }));  // geometry not submitted
+    CHECK(throws([&] { s->removeGeometry(cube); }));                      // "Not implemented!"
+    CHECK(throws([&] { s->getGeometryInstance(5); }));
+    Config small;
+    small.setGeometryLimit(16);  // -> 17 slots
+    small.setPresent(false);
+    Kuafu k2(std::make_shared<Config>(small));
+    for (int i = 0; i < 17; ++i) k2.getScene()->submitGeometry(cube);
+    CHECK(throws([&] { k2.getScene()->submitGeometry(cube); }));
+    for (int i = 0; i < 40; ++i) s->addPointLight(std::make_shared<PointLight>());  // extra lights are dropped
+
+    // camera: pose round trip and pinhole projection (camera.cpp:92-139)
+    Camera* cam = s->createCamera(64, 48);
+    s->setCamera(cam);
+    cam->setFullPerspective(64, 48, 50, 52, 30, 20, 0);
+    cam->setPosition({1, 2, 3});
+    cam->setFront({0, 0, -1});
+    cam->setUp({0, 1, 0});
+    glm::vec4 pv = cam->getViewMatrix() * glm::vec4(1.5f, 2.5f, 1.0f, 1.0f);
+    glm::vec4 clip = cam->getProjectionMatrix() * pv;
+    float px = (clip.x / clip.w + 1) / 2 * 64, py = (clip.y / clip.w + 1) / 2 * 48;
+    CHECK(std::fabs(px - (30 + 50 * 0.5f / 2.0f)) < 1e-3f && std::fabs(py - (20 - 52 * 0.5f / 2.0f)) < 1e-3f);
+    global::frameCount = 5;
+    cam->setPose(cam->getPose());
+    CHECK(global::frameCount == -1);
+    CHECK(throws([&] { cam->downloadLatestFrame(); }));  // nothing rendered yet
+    CHECK(throws([&] { k.setWindow(100, 100); }));
+
+    // OBJ import (triangulated, normals generated when absent)
+    {
+        std::ofstream f("/tmp/kf_facade_test.obj");
+        f << "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nf 1/1 2/2 3/3 4/4\n";
+    }
+    auto obj = loadObj("/tmp/kf_facade_test.obj");
+    CHECK(obj->indices.size() == 6 && obj->vertices.size() == 4);
+    CHECK(std::fabs(obj->vertices[0].normal.z - 1.0f) < 1e-6f && obj->vertices[2].texCoord.y == 0.0f);
+    CHECK(throws([&] { loadObj("/tmp/does_not_exist.obj"); }));
+    std::cout << "OK" << std::endl;
+    return 0;
+}
+
+static void json_vec(std::ostream& o, const float* v, int n) {
+    o << "[";
+    for (int i = 0; i < n; ++i) o << (i ? "," : "") << v[i];
+    o << "]";
+}
+
+static int gpu_run(const std::string& dir) {
+    Config c;
+    c.setPresent(false);
+    c.setPerPixelSampleRate(4);
+    c.setPathDepth(4);
+    c.setGeometryLimit(64);
+    auto k = std::make_unique<Kuafu>(std::make_shared<Config>(c));
+    Scene* s = k->getScene();
+    NiceMaterial floorM;
+    floorM.diffuseColor = {0.7f, 0.7f, 0.7f};
+    floorM.roughness = 0.8f;
+    auto floor = createYZPlane(true, floorM);
+    NiceMaterial red;
+    red.diffuseColor = {0.8f, 0.2f, 0.2f};
+    auto cube = createCube(true, red);
+    NiceMaterial metal;
+    metal.metallic = 1.0f;
+    metal.roughness = 0.3f;
+    auto sphere = createSphere(true, metal);
+    NiceMaterial glass;
+    glass.transmission = 1.0f;
+    glass.ior = 1.45f;
+    auto caps = createCapsule(0.6f, 0.3f, true, glass);
+    s->setGeometries({floor, cube, sphere, caps});
+    glm::mat4 ft = glm::translate(glm::mat4(1.0f), {0, 0, -0.5f});
+    ft = glm::rotate(ft, glm::radians(90.f), {0., -1., 0.});
+    ft = glm::scale(ft, glm::vec3(6.0f));
+    s->setGeometryInstances({instance(floor, ft), instance(cube, glm::scale(glm::translate(glm::mat4(1.0f), {0, -1.5f, 0}), glm::vec3(0.4f))),
+                             instance(sphere, glm::scale(glm::translate(glm::mat4(1.0f), {0.2f, 0.2f, 0.1f}), glm::vec3(0.5f))),
+                             instance(caps, glm::rotate(glm::translate(glm::mat4(1.0f), {0.5f, 1.5f, 0.2f}), 0.7f, {0, 0, 1}))});
+    auto dl = std::make_shared<DirectionalLight>();
+    dl->direction = {-0.3f, 0.2f, -1.0f};
+    dl->strength = 2.0f;
+    s->setDirectionalLight(dl);
+    auto pl = std::make_shared<PointLight>();
+    pl->position = {1, -1, 2};
+    pl->strength = 6;
+    pl->radius = 0;
+    s->addPointLight(pl);
+    auto al = std::make_shared<ActiveLight>();
+    al->viewMat = glm::lookAt(glm::vec3(3, 0, 3), glm::vec3(0, 0, 0), glm::vec3(0, 0, 1));
+    al->fov = 0.6f;
+    al->strength = 20.0f;
+    s->addActiveLight(al);
+    s->setClearColor({0.6f, 0.7f, 0.8f, 0.5f});
+    Camera* cam = s->createCamera(96, 64);
+    s->setCamera(cam);
+    cam->setFullPerspective(96, 64, 80, 80, 48, 32, 0);
+    cam->setPosition({-4.0f, 0.0f, 1.5f});
+    cam->setFront(glm::normalize(glm::vec3(1.0f, 0.0f, -0.3f)));
+    cam->setUp({0, 0, 1});
+    k->run();
+    k->run();
+    CHECK(global::frameCount == 1);
+    auto bgra = k->downloadLatestFrame(cam);
+    auto rgba = cam->downloadTarget(0);
+    CHECK(bgra.size() == 96u * 64u * 4u && rgba.size() == 96u * 64u * 4u);
+    {
+        std::ofstream f(dir + "/rgba.bin", std::ios::binary);
+        f.write((const char*)rgba.data(), rgba.size() * 4);
+        std::ofstream g(dir + "/bgra.bin", std::ios::binary);
+        g.write((const char*)bgra.data(), bgra.size());
+    }
+    // the scene, read back through public fields only
+    std::ofstream j(dir + "/scene.json");
+    j.precision(9);
+    j << "{\"geometries\":[";
+    for (size_t g = 0; g < s->getGeometries().size(); ++g) {
+        auto& G = *s->getGeometries()[g];
+        j << (g ? "," : "") << "{\"pos\":[";
+        for (size_t v = 0; v < G.vertices.size(); ++v) {
+            j << (v ? "," : "");
+            json_vec(j, &G.vertices[v].pos.x, 3);
+        }
+        j << "],\"normal\":[";
+        for (size_t v = 0; v < G.vertices.size(); ++v) {
+            j << (v ? "," : "");
+            json_vec(j, &G.vertices[v].normal.x, 3);
+        }
+        j << "],\"indices\":[";
+        for (size_t i = 0; i < G.indices.size(); ++i) j << (i ? "," : "") << G.indices[i];
+        j << "],\"material\":" << G.matIndex[0] << "}";
+    }
+    j << "],\"instances\":[";
+    for (size_t i = 0; i < s->getGeometryInstances().size(); ++i) {
+        auto& I = *s->getGeometryInstances()[i];
+        j << (i ? "," : "") << "{\"geometry\":" << I.geometryIndex << ",\"transform\":";
+        json_vec(j, glm::value_ptr(I.transform), 16);
+        j << "}";
+    }
+    j << "],\"materials\":[";
+    for (size_t m = 0; m < global::materials.size(); ++m) {
+        auto& M = global::materials[m];
+        j << (m ? "," : "") << "{\"diffuse\":";
+        json_vec(j, &M.diffuseColor.x, 3);
+        j << ",\"metallic\":" << M.metallic << ",\"specular\":" << M.specular << ",\"roughness\":" << M.roughness
+          << ",\"ior\":" << M.ior << ",\"transmission\":" << M.transmission << ",\"alpha\":" << M.alpha
+          << ",\"emission\":";
+        json_vec(j, &M.emission.x, 3);
+        j << ",\"emissionStrength\":" << M.emissionStrength << "}";
+    }
+    j << "],\"dir_light\":{\"direction\":";
+    json_vec(j, &dl->direction.x, 3);
+    j << ",\"color\":";
+    json_vec(j, &dl->color.x, 3);
+    j << ",\"strength\":" << dl->strength << ",\"softness\":" << dl->softness << "},\"point_light\":{\"position\":";
+    json_vec(j, &pl->position.x, 3);
+    j << ",\"color\":";
+    json_vec(j, &pl->color.x, 3);
+    j << ",\"strength\":" << pl->strength << ",\"radius\":" << pl->radius << "},\"active_light\":{\"view\":";
+    json_vec(j, glm::value_ptr(al->viewMat), 16);
+    j << ",\"color\":";
+    json_vec(j, &al->color.x, 3);
+    j << ",\"fov\":" << al->fov << ",\"strength\":" << al->strength << "},\"clear_color\":";
+    glm::vec4 cc = s->getClearColor();
+    json_vec(j, &cc.x, 4);
+    j << ",\"camera\":{\"position\":";
+    json_vec(j, &cam->getPosition().x, 3);
+    j << ",\"front\":";
+    json_vec(j, &cam->getFront().x, 3);
+    j << ",\"up\":[0,0,1],\"width\":" << cam->getWidth() << ",\"height\":" << cam->getHeight()
+      << ",\"fx\":" << cam->getFocalX() << ",\"fy\":" << cam->getFocalY() << ",\"cx\":" << cam->getPrincipalPointX()
+      << ",\"cy\":" << cam->getPrincipalPointY() << "},\"spp\":4,\"path_depth\":4,\"frames\":2}";
+    j.close();
+    // the denoise stage through the facade (random weights unless a weight file ships)
+    k->getConfig().setUseDenoiser(true);
+    k->run();
+    auto den = cam->downloadTarget(0);
+    for (float v : den) CHECK(std::isfinite(v));
+    std::cout << "OK" << std::endl;
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    try {
+        if (argc >= 2 && std::string(argv[1]) == "cpu") return cpu_checks();
+        if (argc >= 3 && std::string(argv[1]) == "gpu") return gpu_run(argv[2]);
+        std::fprintf(stderr, "usage: facade_test cpu | gpu <dir>\n");
+        return 2;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "exception: %s\n", e.what());
+        return 1;
+    }
+}
