@@ -46,6 +46,60 @@ def cpu_baseline(workload, budget_s=15.0):
                       "%d spp, %d rays in %.1f s" % (y0, y0 + rows, W, H, workload.pc.spp, rays, dt)}
 
 
+def relmse(x, y):
+    import numpy as np
+    return float(np.mean((x.astype(np.float64) - y) ** 2 / (y.astype(np.float64) ** 2 + 1e-2)))
+
+
+def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
+    """Wall time to reach relMSE <= target against a high-spp reference of the
+    same frame: plain tracer (spp doubling) vs tracer + denoiser."""
+    import numpy as np
+    from kuafu_amd.denoiser import Denoiser
+    pc = w.pc
+    chunk = 128
+    pc.spp = chunk
+    for f in range(ref_spp // chunk):
+        pc.frame_count = f
+        rd.render(gs, w.cam, pc, seed=900000 + f)
+    ref = rd.read(0)[..., :3].astype(np.float64).copy()
+    pc.frame_count = 0
+    plain = []
+    spp = 1
+    while spp <= 1024:
+        pc.spp = spp
+        t = time.perf_counter()
+        rd.render(gs, w.cam, pc, seed=31 * spp + 1)
+        dt = time.perf_counter() - t
+        e = relmse(rd.read(0)[..., :3], ref)
+        plain.append({"spp": spp, "ms": round(dt * 1e3, 2), "relmse": round(e, 5)})
+        if e <= target:
+            break
+        spp *= 2
+    den = Denoiser()
+    dn = []
+    spp = 1
+    while spp <= 64:
+        pc.spp = spp
+        t = time.perf_counter()
+        rd.render(gs, w.cam, pc, seed=37 * spp + 3)
+        den.apply(rd)
+        dt = time.perf_counter() - t
+        e = relmse(rd.read(0)[..., :3], ref)
+        dn.append({"spp": spp, "ms": round(dt * 1e3, 2), "relmse": round(e, 5)})
+        if e <= target:
+            break
+        spp *= 2
+    pc.spp = 16
+    hit_p = [p for p in plain if p["relmse"] <= target]
+    hit_d = [d for d in dn if d["relmse"] <= target]
+    res = {"target_relmse": target, "reference": "GPU FP32 tracer, %d spp, same frame" % ref_spp,
+           "plain": plain, "denoised": dn}
+    if hit_p and hit_d:
+        res["speedup"] = round(hit_p[0]["ms"] / hit_d[0]["ms"], 2)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +107,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--samples-per-batch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-relmse", action="store_true")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     args = ap.parse_args()
@@ -97,7 +152,8 @@ def main():
     nodes_per_ray = cst.node_visits / max(closest, 1)
     tris_per_ray = cst.tri_tests / max(closest, 1)
     bytes_per_ray = 32 + 20 + 64 * nodes_per_ray + 48 * tris_per_ray
-    launches = args.steps * (pc.path_depth + 1)
+    spb = args.samples_per_batch or max(1, min(pc.spp, (8 << 20) // (w.width * w.height)))
+    launches = args.steps * (pc.path_depth + 1) * ((pc.spp + spb - 1) // spb)
     achieved = bytes_per_ray * ext_rays / (ext_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "extend_traffic_config2.json")
@@ -120,6 +176,14 @@ def main():
                    "kernel_mrays_s": round(rays / (trace_ms * 1e-3) / 1e6, 2)},
         "roofline": roof,
     }
+    import ctypes
+    bw = ctypes.c_float()
+    check(lib().kf_bench_copy(1 << 30, 10, ctypes.byref(bw)))
+    roof["achievable_copy_gbs"] = round(bw.value, 1)
+    roof["frac_of_achievable"] = round(achieved / bw.value, 4)
+    if not args.no_relmse:
+        print("bench: time to relMSE", file=sys.stderr, flush=True)
+        out["time_to_relmse"] = time_to_relmse(w, gs, rd)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w)

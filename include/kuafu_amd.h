@@ -282,6 +282,12 @@ float* kf_renderer_device_target(kf_renderer* r, int target);
  * config.hpp:144, vkCore.hpp:3008-3111); W*H*4 bytes, order B,G,R,A. */
 int kf_renderer_download_bgra8(kf_renderer* r, uint8_t* out);
 
+/* Measurement utility (no reference counterpart): streaming copy of `bytes`
+ * between two HBM buffers, `iters` times; reports the achieved GB/s (read +
+ * write bytes / kernel time).  The achievable-bandwidth denominator of the
+ * traversal roofline. */
+int kf_bench_copy(uint64_t bytes, int iters, float* gbps);
+
 /* ------------------------------------------------------------------------ */
 /* Denoise stage (replaces DenoiserOptix, denoiser.cpp:24-344)               */
 /* ------------------------------------------------------------------------ */

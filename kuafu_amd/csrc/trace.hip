@@ -52,10 +52,60 @@ __global__ __launch_bounds__(TB) void k_any(SceneView S, const kf_ray* __restric
     if (ovf) atomicAdd(&cnt[2], 1ull);
 }
 
+// Streaming copy, four independent 16-byte loads in flight per thread.
+__global__ __launch_bounds__(TB) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
 }  // namespace kf
 
 using namespace kf;
+
+extern "C" int kf_bench_copy(uint64_t bytes, int iters, float* gbps) {
+    try {
+        if (!gbps || bytes < 16 || iters < 1) throw Error("kf_bench_copy: bad argument");
+        hipStream_t st = stream();
+        size_t n = bytes / 16;
+        uint4 *a = nullptr, *b = nullptr;
+        KF_CHECK(hipMalloc((void**)&a, n * 16));
+        KF_CHECK(hipMalloc((void**)&b, n * 16));
+        KF_CHECK(hipMemsetAsync(a, 1, n * 16, st));
+        int dev = 0, cus = 256;
+        KF_CHECK(hipGetDevice(&dev));
+        KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        unsigned grid = (unsigned)cus * 16;
+        k_copy<<<grid, TB, 0, st>>>(a, b, n);  // warm-up
+        hipEvent_t e0, e1;
+        KF_CHECK(hipEventCreate(&e0));
+        KF_CHECK(hipEventCreate(&e1));
+        KF_CHECK(hipEventRecord(e0, st));
+        for (int i = 0; i < iters; ++i) k_copy<<<grid, TB, 0, st>>>(a, b, n);
+        KF_CHECK(hipEventRecord(e1, st));
+        KF_CHECK(hipEventSynchronize(e1));
+        KF_CHECK(hipGetLastError());
+        float ms = 0.f;
+        KF_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        *gbps = (float)(2.0 * n * 16 * iters / (ms * 1e-3) / 1e9);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        KF_CHECK(hipFree(a));
+        KF_CHECK(hipFree(b));
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
 
 namespace {
 unsigned long long* trace_counters() {
