@@ -137,6 +137,7 @@ def main():
     ext_rays = 0
     ext_ms = 0.0
     trace_ms = 0.0
+    launches = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
         st = rd.render(gs, w.cam, pc, seed=100 + k, samples_per_batch=args.samples_per_batch)
@@ -144,6 +145,7 @@ def main():
         ext_rays += st.primary_rays + st.extension_rays
         ext_ms += st.extend_ms
         trace_ms += st.trace_ms
+        launches += st.extend_launches
     wall = time.perf_counter() - t0
     ms_per_step = wall / args.steps * 1e3
     value = rays / wall / 1e6
@@ -152,8 +154,6 @@ def main():
     nodes_per_ray = cst.node_visits / max(closest, 1)
     tris_per_ray = cst.tri_tests / max(closest, 1)
     bytes_per_ray = 32 + 20 + 64 * nodes_per_ray + 48 * tris_per_ray
-    spb = args.samples_per_batch or max(1, min(pc.spp, (8 << 20) // (w.width * w.height)))
-    launches = args.steps * (pc.path_depth + 1) * ((pc.spp + spb - 1) // spb)
     achieved = bytes_per_ray * ext_rays / (ext_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "extend_traffic_config2.json")

@@ -172,6 +172,8 @@ typedef struct kf_render_stats {
     float extend_ms;          /* closest-hit traversal kernels */
     float shadow_ms;          /* any-hit traversal kernels */
     float other_ms;
+    uint32_t extend_launches; /* closest-hit traversal kernel launches */
+    uint32_t shadow_launches; /* any-hit traversal kernel launches */
 } kf_render_stats;
 
 /* Render options that the reference takes from Config / push constants. */

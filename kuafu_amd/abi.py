@@ -80,7 +80,8 @@ class SceneDesc(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("trace_ms", C.c_float),
-                ("extend_ms", C.c_float), ("shadow_ms", C.c_float), ("other_ms", C.c_float)]
+                ("extend_ms", C.c_float), ("shadow_ms", C.c_float), ("other_ms", C.c_float),
+                ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32)]
 
     def rays(self):
         return int(self.primary_rays + self.extension_rays + self.shadow_rays)

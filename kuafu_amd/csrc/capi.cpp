@@ -7,6 +7,8 @@
 // here, on the host, before any kernel can read it.
 #include "kf_internal.h"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -22,6 +24,17 @@ std::mutex g_mu;
 }  // namespace
 
 void set_error(const std::string& m) { g_error = m; }
+
+uint32_t traversal_refill() {
+    // Idle lanes of a wave that make the persistent traversal kernels fetch
+    // new rays (1..64); KUAFU_AMD_REFILL overrides it for tuning.
+    static const uint32_t v = [] {
+        const char* e = getenv("KUAFU_AMD_REFILL");
+        long x = e ? strtol(e, nullptr, 10) : 0;
+        return (x >= 1 && x <= 64) ? (uint32_t)x : 16u;
+    }();
+    return v;
+}
 
 hipStream_t stream() {
     if (!g_stream) {
@@ -308,6 +321,9 @@ int kf_scene_create(const kf_scene_desc* d, kf_scene** out) {
         s->n_tri = s->h_toff.back();
         s->n_mat = d->num_materials;
         if (s->h_voff[0] != 0 || s->h_toff[0] != 0) throw Error("kf_scene_create: offsets must start at 0");
+        // traversal items carry a 30-bit record index
+        if (s->n_tri >= (1u << 29) || d->num_instances >= (1u << 29))
+            throw Error("kf_scene_create: at most 2^29 triangles and 2^29 instances");
         for (uint32_t g = 0; g < s->n_geom; ++g)
             if (s->h_voff[g + 1] < s->h_voff[g] || s->h_toff[g + 1] < s->h_toff[g])
                 throw Error("kf_scene_create: offsets must be non-decreasing");

@@ -23,6 +23,7 @@ struct Error : std::exception {
 
 void set_error(const std::string& m);
 hipStream_t stream();
+uint32_t traversal_refill();
 
 // tea (Random.glsl:7-21) on the host, for the frame seed.
 inline uint32_t tea_host(uint32_t v0, uint32_t v1) {

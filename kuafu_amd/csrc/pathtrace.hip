@@ -30,12 +30,16 @@ namespace kf {
 namespace {
 
 constexpr int TB = 256;
+static_assert(TB == (int)kRingThreads, "traversal kernels index their LDS stack ring by a 256-thread block");
 
-// Device counters of one frame: per bounce b: [0] active paths, [1] shadow
-// rays, [2] next queue.  Global: primary / extension / shadow / nodes / tris
-// / stack overflow at the end.
+// Device counters of one batch: per bounce b: [0] active paths, [1] shadow
+// rays, [2] next queue, [3..10] work chunks of k_extend, [11..18] work chunks
+// of k_shadow.  Global: primary / extension / shadow / nodes / tris / stack
+// overflow at the end.
 constexpr int kMaxBounces = 64;
-constexpr int kCntPerBounce = 3;
+constexpr int kCntPerBounce = 19;
+constexpr int kCntExtendChunks = 3;
+constexpr int kCntShadowChunks = 11;
 constexpr int kCntGlobal = kMaxBounces * kCntPerBounce;
 constexpr int kCntSize = kCntGlobal + 8;
 
@@ -139,23 +143,30 @@ template <bool COUNT>
 __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t cap, const uint32_t* __restrict__ queue,
                                                const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
                                                const uint32_t* __restrict__ seed, float4* __restrict__ hit,
-                                               uint32_t* __restrict__ hit_prim, unsigned long long* __restrict__ cnt) {
-    uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)cap);
-    uint32_t nodes = 0, tris = 0, ovf = 0;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        uint32_t p = queue[q];
-        float4 o = ray_o[p], d = ray_d[p];
-        THit h;
-        TCount c{0, 0};
-        traverse<false, COUNT>(S, xyz(o), xyz(d), 0.001f, 10000.0f, true, seed[p], h, c, ovf);
-        hit[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.inst));
-        hit_prim[p] = h.prim | (h.back << 31);
-        nodes += c.nodes;
-        tris += c.tris;
-    }
+                                               uint32_t* __restrict__ hit_prim, unsigned long long* __restrict__ cnt,
+                                               uint32_t refill) {
+    const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)cap);
+    uint32_t ovf = 0;
+    TCount tot{0, 0};
+    auto load = [&](uint32_t q, TRay& r) __attribute__((always_inline)) {
+        const uint32_t p = queue[q];
+        const float4 o = ray_o[p], d = ray_d[p];
+        r.o = xyz(o);
+        r.d = xyz(d);
+        r.tmin = 0.001f;
+        r.tmax = 10000.0f;
+        r.alpha_seed = seed[p];
+        r.payload = p;
+    };
+    auto store = [&](uint32_t p, THit b, bool) __attribute__((always_inline)) {
+        hit[p] = make_float4(b.t, b.b1, b.b2, __uint_as_float(b.inst));
+        hit_prim[p] = b.prim | (b.back << 31);
+    };
+    trace_persistent<false, COUNT>(S, n, cnt + bounce * kCntPerBounce + kCntExtendChunks, refill, true, load, store, ovf,
+                                   tot);
     if (COUNT) {
-        wave_add(&cnt[kCntGlobal + 3], nodes);
-        wave_add(&cnt[kCntGlobal + 4], tris);
+        wave_add(&cnt[kCntGlobal + 3], tot.nodes);
+        wave_add(&cnt[kCntGlobal + 4], tot.tris);
     }
     if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
@@ -422,16 +433,24 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
 
 __global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t cap, const float4* __restrict__ sh_o,
                                                const float4* __restrict__ sh_d, uint8_t* __restrict__ vis,
-                                               unsigned long long* __restrict__ cnt) {
-    uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 1], (unsigned long long)cap);
+                                               unsigned long long* __restrict__ cnt, uint32_t refill) {
+    const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 1], (unsigned long long)cap);
     uint32_t ovf = 0;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        float4 o = sh_o[q], d = sh_d[q];
-        THit h;
-        TCount c{0, 0};
-        bool occ = traverse<true, false>(S, xyz(o), xyz(d), 0.001f, d.w, false, 0u, h, c, ovf);
-        if (!occ) vis[__float_as_uint(o.w)] = 1;
-    }
+    TCount tot{0, 0};
+    auto load = [&](uint32_t q, TRay& r) __attribute__((always_inline)) {
+        const float4 o = sh_o[q], d = sh_d[q];
+        r.o = xyz(o);
+        r.d = xyz(d);
+        r.tmin = 0.001f;
+        r.tmax = d.w;
+        r.alpha_seed = 0u;
+        r.payload = __float_as_uint(o.w);
+    };
+    auto store = [&](uint32_t slot, THit, bool found) __attribute__((always_inline)) {
+        if (!found) vis[slot] = 1;
+    };
+    trace_persistent<true, false>(S, n, cnt + bounce * kCntPerBounce + kCntShadowChunks, refill, false, load, store, ovf,
+                                  tot);
     if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
 
@@ -683,8 +702,11 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
         uint32_t WH = r->W * r->H;
         uint32_t spb = o.samples_per_batch ? o.samples_per_batch : 0;
         if (spb == 0) {
-            // default: about 8M paths per batch
-            spb = std::max<uint32_t>(1, std::min<uint32_t>(pc->spp, (8u << 20) / std::max<uint32_t>(WH, 1)));
+            // default: up to 32M paths per batch (16 spp at 1920x1080).  Each
+            // bounce's traversal launch then holds tens of rays per lane, so
+            // the tail of the slowest rays is a small part of it; the state
+            // costs about 200 B per path (6.6 GB at 32M paths).
+            spb = std::max<uint32_t>(1, std::min<uint32_t>(pc->spp, (32u << 20) / std::max<uint32_t>(WH, 1)));
         }
         spb = std::min(spb, pc->spp);
         uint32_t K = inline_nee ? 1u : (uint32_t)std::max(L.n_slots, 1);
@@ -743,8 +765,10 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
         int n_cu = 256;
         KF_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
         const uint32_t grid = (uint32_t)n_cu * 8;
+        const uint32_t refill = traversal_refill();
 
         unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t n_ext = 0, n_sh = 0;
         float ms_total = 0.f, ms_ext = 0.f, ms_sh = 0.f;
         for (uint32_t s0 = 0; s0 < pc->spp; s0 += spb) {
             F.s0 = s0;
@@ -761,11 +785,12 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
                 if (o.count_traversal)
                     k_extend<true><<<grid, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
-                                                         r->hit_prim, r->counters);
+                                                         r->hit_prim, r->counters, refill);
                 else
                     k_extend<false><<<grid, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
-                                                          r->hit_prim, r->counters);
+                                                          r->hit_prim, r->counters, refill);
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 1], st));
+                ++n_ext;
                 if (inline_nee)
                     k_shade<true><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->hit,
                                                        r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis,
@@ -776,8 +801,10 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
                                                         r->nee_b, r->vis, r->sh_o, r->sh_d, r->d_albedo, r->d_normal,
                                                         r->counters);
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], st));
-                if (!inline_nee)
-                    k_shadow<<<grid, TB, 0, st>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->vis, r->counters);
+                if (!inline_nee) {
+                    k_shadow<<<grid, TB, 0, st>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->vis, r->counters, refill);
+                    ++n_sh;
+                }
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], st));
                 if (inline_nee)
                     k_connect<true><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
@@ -819,6 +846,8 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
             stats->extend_ms = ms_ext;
             stats->shadow_ms = ms_sh;
             stats->other_ms = ms_total - ms_ext - ms_sh;
+            stats->extend_launches = n_ext;
+            stats->shadow_launches = n_sh;
         }
         return 0;
     } catch (const std::exception& e) {

@@ -10,46 +10,67 @@ namespace kf {
 namespace {
 
 constexpr int TB = 256;
+static_assert(TB == (int)kRingThreads, "traversal kernels index their LDS stack ring by a 256-thread block");
+
+// Counters of a trace call: [0] node records, [1] triangle tests, [2] stack
+// overflow / malformed tree, [4..11] work-chunk counters of the persistent grid.
+constexpr int kTraceCnt = 12;
+
+KF_DEV void ray_of(const kf_ray& r, uint32_t q, TRay& t) {
+    t.o = mk(r.org[0], r.org[1], r.org[2]);
+    t.d = mk(r.dir[0], r.dir[1], r.dir[2]);
+    t.tmin = r.tmin;
+    t.tmax = r.tmax;
+    t.alpha_seed = 0u;
+    t.payload = q;
+}
 
 template <bool COUNT>
 __global__ __launch_bounds__(TB) void k_closest(SceneView S, const kf_ray* __restrict__ rays, uint32_t n,
-                                                kf_hit* __restrict__ hits, unsigned long long* __restrict__ cnt) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                kf_hit* __restrict__ hits, unsigned long long* __restrict__ cnt,
+                                                uint32_t refill) {
     uint32_t ovf = 0;
-    if (i < n) {
-        kf_ray r = rays[i];
-        THit h;
-        TCount c{0, 0};
-        traverse<false, COUNT>(S, mk(r.org[0], r.org[1], r.org[2]), mk(r.dir[0], r.dir[1], r.dir[2]), r.tmin, r.tmax,
-                               false, 0u, h, c, ovf);
+    TCount tot{0, 0};
+    auto load = [&](uint32_t q, TRay& t) __attribute__((always_inline)) { ray_of(rays[q], q, t); };
+    auto store = [&](uint32_t q, THit b, bool) __attribute__((always_inline)) {
         kf_hit o;
-        o.t = h.t;
-        o.inst = h.inst;
-        o.prim = h.prim;
-        o.b1 = h.b1;
-        o.b2 = h.b2;
-        o.back = h.back;
-        hits[i] = o;
-        if (COUNT) {
-            atomicAdd(&cnt[0], (unsigned long long)c.nodes);
-            atomicAdd(&cnt[1], (unsigned long long)c.tris);
-        }
+        o.t = b.t;
+        o.inst = b.inst;
+        o.prim = b.prim;
+        o.b1 = b.b1;
+        o.b2 = b.b2;
+        o.back = b.back;
+        hits[q] = o;
+    };
+    trace_persistent<false, COUNT>(S, n, cnt + 4, refill, false, load, store, ovf, tot);
+    if (COUNT) {
+        atomicAdd(&cnt[0], (unsigned long long)tot.nodes);
+        atomicAdd(&cnt[1], (unsigned long long)tot.tris);
     }
     if (ovf) atomicAdd(&cnt[2], 1ull);
 }
 
 __global__ __launch_bounds__(TB) void k_any(SceneView S, const kf_ray* __restrict__ rays, uint32_t n,
-                                            uint8_t* __restrict__ occ, unsigned long long* __restrict__ cnt) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                            uint8_t* __restrict__ occ, unsigned long long* __restrict__ cnt,
+                                            uint32_t refill) {
     uint32_t ovf = 0;
-    if (i < n) {
-        kf_ray r = rays[i];
-        THit h;
-        TCount c{0, 0};
-        occ[i] = traverse<true, false>(S, mk(r.org[0], r.org[1], r.org[2]), mk(r.dir[0], r.dir[1], r.dir[2]), r.tmin,
-                                       r.tmax, false, 0u, h, c, ovf) ? 1 : 0;
-    }
+    TCount tot{0, 0};
+    auto load = [&](uint32_t q, TRay& t) __attribute__((always_inline)) { ray_of(rays[q], q, t); };
+    auto store = [&](uint32_t q, THit, bool found) __attribute__((always_inline)) { occ[q] = found ? 1 : 0; };
+    trace_persistent<true, false>(S, n, cnt + 4, refill, false, load, store, ovf, tot);
     if (ovf) atomicAdd(&cnt[2], 1ull);
+}
+
+// Persistent grid: enough resident waves to fill every CU.
+unsigned trace_grid() {
+    static unsigned g = 0;
+    if (!g) {
+        int dev = 0, cus = 256;
+        KF_CHECK(hipGetDevice(&dev));
+        KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        g = (unsigned)cus * 8u;
+    }
+    return g;
 }
 
 // Streaming copy, four independent 16-byte loads in flight per thread.
@@ -110,7 +131,7 @@ extern "C" int kf_bench_copy(uint64_t bytes, int iters, float* gbps) {
 namespace {
 unsigned long long* trace_counters() {
     static unsigned long long* c = nullptr;
-    if (!c) KF_CHECK(hipMalloc((void**)&c, 4 * sizeof(unsigned long long)));
+    if (!c) KF_CHECK(hipMalloc((void**)&c, kTraceCnt * sizeof(unsigned long long)));
     return c;
 }
 }  // namespace
@@ -125,11 +146,11 @@ extern "C" int kf_trace_closest(kf_scene* s, const kf_ray* rays, uint32_t n, kf_
         KF_CHECK(hipMalloc((void**)&d_r, n * sizeof(kf_ray)));
         KF_CHECK(hipMalloc((void**)&d_h, n * sizeof(kf_hit)));
         unsigned long long* c = trace_counters();
-        KF_CHECK(hipMemsetAsync(c, 0, 4 * sizeof(unsigned long long), st));
+        KF_CHECK(hipMemsetAsync(c, 0, kTraceCnt * sizeof(unsigned long long), st));
         KF_CHECK(hipMemcpyAsync(d_r, rays, n * sizeof(kf_ray), hipMemcpyHostToDevice, st));
-        k_closest<false><<<(n + TB - 1) / TB, TB, 0, st>>>(s->view(), d_r, n, d_h, c);
+        k_closest<false><<<trace_grid(), TB, 0, st>>>(s->view(), d_r, n, d_h, c, traversal_refill());
         KF_CHECK(hipGetLastError());
-        unsigned long long h[4];
+        unsigned long long h[kTraceCnt];
         KF_CHECK(hipMemcpyAsync(hits, d_h, n * sizeof(kf_hit), hipMemcpyDeviceToHost, st));
         KF_CHECK(hipMemcpyAsync(h, c, sizeof(h), hipMemcpyDeviceToHost, st));
         KF_CHECK(hipStreamSynchronize(st));
@@ -153,11 +174,11 @@ extern "C" int kf_trace_any(kf_scene* s, const kf_ray* rays, uint32_t n, uint8_t
         KF_CHECK(hipMalloc((void**)&d_r, n * sizeof(kf_ray)));
         KF_CHECK(hipMalloc((void**)&d_o, n));
         unsigned long long* c = trace_counters();
-        KF_CHECK(hipMemsetAsync(c, 0, 4 * sizeof(unsigned long long), st));
+        KF_CHECK(hipMemsetAsync(c, 0, kTraceCnt * sizeof(unsigned long long), st));
         KF_CHECK(hipMemcpyAsync(d_r, rays, n * sizeof(kf_ray), hipMemcpyHostToDevice, st));
-        k_any<<<(n + TB - 1) / TB, TB, 0, st>>>(s->view(), d_r, n, d_o, c);
+        k_any<<<trace_grid(), TB, 0, st>>>(s->view(), d_r, n, d_o, c, traversal_refill());
         KF_CHECK(hipGetLastError());
-        unsigned long long h[4];
+        unsigned long long h[kTraceCnt];
         KF_CHECK(hipMemcpyAsync(occluded, d_o, n, hipMemcpyDeviceToHost, st));
         KF_CHECK(hipMemcpyAsync(h, c, sizeof(h), hipMemcpyDeviceToHost, st));
         KF_CHECK(hipStreamSynchronize(st));
@@ -178,12 +199,12 @@ extern "C" int kf_trace_closest_device(kf_scene* s, const kf_ray* d_rays, uint32
         if (n == 0) return 0;
         hipStream_t st = stream();
         unsigned long long* c = trace_counters();
-        KF_CHECK(hipMemsetAsync(c, 0, 4 * sizeof(unsigned long long), st));
+        KF_CHECK(hipMemsetAsync(c, 0, kTraceCnt * sizeof(unsigned long long), st));
         hipEvent_t e0, e1;
         KF_CHECK(hipEventCreate(&e0));
         KF_CHECK(hipEventCreate(&e1));
         KF_CHECK(hipEventRecord(e0, st));
-        k_closest<false><<<(n + TB - 1) / TB, TB, 0, st>>>(s->view(), d_rays, n, d_hits, c);
+        k_closest<false><<<trace_grid(), TB, 0, st>>>(s->view(), d_rays, n, d_hits, c, traversal_refill());
         KF_CHECK(hipEventRecord(e1, st));
         KF_CHECK(hipGetLastError());
         KF_CHECK(hipEventSynchronize(e1));

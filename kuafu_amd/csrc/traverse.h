@@ -20,38 +20,44 @@ namespace kf {
 
 constexpr int kStack = 64;
 constexpr uint32_t kMaxSteps = 1u << 22;  // bound on loop iterations (malformed-tree guard)
-constexpr int kPopRef = (int)0x80000000;      // "pop the stack"
-constexpr int kExitBlasRef = (int)0x80000001; // marker: leave the current instance
-constexpr float kGamma3 = 3.0f * 5.96046448e-8f / (1.0f - 3.0f * 5.96046448e-8f);
+// Slab margin: each computed slab distance (b - o) * rcp(d) carries three
+// roundings, one of them the 1-ulp hardware reciprocal (2 units of 2^-24),
+// so |relative error| <= gamma_4; near <= far * (1 + 2 gamma_5) then never
+// rejects a box the exact ray meets (Ize, "Robust BVH ray traversal", 2013).
+constexpr float kUnit = 5.96046448e-8f;  // 2^-24
+constexpr float kGamma5 = 5.0f * kUnit / (1.0f - 5.0f * kUnit);
+constexpr float kSlabScale = 1.0f + 2.0f * kGamma5;
 
 struct Pre {
-    v3 o, d, invd;
+    v3 o, invd;  // invd: culling only (hardware reciprocal)
     int kx, ky, kz;
     float Sx, Sy, Sz;
 };
 
 KF_DEV float comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+// Component-wise choice.  (c ? a : b on structure lvalues compiles to a copy
+// from a chosen address, which forces the traversal state into memory.)
+KF_DEV v3 sel3(bool c, v3 a, v3 b) { return mk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
+KF_DEV v3 rcp3(v3 d) { return mk(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z)); }
 
+// Shear constants of the watertight test: kz = dominant axis, (kx, ky)
+// swapped when d[kz] < 0 to keep the winding; Sz = 1 / d[kz] correctly
+// rounded, Sx = d[kx] * Sz, Sy = d[ky] * Sz.
 KF_DEV void precompute(v3 o, v3 d, Pre& p) {
     p.o = o;
-    p.d = d;
-    p.invd = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    p.invd = rcp3(d);
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;
     int ky = kx == 2 ? 0 : kx + 1;
-    float dz = comp(d, kz);
-    if (dz < 0.0f) {
-        int t = kx;
-        kx = ky;
-        ky = t;
-    }
-    p.kx = kx;
-    p.ky = ky;
+    const float dz = comp(d, kz);
+    const bool swap = dz < 0.0f;
+    p.kx = swap ? ky : kx;
+    p.ky = swap ? kx : ky;
     p.kz = kz;
-    p.Sx = comp(d, kx) / dz;
-    p.Sy = comp(d, ky) / dz;
     p.Sz = 1.0f / dz;
+    p.Sx = comp(d, p.kx) * p.Sz;
+    p.Sy = comp(d, p.ky) * p.Sz;
 }
 
 struct TriHit {
@@ -59,7 +65,10 @@ struct TriHit {
     uint32_t back;
 };
 
-KF_DEV bool intersect(const Pre& p, float4 v0, float4 v1, float4 v2, float tmin, float tmax, TriHit& h) {
+// Watertight ray/triangle test without early exits (the caller runs it for
+// every lane of a wave).  en gates the double-precision fallback so that
+// lanes holding other records never take it.
+KF_DEV bool intersect(const Pre& p, float4 v0, float4 v1, float4 v2, float tmin, float tmax, bool en, TriHit& h) {
     v3 A = mk(v0.x, v0.y, v0.z) - p.o;
     v3 B = mk(v1.x, v1.y, v1.z) - p.o;
     v3 C = mk(v2.x, v2.y, v2.z) - p.o;
@@ -73,31 +82,28 @@ KF_DEV bool intersect(const Pre& p, float4 v0, float4 v1, float4 v2, float tmin,
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
-    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    if (en && (U == 0.0f || V == 0.0f || W == 0.0f)) {
         U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
         V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
         W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
     }
-    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
-    float det = U + V + W;
-    if (det == 0.0f) return false;
-    float Az = p.Sz * Akz;
-    float Bz = p.Sz * Bkz;
-    float Cz = p.Sz * Ckz;
-    float Tn = U * Az + V * Bz + W * Cz;
-    float rcp = 1.0f / det;
-    float t = Tn * rcp;
-    if (!(t >= tmin && t <= tmax)) return false;
+    const bool outside = (U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f);
+    const float det = U + V + W;
+    const float Az = p.Sz * Akz;
+    const float Bz = p.Sz * Bkz;
+    const float Cz = p.Sz * Ckz;
+    const float Tn = U * Az + V * Bz + W * Cz;
+    const float rcp = 1.0f / det;
+    const float t = Tn * rcp;
     h.t = t;
     h.b1 = V * rcp;
     h.b2 = W * rcp;
     h.back = det < 0.0f ? 1u : 0u;
-    return true;
+    return !outside && det != 0.0f && t >= tmin && t <= tmax;
 }
 
-// Conservative slab test of the two child boxes of a node (Ize 2013).
+// Conservative slab test of the two child boxes of a node.
 KF_DEV void slab2(const Node& n, v3 o, v3 invd, float tmin, float tmax, bool& hl, bool& hr, float& tl, float& tr) {
-    const float s = 1.0f + 2.0f * kGamma3;
     float l0x = (n.a.x - o.x) * invd.x, l1x = (n.a.y - o.x) * invd.x;
     float l0y = (n.a.z - o.y) * invd.y, l1y = (n.a.w - o.y) * invd.y;
     float l0z = (n.c.x - o.z) * invd.z, l1z = (n.c.y - o.z) * invd.z;
@@ -108,8 +114,8 @@ KF_DEV void slab2(const Node& n, v3 o, v3 invd, float tmin, float tmax, bool& hl
     float lf = fminf(fminf(fmaxf(l0x, l1x), fmaxf(l0y, l1y)), fminf(fmaxf(l0z, l1z), tmax));
     float rn = fmaxf(fmaxf(fminf(r0x, r1x), fminf(r0y, r1y)), fmaxf(fminf(r0z, r1z), tmin));
     float rf = fminf(fminf(fmaxf(r0x, r1x), fmaxf(r0y, r1y)), fminf(fmaxf(r0z, r1z), tmax));
-    hl = ln <= lf * s;
-    hr = rn <= rf * s;
+    hl = ln <= lf * kSlabScale;
+    hr = rn <= rf * kSlabScale;
     tl = ln;
     tr = rn;
 }
@@ -139,155 +145,424 @@ KF_DEV v3 xform_dir(const Inst& I, v3 v) {
               I.w2o[2].x * v.x + I.w2o[2].y * v.y + I.w2o[2].z * v.z);
 }
 
+// ---------------------------------------------------------------- traversal
+// The traversal is a state machine over 32-bit items: bits 31:30 the kind,
+// bits 29:0 an index into that kind's array.  Every step fetches exactly one
+// record (64 B for nodes and instances, 48 B for triangles) and processes it,
+// so all lanes of a wave wait on one memory round trip per step whatever kind
+// of record each lane is on.  The stack lives in private memory with its top
+// cached in a register: a pop takes the register and refills it from memory,
+// and that load overlaps the next record fetch.
+enum : uint32_t { kTlasNode = 0u, kBlasNode = 1u, kInstance = 2u, kTriangle = 3u };
+constexpr uint32_t kIdxMask = 0x3fffffffu;  // scenes are limited to 2^30 records of each kind
+
+KF_DEV uint32_t make_item(uint32_t kind, uint32_t idx) { return (kind << 30) | (idx & kIdxMask); }
+// A child ref of a node of kind k (0 = TLAS, 1 = BLAS): >= 0 is a node of the
+// same kind, < 0 is ~leaf (instance for the TLAS, triangle slot for a BLAS).
+KF_DEV uint32_t child_item(uint32_t k, int ref) { return ref >= 0 ? make_item(k, (uint32_t)ref) : make_item(k + 2u, (uint32_t)~ref); }
+
+// Record arrays of the traversal.  Where a lane chooses between them by the
+// kind of its record, each field passes through an empty asm first: that
+// keeps the choice a register select, which the optimiser otherwise turns
+// into a dynamically indexed private-memory copy of the whole structure.
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) vf4 gf4;  // vf4 in the global address space
+
+template <class T>
+KF_DEV T opaque_sgpr(T v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
+__device__ const float4 kZeroRecord[4] = {};
+
+struct TravScene {
+    const float4 *tlas, *blas, *inst, *tri;
+    uint32_t n_tlas, n_blas, n_inst, n_tri;
+    int root;
+    const float4* zero;  // 64 zero bytes, read in place of an out-of-range record
+    const kf_material* mats;
+    const uint32_t *matidx, *toff;
+    KF_DEV explicit TravScene(const SceneView& S)
+        : tlas((const float4*)S.tlas), blas((const float4*)S.blas), inst((const float4*)S.inst), tri(S.tri),
+          n_tlas(S.n_tlas_nodes), n_blas(S.n_blas_nodes), n_inst(S.n_inst), n_tri(S.n_tri), root(S.tlas_root),
+          zero(kZeroRecord), mats(S.mats), matidx(S.matidx), toff(S.toff) {}
+};
+
+// ------------------------------------------------------------- stacks
+// A lane's traversal stack below its register-cached top.  Entry i is the
+// i-th oldest; sp counts the entries.
+//   peek(sp): entry sp - 1 (read speculatively every step; any in-range
+//             address when sp == 0);  push(sp, v): store v as entry sp;
+//   popped(sp): entry sp has just become the register top (sp is the new
+//             count).
+struct PrivStack {  // everything in private memory
+    static constexpr int kCap = kStack;
+    uint32_t a[kStack];
+    KF_DEV uint32_t peek(int sp) const { return a[sp > 0 ? sp - 1 : 0]; }
+    KF_DEV void push(int sp, uint32_t v, uint32_t& overflow) {
+        if (sp < kStack) a[sp] = v;
+        else overflow = 1;
+    }
+    KF_DEV void popped(int) {}
+};
+
+// The newest kRing entries in an LDS ring, one column per thread of the
+// block (slot-major, so a wave's accesses hit 64 distinct banks whatever
+// slots its lanes use); older entries spill to private memory.  Private
+// stores dirty whole cache lines that end up written back to HBM, so the
+// ring keeps the common shallow stack off the memory system entirely.
+constexpr int kRing = 16;                 // power of two
+constexpr uint32_t kRingThreads = 256;    // block size of kernels using RingStack
+struct RingStack {
+    static constexpr int kCap = kRing + kStack;
+    uint32_t* ring;  // this thread's column: slot k at ring[k * kRingThreads]
+    uint32_t spill[kStack];
+    KF_DEV uint32_t& slot(int i) const { return ring[(uint32_t)(i & (kRing - 1)) * kRingThreads]; }
+    KF_DEV uint32_t peek(int sp) const { return slot(sp - 1); }
+    KF_DEV void push(int sp, uint32_t v, uint32_t& overflow) {
+        if (sp >= kCap) {
+            overflow = 1;
+            return;
+        }
+        if (sp >= kRing) spill[sp - kRing] = slot(sp);  // evict the entry that shares the slot
+        slot(sp) = v;
+    }
+    KF_DEV void popped(int sp) {
+        if (sp >= kRing) slot(sp) = spill[sp - kRing];  // bring the next older entry back into the ring
+    }
+};
+
+// One query: ray, t range, alpha coin seed and an opaque payload for the
+// caller (where to write the result).
+struct TRay {
+    v3 o, d;
+    float tmin, tmax;
+    uint32_t alpha_seed, payload;
+};
+
+KF_DEV uint32_t root_item(int root) {
+    return root >= 0 ? make_item(kTlasNode, (uint32_t)root) : make_item(kInstance, (uint32_t)~root);
+}
+
 // ANY: shadow-ray query (opaque, stop at the first hit in [tmin, tmax]).
-// ALPHA: run the any-hit alpha test (PathTrace.rahit:30-48) for non-opaque
-// instances, with an order-independent coin (alpha_coin).
+// Closest-hit queries with use_alpha run the any-hit alpha test
+// (PathTrace.rahit:30-48) for non-opaque instances, with an order-independent
+// coin (alpha_coin).
+template <bool ANY, bool COUNT>
+struct Trav {
+    v3 o, d, winvd;  // world-space ray
+    Pre obj;         // object-space ray of the current instance
+    float tmin;
+    THit best;
+    uint32_t cur_inst, cur_flags, cur_geom;
+    uint32_t cur;    // item processed by the next step
+    uint32_t top;    // stack top, cached in a register (valid when has_top)
+    int sp;          // entries in the private array below the cached top
+    bool has_top, found, use_alpha;
+    uint32_t alpha_seed, steps;
+    TCount cnt;
+
+    // Benign state for a lane without a query (it then reads only the zero
+    // record until start() gives it one).
+    KF_DEV void reset() {
+        o = d = winvd = mk(0.0f, 0.0f, 0.0f);
+        obj.o = obj.invd = o;
+        obj.kx = 0;
+        obj.ky = 1;
+        obj.kz = 2;
+        obj.Sx = obj.Sy = obj.Sz = 0.0f;
+        tmin = 0.0f;
+        best.t = best.b1 = best.b2 = 0.0f;
+        best.inst = best.prim = ~0u;
+        best.back = 0u;
+        cur_inst = cur_flags = cur_geom = 0u;
+        cur = top = 0u;
+        sp = 0;
+        has_top = found = false;
+        use_alpha = false;
+        alpha_seed = steps = 0u;
+        cnt.nodes = cnt.tris = 0u;
+    }
+
+    // Lanes with f take a new query r starting at item root (the scene is
+    // not empty); the others keep their state.  Written as selects so that
+    // refilling some lanes does not copy the state of all of them.
+    KF_DEV void start(bool f, const TRay& r, uint32_t root, bool use_alpha_) {
+        o = sel3(f, r.o, o);
+        d = sel3(f, r.d, d);
+        winvd = sel3(f, rcp3(r.d), winvd);  // culling only
+        tmin = f ? r.tmin : tmin;
+        best.t = f ? r.tmax : best.t;
+        best.inst = f ? ~0u : best.inst;
+        best.prim = f ? ~0u : best.prim;
+        best.b1 = f ? 0.0f : best.b1;
+        best.b2 = f ? 0.0f : best.b2;
+        best.back = f ? 0u : best.back;
+        cur = f ? root : cur;
+        sp = f ? 0 : sp;
+        has_top = f ? false : has_top;
+        found = f ? false : found;
+        use_alpha = use_alpha_;
+        alpha_seed = f ? r.alpha_seed : alpha_seed;
+        steps = f ? 0u : steps;
+        cnt.nodes = f ? 0u : cnt.nodes;
+        cnt.tris = f ? 0u : cnt.tris;
+    }
+
+    // One record; returns false when the query has finished.  stk: kStack
+    // private entries owned by this query (a separate array so that the
+    // scalar state above stays in registers).
+    //
+    // The entry below the cached top is loaded speculatively together with
+    // the record, so a pop needs no memory access of its own: every step
+    // costs the wave one memory round trip.  Each branch only decides an
+    // action; the stack is updated once at the end.
+    // live: the lane holds a query.  Lanes without one read the zero record
+    // and have no side effects beyond their own (discarded) state.
+    template <class Stack>
+    KF_DEV bool step(const TravScene& S, Stack& stk, uint32_t& overflow, bool live) {
+        enum { kNext, kPush, kPop, kDone };
+        const uint32_t kind = cur >> 30, idx = cur & kIdxMask;
+        // All operands of the per-lane choices are evaluated unconditionally,
+        // so the choices stay selects (an asm cannot be speculated).
+        const uint32_t n_tlas = opaque_sgpr(S.n_tlas), n_blas = opaque_sgpr(S.n_blas);
+        const uint32_t n_inst = opaque_sgpr(S.n_inst), n_tri = opaque_sgpr(S.n_tri);
+        const float4 *p_tlas = opaque_sgpr(S.tlas), *p_blas = opaque_sgpr(S.blas);
+        const float4 *p_inst = opaque_sgpr(S.inst), *p_tri = opaque_sgpr(S.tri), *p_zero = opaque_sgpr(S.zero);
+        const uint32_t lim = kind == kTlasNode ? n_tlas : kind == kBlasNode ? n_blas : kind == kInstance ? n_inst : n_tri;
+        // Out-of-range record or step budget spent (malformed tree): read the
+        // zero record instead and finish the query with the overflow flag.
+        // Kept free of branches: an early exit here makes the compiler copy
+        // the whole traversal state on every step.
+        const bool bad = live && (idx >= lim || ++steps > kMaxSteps);
+        const bool on = live && !bad;
+        const float4* base = kind == kTlasNode ? p_tlas : kind == kBlasNode ? p_blas : kind == kInstance ? p_inst : p_tri;
+        const float4* P = on ? base + (kind == kTriangle ? 3u : 4u) * (size_t)idx : p_zero;
+        // Four 16-byte global loads issued together; triangles are 48 B, so
+        // their fourth load repeats the third (same cache line).  The empty
+        // asm needs all results, which keeps the compiler from sinking parts
+        // of the fetch into the kind-specific branches below (each sunk load
+        // would cost the wave another memory round trip).
+        const gf4* G = (const gf4*)P;
+        const vf4 r0 = G[0], r1 = G[1], r2 = G[2], r3 = G[(kind == kTriangle && on) ? 2 : 3];
+        const uint32_t below = stk.peek(sp);
+        asm volatile("" ::"v"(r0), "v"(r1), "v"(r2), "v"(r3), "v"(below));
+        const float4 q0 = make_float4(r0.x, r0.y, r0.z, r0.w), q1 = make_float4(r1.x, r1.y, r1.z, r1.w);
+        const float4 q2 = make_float4(r2.x, r2.y, r2.z, r2.w), q3 = make_float4(r3.x, r3.y, r3.z, r3.w);
+
+        // Every lane evaluates the record as a node, an instance and a
+        // triangle and keeps the result of its kind: in a wave64 nearly every
+        // step has lanes of all three kinds, so branching would run all three
+        // anyway and would copy the traversal state at each join.
+        const bool is_node = kind <= kBlasNode && on;
+        const bool is_inst = kind == kInstance && on;
+        const bool is_tri = kind == kTriangle && on;
+
+        // triangle, against the current instance's object-space ray
+        TriHit h;
+        const bool tri_hit = intersect(obj, q0, q1, q2, tmin, best.t, is_tri, h) && is_tri;
+        const uint32_t prim = __float_as_uint(q0.w);
+        bool take = tri_hit;
+        if (!ANY) {
+            take = take && (h.t < best.t || cur_inst < best.inst || (cur_inst == best.inst && prim < best.prim));
+            if (take && use_alpha && !(cur_flags & kInstOpaque)) {
+                const float a = S.mats[S.matidx[S.toff[cur_geom] + prim]].alpha;
+                take = !(a == 0.0f || alpha_coin(alpha_seed, cur_inst, prim) > a);
+            }
+        }
+        best.t = take ? h.t : best.t;
+        if (!ANY) {
+            best.b1 = take ? h.b1 : best.b1;
+            best.b2 = take ? h.b2 : best.b2;
+            best.back = take ? h.back : best.back;
+            best.inst = take ? cur_inst : best.inst;
+            best.prim = take ? prim : best.prim;
+        }
+        found = found || take;
+
+        // node: two child boxes
+        Node n;
+        n.a = q0;
+        n.b = q1;
+        n.c = q2;
+        n.d = make_int4(__float_as_int(q3.x), __float_as_int(q3.y), 0, 0);
+        const bool blas = kind == kBlasNode;
+        bool hl, hr;
+        float tl, tr;
+        slab2(n, sel3(blas, obj.o, o), sel3(blas, obj.invd, winvd), tmin, best.t, hl, hr, tl, tr);
+        const uint32_t li = child_item(kind & 1u, n.d.x), rj = child_item(kind & 1u, n.d.y);
+        // Nearer child first; any-hit queries take a leaf child (triangle or
+        // instance) before an inner one, which finds occluders sooner.
+        bool lfirst = !hr || (hl && tl <= tr);
+        if (ANY && hl && hr && (n.d.x < 0) != (n.d.y < 0)) lfirst = n.d.x < 0;
+
+        // instance: enter its BLAS with the object-space ray
+        const int root = __float_as_int(q3.x);
+        const uint32_t iflags = __float_as_uint(q3.z);
+        const bool enter = is_inst && (iflags & kInstValid) && root != kEmptyRef;
+        Inst I;
+        I.w2o[0] = q0;
+        I.w2o[1] = q1;
+        I.w2o[2] = q2;
+        Pre nobj;
+        precompute(xform_point(I, o), xform_dir(I, d), nobj);
+        obj.o = sel3(enter, nobj.o, obj.o);
+        obj.invd = sel3(enter, nobj.invd, obj.invd);
+        obj.kx = enter ? nobj.kx : obj.kx;
+        obj.ky = enter ? nobj.ky : obj.ky;
+        obj.kz = enter ? nobj.kz : obj.kz;
+        obj.Sx = enter ? nobj.Sx : obj.Sx;
+        obj.Sy = enter ? nobj.Sy : obj.Sy;
+        obj.Sz = enter ? nobj.Sz : obj.Sz;
+        cur_inst = enter ? idx : cur_inst;
+        cur_flags = enter ? iflags : cur_flags;
+        cur_geom = enter ? __float_as_uint(q3.y) : cur_geom;
+        const uint32_t inst_root = root >= 0 ? make_item(kBlasNode, (uint32_t)root) : make_item(kTriangle, (uint32_t)~root);
+
+        if (COUNT) {
+            cnt.nodes += (is_node || is_inst) ? 1u : 0u;
+            cnt.tris += is_tri ? 1u : 0u;
+        }
+        const int node_act = (hl && hr) ? kPush : ((hl || hr) ? kNext : kPop);
+        int act = is_node ? node_act : (enter ? kNext : ((ANY && take) ? kDone : kPop));
+        const uint32_t next = is_node ? (lfirst ? li : rj) : inst_root;
+        const uint32_t pushed = lfirst ? rj : li;
+        if (bad) {
+            overflow = 1;
+            act = kDone;
+        }
+
+        // Stack update, resolved with selects.
+        const bool push = act == kPush, popping = act == kPop;
+        if (push && has_top) stk.push(sp, top, overflow);
+        if (!live) act = kDone;
+        const bool alive = act != kDone && (!popping || has_top);
+        cur = popping ? top : next;
+        const int sp_in = sp;
+        sp = push ? (has_top && sp_in < Stack::kCap ? sp_in + 1 : sp_in) : (popping && sp_in > 0 ? sp_in - 1 : sp_in);
+        top = push ? pushed : (popping ? below : top);
+        has_top = push ? true : (popping ? sp_in > 0 : has_top);
+        if (popping && sp_in > 0) stk.popped(sp_in - 1);
+        return alive;
+    }
+};
+
+// Whole query for one lane (used where rays are generated inside a kernel).
 template <bool ANY, bool COUNT>
 KF_DEV bool traverse(const SceneView& S, v3 o, v3 d, float tmin, float tmax, bool use_alpha, uint32_t alpha_seed,
                      THit& best, TCount& cnt, uint32_t& overflow) {
-    best.t = tmax;
-    best.inst = ~0u;
-    best.prim = ~0u;
-    best.b1 = 0.0f;
-    best.b2 = 0.0f;
-    best.back = 0u;
-    if (S.tlas_root == kEmptyRef) return false;
-    v3 winvd = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    Pre obj;
-    v3 co = o, cinvd = winvd;
-    bool in_blas = false;
-    uint32_t cur_inst = 0;
-    int cur_flags = 0;
-    uint32_t cur_geom = 0;
-    int stack[kStack];
-    int sp = 0;
-    int ref = S.tlas_root;
-    bool found = false;
-
-    auto test_tri = [&](int slot) -> bool {
-        if (COUNT) cnt.tris++;
-        if ((uint32_t)slot >= S.n_tri) {
-            overflow = 1;
-            return false;
+    const TravScene TS(S);
+    Trav<ANY, COUNT> T;
+    PrivStack stk;
+    T.reset();
+    TRay r;
+    r.o = o;
+    r.d = d;
+    r.tmin = tmin;
+    r.tmax = tmax;
+    r.alpha_seed = alpha_seed;
+    r.payload = 0u;
+    T.start(true, r, root_item(TS.root), use_alpha);
+    if (TS.root != kEmptyRef)
+        while (T.step(TS, stk, overflow, true)) {
         }
-        const float4* t = S.tri + 3 * (size_t)slot;
-        float4 v0 = t[0], v1 = t[1], v2 = t[2];
-        TriHit h;
-        if (!intersect(obj, v0, v1, v2, tmin, best.t, h)) return false;
-        uint32_t prim = __float_as_uint(v0.w);
-        if (!ANY) {
-            if (!(h.t < best.t || cur_inst < best.inst || (cur_inst == best.inst && prim < best.prim))) return false;
-            if (use_alpha && !(cur_flags & kInstOpaque)) {
-                float a = S.mats[S.matidx[S.toff[cur_geom] + prim]].alpha;
-                if (a == 0.0f || alpha_coin(alpha_seed, cur_inst, prim) > a) return false;
-            }
-        }
-        best.t = h.t;
-        best.b1 = h.b1;
-        best.b2 = h.b2;
-        best.back = h.back;
-        best.inst = cur_inst;
-        best.prim = prim;
-        return true;
-    };
+    best = T.best;
+    cnt.nodes += T.cnt.nodes;
+    cnt.tris += T.cnt.tris;
+    return T.found;
+}
 
-    uint32_t steps = 0;
+// ------------------------------------------------------- persistent driver
+// Queries [0, n) are split into 8 contiguous chunks, one per XCD (workgroups
+// are dispatched to the 8 XCDs round-robin, so block b runs on XCD b % 8 and
+// neighbouring queries share that XCD's L2).  A wave refills its idle lanes
+// from its chunk's counter once at least `refill` lanes are idle (or all
+// are), and moves on to the next chunk when its own is used up, so no lane
+// waits for the slowest ray of a fixed group.
+//   ctr[8]: per-chunk counters, zero before the launch.
+//   load(q, TRay&) fills the ray of query q and its payload;
+//   store(payload, best, found) writes a finished query.
+KF_DEV uint64_t chunk_begin(uint32_t n, uint32_t c) { return (uint64_t)n * c / 8u; }
+
+template <bool ANY, bool COUNT, class Load, class Store>
+KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long* __restrict__ ctr, uint32_t refill,
+                             bool use_alpha, Load load, Store store, uint32_t& overflow, TCount& total) {
+    const TravScene TS(S);
+    if (TS.root == kEmptyRef) {  // empty scene: every query misses
+        THit miss;
+        miss.b1 = miss.b2 = 0.0f;
+        miss.inst = miss.prim = ~0u;
+        miss.back = 0u;
+        for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+            TRay r;
+            load(q, r);
+            miss.t = r.tmax;
+            store(r.payload, miss, false);
+        }
+        return;
+    }
+    const uint32_t root = root_item(TS.root);
+    Trav<ANY, COUNT> T;
+    T.reset();
+    __shared__ uint32_t ring[kRing * kRingThreads];
+    RingStack stk;
+    stk.ring = ring + threadIdx.x;
+    TRay r;  // last ray fetched by this lane
+    r.o = r.d = mk(0.0f, 0.0f, 0.0f);
+    r.tmin = r.tmax = 0.0f;
+    r.alpha_seed = r.payload = 0u;
+    bool live = false;
+    uint32_t payload = 0;
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t chunk = blockIdx.x & 7u;  // wave-uniform
+    uint32_t moved = 0;                // chunks found empty (wave-uniform)
     while (true) {
-        if (++steps > kMaxSteps) {
-            overflow = 1;
-            break;
-        }
-        if (ref == kPopRef) {
-            if (sp == 0) break;
-            ref = stack[--sp];
-            if (ref == kExitBlasRef) {
-                in_blas = false;
-                co = o;
-                cinvd = winvd;
-                ref = kPopRef;
+        const unsigned long long idle = __ballot(!live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        bool fresh = false;
+        if (moved < 8u && (nidle >= refill || nidle == 64u)) {
+            const uint32_t rank = (uint32_t)__popcll(idle & below);
+            const bool wants = !live;
+            uint32_t got = 0;
+            while (got < nidle && moved < 8u) {
+                const uint32_t want = nidle - got;
+                const uint64_t c0 = chunk_begin(n, chunk), len = chunk_begin(n, chunk + 1u) - c0;
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&ctr[chunk], (unsigned long long)want);
+                base = __shfl(base, 0);
+                if (base < len) {
+                    const uint32_t take = (uint32_t)min((unsigned long long)want, len - base);
+                    if (wants && rank >= got && rank < got + take) {
+                        load((uint32_t)(c0 + base + (rank - got)), r);
+                        fresh = true;
+                    }
+                    got += take;
+                }
+                if (base + want >= len) {
+                    chunk = (chunk + 1u) & 7u;
+                    ++moved;
+                }
             }
+        }
+        T.start(fresh, r, root, use_alpha);
+        payload = fresh ? r.payload : payload;
+        live = live || fresh;
+        if (__ballot(live) == 0ull) {
+            if (moved >= 8u) break;
             continue;
         }
-        if (ref >= 0) {
-            if (COUNT) cnt.nodes++;
-            if ((uint32_t)ref >= (in_blas ? S.n_blas_nodes : S.n_tlas_nodes)) {
-                overflow = 1;
-                ref = kPopRef;
-                continue;
+        const bool alive = T.step(TS, stk, overflow, live);
+        if (live && !alive) {
+            live = false;
+            store(payload, T.best, T.found);
+            if (COUNT) {
+                total.nodes += T.cnt.nodes;
+                total.tris += T.cnt.tris;
             }
-            Node n = in_blas ? S.blas[ref] : S.tlas[ref];
-            bool hl, hr;
-            float tl, tr;
-            slab2(n, co, cinvd, tmin, best.t, hl, hr, tl, tr);
-            int lref = n.d.x, rref = n.d.y;
-            if (in_blas) {
-                if (hl && lref < 0) {
-                    hl = false;
-                    if (test_tri(~lref)) {
-                        found = true;
-                        if (ANY) return true;
-                    }
-                }
-                if (hr && rref < 0) {
-                    hr = false;
-                    if (test_tri(~rref)) {
-                        found = true;
-                        if (ANY) return true;
-                    }
-                }
-            }
-            if (hl && hr) {
-                int nearr = tl <= tr ? lref : rref;
-                int farr = tl <= tr ? rref : lref;
-                if (sp < kStack) stack[sp++] = farr; else overflow = 1;
-                ref = nearr;
-            } else if (hl) {
-                ref = lref;
-            } else if (hr) {
-                ref = rref;
-            } else {
-                ref = kPopRef;
-            }
-            continue;
-        }
-        // leaf
-        if (!in_blas) {
-            uint32_t ii = (uint32_t)(~ref);
-            ref = kPopRef;
-            if (ii >= S.n_inst) {
-                overflow = 1;
-                continue;
-            }
-            Inst I = S.inst[ii];
-            if (!(I.info.z & kInstValid)) continue;
-            v3 oo = xform_point(I, o);
-            v3 od = xform_dir(I, d);
-            precompute(oo, od, obj);
-            cur_inst = ii;
-            cur_flags = I.info.z;
-            cur_geom = (uint32_t)I.info.y;
-            int root = I.info.x;
-            if (root < 0) {  // single-triangle BLAS
-                if (test_tri(~root)) {
-                    found = true;
-                    if (ANY) return true;
-                }
-                continue;
-            }
-            if (sp < kStack) stack[sp++] = kExitBlasRef; else { overflow = 1; continue; }
-            in_blas = true;
-            co = obj.o;
-            cinvd = obj.invd;
-            ref = root;
-        } else {
-            if (test_tri(~ref)) {
-                found = true;
-                if (ANY) return true;
-            }
-            ref = kPopRef;
         }
     }
-    return found;
 }
 
 }  // namespace kf

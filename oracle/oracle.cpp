@@ -241,9 +241,11 @@ template <class T> Pre<T> precompute(V3<T> o, V3<T> d) {
     int ky = kx == 2 ? 0 : kx + 1;
     if (d[kz] < 0) std::swap(kx, ky);
     p.kx = kx; p.ky = ky; p.kz = kz;
-    p.Sx = d[kx] / d[kz];
-    p.Sy = d[ky] / d[kz];
+    // Sz correctly rounded; Sx, Sy as products with it (any per-ray shear
+    // constants keep the test watertight, the kernels use these).
     p.Sz = T(1) / d[kz];
+    p.Sx = d[kx] * p.Sz;
+    p.Sy = d[ky] * p.Sz;
     return p;
 }
 
