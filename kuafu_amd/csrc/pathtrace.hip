@@ -644,6 +644,14 @@ struct FrameOut {
     float trace_ms, extend_ms, shadow_ms;
 };
 
+// Blocks of a TB-thread kernel that are resident at once on n_cu CUs.
+template <class Kernel>
+uint32_t resident_grid(Kernel kernel, int n_cu) {
+    int nb = 0;
+    KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, TB, 0));
+    return (uint32_t)std::max(nb, 1) * (uint32_t)n_cu;
+}
+
 void build_light_list(const Scene& s, DevLights& L, bool& inline_nee) {
     L.dl = s.dl;
     L.pl = s.pl;
@@ -766,6 +774,10 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
         KF_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
         const uint32_t grid = (uint32_t)n_cu * 8;
         const uint32_t refill = traversal_refill();
+        // Persistent traversal kernels: exactly the blocks that fit at once.
+        static const uint32_t grid_ext0 = resident_grid(k_extend<false>, n_cu);
+        static const uint32_t grid_ext1 = resident_grid(k_extend<true>, n_cu);
+        static const uint32_t grid_sh = resident_grid(k_shadow, n_cu);
 
         unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
         uint32_t n_ext = 0, n_sh = 0;
@@ -784,10 +796,10 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
             for (uint32_t b = 0; b < nb; ++b) {
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
                 if (o.count_traversal)
-                    k_extend<true><<<grid, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
+                    k_extend<true><<<grid_ext1, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
                                                          r->hit_prim, r->counters, refill);
                 else
-                    k_extend<false><<<grid, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
+                    k_extend<false><<<grid_ext0, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
                                                           r->hit_prim, r->counters, refill);
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 1], st));
                 ++n_ext;
@@ -802,7 +814,7 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
                                                         r->counters);
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], st));
                 if (!inline_nee) {
-                    k_shadow<<<grid, TB, 0, st>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->vis, r->counters, refill);
+                    k_shadow<<<grid_sh, TB, 0, st>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->vis, r->counters, refill);
                     ++n_sh;
                 }
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], st));

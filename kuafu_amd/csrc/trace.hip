@@ -6,6 +6,8 @@
 #include "kf_internal.h"
 #include "traverse.h"
 
+#include <algorithm>
+
 namespace kf {
 namespace {
 
@@ -61,16 +63,14 @@ __global__ __launch_bounds__(TB) void k_any(SceneView S, const kf_ray* __restric
     if (ovf) atomicAdd(&cnt[2], 1ull);
 }
 
-// Persistent grid: enough resident waves to fill every CU.
-unsigned trace_grid() {
-    static unsigned g = 0;
-    if (!g) {
-        int dev = 0, cus = 256;
-        KF_CHECK(hipGetDevice(&dev));
-        KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        g = (unsigned)cus * 8u;
-    }
-    return g;
+// Persistent grid: exactly the blocks of `kernel` that are resident at once.
+template <class Kernel>
+unsigned trace_grid(Kernel kernel) {
+    int dev = 0, cus = 256, nb = 0;
+    KF_CHECK(hipGetDevice(&dev));
+    KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, TB, 0));
+    return (unsigned)std::max(nb, 1) * (unsigned)cus;
 }
 
 // Streaming copy, four independent 16-byte loads in flight per thread.
@@ -148,7 +148,7 @@ extern "C" int kf_trace_closest(kf_scene* s, const kf_ray* rays, uint32_t n, kf_
         unsigned long long* c = trace_counters();
         KF_CHECK(hipMemsetAsync(c, 0, kTraceCnt * sizeof(unsigned long long), st));
         KF_CHECK(hipMemcpyAsync(d_r, rays, n * sizeof(kf_ray), hipMemcpyHostToDevice, st));
-        k_closest<false><<<trace_grid(), TB, 0, st>>>(s->view(), d_r, n, d_h, c, traversal_refill());
+        k_closest<false><<<trace_grid(k_closest<false>), TB, 0, st>>>(s->view(), d_r, n, d_h, c, traversal_refill());
         KF_CHECK(hipGetLastError());
         unsigned long long h[kTraceCnt];
         KF_CHECK(hipMemcpyAsync(hits, d_h, n * sizeof(kf_hit), hipMemcpyDeviceToHost, st));
@@ -176,7 +176,7 @@ extern "C" int kf_trace_any(kf_scene* s, const kf_ray* rays, uint32_t n, uint8_t
         unsigned long long* c = trace_counters();
         KF_CHECK(hipMemsetAsync(c, 0, kTraceCnt * sizeof(unsigned long long), st));
         KF_CHECK(hipMemcpyAsync(d_r, rays, n * sizeof(kf_ray), hipMemcpyHostToDevice, st));
-        k_any<<<trace_grid(), TB, 0, st>>>(s->view(), d_r, n, d_o, c, traversal_refill());
+        k_any<<<trace_grid(k_any), TB, 0, st>>>(s->view(), d_r, n, d_o, c, traversal_refill());
         KF_CHECK(hipGetLastError());
         unsigned long long h[kTraceCnt];
         KF_CHECK(hipMemcpyAsync(occluded, d_o, n, hipMemcpyDeviceToHost, st));
@@ -204,7 +204,7 @@ extern "C" int kf_trace_closest_device(kf_scene* s, const kf_ray* d_rays, uint32
         KF_CHECK(hipEventCreate(&e0));
         KF_CHECK(hipEventCreate(&e1));
         KF_CHECK(hipEventRecord(e0, st));
-        k_closest<false><<<trace_grid(), TB, 0, st>>>(s->view(), d_rays, n, d_hits, c, traversal_refill());
+        k_closest<false><<<trace_grid(k_closest<false>), TB, 0, st>>>(s->view(), d_rays, n, d_hits, c, traversal_refill());
         KF_CHECK(hipEventRecord(e1, st));
         KF_CHECK(hipGetLastError());
         KF_CHECK(hipEventSynchronize(e1));

@@ -476,13 +476,18 @@ KF_DEV bool traverse(const SceneView& S, v3 o, v3 d, float tmin, float tmax, boo
 // ------------------------------------------------------- persistent driver
 // Queries [0, n) are split into 8 contiguous chunks, one per XCD (workgroups
 // are dispatched to the 8 XCDs round-robin, so block b runs on XCD b % 8 and
-// neighbouring queries share that XCD's L2).  A wave refills its idle lanes
-// from its chunk's counter once at least `refill` lanes are idle (or all
-// are), and moves on to the next chunk when its own is used up, so no lane
-// waits for the slowest ray of a fixed group.
+// neighbouring queries share that XCD's L2).  A wave reserves kGrab queries
+// at a time from its chunk's counter and refills its idle lanes from that
+// reserve once at least `refill` lanes are idle (or all are), so no lane
+// waits for the slowest ray of a fixed group.  When its chunk is used up the
+// wave moves on to the next one; a plain load of the counter tells a used-up
+// chunk without an atomic, which keeps the end of a launch (every wave
+// passing over every chunk) off the contended counters.
 //   ctr[8]: per-chunk counters, zero before the launch.
 //   load(q, TRay&) fills the ray of query q and its payload;
 //   store(payload, best, found) writes a finished query.
+constexpr uint32_t kGrab = 128;
+
 KF_DEV uint64_t chunk_begin(uint32_t n, uint32_t c) { return (uint64_t)n * c / 8u; }
 
 template <bool ANY, bool COUNT, class Load, class Store>
@@ -516,41 +521,51 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
     uint32_t payload = 0;
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t chunk = blockIdx.x & 7u;  // wave-uniform
-    uint32_t moved = 0;                // chunks found empty (wave-uniform)
+    uint32_t chunk = blockIdx.x & 7u;  // wave-uniform from here on
+    uint32_t moved = 0;                // chunks found used up
+    uint32_t res_q = 0, res_n = 0;     // reserved queries [res_q, res_q + res_n)
     while (true) {
         const unsigned long long idle = __ballot(!live);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         bool fresh = false;
-        if (moved < 8u && (nidle >= refill || nidle == 64u)) {
+        if ((res_n > 0u || moved < 8u) && (nidle >= refill || nidle == 64u)) {
             const uint32_t rank = (uint32_t)__popcll(idle & below);
             const bool wants = !live;
             uint32_t got = 0;
-            while (got < nidle && moved < 8u) {
-                const uint32_t want = nidle - got;
-                const uint64_t c0 = chunk_begin(n, chunk), len = chunk_begin(n, chunk + 1u) - c0;
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(&ctr[chunk], (unsigned long long)want);
-                base = __shfl(base, 0);
-                if (base < len) {
-                    const uint32_t take = (uint32_t)min((unsigned long long)want, len - base);
-                    if (wants && rank >= got && rank < got + take) {
-                        load((uint32_t)(c0 + base + (rank - got)), r);
-                        fresh = true;
+            while (got < nidle && (res_n > 0u || moved < 8u)) {
+                if (res_n == 0u) {
+                    const uint64_t c0 = chunk_begin(n, chunk), len = chunk_begin(n, chunk + 1u) - c0;
+                    unsigned long long base = 0;
+                    if (lane == 0) {
+                        base = __hip_atomic_load(&ctr[chunk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (base < len) base = atomicAdd(&ctr[chunk], (unsigned long long)kGrab);
                     }
-                    got += take;
+                    base = __shfl(base, 0);
+                    if (base < len) {
+                        res_q = (uint32_t)(c0 + base);
+                        res_n = (uint32_t)min((unsigned long long)kGrab, len - base);
+                    }
+                    if (base + kGrab >= len) {
+                        chunk = (chunk + 1u) & 7u;
+                        ++moved;
+                    }
+                    continue;
                 }
-                if (base + want >= len) {
-                    chunk = (chunk + 1u) & 7u;
-                    ++moved;
+                const uint32_t take = min(nidle - got, res_n);
+                if (wants && rank >= got && rank < got + take) {
+                    load(res_q + (rank - got), r);
+                    fresh = true;
                 }
+                got += take;
+                res_q += take;
+                res_n -= take;
             }
         }
         T.start(fresh, r, root, use_alpha);
         payload = fresh ? r.payload : payload;
         live = live || fresh;
         if (__ballot(live) == 0ull) {
-            if (moved >= 8u) break;
+            if (moved >= 8u && res_n == 0u) break;
             continue;
         }
         const bool alive = T.step(TS, stk, overflow, live);
