@@ -8,8 +8,9 @@ HBM before timing starts.
 value = rays traced (primary + extension + shadow, counted on the device) /
 wall time of the K timed frames / 1e6  [Mrays/s].
 roofline = the closest-hit traversal kernel (k_extend): algorithmic bytes per
-ray = 32 (ray) + 20 (hit) + 64 * node visits + 48 * triangle tests, counted by
-the instrumented traversal on the same frame, / the kernel's HIP-event time.
+ray = 32 (ray) + 20 (hit) + 128 * BVH4 node visits + 64 * instance records +
+48 * triangle tests, counted by the instrumented traversal on the same frame,
+/ the kernel's HIP-event time.
 cpu_baseline = the FP32 CPU oracle on a bounded band of rows of the same frame.
 """
 import argparse
@@ -142,7 +143,7 @@ def main():
     for k in range(args.steps):
         st = rd.render(gs, w.cam, pc, seed=100 + k, samples_per_batch=args.samples_per_batch)
         rays += st.rays()
-        ext_rays += st.primary_rays + st.extension_rays
+        ext_rays += st.primary_rays + st.extension_rays + st.speculative_rays
         ext_ms += st.extend_ms
         trace_ms += st.trace_ms
         launches += st.extend_launches
@@ -150,10 +151,11 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     value = rays / wall / 1e6
 
-    closest = cst.primary_rays + cst.extension_rays
+    closest = cst.primary_rays + cst.extension_rays + cst.speculative_rays  # rays k_extend traced
     nodes_per_ray = cst.node_visits / max(closest, 1)
+    insts_per_ray = cst.instance_visits / max(closest, 1)
     tris_per_ray = cst.tri_tests / max(closest, 1)
-    bytes_per_ray = 32 + 20 + 64 * nodes_per_ray + 48 * tris_per_ray
+    bytes_per_ray = 32 + 20 + 128 * nodes_per_ray + 64 * insts_per_ray + 48 * tris_per_ray
     achieved = bytes_per_ray * ext_rays / (ext_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "extend_traffic_config2.json")
@@ -164,7 +166,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_extend (closest-hit traversal)",
             "bytes_per_ray": round(bytes_per_ray, 1), "node_visits_per_ray": round(nodes_per_ray, 2),
-            "tri_tests_per_ray": round(tris_per_ray, 2),
+            "instance_visits_per_ray": round(insts_per_ray, 2), "tri_tests_per_ray": round(tris_per_ray, 2),
             "avg_launch_ms": round(ext_ms / launches, 4)}
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,

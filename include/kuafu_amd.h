@@ -166,14 +166,17 @@ typedef struct kf_render_stats {
     uint64_t primary_rays;
     uint64_t extension_rays;
     uint64_t shadow_rays;
-    uint64_t node_visits;     /* only when the counting traversal is enabled */
-    uint64_t tri_tests;
+    uint64_t node_visits;     /* BVH4 node records (128 B); only with the counting traversal */
+    uint64_t tri_tests;       /* triangle records (48 B) */
     float trace_ms;           /* kernel wall time of the frame (HIP events) */
     float extend_ms;          /* closest-hit traversal kernels */
     float shadow_ms;          /* any-hit traversal kernels */
     float other_ms;
     uint32_t extend_launches; /* closest-hit traversal kernel launches */
     uint32_t shadow_launches; /* any-hit traversal kernel launches */
+    uint64_t instance_visits; /* instance records (64 B) */
+    uint64_t speculative_rays; /* next-bounce rays traced for paths Russian roulette
+                                  then ended (not in extension_rays) */
 } kf_render_stats;
 
 /* Render options that the reference takes from Config / push constants. */
@@ -234,9 +237,10 @@ int kf_scene_set_lights(kf_scene* s, const kf_dir_light* d, const kf_point_light
 
 /* Statistics of the built BVHs. */
 typedef struct kf_bvh_info {
-    uint32_t blas_nodes, tlas_nodes, triangles, instances;
+    uint32_t blas_nodes, tlas_nodes, triangles, instances;  /* BVH2 nodes */
     uint32_t max_blas_depth, max_tlas_depth;
     float build_ms;
+    uint32_t blas4_nodes, tlas4_nodes;  /* BVH4 nodes read by the traversal */
 } kf_bvh_info;
 int kf_scene_bvh_info(kf_scene* s, kf_bvh_info* out);
 
@@ -245,6 +249,18 @@ int kf_scene_bvh_info(kf_scene* s, kf_bvh_info* out);
  * int bits) and the triangle order.  For BVH invariant tests. */
 int kf_scene_read_blas(kf_scene* s, uint32_t geometry, float* nodes, uint32_t max_nodes,
                        uint32_t* n_nodes, int32_t* root_ref, uint32_t* tri_order, uint32_t max_tris);
+
+/* Copy a BVH4 back (which = 0: the BLAS of all geometries, roots[g] per
+ * geometry; 1: the TLAS, roots[0]).  Nodes are 32 floats each: lo.x, hi.x,
+ * lo.y, hi.y, lo.z, hi.z of the four children (4 floats each), then the four
+ * children and 4 pad words as uint32 bits.  A child is kind << 30 | index:
+ * kind 0 / 1 an inner TLAS / BLAS node of this array, 2 an instance, 3 a
+ * triangle slot; 0xffffffff an unused slot.  Roots are refs: >= 0 a node of
+ * this array, < 0 ~leaf, 0x7fffffff empty.
+ * For BVH invariant tests; no counterpart in the reference, whose
+ * acceleration structures are opaque driver objects. */
+int kf_scene_read_bvh4(kf_scene* s, int which, float* nodes, uint32_t max_nodes, uint32_t* n_nodes, int32_t* roots,
+                       uint32_t max_roots);
 
 /* Fixed ray batches (host pointers, synchronous): the traceRayEXT calls of
  * PathTrace.rgen:97-107 (closest hit) and PathTrace.rchit:186-197

@@ -81,7 +81,8 @@ class RenderStats(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("trace_ms", C.c_float),
                 ("extend_ms", C.c_float), ("shadow_ms", C.c_float), ("other_ms", C.c_float),
-                ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32)]
+                ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
+                ("instance_visits", C.c_uint64), ("speculative_rays", C.c_uint64)]
 
     def rays(self):
         return int(self.primary_rays + self.extension_rays + self.shadow_rays)
@@ -95,7 +96,7 @@ class RenderOpts(C.Structure):
 class BvhInfo(C.Structure):
     _fields_ = [("blas_nodes", C.c_uint32), ("tlas_nodes", C.c_uint32), ("triangles", C.c_uint32),
                 ("instances", C.c_uint32), ("max_blas_depth", C.c_uint32), ("max_tlas_depth", C.c_uint32),
-                ("build_ms", C.c_float)]
+                ("build_ms", C.c_float), ("blas4_nodes", C.c_uint32), ("tlas4_nodes", C.c_uint32)]
 
 
 assert C.sizeof(DirLight) == 32 and C.sizeof(PointLights) == 1024 and C.sizeof(ActiveLights) == 1536
@@ -128,6 +129,8 @@ EXPORTS = [
     ("kf_scene_bvh_info", C.c_int, [C.c_void_p, C.c_void_p]),
     ("kf_scene_read_blas", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_uint32]),
+    ("kf_scene_read_bvh4", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_uint32]),
     ("kf_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("kf_trace_any", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("kf_trace_closest_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),

@@ -323,6 +323,17 @@ class GpuScene:
                                        ptr(order), len(order)))
         return nodes[:nn.value], root.value, order[:nt]
 
+    def read_bvh4(self, which):
+        """BVH4 nodes (n, 32) float32 and root refs (per geometry for which=0,
+        one for which=1 = TLAS)."""
+        nn = C.c_uint32()
+        check(lib().kf_scene_read_bvh4(self.h, which, None, 0, C.byref(nn), None, 0))
+        nodes = np.zeros((max(nn.value, 1), 32), np.float32)
+        nr = max(len(self.data.geoms), 1) if which == 0 else 1
+        roots = np.zeros(nr, np.int32)
+        check(lib().kf_scene_read_bvh4(self.h, which, ptr(nodes), nodes.shape[0], C.byref(nn), ptr(roots), nr))
+        return nodes[:nn.value], roots
+
 
 class Renderer:
     """RGBA32F render targets + one-frame render (RayTracer::trace)."""

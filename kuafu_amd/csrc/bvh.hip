@@ -20,6 +20,8 @@
 
 namespace kf {
 
+#define KF_DEV_INLINE __device__ __forceinline__
+
 namespace {
 
 constexpr int kSortThreads = 256;
@@ -415,6 +417,99 @@ __global__ void k_instances(uint32_t n, const kf_instance* __restrict__ src, con
     iseg[i] = 0;
 }
 
+// ---------------------------------------------------------------- BVH4
+KF_DEV_INLINE float half_area(const float lo[3], const float hi[3]) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// One level of the collapse: item (BVH2 node b, BVH4 slot w) writes BVH4
+// node w and queues its inner children for the next level.  err is set on
+// any index outside the arrays (malformed input), and nothing is written
+// out of bounds.
+__global__ void k_collapse(const Node* __restrict__ b2, uint32_t n_b2, const int2* __restrict__ in, uint32_t n_in,
+                           int2* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
+                           Node4* __restrict__ wide, uint32_t* __restrict__ wide_count, uint32_t wide_cap,
+                           uint32_t inner_kind, int* __restrict__ err) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_in) return;
+    const int2 it = in[i];
+    if ((uint32_t)it.x >= n_b2 || (uint32_t)it.y >= wide_cap) {
+        *err = 1;
+        return;
+    }
+    float lo[4][3], hi[4][3];
+    int ref[4];
+    auto take = [&](int node, int s0, int s1) {
+        const Node n = b2[node];
+        lo[s0][0] = n.a.x; hi[s0][0] = n.a.y; lo[s0][1] = n.a.z; hi[s0][1] = n.a.w; lo[s0][2] = n.c.x; hi[s0][2] = n.c.y;
+        lo[s1][0] = n.b.x; hi[s1][0] = n.b.y; lo[s1][1] = n.b.z; hi[s1][1] = n.b.w; lo[s1][2] = n.c.z; hi[s1][2] = n.c.w;
+        ref[s0] = n.d.x;
+        ref[s1] = n.d.y;
+    };
+    take(it.x, 0, 1);
+    int cnt = 2;
+    while (cnt < 4) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int j = 0; j < cnt; ++j)
+            if (ref[j] >= 0) {
+                const float a = half_area(lo[j], hi[j]);
+                if (a > ba) {
+                    ba = a;
+                    best = j;
+                }
+            }
+        if (best < 0) break;
+        if ((uint32_t)ref[best] >= n_b2) {
+            *err = 1;
+            return;
+        }
+        take(ref[best], best, cnt);
+        ++cnt;
+    }
+    int k = 0;
+    for (int j = 0; j < cnt; ++j) k += ref[j] >= 0 ? 1 : 0;
+    const uint32_t base = k ? atomicAdd(wide_count, (uint32_t)k) : 0u;
+    if (base + (uint32_t)k > wide_cap) {
+        *err = 1;
+        return;
+    }
+    float blo[3][4], bhi[3][4];
+    int wref[4];
+    int m = 0;
+    for (int j = 0; j < 4; ++j) {
+        const bool used = j < cnt;
+        for (int a = 0; a < 3; ++a) {
+            blo[a][j] = used ? lo[j][a] : 0.0f;
+            bhi[a][j] = used ? hi[j][a] : 0.0f;
+        }
+        // child refs are stored as traversal items: kind (inner_kind for an
+        // inner node, inner_kind + 2 for a leaf) in bits 31:30, index below
+        if (!used) {
+            wref[j] = (int)kEmptyItem;
+        } else if (ref[j] >= 0) {
+            wref[j] = (int)((inner_kind << 30) | (base + m));
+            const uint32_t q = atomicAdd(out_count, 1u);
+            if (q < out_cap) out[q] = make_int2(ref[j], (int)(base + m));
+            else *err = 1;
+            ++m;
+        } else {
+            wref[j] = (int)(((inner_kind + 2u) << 30) | (uint32_t)~ref[j]);
+        }
+    }
+    Node4 W;
+    W.lox = make_float4(blo[0][0], blo[0][1], blo[0][2], blo[0][3]);
+    W.hix = make_float4(bhi[0][0], bhi[0][1], bhi[0][2], bhi[0][3]);
+    W.loy = make_float4(blo[1][0], blo[1][1], blo[1][2], blo[1][3]);
+    W.hiy = make_float4(bhi[1][0], bhi[1][1], bhi[1][2], bhi[1][3]);
+    W.loz = make_float4(blo[2][0], blo[2][1], blo[2][2], blo[2][3]);
+    W.hiz = make_float4(bhi[2][0], bhi[2][1], bhi[2][2], bhi[2][3]);
+    W.ref = make_int4(wref[0], wref[1], wref[2], wref[3]);
+    W.pad = make_int4(0, 0, 0, 0);
+    wide[it.y] = W;
+}
+
 }  // namespace
 
 void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_seg, uint32_t n_seg,
@@ -473,6 +568,56 @@ void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32
     dfree(parent_node); dfree(parent_leaf); dfree(flags); dfree(hist);
 }
 
+uint32_t collapse_bvh4(const Node* d_b2, uint32_t n_b2, const std::vector<int>& roots2, std::vector<int>& roots4,
+                       Node4* d_out, uint32_t cap, uint32_t inner_kind, hipStream_t st) {
+    const uint32_t TB = 256;
+    roots4.assign(roots2.size(), kEmptyRef);
+    std::vector<int2> items;
+    for (size_t g = 0; g < roots2.size(); ++g) {
+        const int r = roots2[g];
+        if (r >= 0 && r != kEmptyRef) {
+            roots4[g] = (int)items.size();
+            items.push_back(make_int2(r, (int)items.size()));
+        } else {
+            roots4[g] = r;  // single leaf or empty
+        }
+    }
+    uint32_t n_in = (uint32_t)items.size();
+    if (n_in == 0) return 0;
+    if (n_in > cap || n_b2 == 0) throw Error("collapse_bvh4: inconsistent BVH2");
+    const uint32_t qcap = std::max<uint32_t>(n_b2, 1);
+    int2* qa = dalloc<int2>(qcap);
+    int2* qb = dalloc<int2>(qcap);
+    uint32_t* cnt = dalloc<uint32_t>(2);  // [0] next queue, [1] BVH4 nodes
+    int* err = dalloc<int>(1);
+    uint32_t h2[2] = {0, n_in};
+    int zero = 0;
+    KF_CHECK(hipMemcpyAsync(qa, items.data(), n_in * sizeof(int2), hipMemcpyHostToDevice, st));
+    KF_CHECK(hipMemcpyAsync(cnt, h2, sizeof(h2), hipMemcpyHostToDevice, st));
+    KF_CHECK(hipMemcpyAsync(err, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+    // Every level consumes at least one BVH2 level, so n_b2 + 1 levels bound
+    // a well-formed tree.
+    for (uint32_t level = 0; n_in > 0; ++level) {
+        if (level > n_b2 + 1) throw Error("collapse_bvh4: BVH2 deeper than its node count");
+        k_collapse<<<blocks(n_in, TB), TB, 0, st>>>(d_b2, n_b2, qa, n_in, qb, cnt, qcap, d_out, cnt + 1, cap,
+                                                    inner_kind, err);
+        KF_CHECK(hipGetLastError());
+        uint32_t next = 0;
+        KF_CHECK(hipMemcpyAsync(&next, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipMemsetAsync(cnt, 0, sizeof(uint32_t), st));
+        KF_CHECK(hipStreamSynchronize(st));
+        n_in = std::min(next, qcap);
+        std::swap(qa, qb);
+    }
+    int h_err = 0;
+    KF_CHECK(hipMemcpyAsync(h2, cnt, sizeof(h2), hipMemcpyDeviceToHost, st));
+    KF_CHECK(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
+    KF_CHECK(hipStreamSynchronize(st));
+    dfree(qa); dfree(qb); dfree(cnt); dfree(err);
+    if (h_err) throw Error("collapse_bvh4: malformed BVH2");
+    return h2[1];
+}
+
 void build_blas(Scene& s) {
     hipStream_t st = stream();
     const uint32_t TB = 256;
@@ -513,6 +658,16 @@ void build_blas(Scene& s) {
         KF_CHECK(hipMemcpyAsync(s.h_blas_root.data(), s.d_blas_root, s.n_geom * sizeof(int), hipMemcpyDeviceToHost, st));
     dfree(d_node_off); dfree(lo); dfree(hi); dfree(order);
     KF_CHECK(hipStreamSynchronize(st));
+    // BVH4 for the traversal
+    dfree(s.d_blas4);
+    dfree(s.d_blas_root4);
+    const uint32_t cap4 = std::max<uint32_t>(acc, 1);
+    s.d_blas4 = dalloc<Node4>(cap4);
+    s.n_blas4 = collapse_bvh4(s.d_blas, acc, s.h_blas_root, s.h_blas_root4, s.d_blas4, cap4, 1u, st);
+    s.d_blas_root4 = dalloc<int>(std::max<uint32_t>(s.n_geom, 1));
+    if (s.n_geom)
+        KF_CHECK(hipMemcpyAsync(s.d_blas_root4, s.h_blas_root4.data(), s.n_geom * sizeof(int), hipMemcpyHostToDevice, st));
+    KF_CHECK(hipStreamSynchronize(st));
 }
 
 void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
@@ -540,7 +695,7 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     float4* ihi = dalloc<float4>(std::max<uint32_t>(n, 1));
     uint32_t* iseg = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
     if (n)
-        k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, d_gflags, s.d_blas_root, glo, ghi,
+        k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, d_gflags, s.d_blas_root4, glo, ghi,
                                                   s.d_inst, ilo, ihi, iseg);
     uint32_t seg_off_h[2] = {0, n}, node_off_h[1] = {0};
     uint32_t* d_seg_off = dalloc<uint32_t>(2);
@@ -554,6 +709,15 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     KF_CHECK(hipMemcpyAsync(&root, d_root, sizeof(int), hipMemcpyDeviceToHost, st));
     KF_CHECK(hipStreamSynchronize(st));
     s.tlas_root = root;
+    const uint32_t n_tlas2 = n > 1 ? n - 1 : 0;
+    if (!s.d_tlas4 || s.tlas4_cap < std::max<uint32_t>(n_tlas2, 1)) {
+        dfree(s.d_tlas4);
+        s.tlas4_cap = std::max<uint32_t>(n_tlas2, 1);
+        s.d_tlas4 = dalloc<Node4>(s.tlas4_cap);
+    }
+    std::vector<int> r2{root}, r4;
+    s.n_tlas4 = collapse_bvh4(s.d_tlas, n_tlas2, r2, r4, s.d_tlas4, s.tlas4_cap, 0u, st);
+    s.tlas_root4 = r4[0];
     dfree(d_w2o); dfree(d_gflags); dfree(glo); dfree(ghi); dfree(ilo); dfree(ihi); dfree(iseg);
     dfree(d_seg_off); dfree(d_node_off); dfree(order); dfree(d_root);
     KF_CHECK(hipStreamSynchronize(st));

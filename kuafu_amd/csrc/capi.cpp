@@ -65,15 +65,18 @@ template <class T> static void release(T*& p) {
 SceneView Scene::view() const {
     SceneView v;
     memset(&v, 0, sizeof(v));
-    v.blas = d_blas;
+    v.blas = d_blas4;
     v.tri = d_tri;
-    v.tlas = d_tlas;
-    v.tlas_root = n_inst ? tlas_root : kEmptyRef;
-    v.n_blas_nodes = n_blas_nodes;
+    v.tlas = d_tlas4;
+    v.tlas_root = n_inst ? tlas_root4 : kEmptyRef;
+    v.n_blas_nodes = n_blas4;
     v.n_tri = n_tri;
-    v.n_tlas_nodes = n_inst > 1 ? n_inst - 1 : 0;
+    v.n_tlas_nodes = n_tlas4;
     v.inst = d_inst;
     v.n_inst = n_inst;
+    v.any_alpha = 0;
+    for (uint32_t f : h_gflags)
+        if (!(f & KF_GEOM_OPAQUE)) v.any_alpha = 1;
     v.verts = d_verts;
     v.idx = d_idx;
     v.matidx = d_matidx;
@@ -95,6 +98,7 @@ Scene::~Scene() {
     if (g_stream) (void)hipStreamSynchronize(g_stream);
     release(d_verts); release(d_idx); release(d_matidx); release(d_voff); release(d_toff); release(d_tri_geom);
     release(d_mats); release(d_blas); release(d_tri); release(d_blas_root); release(d_tlas); release(d_inst);
+    release(d_blas4); release(d_blas_root4); release(d_tlas4);
     release(d_inst_src); release(d_tex); release(d_tex_info); release(d_env_info); release(d_srgb);
 }
 
@@ -454,6 +458,35 @@ int kf_scene_bvh_info(kf_scene* s, kf_bvh_info* out) {
         std::vector<Node> tn(out->tlas_nodes);
         if (!tn.empty()) KF_CHECK(hipMemcpy(tn.data(), s->d_tlas, tn.size() * sizeof(Node), hipMemcpyDeviceToHost));
         out->max_tlas_depth = (s->n_inst > 1) ? subtree_depth(tn, s->tlas_root) : 0;
+        out->blas4_nodes = s->n_blas4;
+        out->tlas4_nodes = s->n_tlas4;
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_scene_read_bvh4(kf_scene* s, int which, float* nodes, uint32_t max_nodes, uint32_t* n_nodes, int32_t* roots,
+                       uint32_t max_roots) {
+    try {
+        if (!s || (which != 0 && which != 1)) throw Error("kf_scene_read_bvh4: bad argument");
+        const uint32_t nn = which == 0 ? s->n_blas4 : s->n_tlas4;
+        const Node4* src = which == 0 ? s->d_blas4 : s->d_tlas4;
+        if (n_nodes) *n_nodes = nn;
+        if (nodes && nn) {
+            if (max_nodes < nn) throw Error("kf_scene_read_bvh4: node buffer too small");
+            KF_CHECK(hipMemcpy(nodes, src, nn * sizeof(Node4), hipMemcpyDeviceToHost));
+        }
+        if (roots) {
+            if (which == 0) {
+                if (max_roots < s->n_geom) throw Error("kf_scene_read_bvh4: root buffer too small");
+                for (uint32_t g = 0; g < s->n_geom; ++g) roots[g] = s->h_blas_root4[g];
+            } else {
+                if (max_roots < 1) throw Error("kf_scene_read_bvh4: root buffer too small");
+                roots[0] = s->n_inst ? s->tlas_root4 : kEmptyRef;
+            }
+        }
         return 0;
     } catch (const std::exception& e) {
         set_error(e.what());

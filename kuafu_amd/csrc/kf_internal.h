@@ -51,6 +51,19 @@ struct alignas(16) Node {
     int4 d;
 };
 
+// BVH4 node, 128 B, collapsed from the BVH2 above: child c's box is
+// (lox[c], hix[c]) x (loy[c], hiy[c]) x (loz[c], hiz[c]); ref[c] is the
+// child as a traversal item: bits 31:30 the kind (0 TLAS node, 1 BLAS node,
+// 2 instance, 3 triangle slot), bits 29:0 its index; kEmptyItem marks an
+// unused slot (its box is all zero).  The traversal reads these; the BVH2
+// stays for the builder and its tests.
+constexpr uint32_t kEmptyItem = 0xffffffffu;
+struct alignas(16) Node4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 ref;
+    int4 pad;
+};
+
 // Instance record read by the traversal, 64 B: world->object rows and
 // (root_ref, geometry, flags, 0).  flags bit0 = opaque, bit1 = valid.
 struct alignas(16) Inst {
@@ -63,13 +76,14 @@ constexpr int kEmptyRef = 0x7fffffff;  // root ref of a BLAS without triangles
 
 // Everything the traversal and shading kernels read, by value.
 struct SceneView {
-    const Node* blas;
+    const Node4* blas;        // BVH4 of all geometries
     const float4* tri;        // 3 float4 per triangle slot; tri[3s].w = local prim id bits
-    const Node* tlas;
-    int tlas_root;
+    const Node4* tlas;        // BVH4 over the instances
+    int tlas_root;            // BVH4 root ref of the TLAS
     uint32_t n_blas_nodes, n_tri, n_tlas_nodes;  // bounds every traversal index is checked against
     const Inst* inst;
     uint32_t n_inst;
+    uint32_t any_alpha;       // some geometry is not opaque: path rays run the alpha test
     // shading data (reference SSBO layouts)
     const float4* verts;      // kf_vertex as 3 x float4
     const uint32_t* idx;      // 3 per triangle (local)
@@ -102,6 +116,14 @@ struct Scene {
     int* d_blas_root = nullptr;
     uint32_t n_blas_nodes = 0;
     Node* d_tlas = nullptr;
+    // BVH4 copies read by the traversal
+    Node4* d_blas4 = nullptr;
+    int* d_blas_root4 = nullptr;
+    std::vector<int> h_blas_root4;
+    uint32_t n_blas4 = 0;
+    Node4* d_tlas4 = nullptr;
+    uint32_t n_tlas4 = 0, tlas4_cap = 0;
+    int tlas_root4 = kEmptyRef;
     Inst* d_inst = nullptr;
     kf_instance* d_inst_src = nullptr;
     int tlas_root = kEmptyRef;
@@ -131,6 +153,13 @@ struct Scene {
 void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_seg, uint32_t n_seg,
                 const uint32_t* d_seg_off, const uint32_t* d_node_off, Node* d_nodes, uint32_t* d_order,
                 int* d_root, bool leaf_is_slot, hipStream_t st);
+// Collapse a BVH2 into a BVH4 (level by level on the GPU): each BVH4 node
+// takes the two children of a BVH2 node and then repeatedly opens its inner
+// child of largest surface area until it has four.  roots2[i] (BVH2 root
+// refs, host) -> roots4[i] (BVH4 root refs); leaf and empty roots pass
+// through.  Returns the number of BVH4 nodes written to out (capacity cap).
+uint32_t collapse_bvh4(const Node* d_b2, uint32_t n_b2, const std::vector<int>& roots2, std::vector<int>& roots4,
+                       Node4* d_out, uint32_t cap, uint32_t inner_kind, hipStream_t st);
 void build_blas(Scene& s);
 void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n);
 
@@ -147,6 +176,7 @@ struct Renderer {
     uint32_t* hit_prim = nullptr;
     uint32_t* seed = nullptr;
     uint32_t *queue_a = nullptr, *queue_b = nullptr;
+    uint32_t* queue_e = nullptr;               // paths k_extend traces at the next bounce
     float4 *sh_o = nullptr, *sh_d = nullptr;   // shadow queue (o.xyz, slot bits), (d.xyz, tmax)
     float4 *nee_a = nullptr, *nee_b = nullptr; // per slot (c_a.xyz, theta), (c_b.xyz, mode|group bits)
     uint8_t* vis = nullptr;
@@ -161,6 +191,8 @@ struct Renderer {
     float4* d_out = nullptr;                   // per-path final radiance of a batch
     size_t out_cap = 0;
     std::vector<hipEvent_t> ev;
+    hipStream_t st2 = nullptr;                 // any-hit traversal, beside the main stream
+    std::vector<hipEvent_t> sync_ev;           // per bounce: shading done, shadow rays done
     ~Renderer();
     void ensure(size_t paths, uint32_t nslots, uint32_t spp);
 };

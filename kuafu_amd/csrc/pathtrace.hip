@@ -34,12 +34,15 @@ static_assert(TB == (int)kRingThreads, "traversal kernels index their LDS stack 
 
 // Device counters of one batch: per bounce b: [0] active paths, [1] shadow
 // rays, [2] next queue, [3..10] work chunks of k_extend, [11..18] work chunks
-// of k_shadow.  Global: primary / extension / shadow / nodes / tris / stack
-// overflow at the end.
+// of k_shadow, [19] rays k_extend traces at bounce b.  Global, at the end:
+// primary / extension / shadow rays, BVH4 node records / triangle tests
+// (closest-hit traversal), stack overflow, instance records, speculative
+// rays.
 constexpr int kMaxBounces = 64;
-constexpr int kCntPerBounce = 19;
+constexpr int kCntPerBounce = 20;
 constexpr int kCntExtendChunks = 3;
 constexpr int kCntShadowChunks = 11;
+constexpr int kCntExtQueue = 19;
 constexpr int kCntGlobal = kMaxBounces * kCntPerBounce;
 constexpr int kCntSize = kCntGlobal + 8;
 
@@ -135,6 +138,7 @@ __global__ __launch_bounds__(TB) void k_raygen(Frame F, const uint2* __restrict_
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         cnt[0 * kCntPerBounce + 0] = n;  // active paths at bounce 0
+        cnt[0 * kCntPerBounce + kCntExtQueue] = n;
         atomicAdd(&cnt[kCntGlobal + 0], (unsigned long long)n);
     }
 }
@@ -144,10 +148,10 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
                                                const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
                                                const uint32_t* __restrict__ seed, float4* __restrict__ hit,
                                                uint32_t* __restrict__ hit_prim, unsigned long long* __restrict__ cnt,
-                                               uint32_t refill) {
-    const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)cap);
+                                               uint32_t refill, uint32_t WH, uint32_t s0, uint32_t S0) {
+    const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + kCntExtQueue], (unsigned long long)cap);
     uint32_t ovf = 0;
-    TCount tot{0, 0};
+    TCount tot{0, 0, 0};
     auto load = [&](uint32_t q, TRay& r) __attribute__((always_inline)) {
         const uint32_t p = queue[q];
         const float4 o = ray_o[p], d = ray_d[p];
@@ -155,7 +159,8 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
         r.d = xyz(d);
         r.tmin = 0.001f;
         r.tmax = 10000.0f;
-        r.alpha_seed = seed[p];
+        // alpha coin seed: tea(initial seed of the sample's stream, bounce)
+        r.alpha_seed = S.any_alpha ? tea(tea(p % WH, S0 + 1u + s0 + p / WH), (uint32_t)bounce) : 0u;
         r.payload = p;
     };
     auto store = [&](uint32_t p, THit b, bool) __attribute__((always_inline)) {
@@ -167,6 +172,7 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
     if (COUNT) {
         wave_add(&cnt[kCntGlobal + 3], tot.nodes);
         wave_add(&cnt[kCntGlobal + 4], tot.tris);
+        wave_add(&cnt[kCntGlobal + 6], tot.insts);
     }
     if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
@@ -183,7 +189,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                                               float4* __restrict__ nee_b, uint8_t* __restrict__ vis,
                                               float4* __restrict__ sh_o, float4* __restrict__ sh_d,
                                               float4* __restrict__ albedo_out, float4* __restrict__ normal_out,
-                                              unsigned long long* __restrict__ cnt) {
+                                              uint32_t* __restrict__ ext_queue, unsigned long long* __restrict__ cnt) {
     uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t iters = (n + stride - 1) / stride;
@@ -193,7 +199,12 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
         uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool active = q < n;
         uint32_t p = active ? queue[q] : 0;
-        uint32_t depth = active ? __float_as_uint(thr[p].w) : 0;
+        const float4 t4 = active ? thr[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t depth = __float_as_uint(t4.w);
+        // The path is traced at the next bounce unless k_connect ends it for
+        // a reason already known here (miss, emitter, zero throughput, depth):
+        // only Russian roulette, decided in k_connect, can still end it.
+        bool ext = false;
         bool aov = active && depth == 0 && F.s0 == 0 && p < F.WH;  // sample 0, first segment
         if (active) {
             float4 h = hit[p];
@@ -385,7 +396,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                         if (!(NdotL > 0.0f)) continue;
                         if (NEE_INLINE) {
                             THit hh;
-                            TCount cc{0, 0};
+                            TCount cc{0, 0, 0};
                             inline_shadow++;
                             bool occ = traverse<true, false>(S, worldPos, Ld, 0.001f, maxd, false, 0u, hh, cc, ovf);
                             if (!occ) {
@@ -416,6 +427,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                     ray_d[p] = f4(L, 0.0f);
                     em[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
                     rw[p] = f4(weight, 0.0f);
+                    ext = !all_zero(xyz(t4) * weight) && depth + 1u <= F.pc.path_depth;  // as in k_connect
                     if (aov) {
                         albedo_out[p] = f4(base, 1.0f);
                         normal_out[p] = f4(N, 1.0f);
@@ -424,6 +436,8 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
             }
             seed[p] = rs;
         }
+        const uint32_t at = wave_append(&cnt[(bounce + 1) * kCntPerBounce + kCntExtQueue], ext);
+        if (ext) ext_queue[at] = p;
     }
     if (NEE_INLINE) {
         wave_add(&cnt[kCntGlobal + 2], inline_shadow);
@@ -436,7 +450,7 @@ __global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t
                                                unsigned long long* __restrict__ cnt, uint32_t refill) {
     const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 1], (unsigned long long)cap);
     uint32_t ovf = 0;
-    TCount tot{0, 0};
+    TCount tot{0, 0, 0};
     auto load = [&](uint32_t q, TRay& r) __attribute__((always_inline)) {
         const float4 o = sh_o[q], d = sh_d[q];
         r.o = xyz(o);
@@ -533,6 +547,8 @@ __global__ void k_next_count(int bounce, unsigned long long* cnt) {
     unsigned long long nx = cnt[bounce * kCntPerBounce + 2];
     cnt[(bounce + 1) * kCntPerBounce + 0] = nx;
     cnt[kCntGlobal + 1] += nx;
+    // next-bounce rays traced for paths Russian roulette then ended
+    cnt[kCntGlobal + 7] += cnt[(bounce + 1) * kCntPerBounce + kCntExtQueue] - nx;
     cnt[kCntGlobal + 2] += cnt[bounce * kCntPerBounce + 1];
 }
 
@@ -598,16 +614,22 @@ template <class T> T* dmalloc(size_t n) {
 }  // namespace
 
 Renderer::~Renderer() {
+    if (st2) {
+        (void)hipStreamSynchronize(st2);
+        (void)hipStreamDestroy(st2);
+    }
     for (auto e : ev) (void)hipEventDestroy(e);
+    for (auto e : sync_ev) (void)hipEventDestroy(e);
     void* ptrs[] = {d_rgba, d_albedo, d_normal, d_bgra, ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed,
-                    queue_a, queue_b, sh_o, sh_d, nee_a, nee_b, vis, samples, counters, d_lights, d_jump, d_acc, d_out};
+                    queue_a, queue_b, queue_e, sh_o, sh_d, nee_a, nee_b, vis, samples, counters, d_lights, d_jump,
+                    d_acc, d_out};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
 
 void Renderer::ensure(size_t paths, uint32_t nslots, uint32_t spp) {
     if (paths > cap || nslots > slots) {
-        void* ptrs[] = {ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed, queue_a, queue_b,
+        void* ptrs[] = {ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed, queue_a, queue_b, queue_e,
                         sh_o, sh_d, nee_a, nee_b, vis};
         for (void* p : ptrs)
             if (p) KF_CHECK(hipFree(p));
@@ -624,6 +646,7 @@ void Renderer::ensure(size_t paths, uint32_t nslots, uint32_t spp) {
         seed = dmalloc<uint32_t>(cap);
         queue_a = dmalloc<uint32_t>(cap);
         queue_b = dmalloc<uint32_t>(cap);
+        queue_e = dmalloc<uint32_t>(cap);
         sh_o = dmalloc<float4>(cap * slots);
         sh_d = dmalloc<float4>(cap * slots);
         nee_a = dmalloc<float4>(cap * slots);
@@ -767,6 +790,9 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
         if (r->ev.empty()) {
             r->ev.resize(2 + 4 * kMaxBounces);
             for (auto& e : r->ev) KF_CHECK(hipEventCreate(&e));
+            r->sync_ev.resize(2 * kMaxBounces);
+            for (auto& e : r->sync_ev) KF_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            KF_CHECK(hipStreamCreateWithFlags(&r->st2, hipStreamNonBlocking));  // default priority
         }
         int dev = 0;
         KF_CHECK(hipGetDevice(&dev));
@@ -779,7 +805,7 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
         static const uint32_t grid_ext1 = resident_grid(k_extend<true>, n_cu);
         static const uint32_t grid_sh = resident_grid(k_shadow, n_cu);
 
-        unsigned long long totals[6] = {0, 0, 0, 0, 0, 0};
+        unsigned long long totals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         uint32_t n_ext = 0, n_sh = 0;
         float ms_total = 0.f, ms_ext = 0.f, ms_sh = 0.f;
         for (uint32_t s0 = 0; s0 < pc->spp; s0 += spb) {
@@ -793,31 +819,47 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
             uint32_t* qa = r->queue_a;
             uint32_t* qb = r->queue_b;
             uint32_t nb = pc->path_depth + 1;
-            for (uint32_t b = 0; b < nb; ++b) {
+            // Per bounce b:  A: shade(b) -> [B: shadow(b)] , A: extend(b+1)
+            // -> (wait B) connect(b).  k_shade lists the paths that continue
+            // unless Russian roulette ends them, so the closest-hit rays of
+            // bounce b+1 are traced while the shadow rays of bounce b are,
+            // on the second stream; k_connect then waits for both.
+            auto launch_extend = [&](uint32_t b, const uint32_t* queue) {
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
                 if (o.count_traversal)
-                    k_extend<true><<<grid_ext1, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
-                                                         r->hit_prim, r->counters, refill);
+                    k_extend<true><<<grid_ext1, TB, 0, st>>>(S, (int)b, B, queue, r->ray_o, r->ray_d, r->seed, r->hit,
+                                                             r->hit_prim, r->counters, refill, WH, F.s0, F.S0);
                 else
-                    k_extend<false><<<grid_ext0, TB, 0, st>>>(S, (int)b, B, qa, r->ray_o, r->ray_d, r->seed, r->hit,
-                                                          r->hit_prim, r->counters, refill);
+                    k_extend<false><<<grid_ext0, TB, 0, st>>>(S, (int)b, B, queue, r->ray_o, r->ray_d, r->seed,
+                                                              r->hit, r->hit_prim, r->counters, refill, WH, F.s0,
+                                                              F.S0);
                 KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 1], st));
                 ++n_ext;
+            };
+            launch_extend(0, qa);
+            for (uint32_t b = 0; b < nb; ++b) {
                 if (inline_nee)
                     k_shade<true><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->hit,
                                                        r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis,
-                                                       r->sh_o, r->sh_d, r->d_albedo, r->d_normal, r->counters);
+                                                       r->sh_o, r->sh_d, r->d_albedo, r->d_normal, r->queue_e,
+                                                       r->counters);
                 else
                     k_shade<false><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr,
                                                         r->hit, r->hit_prim, r->rw, r->em, r->seed, r->nee_a,
                                                         r->nee_b, r->vis, r->sh_o, r->sh_d, r->d_albedo, r->d_normal,
-                                                        r->counters);
-                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], st));
+                                                        r->queue_e, r->counters);
                 if (!inline_nee) {
-                    k_shadow<<<grid_sh, TB, 0, st>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->vis, r->counters, refill);
+                    KF_CHECK(hipEventRecord(r->sync_ev[2 * b], st));
+                    KF_CHECK(hipStreamWaitEvent(r->st2, r->sync_ev[2 * b], 0));
+                    KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], r->st2));
+                    k_shadow<<<grid_sh, TB, 0, r->st2>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->vis, r->counters,
+                                                         refill);
+                    KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], r->st2));
+                    KF_CHECK(hipEventRecord(r->sync_ev[2 * b + 1], r->st2));
                     ++n_sh;
                 }
-                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], st));
+                if (b + 1 < nb) launch_extend(b + 1, r->queue_e);
+                if (!inline_nee) KF_CHECK(hipStreamWaitEvent(st, r->sync_ev[2 * b + 1], 0));
                 if (inline_nee)
                     k_connect<true><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
                                                          r->nee_a, r->nee_b, r->vis, d_out, r->counters);
@@ -835,14 +877,14 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
             unsigned long long h[kCntSize];
             KF_CHECK(hipMemcpyAsync(h, r->counters, sizeof(h), hipMemcpyDeviceToHost, st));
             KF_CHECK(hipStreamSynchronize(st));
-            for (int k = 0; k < 6; ++k) totals[k] += h[kCntGlobal + k];
+            for (int k = 0; k < 8; ++k) totals[k] += h[kCntGlobal + k];
             float ms = 0.f;
             KF_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
             ms_total += ms;
             for (uint32_t b = 0; b < nb; ++b) {
                 float a = 0.f, c = 0.f;
                 KF_CHECK(hipEventElapsedTime(&a, r->ev[2 + 4 * b + 0], r->ev[2 + 4 * b + 1]));
-                KF_CHECK(hipEventElapsedTime(&c, r->ev[2 + 4 * b + 2], r->ev[2 + 4 * b + 3]));
+                if (!inline_nee) KF_CHECK(hipEventElapsedTime(&c, r->ev[2 + 4 * b + 2], r->ev[2 + 4 * b + 3]));
                 ms_ext += a;
                 ms_sh += c;
             }
@@ -854,6 +896,8 @@ extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, 
             stats->shadow_rays = totals[2];
             stats->node_visits = totals[3];
             stats->tri_tests = totals[4];
+            stats->instance_visits = totals[6];
+            stats->speculative_rays = totals[7];
             stats->trace_ms = ms_total;
             stats->extend_ms = ms_ext;
             stats->shadow_ms = ms_sh;

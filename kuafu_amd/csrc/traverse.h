@@ -102,24 +102,6 @@ KF_DEV bool intersect(const Pre& p, float4 v0, float4 v1, float4 v2, float tmin,
     return !outside && det != 0.0f && t >= tmin && t <= tmax;
 }
 
-// Conservative slab test of the two child boxes of a node.
-KF_DEV void slab2(const Node& n, v3 o, v3 invd, float tmin, float tmax, bool& hl, bool& hr, float& tl, float& tr) {
-    float l0x = (n.a.x - o.x) * invd.x, l1x = (n.a.y - o.x) * invd.x;
-    float l0y = (n.a.z - o.y) * invd.y, l1y = (n.a.w - o.y) * invd.y;
-    float l0z = (n.c.x - o.z) * invd.z, l1z = (n.c.y - o.z) * invd.z;
-    float r0x = (n.b.x - o.x) * invd.x, r1x = (n.b.y - o.x) * invd.x;
-    float r0y = (n.b.z - o.y) * invd.y, r1y = (n.b.w - o.y) * invd.y;
-    float r0z = (n.c.z - o.z) * invd.z, r1z = (n.c.w - o.z) * invd.z;
-    float ln = fmaxf(fmaxf(fminf(l0x, l1x), fminf(l0y, l1y)), fmaxf(fminf(l0z, l1z), tmin));
-    float lf = fminf(fminf(fmaxf(l0x, l1x), fmaxf(l0y, l1y)), fminf(fmaxf(l0z, l1z), tmax));
-    float rn = fmaxf(fmaxf(fminf(r0x, r1x), fminf(r0y, r1y)), fmaxf(fminf(r0z, r1z), tmin));
-    float rf = fminf(fminf(fmaxf(r0x, r1x), fmaxf(r0y, r1y)), fminf(fmaxf(r0z, r1z), tmax));
-    hl = ln <= lf * kSlabScale;
-    hr = rn <= rf * kSlabScale;
-    tl = ln;
-    tr = rn;
-}
-
 KF_DEV float alpha_coin(uint32_t seed, uint32_t inst, uint32_t prim) {
     uint32_t h = tea(seed ^ (inst * 0x9e3779b9u), prim);
     return float(h & 0x00FFFFFFu) / float(0x01000000);
@@ -131,7 +113,7 @@ struct THit {
 };
 
 struct TCount {
-    uint32_t nodes, tris;
+    uint32_t nodes, tris, insts;  // BVH4 node records, triangle tests, instance records
 };
 
 KF_DEV v3 xform_point(const Inst& I, v3 v) {
@@ -157,9 +139,6 @@ enum : uint32_t { kTlasNode = 0u, kBlasNode = 1u, kInstance = 2u, kTriangle = 3u
 constexpr uint32_t kIdxMask = 0x3fffffffu;  // scenes are limited to 2^30 records of each kind
 
 KF_DEV uint32_t make_item(uint32_t kind, uint32_t idx) { return (kind << 30) | (idx & kIdxMask); }
-// A child ref of a node of kind k (0 = TLAS, 1 = BLAS): >= 0 is a node of the
-// same kind, < 0 is ~leaf (instance for the TLAS, triangle slot for a BLAS).
-KF_DEV uint32_t child_item(uint32_t k, int ref) { return ref >= 0 ? make_item(k, (uint32_t)ref) : make_item(k + 2u, (uint32_t)~ref); }
 
 // Record arrays of the traversal.  Where a lane chooses between them by the
 // kind of its record, each field passes through an empty asm first: that
@@ -174,13 +153,13 @@ KF_DEV T opaque_sgpr(T v) {
     return v;
 }
 
-__device__ const float4 kZeroRecord[4] = {};
+__device__ const float4 kZeroRecord[8] = {};
 
 struct TravScene {
     const float4 *tlas, *blas, *inst, *tri;
     uint32_t n_tlas, n_blas, n_inst, n_tri;
     int root;
-    const float4* zero;  // 64 zero bytes, read in place of an out-of-range record
+    const float4* zero;  // 128 zero bytes, read in place of an out-of-range record
     const kf_material* mats;
     const uint32_t *matidx, *toff;
     KF_DEV explicit TravScene(const SceneView& S)
@@ -251,7 +230,7 @@ KF_DEV uint32_t root_item(int root) {
 // coin (alpha_coin).
 template <bool ANY, bool COUNT>
 struct Trav {
-    v3 o, d, winvd;  // world-space ray
+    v3 o, d;         // world-space ray
     Pre obj;         // object-space ray of the current instance
     float tmin;
     THit best;
@@ -266,7 +245,7 @@ struct Trav {
     // Benign state for a lane without a query (it then reads only the zero
     // record until start() gives it one).
     KF_DEV void reset() {
-        o = d = winvd = mk(0.0f, 0.0f, 0.0f);
+        o = d = mk(0.0f, 0.0f, 0.0f);
         obj.o = obj.invd = o;
         obj.kx = 0;
         obj.ky = 1;
@@ -282,7 +261,7 @@ struct Trav {
         has_top = found = false;
         use_alpha = false;
         alpha_seed = steps = 0u;
-        cnt.nodes = cnt.tris = 0u;
+        cnt.nodes = cnt.tris = cnt.insts = 0u;
     }
 
     // Lanes with f take a new query r starting at item root (the scene is
@@ -291,7 +270,6 @@ struct Trav {
     KF_DEV void start(bool f, const TRay& r, uint32_t root, bool use_alpha_) {
         o = sel3(f, r.o, o);
         d = sel3(f, r.d, d);
-        winvd = sel3(f, rcp3(r.d), winvd);  // culling only
         tmin = f ? r.tmin : tmin;
         best.t = f ? r.tmax : best.t;
         best.inst = f ? ~0u : best.inst;
@@ -308,6 +286,7 @@ struct Trav {
         steps = f ? 0u : steps;
         cnt.nodes = f ? 0u : cnt.nodes;
         cnt.tris = f ? 0u : cnt.tris;
+        cnt.insts = f ? 0u : cnt.insts;
     }
 
     // One record; returns false when the query has finished.  stk: kStack
@@ -338,16 +317,20 @@ struct Trav {
         const bool bad = live && (idx >= lim || ++steps > kMaxSteps);
         const bool on = live && !bad;
         const float4* base = kind == kTlasNode ? p_tlas : kind == kBlasNode ? p_blas : kind == kInstance ? p_inst : p_tri;
-        const float4* P = on ? base + (kind == kTriangle ? 3u : 4u) * (size_t)idx : p_zero;
-        // Four 16-byte global loads issued together; triangles are 48 B, so
-        // their fourth load repeats the third (same cache line).  The empty
-        // asm needs all results, which keeps the compiler from sinking parts
-        // of the fetch into the kind-specific branches below (each sunk load
-        // would cost the wave another memory round trip).
+        const bool node_kind = kind <= kBlasNode;
+        const float4* P = on ? base + (node_kind ? 8u : (kind == kTriangle ? 3u : 4u)) * (size_t)idx : p_zero;
+        // Seven 16-byte global loads issued together.  BVH4 nodes are 128 B;
+        // instances (64 B) and triangles (48 B) repeat loads of their own
+        // first lines instead of reading past the record.  The empty asm
+        // needs all results, which keeps the compiler from sinking parts of
+        // the fetch into the kind-specific code below (each sunk load would
+        // cost the wave another memory round trip).
         const gf4* G = (const gf4*)P;
+        const bool wide = node_kind && on;
         const vf4 r0 = G[0], r1 = G[1], r2 = G[2], r3 = G[(kind == kTriangle && on) ? 2 : 3];
+        const vf4 r4 = G[wide ? 4 : 0], r5 = G[wide ? 5 : 1], r6 = G[wide ? 6 : 2];  // line 7 is padding
         const uint32_t below = stk.peek(sp);
-        asm volatile("" ::"v"(r0), "v"(r1), "v"(r2), "v"(r3), "v"(below));
+        asm volatile("" ::"v"(r0), "v"(r1), "v"(r2), "v"(r3), "v"(r4), "v"(r5), "v"(r6), "v"(below));
         const float4 q0 = make_float4(r0.x, r0.y, r0.z, r0.w), q1 = make_float4(r1.x, r1.y, r1.z, r1.w);
         const float4 q2 = make_float4(r2.x, r2.y, r2.z, r2.w), q3 = make_float4(r3.x, r3.y, r3.z, r3.w);
 
@@ -381,21 +364,44 @@ struct Trav {
         }
         found = found || take;
 
-        // node: two child boxes
-        Node n;
-        n.a = q0;
-        n.b = q1;
-        n.c = q2;
-        n.d = make_int4(__float_as_int(q3.x), __float_as_int(q3.y), 0, 0);
+        // node: four child boxes (Node4 layout: lox, hix, loy, hiy, loz, hiz, ref)
         const bool blas = kind == kBlasNode;
-        bool hl, hr;
-        float tl, tr;
-        slab2(n, sel3(blas, obj.o, o), sel3(blas, obj.invd, winvd), tmin, best.t, hl, hr, tl, tr);
-        const uint32_t li = child_item(kind & 1u, n.d.x), rj = child_item(kind & 1u, n.d.y);
-        // Nearer child first; any-hit queries take a leaf child (triangle or
-        // instance) before an inner one, which finds occluders sooner.
-        bool lfirst = !hr || (hl && tl <= tr);
-        if (ANY && hl && hr && (n.d.x < 0) != (n.d.y < 0)) lfirst = n.d.x < 0;
+        const v3 ro = sel3(blas, obj.o, o), ri = sel3(blas, obj.invd, rcp3(d));  // rcp: culling only
+        const float lox[4] = {r0.x, r0.y, r0.z, r0.w}, hix[4] = {r1.x, r1.y, r1.z, r1.w};
+        const float loy[4] = {r2.x, r2.y, r2.z, r2.w}, hiy[4] = {r3.x, r3.y, r3.z, r3.w};
+        const float loz[4] = {r4.x, r4.y, r4.z, r4.w}, hiz[4] = {r5.x, r5.y, r5.z, r5.w};
+        uint32_t citem[4] = {__float_as_uint(r6.x), __float_as_uint(r6.y), __float_as_uint(r6.z), __float_as_uint(r6.w)};
+        float key[4];
+        int nhit = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float l0x = (lox[c] - ro.x) * ri.x, l1x = (hix[c] - ro.x) * ri.x;
+            const float l0y = (loy[c] - ro.y) * ri.y, l1y = (hiy[c] - ro.y) * ri.y;
+            const float l0z = (loz[c] - ro.z) * ri.z, l1z = (hiz[c] - ro.z) * ri.z;
+            const float tn = fmaxf(fmaxf(fminf(l0x, l1x), fminf(l0y, l1y)), fmaxf(fminf(l0z, l1z), tmin));
+            const float tf = fminf(fminf(fmaxf(l0x, l1x), fmaxf(l0y, l1y)), fminf(fmaxf(l0z, l1z), best.t));
+            const bool h = tn <= tf * kSlabScale && citem[c] != kEmptyItem;
+            // Nearer child first; any-hit queries take leaf children
+            // (triangles, instances: kinds 2 and 3) before inner ones, which
+            // finds occluders sooner.
+            key[c] = h ? ((ANY && (citem[c] >> 31)) ? -INFINITY : tn) : INFINITY;
+            nhit += h ? 1 : 0;
+        }
+        // sorting network on (key, item), ascending
+        auto cx = [&](int i, int j) {
+            const bool sw = key[j] < key[i];
+            const float ki = key[i], kj = key[j];
+            const uint32_t ii = citem[i], ij = citem[j];
+            key[i] = sw ? kj : ki;
+            key[j] = sw ? ki : kj;
+            citem[i] = sw ? ij : ii;
+            citem[j] = sw ? ii : ij;
+        };
+        cx(0, 1);
+        cx(2, 3);
+        cx(0, 2);
+        cx(1, 3);
+        cx(1, 2);
 
         // instance: enter its BLAS with the object-space ray
         const int root = __float_as_int(q3.x);
@@ -421,28 +427,38 @@ struct Trav {
         const uint32_t inst_root = root >= 0 ? make_item(kBlasNode, (uint32_t)root) : make_item(kTriangle, (uint32_t)~root);
 
         if (COUNT) {
-            cnt.nodes += (is_node || is_inst) ? 1u : 0u;
+            cnt.nodes += is_node ? 1u : 0u;
             cnt.tris += is_tri ? 1u : 0u;
+            cnt.insts += is_inst ? 1u : 0u;
         }
-        const int node_act = (hl && hr) ? kPush : ((hl || hr) ? kNext : kPop);
+        const int node_act = nhit >= 2 ? kPush : (nhit == 1 ? kNext : kPop);
         int act = is_node ? node_act : (enter ? kNext : ((ANY && take) ? kDone : kPop));
-        const uint32_t next = is_node ? (lfirst ? li : rj) : inst_root;
-        const uint32_t pushed = lfirst ? rj : li;
+        const uint32_t next = is_node ? citem[0] : inst_root;
         if (bad) {
             overflow = 1;
             act = kDone;
         }
 
-        // Stack update, resolved with selects.
+        // Stack update, resolved with selects.  A node with k >= 2 hit
+        // children pushes the k - 1 farther ones, farthest first, so the
+        // nearest of them ends up as the cached top.
         const bool push = act == kPush, popping = act == kPop;
-        if (push && has_top) stk.push(sp, top, overflow);
         if (!live) act = kDone;
         const bool alive = act != kDone && (!popping || has_top);
-        cur = popping ? top : next;
+        const uint32_t popped_item = top;
+#pragma unroll
+        for (int j = 3; j >= 1; --j) {
+            const bool pj = push && j < nhit;
+            if (pj && has_top) stk.push(sp, top, overflow);
+            sp = (pj && has_top && sp < Stack::kCap) ? sp + 1 : sp;
+            top = pj ? citem[j] : top;
+            has_top = has_top || pj;
+        }
+        cur = popping ? popped_item : next;
         const int sp_in = sp;
-        sp = push ? (has_top && sp_in < Stack::kCap ? sp_in + 1 : sp_in) : (popping && sp_in > 0 ? sp_in - 1 : sp_in);
-        top = push ? pushed : (popping ? below : top);
-        has_top = push ? true : (popping ? sp_in > 0 : has_top);
+        sp = (popping && sp_in > 0) ? sp_in - 1 : sp_in;
+        top = popping ? below : top;
+        has_top = popping ? sp_in > 0 : has_top;
         if (popping && sp_in > 0) stk.popped(sp_in - 1);
         return alive;
     }
@@ -470,6 +486,7 @@ KF_DEV bool traverse(const SceneView& S, v3 o, v3 d, float tmin, float tmax, boo
     best = T.best;
     cnt.nodes += T.cnt.nodes;
     cnt.tris += T.cnt.tris;
+    cnt.insts += T.cnt.insts;
     return T.found;
 }
 
@@ -513,10 +530,6 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
     __shared__ uint32_t ring[kRing * kRingThreads];
     RingStack stk;
     stk.ring = ring + threadIdx.x;
-    TRay r;  // last ray fetched by this lane
-    r.o = r.d = mk(0.0f, 0.0f, 0.0f);
-    r.tmin = r.tmax = 0.0f;
-    r.alpha_seed = r.payload = 0u;
     bool live = false;
     uint32_t payload = 0;
     const uint32_t lane = threadIdx.x & 63u;
@@ -528,6 +541,10 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
         const unsigned long long idle = __ballot(!live);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         bool fresh = false;
+        TRay r;  // the ray this lane fetches in this iteration, if any
+        r.o = r.d = mk(0.0f, 0.0f, 0.0f);
+        r.tmin = r.tmax = 0.0f;
+        r.alpha_seed = r.payload = 0u;
         if ((res_n > 0u || moved < 8u) && (nidle >= refill || nidle == 64u)) {
             const uint32_t rank = (uint32_t)__popcll(idle & below);
             const bool wants = !live;
@@ -561,9 +578,11 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
                 res_n -= take;
             }
         }
-        T.start(fresh, r, root, use_alpha);
-        payload = fresh ? r.payload : payload;
-        live = live || fresh;
+        if (__ballot(fresh) != 0ull) {  // wave-uniform: most iterations fetch nothing
+            T.start(fresh, r, root, use_alpha);
+            payload = fresh ? r.payload : payload;
+            live = live || fresh;
+        }
         if (__ballot(live) == 0ull) {
             if (moved >= 8u && res_n == 0u) break;
             continue;
@@ -575,6 +594,7 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
             if (COUNT) {
                 total.nodes += T.cnt.nodes;
                 total.tris += T.cnt.tris;
+                total.insts += T.cnt.insts;
             }
         }
     }

@@ -869,6 +869,7 @@ void render_pixel(const ko_scene& s, const kf_camera_ubo& cam, const kf_push_con
     for (uint32_t i = 0; i < pc.spp; ++i) {
         Payload<T> ray;
         ray.seed = tea(pixel, S0 + 1u + i);
+        const uint32_t sample_seed = ray.seed;
         T r0 = T(rnd(seed));
         T r1 = T(rnd(seed));
         T posx = T(float(x)) + r0, posy = T(float(y)) + r1;
@@ -910,7 +911,10 @@ void render_pixel(const ko_scene& s, const kf_camera_ubo& cam, const kf_push_con
         for (ray.depth = 0; ray.depth <= maxDepth; ++ray.depth) {
             if (ray.depth == 0) cnt.primary++; else cnt.extension++;
             Hit<T> h;
-            bool hit = trace(s, ray.origin, ray.dir, T(0.001f), T(10000.0f), h, false, true, ray.seed, false);
+            // alpha coin seed: tea(initial seed of the sample's stream, depth),
+            // independent of how many numbers the path has drawn so far
+            bool hit = trace(s, ray.origin, ray.dir, T(0.001f), T(10000.0f), h, false, true,
+                             tea(sample_seed, ray.depth), false);
             shade_segment(s, pc, h, hit, ray, cnt);
             color = color + ray.emission * weight;
             weight = weight * ray.weight;

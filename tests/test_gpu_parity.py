@@ -82,6 +82,94 @@ def test_blas_invariants(gpu):
     assert info.triangles == 5120 + 2 + 1 and info.max_blas_depth == d
 
 
+EMPTY_ITEM = 0xFFFFFFFF
+
+
+def walk_bvh4(nodes, root, leaf_box_check, inner_kind):
+    """Visit a BVH4 from root: every node once, returns the leaf indices seen.
+    Children are items kind << 30 | index (inner_kind for inner nodes,
+    inner_kind + 2 for leaves).  leaf_box_check(leaf, lo, hi) asserts the
+    leaf lies in its child box."""
+    items = nodes[:, 24:28].copy().view(np.uint32)
+    seen = np.zeros(len(nodes), int)
+    leaves = []
+    stack = [root]
+    while stack:
+        r = stack.pop()
+        seen[r] += 1
+        n = nodes[r]
+        lo = np.stack([n[0:4], n[8:12], n[16:20]], 1).astype(np.float64)
+        hi = np.stack([n[4:8], n[12:16], n[20:24]], 1).astype(np.float64)
+        used = items[r] != EMPTY_ITEM
+        assert used.sum() >= 2
+        assert used[:used.sum()].all(), "unused child slots come last"
+        for c in range(4):
+            if not used[c]:
+                continue
+            kind, idx = int(items[r, c]) >> 30, int(items[r, c]) & 0x3FFFFFFF
+            assert kind in (inner_kind, inner_kind + 2)
+            assert (lo[c] <= hi[c]).all()
+            if kind == inner_kind:
+                child = nodes[idx]
+                clo = np.stack([child[0:4], child[8:12], child[16:20]], 1)
+                chi = np.stack([child[4:8], child[12:16], child[20:24]], 1)
+                cu = items[idx] != EMPTY_ITEM
+                assert (clo[cu] >= lo[c]).all() and (chi[cu] <= hi[c]).all()
+                stack.append(idx)
+            else:
+                leaves.append(idx)
+                leaf_box_check(idx, lo[c], hi[c])
+    return seen, leaves
+
+
+def test_bvh4_invariants(gpu):
+    """The BVH4 the traversal reads: every triangle slot of a geometry in
+    exactly one leaf whose box holds the triangle, every node reached once,
+    fewer nodes than the BVH2; the TLAS BVH4 holds every instance once."""
+    s = scenes.SceneData()
+    v, t = scenes.icosphere(4)
+    s.add_geometry(v, t, scenes.material())
+    s.add_geometry(*scenes.quad(1.0), scenes.material())
+    one_v, one_t = scenes.quad(1.0)
+    s.add_geometry(one_v, one_t[:1], scenes.material())
+    cube_v, cube_t = scenes.box((0.5, 0.5, 0.5))
+    s.add_geometry(cube_v, cube_t, scenes.material())
+    rng = np.random.default_rng(3)
+    for i in range(37):
+        m = np.eye(4, dtype=np.float32)
+        m[:3, 3] = rng.uniform(-5, 5, 3)
+        s.add_instance(i % 4, m)
+    gs = GpuScene(s)
+    nodes4, roots4 = gs.read_bvh4(0)
+    info = gs.bvh_info()
+    assert info.blas4_nodes == len(nodes4) and info.blas4_nodes < info.blas_nodes
+    toff = np.cumsum([0] + [len(g[1]) for g in s.geoms])
+    all_seen = np.zeros(len(nodes4), int)
+    for g, (verts, tris, _, _) in enumerate(s.geoms):
+        nt = len(tris)
+        _, _, order = gs.read_blas(g)
+        pos = verts["pos"].astype(np.float64)
+        if nt == 1:
+            assert roots4[g] == ~int(toff[g])
+            continue
+
+        def tri_in_box(slot, lo, hi):
+            local = slot - toff[g]
+            assert 0 <= local < nt
+            p = pos[tris[order[local]]]
+            assert (p.min(0) >= lo - 1e-7).all() and (p.max(0) <= hi + 1e-7).all()
+
+        seen, leaves = walk_bvh4(nodes4, int(roots4[g]), tri_in_box, 1)
+        all_seen += seen
+        assert sorted(leaves) == list(range(toff[g], toff[g] + nt))
+    assert (all_seen == 1).all()
+    # TLAS
+    tl, troot = gs.read_bvh4(1)
+    assert info.tlas4_nodes == len(tl) and len(tl) < 36
+    seen, leaves = walk_bvh4(tl, int(troot[0]), lambda i, lo, hi: None, 0)
+    assert (seen == 1).all() and sorted(leaves) == list(range(37))
+
+
 def rays_toward(rng, n, center, spread, origin_box):
     r = np.zeros(n, abi.RAY_DTYPE)
     r["org"] = rng.uniform(-origin_box, origin_box, (n, 3))
@@ -195,6 +283,23 @@ def test_render_parity_tiny_lights(gpu, soft):
     cam = camera_ubo((0.0, 2.5, 5.0), (0, -0.4, -1.0), (0, 1, 0), 48, 40, fx=50.0, fy=50.0)
     pc = push_constants(spp=3, path_depth=6, clear_color=(0.8, 0.9, 1.0, 0.5))
     w = scenes.Workload("tiny", data, cam, pc, 48, 40)
+    (res,) = render_both(w)
+    assert_frame_parity(*res)
+
+
+def test_render_parity_alpha_tested_geometry(gpu):
+    # non-opaque spheres with alpha 0.5 and 0.2 run the any-hit alpha test
+    # on path rays; the coin is seeded by (sample stream, bounce) and so is
+    # the same whatever order the traversal meets the triangles in
+    data = scenes.tiny_scene(2, n_spheres=2)
+    sv, si = scenes.uv_sphere(12, 8, 0.6)
+    for a, pos in ((0.5, (-0.6, 0.7, 0.8)), (0.2, (0.7, 0.6, 1.2))):
+        g = data.add_geometry(sv, si, scenes.material(diffuse=(0.3, 0.8, 0.4), alpha=a), opaque=False)
+        data.add_instance(g, scenes.mat_from_rows(np.eye(3), pos))
+    from kuafu_amd.api import camera_ubo
+    cam = camera_ubo((0.0, 2.0, 5.0), (0, -0.3, -1.0), (0, 1, 0), 48, 40, fx=50.0, fy=50.0)
+    pc = push_constants(spp=4, path_depth=5, clear_color=(0.8, 0.9, 1.0, 0.5))
+    w = scenes.Workload("alpha", data, cam, pc, 48, 40)
     (res,) = render_both(w)
     assert_frame_parity(*res)
 
