@@ -75,6 +75,29 @@ KF_DEV uint32_t wave_append(unsigned long long* counter, bool pred) {
     return base + (uint32_t)__popcll(m & lt);
 }
 
+// Block-aggregated queue append: one atomic per TB-thread block instead of
+// one per wave (appends to a single counter serialise in the memory system).
+// Every thread of the block must call it at the same point.
+KF_DEV uint32_t block_append(unsigned long long* counter, bool pred) {
+    __shared__ uint32_t s_wave[TB / 64];
+    __shared__ uint32_t s_base;
+    const unsigned long long m = __ballot(pred);
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    if (lane == 0) s_wave[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < TB / 64; ++i) tot += s_wave[i];
+        s_base = tot ? (uint32_t)atomicAdd(counter, (unsigned long long)tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = s_base;
+    for (uint32_t i = 0; i < w; ++i) off += s_wave[i];
+    const uint32_t r = off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();  // s_wave and s_base are reused by the next call
+    return r;
+}
+
 __global__ void k_zero(unsigned long long* c, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) c[i] = 0;
@@ -194,7 +217,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t iters = (n + stride - 1) / stride;
     const uint32_t K = F.slots;
-    uint32_t inline_shadow = 0, ovf = 0;
+    uint32_t n_shadow = 0, ovf = 0;  // shadow rays of this thread
     for (uint32_t it = 0; it < iters; ++it) {
         uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool active = q < n;
@@ -397,7 +420,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                         if (NEE_INLINE) {
                             THit hh;
                             TCount cc{0, 0, 0};
-                            inline_shadow++;
+                            n_shadow++;
                             bool occ = traverse<true, false>(S, worldPos, Ld, 0.001f, maxd, false, 0u, hh, cc, ovf);
                             if (!occ) {
                                 v3 ca, cb;
@@ -420,6 +443,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                             uint32_t at = (uint32_t)atomicAdd(&cnt[bounce * kCntPerBounce + 1], 1ull);
                             sh_o[at] = f4(worldPos, __uint_as_float((uint32_t)slot));
                             sh_d[at] = f4(Ld, maxd);
+                            n_shadow++;
                         }
                     }
                     if (NEE_INLINE) nee_a[(size_t)p * K] = f4((sum_d + sum_p) + sum_a, 0.0f);
@@ -436,13 +460,11 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
             }
             seed[p] = rs;
         }
-        const uint32_t at = wave_append(&cnt[(bounce + 1) * kCntPerBounce + kCntExtQueue], ext);
+        const uint32_t at = block_append(&cnt[(bounce + 1) * kCntPerBounce + kCntExtQueue], ext);
         if (ext) ext_queue[at] = p;
     }
-    if (NEE_INLINE) {
-        wave_add(&cnt[kCntGlobal + 2], inline_shadow);
-        if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
-    }
+    wave_add(&cnt[kCntGlobal + 2], n_shadow);
+    if (NEE_INLINE && ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
 
 __global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t cap, const float4* __restrict__ sh_o,
@@ -538,7 +560,7 @@ __global__ __launch_bounds__(TB) void k_connect(Frame F, int bounce, const uint3
                 out[p] = f4(color, 0.0f);
             }
         }
-        uint32_t at = wave_append(&cnt[bounce * kCntPerBounce + 2], cont);
+        uint32_t at = block_append(&cnt[bounce * kCntPerBounce + 2], cont);
         if (cont) next[at] = p;
     }
 }
@@ -549,7 +571,6 @@ __global__ void k_next_count(int bounce, unsigned long long* cnt) {
     cnt[kCntGlobal + 1] += nx;
     // next-bounce rays traced for paths Russian roulette then ended
     cnt[kCntGlobal + 7] += cnt[(bounce + 1) * kCntPerBounce + kCntExtQueue] - nx;
-    cnt[kCntGlobal + 2] += cnt[bounce * kCntPerBounce + 1];
 }
 
 // Sum the batch's samples into acc in sample order (rgen:140), then on the

@@ -581,7 +581,11 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
         if (__ballot(fresh) != 0ull) {  // wave-uniform: most iterations fetch nothing
             T.start(fresh, r, root, use_alpha);
             payload = fresh ? r.payload : payload;
-            live = live || fresh;
+            // An empty t range (tmax < tmin, or NaN) has nothing to trace:
+            // finish it at once as a miss.
+            const bool empty = fresh && !(r.tmin <= r.tmax);
+            if (empty) store(r.payload, T.best, false);
+            live = live || (fresh && !empty);
         }
         if (__ballot(live) == 0ull) {
             if (moved >= 8u && res_n == 0u) break;
