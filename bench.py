@@ -27,24 +27,28 @@ METRIC = "Mrays/s at 1080p; time-to-relMSE 1e-2, denoised vs plain spp, 1 GPU"
 
 
 def cpu_baseline(workload, budget_s=15.0):
-    """Time the FP32 oracle on a band of rows (bounded CPU sample)."""
+    """Time the FP32 oracle on bands of rows of the same frame (bounded CPU
+    sample): 64-row bands from the middle of the frame, wrapping around, until
+    about budget_s seconds of rendering have run."""
     import oracle
     threads = min(os.cpu_count() or 1, 16)
     sc = oracle.Scene(workload.data)
     W, H = workload.width, workload.height
-    y0 = H // 2
-    # calibrate on 2 rows, then size the band to about budget_s
-    t = time.perf_counter()
-    r = sc.render(workload.cam, workload.pc, W, H, rows=(y0, y0 + 2), threads=threads)
-    dt = time.perf_counter() - t
-    rows = int(max(2, min(H // 2, 2 * budget_s / max(dt, 1e-3))))
-    t = time.perf_counter()
-    r = sc.render(workload.cam, workload.pc, W, H, rows=(y0, y0 + rows), threads=threads)
-    dt = time.perf_counter() - t
-    rays = int(r["counts"].sum())
+    band = min(64, H)
+    y = (H // 2) // band * band
+    rays, dt, rows = 0, 0.0, 0
+    while dt < budget_s:
+        y1 = min(y + band, H)
+        t = time.perf_counter()
+        r = sc.render(workload.cam, workload.pc, W, H, rows=(y, y1), threads=threads)
+        dt += time.perf_counter() - t
+        rays += int(r["counts"].sum())
+        rows += y1 - y
+        y = 0 if y1 >= H else y1
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": "FP32 CPU oracle (restatement of the reference shaders), rows %d-%d of the %dx%d frame, "
-                      "%d spp, %d rays in %.1f s" % (y0, y0 + rows, W, H, workload.pc.spp, rays, dt)}
+            "sample": "FP32 CPU oracle (restatement of the reference shaders), %d rows of the %dx%d frame in "
+                      "64-row bands from row %d, %d spp, %d rays in %.1f s" % (rows, W, H, (H // 2) // band * band,
+                                                                                workload.pc.spp, rays, dt)}
 
 
 def relmse(x, y):
