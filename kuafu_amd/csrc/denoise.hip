@@ -49,70 +49,115 @@ __host__ __device__ inline float bf2f(uint16_t b) {
 
 // ---------------------------------------------------------------- convolution
 // y[p, co] = sum_{tap, ci} x[p + off(tap), ci] * w[tap, co, ci]   (zero padding)
-// Block tile 64 pixels x 64 output channels, K step 32 (one tap, 32 input
-// channels).  Wave w owns pixels 16w..16w+15 and all 64 channels (4 MFMA
-// tiles).  Cin must be a multiple of 32; ldx, ldy multiples of 8.
-constexpr int CBM = 64, CBN = 64, CBK = 32, CLD = CBK + 8;
+// Block tile: 32 x 8 pixels of one image x BN output channels (BN = 16 or
+// 32, chosen from the channels actually stored).  Per 32-channel input chunk
+// the block stages in LDS the 34 x 10 input halo of its tile and the weights
+// of all 9 taps, so one barrier pair covers 9 taps x 4 x BN/16 MFMAs per
+// wave.  Wave w owns pixel rows 2w and 2w + 1 (four 16-pixel MFMA tiles).
+// Cin must be a multiple of 32; ldx, ldy multiples of 8.
+constexpr int CBK = 32, CLD = CBK + 8;
+constexpr int TX = 32, TY = 8, HXX = TX + 2, HYY = TY + 2, NHALO = HXX * HYY;
 
+template <int BN>
 __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int ldx, int Cin,
                                              const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
                                              int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
                                              float* __restrict__ Yf, int ldyf, int cstore,
                                              const uint16_t* __restrict__ M, int ldm, int relu) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[CBM * CLD];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[CBN * CLD];
-    const int P = N * H * W;
-    const int p0 = blockIdx.x * CBM, co0 = blockIdx.y * CBN;
+    __shared__ __attribute__((aligned(16))) uint16_t Hs[NHALO * CLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Ws[9 * BN * CLD];
+    const int tiles_x = (W + TX - 1) / TX, tiles_y = (H + TY - 1) / TY;
+    int bi = blockIdx.x;
+    const int x0 = (bi % tiles_x) * TX;
+    bi /= tiles_x;
+    const int y0 = (bi % tiles_y) * TY;
+    const int n_ = bi / tiles_y;
+    const int co0 = blockIdx.y * BN;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    f32x4 acc[4];
-    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int lr = t >> 2, lk = (t & 3) * 8;
-    const int p = p0 + lr;
-    const bool pv = p < P;
-    int n_ = 0, y = 0, x = 0;
-    if (pv) {
-        n_ = p / (H * W);
-        int rem = p - n_ * H * W;
-        y = rem / W;
-        x = rem - y * W;
-    }
-    const int co = co0 + lr;
-    const bool cv = co < Cout;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
     const uint4 zero = make_uint4(0, 0, 0, 0);
-    for (int tap = 0; tap < 9; ++tap) {
-        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-        const bool in = pv && yy >= 0 && yy < H && xx >= 0 && xx < W;
-        const uint16_t* xrow = X + (size_t)((n_ * H + (in ? yy : 0)) * W + (in ? xx : 0)) * ldx;
-        const uint16_t* wrow = Wt + ((size_t)tap * Cout + (cv ? co : 0)) * Cin;
-        for (int ci0 = 0; ci0 < Cin; ci0 += CBK) {
-            uint4 av = in ? *(const uint4*)(xrow + ci0 + lk) : zero;
-            uint4 bv = cv ? *(const uint4*)(wrow + ci0 + lk) : zero;
-            __syncthreads();
-            *(uint4*)&As[lr * CLD + lk] = av;
-            *(uint4*)&Bs[lr * CLD + lk] = bv;
-            __syncthreads();
-            bf16x8 a = __builtin_bit_cast(bf16x8, *(const uint4*)&As[(wv * 16 + fr) * CLD + fk]);
+    f32x4 acc[4][BN / 16];  // [row * 2 + 16-pixel half][channel tile]
 #pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                bf16x8 b = __builtin_bit_cast(bf16x8, *(const uint4*)&Bs[(nb * 16 + fr) * CLD + fk]);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nb = 0; nb < BN / 16; ++nb) acc[mt][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ci0 = 0; ci0 < Cin; ci0 += CBK) {
+        __syncthreads();  // the previous chunk's LDS reads are done
+        for (int i = t; i < NHALO * 4; i += TB) {
+            const int hp = i >> 2, q = i & 3;
+            const int hy = hp / HXX, hx = hp - hy * HXX;
+            const int yy = y0 + hy - 1, xx = x0 + hx - 1;
+            uint4 v = zero;
+            if (n_ < N && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                v = *(const uint4*)(X + ((size_t)(n_ * H + yy) * W + xx) * ldx + ci0 + q * 8);
+            *(uint4*)&Hs[hp * CLD + q * 8] = v;
+        }
+        for (int i = t; i < 9 * BN * 4; i += TB) {
+            const int q = i & 3, r = (i >> 2) % BN, tap = (i >> 2) / BN;
+            const int co = co0 + r;
+            uint4 v = zero;
+            if (co < Cout) v = *(const uint4*)(Wt + ((size_t)tap * Cout + co) * Cin + ci0 + q * 8);
+            *(uint4*)&Ws[(tap * BN + r) * CLD + q * 8] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;  // offsets inside the halo
+            bf16x8 a[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+                a[mt] = __builtin_bit_cast(
+                    bf16x8, *(const uint4*)&Hs[((2 * wv + (mt >> 1) + dy) * HXX + (mt & 1) * 16 + fr + dx) * CLD + fk]);
+#pragma unroll
+            for (int nb = 0; nb < BN / 16; ++nb) {
+                const bf16x8 b = __builtin_bit_cast(bf16x8, *(const uint4*)&Ws[(tap * BN + nb * 16 + fr) * CLD + fk]);
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt)
+                    acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b, acc[mt][nb], 0, 0, 0);
             }
         }
     }
+    // Epilogue.  BF16 outputs go through LDS so that each lane stores whole
+    // 16-byte channel groups of a pixel; FP32 outputs (the RGB layer) are
+    // stored directly.
+    constexpr int OLD = BN + 8;  // LDS row of one output pixel
+    uint16_t* Os = Hs;           // TX * TY * OLD halves fit in the halo buffer
+    static_assert(TX * TY * OLD <= NHALO * CLD, "output tile must fit in the halo buffer");
+    if (!Yf) __syncthreads();  // every wave is done reading the halo
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb)
+    for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int pp = p0 + wv * 16 + (lane >> 4) * 4 + i;
-            const int c = co0 + nb * 16 + (lane & 15);
-            if (pp >= P || c >= cstore) continue;
-            float v = acc[nb][i] + (bias ? bias[c] : 0.0f);
-            if (relu) v = fmaxf(v, 0.0f);
-            if (M && !(bf2f(M[(size_t)pp * ldm + c]) > 0.0f)) v = 0.0f;
-            if (Yf) Yf[(size_t)pp * ldyf + c] = v;
-            else Y[(size_t)pp * ldy + c] = f2bf(v);
+        for (int nb = 0; nb < BN / 16; ++nb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int ry = 2 * wv + (mt >> 1), rx = (mt & 1) * 16 + (lane >> 4) * 4 + i;
+                const int yo = y0 + ry, xo = x0 + rx;
+                const int cl = nb * 16 + (lane & 15), c = co0 + cl;
+                const bool ok = n_ < N && yo < H && xo < W && c < cstore;
+                const size_t pp = (size_t)(n_ * H + yo) * W + xo;
+                float v = acc[mt][nb][i] + ((bias && c < Cout) ? bias[c] : 0.0f);
+                if (relu) v = fmaxf(v, 0.0f);
+                if (ok && M && !(bf2f(M[pp * ldm + c]) > 0.0f)) v = 0.0f;
+                if (Yf) {
+                    if (ok) Yf[pp * ldyf + c] = v;
+                } else {
+                    Os[(ry * TX + rx) * OLD + cl] = f2bf(v);
+                }
+            }
+    if (Yf) return;
+    __syncthreads();
+    for (int i = t; i < TX * TY * (BN / 8); i += TB) {
+        const int pix = i / (BN / 8), q = i % (BN / 8);
+        const int yo = y0 + pix / TX, xo = x0 + pix % TX;
+        const int c = co0 + q * 8;
+        if (n_ >= N || yo >= H || xo >= W || c >= cstore) continue;
+        const size_t pp = (size_t)(n_ * H + yo) * W + xo;
+        if (c + 8 <= cstore) {
+            *(uint4*)(Y + pp * ldy + c) = *(const uint4*)&Os[pix * OLD + q * 8];
+        } else {
+            for (int j = 0; c + j < cstore; ++j) Y[pp * ldy + c + j] = Os[pix * OLD + q * 8 + j];
         }
+    }
 }
 
 // dW[tap, co, ci] += sum_p dY[p, co] * X[p + off(tap), ci] over one pixel chunk.
@@ -401,18 +446,32 @@ template <class T> T* dalloc(size_t n) {
 }
 
 // ---------------------------------------------------------------- launchers
+// One 3x3 convolution: output tiles of 16 channels when at most 16 are
+// stored (the RGB output layer), else 32.
+void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const float* bias, int Cout, int N, int H,
+                 int W, uint16_t* Y, int ldy, float* Yf, int ldyf, int cstore, const uint16_t* M, int ldm, int relu,
+                 hipStream_t st) {
+    if (Cin % CBK != 0) throw Error("conv: input channels must be a multiple of 32");
+    const unsigned tiles = (unsigned)N * blocks(H, TY) * blocks(W, TX);
+    if (cstore <= 16) {
+        k_conv<16><<<dim3(tiles, blocks(cstore, 16)), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf,
+                                                                   ldyf, cstore, M, ldm, relu);
+    } else {
+        k_conv<32><<<dim3(tiles, blocks(cstore, 32)), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf,
+                                                                   ldyf, cstore, M, ldm, relu);
+    }
+    KF_CHECK(hipGetLastError());
+}
 void conv_fwd(const uint16_t* X, int ldx, int Cin, const uint16_t* Wb, const float* bias, int Cout, int N, int H,
               int W, uint16_t* Y, int ldy, bool relu, hipStream_t st, float* Yf = nullptr, int ldyf = 0,
               int cstore = -1) {
-    dim3 grid(blocks((size_t)N * H * W, CBM), blocks(Cout, CBN));
-    k_conv<<<grid, TB, 0, st>>>(X, ldx, Cin, Wb, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore < 0 ? Cout : cstore,
-                                nullptr, 0, relu ? 1 : 0);
+    conv_launch(X, ldx, Cin, Wb, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore < 0 ? Cout : std::min(cstore, Cout),
+                nullptr, 0, relu ? 1 : 0, st);
 }
 // dX = conv(dY, Wd) masked by (Xf > 0) when Xf is given
 void conv_dgrad(const uint16_t* dY, int ldy, int Cout, const uint16_t* Wd, int Cin, int N, int H, int W, uint16_t* dX,
                 int ldx, const uint16_t* Xf, int ldxf, hipStream_t st) {
-    dim3 grid(blocks((size_t)N * H * W, CBM), blocks(Cin, CBN));
-    k_conv<<<grid, TB, 0, st>>>(dY, ldy, Cout, Wd, nullptr, Cin, N, H, W, dX, ldx, nullptr, 0, Cin, Xf, ldxf, 0);
+    conv_launch(dY, ldy, Cout, Wd, nullptr, Cin, N, H, W, dX, ldx, nullptr, 0, Cin, Xf, ldxf, 0, st);
 }
 void conv_wgrad(const uint16_t* X, int ldx, int Cin, const uint16_t* dY, int ldy, int Cout, int N, int H, int W,
                 float* dW, float* db, hipStream_t st) {

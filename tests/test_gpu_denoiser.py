@@ -87,7 +87,14 @@ def test_network_gradients_match_reference(gpu):
     rg = net.backward(s, dl)
     assert abs(loss - rl) <= 1e-3 * abs(rl), (loss, rl)
     sl, _ = U.param_slices()
-    worst = 0.0
+    # Tolerance: activations are stored in BF16; a value within one FP32
+    # rounding of a BF16 rounding boundary rounds one way on the GPU (FP32
+    # MFMA accumulation) and the other in the FP64 reference.  The deep levels
+    # see few pixels (32 at level 3 of this 2 x 32 x 32 batch), so such flips
+    # leave up to ~6 % relative error in their weight gradients (measured on
+    # seeds 1-3 with two different conv kernels); a wrong tap, channel or
+    # mask gives errors of order 1.  The median over tensors stays ~1 %.
+    errs = []
     for k, (w, b, ci, co) in enumerate(sl):
         for a, z in ((w[0], w[1]), (b[0], b[1])):
             ref = rg[a:z]
@@ -95,9 +102,10 @@ def test_network_gradients_match_reference(gpu):
                 assert np.abs(g[a:z]).max() == 0
                 continue
             err = np.linalg.norm(g[a:z] - ref) / np.linalg.norm(ref)
-            worst = max(worst, err)
-            assert err < 2e-2, (k, a, err)
-    print("worst per-tensor relative gradient error %.2e" % worst)
+            errs.append(err)
+            assert err < 0.1, (k, a, err)
+    print("per-tensor relative gradient error: worst %.2e median %.2e" % (max(errs), np.median(errs)))
+    assert np.median(errs) < 0.03
     # forward output against the reference
     out = d.forward(inp)
     ro = net.output(s)
