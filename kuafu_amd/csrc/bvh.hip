@@ -17,6 +17,7 @@
 #include "kf_internal.h"
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace kf {
 
@@ -39,6 +40,13 @@ __device__ __forceinline__ float ord2f(int o) { return __int_as_float(o >= 0 ? o
 __global__ void k_fill_int(int* p, int v, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Centroid bounds start empty: lo = ord(+inf), hi = ord(-inf) per segment.
+__global__ void k_seg_init(uint32_t n_seg, int* __restrict__ sb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 6 * n_seg) return;
+    sb[i] = (i % 6) < 3 ? 0x7f800000 : (int)(0xff800000u ^ 0x7fffffffu);
 }
 
 __global__ void k_seg_bounds(uint32_t n, const float4* __restrict__ lo, const float4* __restrict__ hi,
@@ -423,17 +431,12 @@ KF_DEV_INLINE float half_area(const float lo[3], const float hi[3]) {
     return dx * dy + dy * dz + dz * dx;
 }
 
-// One level of the collapse: item (BVH2 node b, BVH4 slot w) writes BVH4
-// node w and queues its inner children for the next level.  err is set on
-// any index outside the arrays (malformed input), and nothing is written
-// out of bounds.
-__global__ void k_collapse(const Node* __restrict__ b2, uint32_t n_b2, const int2* __restrict__ in, uint32_t n_in,
-                           int2* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
-                           Node4* __restrict__ wide, uint32_t* __restrict__ wide_count, uint32_t wide_cap,
-                           uint32_t inner_kind, int* __restrict__ err) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_in) return;
-    const int2 it = in[i];
+// One item of the collapse: (BVH2 node b, BVH4 slot w) writes BVH4 node w
+// and queues its inner children for the next level.  err is set on any index
+// outside the arrays (malformed input), and nothing is written out of bounds.
+KF_DEV_INLINE void collapse_item(const Node* __restrict__ b2, uint32_t n_b2, int2 it, int2* __restrict__ out,
+                                 uint32_t* out_count, uint32_t out_cap, Node4* __restrict__ wide,
+                                 uint32_t* wide_count, uint32_t wide_cap, uint32_t inner_kind, int* err) {
     if ((uint32_t)it.x >= n_b2 || (uint32_t)it.y >= wide_cap) {
         *err = 1;
         return;
@@ -510,6 +513,51 @@ __global__ void k_collapse(const Node* __restrict__ b2, uint32_t n_b2, const int
     wide[it.y] = W;
 }
 
+// One level of the collapse over many items (BLAS builds).
+__global__ void k_collapse(const Node* __restrict__ b2, uint32_t n_b2, const int2* __restrict__ in, uint32_t n_in,
+                           int2* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
+                           Node4* __restrict__ wide, uint32_t* __restrict__ wide_count, uint32_t wide_cap,
+                           uint32_t inner_kind, int* __restrict__ err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_in) return;
+    collapse_item(b2, n_b2, in[i], out, out_count, out_cap, wide, wide_count, wide_cap, inner_kind, err);
+}
+
+// The whole collapse of one tree in one block, level after level, with no
+// host round trip (TLAS updates, every frame).  The root takes BVH4 slot 0.
+// cnt: [0] BVH4 nodes written, [1] next-level queue length (scratch).
+constexpr int kTreeThreads = 1024;
+__global__ __launch_bounds__(kTreeThreads) void k_collapse_tree(const Node* __restrict__ b2, uint32_t n_b2, int root2,
+                                                                int2* qa, int2* qb, uint32_t qcap,
+                                                                Node4* __restrict__ wide, uint32_t wide_cap,
+                                                                uint32_t inner_kind, uint32_t* cnt, int* err) {
+    __shared__ uint32_t n_in;
+    if (threadIdx.x == 0) {
+        n_in = (root2 >= 0 && root2 != kEmptyRef) ? 1u : 0u;
+        cnt[0] = n_in;
+        cnt[1] = 0u;
+        if (n_in) qa[0] = make_int2(root2, 0);
+    }
+    __syncthreads();
+    for (uint32_t level = 0; n_in > 0u; ++level) {
+        if (level > n_b2 + 1u) {  // deeper than a well-formed tree can be
+            if (threadIdx.x == 0) *err = 1;
+            break;
+        }
+        for (uint32_t i = threadIdx.x; i < n_in; i += blockDim.x)
+            collapse_item(b2, n_b2, qa[i], qb, &cnt[1], qcap, wide, &cnt[0], wide_cap, inner_kind, err);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            n_in = min(cnt[1], qcap);
+            cnt[1] = 0u;
+        }
+        __syncthreads();
+        int2* t = qa;
+        qa = qb;
+        qb = t;
+    }
+}
+
 }  // namespace
 
 void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32_t* d_seg, uint32_t n_seg,
@@ -531,16 +579,7 @@ void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32
     uint32_t n_tiles = blocks(n, kSortTile);
     uint32_t* hist = dalloc<uint32_t>((size_t)n_tiles * kDigits);
 
-    // centroid bounds per segment: lo = +inf, hi = -inf in ordered-int space
-    {
-        // fill 6 ints per segment: 3 x ord(+inf), 3 x ord(-inf)
-        std::vector<int> init(6 * (size_t)n_seg);
-        int pinf = 0x7f800000, ninf = (int)(0xff800000u ^ 0x7fffffffu);
-        for (uint32_t s = 0; s < n_seg; ++s)
-            for (int k = 0; k < 3; ++k) { init[6 * s + k] = pinf; init[6 * s + 3 + k] = ninf; }
-        KF_CHECK(hipMemcpyAsync(sb, init.data(), init.size() * sizeof(int), hipMemcpyHostToDevice, st));
-        KF_CHECK(hipStreamSynchronize(st));
-    }
+    k_seg_init<<<blocks(6 * n_seg, TB), TB, 0, st>>>(n_seg, sb);
     k_seg_bounds<<<blocks(n, TB), TB, 0, st>>>(n, d_lo, d_hi, d_seg, sb);
     k_morton<<<blocks(n, TB), TB, 0, st>>>(n, d_lo, d_hi, d_seg, sb, k0, v0);
 
@@ -670,6 +709,17 @@ void build_blas(Scene& s) {
     KF_CHECK(hipStreamSynchronize(st));
 }
 
+__global__ void k_tlas_aux(uint32_t n, uint32_t* aux) {
+    aux[0] = 0u;  // segment offsets {0, n}
+    aux[1] = n;
+    aux[2] = 0u;  // node offset
+}
+
+// TLAS (re)build, once per instance update: instance records + world boxes,
+// LBVH over the instances, BVH4 collapse in one block.  Nothing here waits
+// for the GPU: roots are known by construction (an LBVH of n >= 2 leaves has
+// its root at node 0; the collapse puts the root at BVH4 slot 0) and the host
+// inputs go through pinned staging.
 void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     hipStream_t st = stream();
     const uint32_t TB = 256;
@@ -680,14 +730,34 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
         s.d_inst_src = dalloc<kf_instance>(std::max<uint32_t>(n, 1));
         s.n_inst = n;
     }
-    std::vector<float> w2o(12 * (size_t)std::max<uint32_t>(n, 1));
+    if (!s.d_gflags) {  // static per scene
+        s.d_gflags = dalloc<uint32_t>(std::max<uint32_t>(s.n_geom, 1));
+        if (s.n_geom)
+            KF_CHECK(hipMemcpy(s.d_gflags, s.h_gflags.data(), s.n_geom * sizeof(uint32_t), hipMemcpyHostToDevice));
+        s.d_tlas_aux = dalloc<uint32_t>(8);
+        s.d_tlas_err = dalloc<int>(1);
+        KF_CHECK(hipMemsetAsync(s.d_tlas_err, 0, sizeof(int), st));
+        KF_CHECK(hipEventCreateWithFlags(&s.stage_ev, hipEventDisableTiming));
+    }
+    // host inputs -> pinned staging -> device
+    const size_t w2o_bytes = 12 * sizeof(float) * (size_t)n, inst_bytes = sizeof(kf_instance) * (size_t)n;
+    if (s.stage_pending) KF_CHECK(hipEventSynchronize(s.stage_ev));  // the previous update's copies are done
+    s.stage_pending = false;
+    if (w2o_bytes + inst_bytes > s.stage_cap) {
+        if (s.h_stage) KF_CHECK(hipHostFree(s.h_stage));
+        s.stage_cap = std::max<size_t>(w2o_bytes + inst_bytes, 4096);
+        KF_CHECK(hipHostMalloc((void**)&s.h_stage, s.stage_cap, hipHostMallocDefault));
+    }
+    float* w2o = (float*)s.h_stage;
     for (uint32_t i = 0; i < n; ++i) kf_world_to_object(h_inst[i].transform, &w2o[12 * i]);
-    float* d_w2o = dalloc<float>(w2o.size());
-    KF_CHECK(hipMemcpyAsync(d_w2o, w2o.data(), w2o.size() * sizeof(float), hipMemcpyHostToDevice, st));
-    if (n) KF_CHECK(hipMemcpyAsync(s.d_inst_src, h_inst, n * sizeof(kf_instance), hipMemcpyHostToDevice, st));
-    uint32_t* d_gflags = dalloc<uint32_t>(std::max<uint32_t>(s.n_geom, 1));
-    if (s.n_geom)
-        KF_CHECK(hipMemcpyAsync(d_gflags, s.h_gflags.data(), s.n_geom * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    if (n) std::memcpy(s.h_stage + w2o_bytes, h_inst, inst_bytes);
+    float* d_w2o = dalloc<float>(std::max<size_t>(12 * (size_t)n, 1));
+    if (n) {
+        KF_CHECK(hipMemcpyAsync(d_w2o, s.h_stage, w2o_bytes, hipMemcpyHostToDevice, st));
+        KF_CHECK(hipMemcpyAsync(s.d_inst_src, s.h_stage + w2o_bytes, inst_bytes, hipMemcpyHostToDevice, st));
+        KF_CHECK(hipEventRecord(s.stage_ev, st));
+        s.stage_pending = true;
+    }
     float4* glo = dalloc<float4>(std::max<uint32_t>(s.n_geom, 1));
     float4* ghi = dalloc<float4>(std::max<uint32_t>(s.n_geom, 1));
     if (s.n_geom) k_blas_box<<<blocks(s.n_geom, TB), TB, 0, st>>>(s.n_geom, s.d_blas_root, s.d_blas, s.d_tri, glo, ghi);
@@ -695,19 +765,13 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     float4* ihi = dalloc<float4>(std::max<uint32_t>(n, 1));
     uint32_t* iseg = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
     if (n)
-        k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, d_gflags, s.d_blas_root4, glo, ghi,
-                                                  s.d_inst, ilo, ihi, iseg);
-    uint32_t seg_off_h[2] = {0, n}, node_off_h[1] = {0};
-    uint32_t* d_seg_off = dalloc<uint32_t>(2);
-    uint32_t* d_node_off = dalloc<uint32_t>(1);
-    KF_CHECK(hipMemcpyAsync(d_seg_off, seg_off_h, sizeof(seg_off_h), hipMemcpyHostToDevice, st));
-    KF_CHECK(hipMemcpyAsync(d_node_off, node_off_h, sizeof(node_off_h), hipMemcpyHostToDevice, st));
+        k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, s.d_gflags, s.d_blas_root4, glo,
+                                                  ghi, s.d_inst, ilo, ihi, iseg);
+    k_tlas_aux<<<1, 1, 0, st>>>(n, s.d_tlas_aux);
     uint32_t* order = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
     int* d_root = dalloc<int>(1);
-    lbvh_build(n, ilo, ihi, iseg, 1, d_seg_off, d_node_off, s.d_tlas, order, d_root, false, st);
-    int root = kEmptyRef;
-    KF_CHECK(hipMemcpyAsync(&root, d_root, sizeof(int), hipMemcpyDeviceToHost, st));
-    KF_CHECK(hipStreamSynchronize(st));
+    lbvh_build(n, ilo, ihi, iseg, 1, s.d_tlas_aux, s.d_tlas_aux + 2, s.d_tlas, order, d_root, false, st);
+    const int root = n == 0 ? kEmptyRef : (n == 1 ? ~0 : 0);  // as k_seg_root writes it
     s.tlas_root = root;
     const uint32_t n_tlas2 = n > 1 ? n - 1 : 0;
     if (!s.d_tlas4 || s.tlas4_cap < std::max<uint32_t>(n_tlas2, 1)) {
@@ -715,12 +779,16 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
         s.tlas4_cap = std::max<uint32_t>(n_tlas2, 1);
         s.d_tlas4 = dalloc<Node4>(s.tlas4_cap);
     }
-    std::vector<int> r2{root}, r4;
-    s.n_tlas4 = collapse_bvh4(s.d_tlas, n_tlas2, r2, r4, s.d_tlas4, s.tlas4_cap, 0u, st);
-    s.tlas_root4 = r4[0];
-    dfree(d_w2o); dfree(d_gflags); dfree(glo); dfree(ghi); dfree(ilo); dfree(ihi); dfree(iseg);
-    dfree(d_seg_off); dfree(d_node_off); dfree(order); dfree(d_root);
-    KF_CHECK(hipStreamSynchronize(st));
+    const uint32_t qcap = std::max<uint32_t>(n_tlas2, 1);
+    int2* qa = dalloc<int2>(qcap);
+    int2* qb = dalloc<int2>(qcap);
+    k_collapse_tree<<<1, kTreeThreads, 0, st>>>(s.d_tlas, n_tlas2, root, qa, qb, qcap, s.d_tlas4, s.tlas4_cap, 0u,
+                                                 s.d_tlas_aux + 3, s.d_tlas_err);
+    KF_CHECK(hipGetLastError());
+    s.tlas_root4 = root >= 0 ? 0 : root;
+    s.n_tlas4 = n_tlas2 ? s.tlas4_cap : 0;  // index bound; the node count stays on the device
+    dfree(d_w2o); dfree(glo); dfree(ghi); dfree(ilo); dfree(ihi); dfree(iseg);
+    dfree(order); dfree(d_root); dfree(qa); dfree(qb);
 }
 
 }  // namespace kf

@@ -98,7 +98,10 @@ Scene::~Scene() {
     if (g_stream) (void)hipStreamSynchronize(g_stream);
     release(d_verts); release(d_idx); release(d_matidx); release(d_voff); release(d_toff); release(d_tri_geom);
     release(d_mats); release(d_blas); release(d_tri); release(d_blas_root); release(d_tlas); release(d_inst);
-    release(d_blas4); release(d_blas_root4); release(d_tlas4);
+    release(d_blas4); release(d_blas_root4); release(d_tlas4); release(d_gflags); release(d_tlas_aux);
+    release(d_tlas_err);
+    if (h_stage) (void)hipHostFree(h_stage);
+    if (stage_ev) (void)hipEventDestroy(stage_ev);
     release(d_inst_src); release(d_tex); release(d_tex_info); release(d_env_info); release(d_srgb);
 }
 
@@ -440,6 +443,19 @@ static uint32_t subtree_depth(const std::vector<Node>& nodes, int ref) {
     return best;
 }
 
+// BVH4 node count of the TLAS (kept on the device by the asynchronous
+// rebuild); also reports a malformed tree.
+static uint32_t tlas4_count(kf_scene* s) {
+    if (!s->d_tlas_aux || s->n_inst < 2) return 0;
+    uint32_t c = 0;
+    int err = 0;
+    KF_CHECK(hipStreamSynchronize(stream()));
+    KF_CHECK(hipMemcpy(&c, s->d_tlas_aux + 3, sizeof(c), hipMemcpyDeviceToHost));
+    KF_CHECK(hipMemcpy(&err, s->d_tlas_err, sizeof(err), hipMemcpyDeviceToHost));
+    if (err) throw Error("TLAS build: malformed tree");
+    return std::min(c, s->tlas4_cap);
+}
+
 int kf_scene_bvh_info(kf_scene* s, kf_bvh_info* out) {
     try {
         if (!s || !out) throw Error("kf_scene_bvh_info: null argument");
@@ -459,7 +475,7 @@ int kf_scene_bvh_info(kf_scene* s, kf_bvh_info* out) {
         if (!tn.empty()) KF_CHECK(hipMemcpy(tn.data(), s->d_tlas, tn.size() * sizeof(Node), hipMemcpyDeviceToHost));
         out->max_tlas_depth = (s->n_inst > 1) ? subtree_depth(tn, s->tlas_root) : 0;
         out->blas4_nodes = s->n_blas4;
-        out->tlas4_nodes = s->n_tlas4;
+        out->tlas4_nodes = tlas4_count(s);
         return 0;
     } catch (const std::exception& e) {
         set_error(e.what());
@@ -471,7 +487,7 @@ int kf_scene_read_bvh4(kf_scene* s, int which, float* nodes, uint32_t max_nodes,
                        uint32_t max_roots) {
     try {
         if (!s || (which != 0 && which != 1)) throw Error("kf_scene_read_bvh4: bad argument");
-        const uint32_t nn = which == 0 ? s->n_blas4 : s->n_tlas4;
+        const uint32_t nn = which == 0 ? s->n_blas4 : tlas4_count(s);
         const Node4* src = which == 0 ? s->d_blas4 : s->d_tlas4;
         if (n_nodes) *n_nodes = nn;
         if (nodes && nn) {

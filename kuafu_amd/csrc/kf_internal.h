@@ -122,8 +122,17 @@ struct Scene {
     std::vector<int> h_blas_root4;
     uint32_t n_blas4 = 0;
     Node4* d_tlas4 = nullptr;
-    uint32_t n_tlas4 = 0, tlas4_cap = 0;
+    uint32_t n_tlas4 = 0, tlas4_cap = 0;    // n_tlas4: bound for traversal index checks (the capacity)
     int tlas_root4 = kEmptyRef;
+    // TLAS updates run without host round trips: host inputs go through a
+    // pinned staging buffer, reused once the previous update's copies are done
+    uint8_t* h_stage = nullptr;
+    size_t stage_cap = 0;
+    hipEvent_t stage_ev = nullptr;
+    bool stage_pending = false;
+    uint32_t* d_gflags = nullptr;            // per-geometry flags (static)
+    uint32_t* d_tlas_aux = nullptr;          // [0..1] segment offsets, [2] node offset, [3..4] collapse counters
+    int* d_tlas_err = nullptr;
     Inst* d_inst = nullptr;
     kf_instance* d_inst_src = nullptr;
     int tlas_root = kEmptyRef;
