@@ -95,14 +95,47 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
         if e <= target:
             break
         spp *= 2
+    # denoiser at 1, 2, 4, 8 spp: relMSE of the noisy and the denoised frame
+    table = []
+    for spp in (1, 2, 4, 8):
+        pc.spp = spp
+        rd.render(gs, w.cam, pc, seed=41 * spp + 7)
+        noisy = relmse(rd.read(0)[..., :3], ref)
+        t = time.perf_counter()
+        den.apply(rd)
+        dt = time.perf_counter() - t
+        table.append({"spp": spp, "noisy_relmse": round(noisy, 5), "denoised_relmse": round(relmse(rd.read(0)[..., :3], ref), 5),
+                      "denoise_ms": round(dt * 1e3, 2)})
     pc.spp = 16
     hit_p = [p for p in plain if p["relmse"] <= target]
     hit_d = [d for d in dn if d["relmse"] <= target]
     res = {"target_relmse": target, "reference": "GPU FP32 tracer, %d spp, same frame" % ref_spp,
-           "plain": plain, "denoised": dn}
+           "plain": plain, "denoised": dn, "denoiser_by_spp": table}
     if hit_p and hit_d:
         res["speedup"] = round(hit_p[0]["ms"] / hit_d[0]["ms"], 2)
     return res
+
+
+def other_workloads(steps=3):
+    """Mrays/s of the other scene recipes on the same build (not the headline)."""
+    from kuafu_amd import GpuScene, Renderer, scenes
+    out = {}
+    for w in (scenes.config3(camera="left"), scenes.config1()):
+        gs = GpuScene(w.data)
+        rd = Renderer(w.width, w.height)
+        pc = w.pc
+        pc.frame_count = 0
+        rd.render(gs, w.cam, pc, seed=1)
+        rays = 0
+        t = time.perf_counter()
+        for k in range(steps):
+            rays += rd.render(gs, w.cam, pc, seed=2 + k).rays()
+        dt = time.perf_counter() - t
+        out[w.name] = {"width": w.width, "height": w.height, "spp": pc.spp, "path_depth": pc.path_depth,
+                       "mrays_s": round(rays / dt / 1e6, 1), "ms_per_frame": round(dt / steps * 1e3, 2)}
+        gs.close()
+        rd.close()
+    return out
 
 
 def main():
@@ -190,6 +223,8 @@ def main():
     if not args.no_relmse:
         print("bench: time to relMSE", file=sys.stderr, flush=True)
         out["time_to_relmse"] = time_to_relmse(w, gs, rd)
+        print("bench: other scenes", file=sys.stderr, flush=True)
+        out["other_workloads"] = other_workloads()
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w)

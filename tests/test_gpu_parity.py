@@ -336,6 +336,18 @@ def test_instance_update_matches_fresh_scene(gpu):
     assert close.mean() > 0.999
 
 
+@pytest.mark.parametrize("camera", ["left", "right"])
+def test_render_parity_projector_pattern(gpu, camera):
+    # active-light sensing scene: 64 boxes / cylinders, IR projector whose
+    # Bernoulli dot pattern is a texture sampled through the projector's
+    # frustum (rchit:262-316), stereo IR cameras 55 mm apart
+    w = scenes.config3(width=96, height=54, spp=2, path_depth=4, camera=camera)
+    (res,) = render_both(w)
+    assert_frame_parity(*res)
+    # the pattern reaches the image: some pixels are lit by the projector only
+    assert res[2][..., 0].max() > 0.0
+
+
 def test_render_accumulation_frames(gpu):
     w = scenes.config1(32, 32, spp=2)
     for res in render_both(w, frames=3):
