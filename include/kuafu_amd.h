@@ -143,6 +143,14 @@ typedef struct kf_scene_desc {
     uint32_t env_kind;                  /* KF_ENV_* */
     const kf_texture_desc* env_faces;   /* 6 faces +X,-X,+Y,-Y,+Z,-Z for KF_ENV_CUBEMAP */
     float env_gradient[8];              /* bottom rgb, pad, top rgb, pad for KF_ENV_GRADIENT */
+    /* Batched environments (no counterpart in the reference, which traces one
+     * Scene per RayTracer::trace): instances [env_instance_offset[e],
+     * env_instance_offset[e+1]) form environment e, with a TLAS of its own;
+     * rays of environment e see only those instances.  Geometries, materials,
+     * textures, lights and the sky are shared.  num_envs 0 or 1: one
+     * environment holding every instance (env_instance_offset unused). */
+    uint32_t num_envs;
+    const uint32_t* env_instance_offset; /* num_envs + 1, from 0 to num_instances */
 } kf_scene_desc;
 
 /* A ray of a fixed batch: origin, tmin, direction, tmax (32 B). */
@@ -251,7 +259,9 @@ int kf_scene_read_blas(kf_scene* s, uint32_t geometry, float* nodes, uint32_t ma
                        uint32_t* n_nodes, int32_t* root_ref, uint32_t* tri_order, uint32_t max_tris);
 
 /* Copy a BVH4 back (which = 0: the BLAS of all geometries, roots[g] per
- * geometry; 1: the TLAS, roots[0]).  Nodes are 32 floats each: lo.x, hi.x,
+ * geometry; 1: the TLAS, roots[0], or roots[e] per environment of a batched
+ * scene, whose environments' nodes sit in ranges of the array with unused
+ * slots between them).  Nodes are 32 floats each: lo.x, hi.x,
  * lo.y, hi.y, lo.z, hi.z of the four children (4 floats each), then the four
  * children and 4 pad words as uint32 bits.  A child is kind << 30 | index:
  * kind 0 / 1 an inner TLAS / BLAS node of this array, 2 an instance, 3 a
@@ -264,7 +274,8 @@ int kf_scene_read_bvh4(kf_scene* s, int which, float* nodes, uint32_t max_nodes,
 
 /* Fixed ray batches (host pointers, synchronous): the traceRayEXT calls of
  * PathTrace.rgen:97-107 (closest hit) and PathTrace.rchit:186-197
- * (terminate-on-first-hit shadow rays). */
+ * (terminate-on-first-hit shadow rays).  A batched scene's rays see
+ * environment 0. */
 int kf_trace_closest(kf_scene* s, const kf_ray* rays, uint32_t n, kf_hit* hits);
 int kf_trace_any(kf_scene* s, const kf_ray* rays, uint32_t n, uint8_t* occluded);
 
@@ -286,6 +297,19 @@ void kf_renderer_destroy(kf_renderer* r);
  * with the push constants of context.cpp:605-614. */
 int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, const kf_push_constants* pc,
               const kf_render_opts* opts, kf_render_stats* stats);
+
+/* Batched environments in one frame (no counterpart in the reference, where
+ * each environment is its own Scene rendered by its own setCamera + run).
+ * The scene holds num_envs environments (kf_scene_desc.env_instance_offset)
+ * and the renderer's H rows are num_envs images of W x He, He = H / num_envs,
+ * stacked: rows [e*He, (e+1)*He) show environment e through cams[e].  The
+ * per-pixel seeds, jitter and raster coordinates are those of a W x He image,
+ * so for opaque geometry environment e's rows equal kf_render of a scene
+ * holding only e's instances on a W x He renderer (the alpha test's coin is
+ * keyed by the global instance index).  All paths of all environments share
+ * the wavefront queues and persistent traversal launches. */
+int kf_render_envs(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_camera_ubo* cams,
+                   const kf_push_constants* pc, const kf_render_opts* opts, kf_render_stats* stats);
 
 /* Read a render target: 0 = rgba, 1 = albedo, 2 = normal; W*H*4 floats. */
 int kf_renderer_read(kf_renderer* r, int target, float* out);

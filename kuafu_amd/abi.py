@@ -74,7 +74,8 @@ class SceneDesc(C.Structure):
                 ("instances", C.c_void_p), ("num_materials", C.c_uint32), ("materials", C.c_void_p),
                 ("num_textures", C.c_uint32), ("textures", C.c_void_p), ("dir_light", C.c_void_p),
                 ("point_lights", C.c_void_p), ("active_lights", C.c_void_p), ("env_kind", C.c_uint32),
-                ("env_faces", C.c_void_p), ("env_gradient", C.c_float * 8)]
+                ("env_faces", C.c_void_p), ("env_gradient", C.c_float * 8), ("num_envs", C.c_uint32),
+                ("env_instance_offset", C.c_void_p)]
 
 
 class RenderStats(C.Structure):
@@ -137,6 +138,8 @@ EXPORTS = [
     ("kf_renderer_create", C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p]),
     ("kf_renderer_destroy", None, [C.c_void_p]),
     ("kf_render", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("kf_render_envs", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p]),
     ("kf_renderer_read", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_write", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_read_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),

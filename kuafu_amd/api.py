@@ -119,6 +119,7 @@ class SceneData:
         self.env_kind = abi.KF_ENV_CLEAR_COLOR
         self.env_gradient = np.zeros(8, np.float32)
         self.env_faces = None
+        self.env_offsets = None  # batched environments: instance offsets, n_env + 1
         self._n_point = 0
         self._n_active = 0
 
@@ -177,6 +178,19 @@ class SceneData:
         self.env_kind = abi.KF_ENV_GRADIENT
         self.env_gradient[:] = (*bottom, 0.0, *top, 0.0)
 
+    def environment(self, e):
+        """The scene of environment e alone: the same geometries, materials,
+        textures, lights and sky with only e's instances."""
+        import copy
+        sub = copy.copy(self)
+        lo, hi = int(self.env_offsets[e]), int(self.env_offsets[e + 1])
+        sub.instances = list(self.instances[lo:hi])
+        sub.env_offsets = None
+        return sub
+
+    def num_envs(self):
+        return 1 if self.env_offsets is None else len(self.env_offsets) - 1
+
     # -------------------------------------------------------------- flatten
     def arrays(self):
         ng = len(self.geoms)
@@ -232,6 +246,11 @@ class SceneData:
             d.env_faces = C.addressof(faces)
         for i in range(8):
             d.env_gradient[i] = float(self.env_gradient[i])
+        if self.env_offsets is not None:
+            eo = np.ascontiguousarray(self.env_offsets, np.uint32)
+            keep.append(eo)
+            d.num_envs = len(eo) - 1
+            d.env_instance_offset = ptr(eo)
         d._keep = keep
         return d
 
@@ -325,11 +344,11 @@ class GpuScene:
 
     def read_bvh4(self, which):
         """BVH4 nodes (n, 32) float32 and root refs (per geometry for which=0,
-        one for which=1 = TLAS)."""
+        one per environment for which=1 = TLAS)."""
         nn = C.c_uint32()
         check(lib().kf_scene_read_bvh4(self.h, which, None, 0, C.byref(nn), None, 0))
         nodes = np.zeros((max(nn.value, 1), 32), np.float32)
-        nr = max(len(self.data.geoms), 1) if which == 0 else 1
+        nr = max(len(self.data.geoms), 1) if which == 0 else self.data.num_envs()
         roots = np.zeros(nr, np.int32)
         check(lib().kf_scene_read_bvh4(self.h, which, ptr(nodes), nodes.shape[0], C.byref(nn), ptr(roots), nr))
         return nodes[:nn.value], roots
@@ -359,6 +378,15 @@ class Renderer:
         o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal))
         st = abi.RenderStats()
         check(lib().kf_render(self.h, scene.h, C.byref(cam), C.byref(pc), C.byref(o), C.byref(st)))
+        return st
+
+    def render_envs(self, scene, cams, pc, seed=0, samples_per_batch=0, keep_samples=False, count_traversal=False):
+        """One frame of every environment of a batched scene: cams[e] views
+        environment e, whose image is rows [e*H/n, (e+1)*H/n) of the targets."""
+        arr = (abi.CameraUBO * len(cams))(*cams)
+        o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal))
+        st = abi.RenderStats()
+        check(lib().kf_render_envs(self.h, scene.h, len(cams), arr, C.byref(pc), C.byref(o), C.byref(st)))
         return st
 
     def read(self, target=0):

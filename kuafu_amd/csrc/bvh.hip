@@ -384,7 +384,8 @@ __global__ void k_blas_box(uint32_t n_geom, const int* __restrict__ root, const 
 // Instance records + padded world AABBs (corners transformed in double).
 __global__ void k_instances(uint32_t n, const kf_instance* __restrict__ src, const float* __restrict__ w2o,
                             uint32_t n_geom, const uint32_t* __restrict__ gflags, const int* __restrict__ root,
-                            const float4* __restrict__ glo, const float4* __restrict__ ghi, Inst* __restrict__ out,
+                            const float4* __restrict__ glo, const float4* __restrict__ ghi,
+                            const uint32_t* __restrict__ inst_env, Inst* __restrict__ out,
                             float4* __restrict__ ilo, float4* __restrict__ ihi, uint32_t* __restrict__ iseg) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -422,7 +423,7 @@ __global__ void k_instances(uint32_t n, const kf_instance* __restrict__ src, con
     }
     ilo[i] = lo;
     ihi[i] = hi;
-    iseg[i] = 0;
+    iseg[i] = inst_env ? inst_env[i] : 0u;  // the LBVH segment is the environment
 }
 
 // ---------------------------------------------------------------- BVH4
@@ -523,32 +524,49 @@ __global__ void k_collapse(const Node* __restrict__ b2, uint32_t n_b2, const int
     collapse_item(b2, n_b2, in[i], out, out_count, out_cap, wide, wide_count, wide_cap, inner_kind, err);
 }
 
-// The whole collapse of one tree in one block, level after level, with no
-// host round trip (TLAS updates, every frame).  The root takes BVH4 slot 0.
-// cnt: [0] BVH4 nodes written, [1] next-level queue length (scratch).
+// The whole collapse of a forest with no host round trip (TLAS updates,
+// every frame): block e collapses tree e, level after level.  Tree e has the
+// n_e = seg_off[e+1] - seg_off[e] leaves of its LBVH segment; for n_e >= 2
+// its n_e - 1 BVH2 nodes start at node_off[e], with the root first, and its
+// BVH4 takes slots [node_off[e], node_off[e] + n_e - 1) of `wide` (a BVH4
+// never has more nodes than its BVH2), root first; qa / qb hold its level
+// queues in the same index range.  cnt + 2e: [0] next free BVH4 slot,
+// [1] next-level queue length (scratch).
 constexpr int kTreeThreads = 1024;
-__global__ __launch_bounds__(kTreeThreads) void k_collapse_tree(const Node* __restrict__ b2, uint32_t n_b2, int root2,
-                                                                int2* qa, int2* qb, uint32_t qcap,
-                                                                Node4* __restrict__ wide, uint32_t wide_cap,
-                                                                uint32_t inner_kind, uint32_t* cnt, int* err) {
+__global__ __launch_bounds__(kTreeThreads) void k_collapse_forest(const Node* __restrict__ b2, uint32_t n_b2,
+                                                                  const uint32_t* __restrict__ seg_off,
+                                                                  const uint32_t* __restrict__ node_off, int2* qa_all,
+                                                                  int2* qb_all, Node4* __restrict__ wide,
+                                                                  uint32_t inner_kind, uint32_t* cnt_all, int* err) {
+    const uint32_t e = blockIdx.x;
+    const uint32_t n_e = seg_off[e + 1] - seg_off[e];
+    if (n_e < 2u) return;  // a single instance or none: the root is a leaf or empty
+    const uint32_t base = node_off[e], cap_e = n_e - 1u;
+    if (base + cap_e > n_b2) {
+        if (threadIdx.x == 0) *err = 1;
+        return;
+    }
+    int2* qa = qa_all + base;
+    int2* qb = qb_all + base;
+    uint32_t* cnt = cnt_all + 2 * e;
     __shared__ uint32_t n_in;
     if (threadIdx.x == 0) {
-        n_in = (root2 >= 0 && root2 != kEmptyRef) ? 1u : 0u;
-        cnt[0] = n_in;
+        n_in = 1u;
+        cnt[0] = base + 1u;
         cnt[1] = 0u;
-        if (n_in) qa[0] = make_int2(root2, 0);
+        qa[0] = make_int2((int)base, (int)base);
     }
     __syncthreads();
     for (uint32_t level = 0; n_in > 0u; ++level) {
-        if (level > n_b2 + 1u) {  // deeper than a well-formed tree can be
+        if (level > cap_e + 1u) {  // deeper than a well-formed tree can be
             if (threadIdx.x == 0) *err = 1;
             break;
         }
         for (uint32_t i = threadIdx.x; i < n_in; i += blockDim.x)
-            collapse_item(b2, n_b2, qa[i], qb, &cnt[1], qcap, wide, &cnt[0], wide_cap, inner_kind, err);
+            collapse_item(b2, n_b2, qa[i], qb, &cnt[1], cap_e, wide, &cnt[0], base + cap_e, inner_kind, err);
         __syncthreads();
         if (threadIdx.x == 0) {
-            n_in = min(cnt[1], qcap);
+            n_in = min(cnt[1], cap_e);
             cnt[1] = 0u;
         }
         __syncthreads();
@@ -715,14 +733,32 @@ __global__ void k_tlas_aux(uint32_t n, uint32_t* aux) {
     aux[2] = 0u;  // node offset
 }
 
+// Root refs of each environment's BVH4, known by construction: an LBVH of
+// n_e >= 2 leaves has its root at its first node and the collapse puts the
+// BVH4 root at the first slot of the environment's range; one instance is a
+// leaf root, none an empty one.
+void env_roots(const std::vector<uint32_t>& env_off, const std::vector<uint32_t>& node_off, std::vector<int>& roots) {
+    const size_t ne = env_off.size() - 1;
+    roots.assign(ne, kEmptyRef);
+    for (size_t e = 0; e < ne; ++e) {
+        const uint32_t n_e = env_off[e + 1] - env_off[e];
+        roots[e] = n_e >= 2 ? (int)node_off[e] : (n_e == 1 ? ~(int)env_off[e] : kEmptyRef);
+    }
+}
+
 // TLAS (re)build, once per instance update: instance records + world boxes,
-// LBVH over the instances, BVH4 collapse in one block.  Nothing here waits
-// for the GPU: roots are known by construction (an LBVH of n >= 2 leaves has
-// its root at node 0; the collapse puts the root at BVH4 slot 0) and the host
-// inputs go through pinned staging.
+// an LBVH over the instances of each environment (one segment each), the
+// BVH4 collapse of every environment's tree in one launch.  Nothing here
+// waits for the GPU: roots are known by construction (env_roots) and the
+// host inputs go through pinned staging.  One environment: its segment and
+// node offsets are written on the device (the instance count may change);
+// several: they are fixed at scene creation (the count may not).
 void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     hipStream_t st = stream();
     const uint32_t TB = 256;
+    const bool multi = s.n_env > 1;
+    if (multi && n != s.h_env_off.back())
+        throw Error("kf_scene_update_instances: a scene with environments keeps its instance count");
     if (n != s.n_inst || !s.d_inst) {
         dfree(s.d_inst); dfree(s.d_tlas); dfree(s.d_inst_src);
         s.d_inst = dalloc<Inst>(std::max<uint32_t>(n, 1));
@@ -766,14 +802,27 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     uint32_t* iseg = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
     if (n)
         k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, s.d_gflags, s.d_blas_root4, glo,
-                                                  ghi, s.d_inst, ilo, ihi, iseg);
-    k_tlas_aux<<<1, 1, 0, st>>>(n, s.d_tlas_aux);
+                                                  ghi, multi ? s.d_inst_env : nullptr, s.d_inst, ilo, ihi, iseg);
+    const uint32_t n_seg = multi ? s.n_env : 1u;
+    const uint32_t* seg_off = multi ? s.d_env_off : s.d_tlas_aux;
+    const uint32_t* node_off = multi ? s.d_env_node_off : s.d_tlas_aux + 2;
+    uint32_t* cnt = multi ? s.d_env_cnt : s.d_tlas_aux + 3;
+    if (!multi) k_tlas_aux<<<1, 1, 0, st>>>(n, s.d_tlas_aux);
     uint32_t* order = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
-    int* d_root = dalloc<int>(1);
-    lbvh_build(n, ilo, ihi, iseg, 1, s.d_tlas_aux, s.d_tlas_aux + 2, s.d_tlas, order, d_root, false, st);
-    const int root = n == 0 ? kEmptyRef : (n == 1 ? ~0 : 0);  // as k_seg_root writes it
-    s.tlas_root = root;
-    const uint32_t n_tlas2 = n > 1 ? n - 1 : 0;
+    int* d_root = dalloc<int>(n_seg);
+    lbvh_build(n, ilo, ihi, iseg, n_seg, seg_off, node_off, s.d_tlas, order, d_root, false, st);
+    // BVH2 nodes: n_e - 1 per environment of n_e >= 2 instances
+    uint32_t n_tlas2 = 0;
+    if (multi)
+        n_tlas2 = s.h_env_node_off.back();
+    else
+        n_tlas2 = n > 1 ? n - 1 : 0;
+    std::vector<int> roots;
+    if (multi)
+        env_roots(s.h_env_off, s.h_env_node_off, roots);
+    else
+        env_roots({0u, n}, {0u}, roots);
+    s.tlas_root = roots[0];
     if (!s.d_tlas4 || s.tlas4_cap < std::max<uint32_t>(n_tlas2, 1)) {
         dfree(s.d_tlas4);
         s.tlas4_cap = std::max<uint32_t>(n_tlas2, 1);
@@ -782,10 +831,11 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     const uint32_t qcap = std::max<uint32_t>(n_tlas2, 1);
     int2* qa = dalloc<int2>(qcap);
     int2* qb = dalloc<int2>(qcap);
-    k_collapse_tree<<<1, kTreeThreads, 0, st>>>(s.d_tlas, n_tlas2, root, qa, qb, qcap, s.d_tlas4, s.tlas4_cap, 0u,
-                                                 s.d_tlas_aux + 3, s.d_tlas_err);
+    if (n_tlas2)
+        k_collapse_forest<<<n_seg, kTreeThreads, 0, st>>>(s.d_tlas, n_tlas2, seg_off, node_off, qa, qb, s.d_tlas4, 0u,
+                                                          cnt, s.d_tlas_err);
     KF_CHECK(hipGetLastError());
-    s.tlas_root4 = root >= 0 ? 0 : root;
+    s.tlas_root4 = roots[0];  // BVH4 and BVH2 roots coincide by construction
     s.n_tlas4 = n_tlas2 ? s.tlas4_cap : 0;  // index bound; the node count stays on the device
     dfree(d_w2o); dfree(glo); dfree(ghi); dfree(ilo); dfree(ihi); dfree(iseg);
     dfree(order); dfree(d_root); dfree(qa); dfree(qb);

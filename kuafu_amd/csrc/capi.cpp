@@ -69,6 +69,8 @@ SceneView Scene::view() const {
     v.tri = d_tri;
     v.tlas = d_tlas4;
     v.tlas_root = n_inst ? tlas_root4 : kEmptyRef;
+    v.env_root = n_env > 1 ? d_env_root : nullptr;
+    v.n_env = n_env;
     v.n_blas_nodes = n_blas4;
     v.n_tri = n_tri;
     v.n_tlas_nodes = n_tlas4;
@@ -100,9 +102,43 @@ Scene::~Scene() {
     release(d_mats); release(d_blas); release(d_tri); release(d_blas_root); release(d_tlas); release(d_inst);
     release(d_blas4); release(d_blas_root4); release(d_tlas4); release(d_gflags); release(d_tlas_aux);
     release(d_tlas_err);
+    release(d_env_off); release(d_env_node_off); release(d_inst_env); release(d_env_root); release(d_env_cnt);
     if (h_stage) (void)hipHostFree(h_stage);
     if (stage_ev) (void)hipEventDestroy(stage_ev);
     release(d_inst_src); release(d_tex); release(d_tex_info); release(d_env_info); release(d_srgb);
+}
+
+// Environments of a batched scene: instance ranges, BVH2 / BVH4 node ranges
+// of their TLAS trees (n_e - 1 nodes for n_e >= 2 instances), the environment
+// of each instance and the traversal root item of each tree.
+static void setup_envs(Scene& s, uint32_t n_env, const uint32_t* off, uint32_t n_inst) {
+    if (!off) throw Error("kf_scene_create: num_envs > 1 without env_instance_offset");
+    if (off[0] != 0 || off[n_env] != n_inst) throw Error("kf_scene_create: environment offsets must run from 0 to num_instances");
+    for (uint32_t e = 0; e < n_env; ++e)
+        if (off[e + 1] < off[e]) throw Error("kf_scene_create: environment offsets must be non-decreasing");
+    s.n_env = n_env;
+    s.h_env_off.assign(off, off + n_env + 1);
+    s.h_env_node_off.assign(n_env + 1, 0);
+    std::vector<uint32_t> inst_env(n_inst);
+    for (uint32_t e = 0; e < n_env; ++e) {
+        const uint32_t n_e = off[e + 1] - off[e];
+        s.h_env_node_off[e + 1] = s.h_env_node_off[e] + (n_e > 1 ? n_e - 1 : 0);
+        for (uint32_t i = off[e]; i < off[e + 1]; ++i) inst_env[i] = e;
+    }
+    std::vector<int> roots;
+    env_roots(s.h_env_off, s.h_env_node_off, roots);
+    std::vector<uint32_t> items(n_env);
+    for (uint32_t e = 0; e < n_env; ++e) {
+        const int r = roots[e];
+        items[e] = r == kEmptyRef ? kEmptyItem : (r >= 0 ? (uint32_t)r : ((2u << 30) | (uint32_t)~r));
+    }
+    s.d_env_off = upload(s.h_env_off.data(), s.h_env_off.size());
+    s.d_env_node_off = upload(s.h_env_node_off.data(), s.h_env_node_off.size());
+    s.d_inst_env = upload(inst_env.data(), inst_env.size());
+    s.d_env_root = upload(items.data(), items.size());
+    KF_CHECK(hipMalloc((void**)&s.d_env_cnt, 2 * sizeof(uint32_t) * n_env));
+    KF_CHECK(hipMemsetAsync(s.d_env_cnt, 0, 2 * sizeof(uint32_t) * n_env, stream()));
+    KF_CHECK(hipStreamSynchronize(stream()));  // the host vectors above are the copies' sources
 }
 
 }  // namespace kf
@@ -346,6 +382,7 @@ int kf_scene_create(const kf_scene_desc* d, kf_scene** out) {
         for (uint32_t i = 0; i < d->num_instances; ++i)
             if (d->instances[i].geometry_index >= s->n_geom)
                 throw Error("Geometry index is out of bounds.");  // rt.cpp:118-121
+        if (d->num_envs > 1) setup_envs(*s, d->num_envs, d->env_instance_offset, d->num_instances);
         hipStream_t st = stream();
         // vertices as 3 x float4 (the GLSL unpack of rchit:52-64)
         s->d_verts = upload((const float4*)d->vertices, 3 * (size_t)s->n_vert);
@@ -445,14 +482,24 @@ static uint32_t subtree_depth(const std::vector<Node>& nodes, int ref) {
 
 // BVH4 node count of the TLAS (kept on the device by the asynchronous
 // rebuild); also reports a malformed tree.
+// With environments the count is the sum over their trees, whose nodes sit
+// in per-environment ranges of the array (not packed).
 static uint32_t tlas4_count(kf_scene* s) {
     if (!s->d_tlas_aux || s->n_inst < 2) return 0;
-    uint32_t c = 0;
     int err = 0;
     KF_CHECK(hipStreamSynchronize(stream()));
-    KF_CHECK(hipMemcpy(&c, s->d_tlas_aux + 3, sizeof(c), hipMemcpyDeviceToHost));
     KF_CHECK(hipMemcpy(&err, s->d_tlas_err, sizeof(err), hipMemcpyDeviceToHost));
     if (err) throw Error("TLAS build: malformed tree");
+    if (s->n_env > 1) {
+        std::vector<uint32_t> c(2 * s->n_env);
+        KF_CHECK(hipMemcpy(c.data(), s->d_env_cnt, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        uint32_t tot = 0;
+        for (uint32_t e = 0; e < s->n_env; ++e)
+            if (s->h_env_off[e + 1] - s->h_env_off[e] >= 2) tot += c[2 * e] - s->h_env_node_off[e];
+        return tot;
+    }
+    uint32_t c = 0;
+    KF_CHECK(hipMemcpy(&c, s->d_tlas_aux + 3, sizeof(c), hipMemcpyDeviceToHost));
     return std::min(c, s->tlas4_cap);
 }
 
@@ -460,7 +507,7 @@ int kf_scene_bvh_info(kf_scene* s, kf_bvh_info* out) {
     try {
         if (!s || !out) throw Error("kf_scene_bvh_info: null argument");
         out->blas_nodes = s->n_blas_nodes;
-        out->tlas_nodes = s->n_inst > 1 ? s->n_inst - 1 : 0;
+        out->tlas_nodes = s->n_env > 1 ? s->h_env_node_off.back() : (s->n_inst > 1 ? s->n_inst - 1 : 0);
         out->triangles = s->n_tri;
         out->instances = s->n_inst;
         out->build_ms = s->build_ms;
@@ -473,7 +520,15 @@ int kf_scene_bvh_info(kf_scene* s, kf_bvh_info* out) {
         out->max_blas_depth = md;
         std::vector<Node> tn(out->tlas_nodes);
         if (!tn.empty()) KF_CHECK(hipMemcpy(tn.data(), s->d_tlas, tn.size() * sizeof(Node), hipMemcpyDeviceToHost));
-        out->max_tlas_depth = (s->n_inst > 1) ? subtree_depth(tn, s->tlas_root) : 0;
+        out->max_tlas_depth = 0;
+        if (s->n_env > 1) {
+            std::vector<int> roots;
+            env_roots(s->h_env_off, s->h_env_node_off, roots);
+            for (int r : roots)
+                if (r != kEmptyRef) out->max_tlas_depth = std::max(out->max_tlas_depth, subtree_depth(tn, r));
+        } else if (s->n_inst > 1) {
+            out->max_tlas_depth = subtree_depth(tn, s->tlas_root);
+        }
         out->blas4_nodes = s->n_blas4;
         out->tlas4_nodes = tlas4_count(s);
         return 0;
@@ -487,7 +542,8 @@ int kf_scene_read_bvh4(kf_scene* s, int which, float* nodes, uint32_t max_nodes,
                        uint32_t max_roots) {
     try {
         if (!s || (which != 0 && which != 1)) throw Error("kf_scene_read_bvh4: bad argument");
-        const uint32_t nn = which == 0 ? s->n_blas4 : tlas4_count(s);
+        // a batched scene's TLAS array is returned whole (per-environment ranges)
+        const uint32_t nn = which == 0 ? s->n_blas4 : (s->n_env > 1 ? (tlas4_count(s), s->n_tlas4) : tlas4_count(s));
         const Node4* src = which == 0 ? s->d_blas4 : s->d_tlas4;
         if (n_nodes) *n_nodes = nn;
         if (nodes && nn) {
@@ -498,6 +554,11 @@ int kf_scene_read_bvh4(kf_scene* s, int which, float* nodes, uint32_t max_nodes,
             if (which == 0) {
                 if (max_roots < s->n_geom) throw Error("kf_scene_read_bvh4: root buffer too small");
                 for (uint32_t g = 0; g < s->n_geom; ++g) roots[g] = s->h_blas_root4[g];
+            } else if (s->n_env > 1) {
+                if (max_roots < s->n_env) throw Error("kf_scene_read_bvh4: root buffer too small");
+                std::vector<int> r;
+                env_roots(s->h_env_off, s->h_env_node_off, r);
+                for (uint32_t e = 0; e < s->n_env; ++e) roots[e] = r[e];
             } else {
                 if (max_roots < 1) throw Error("kf_scene_read_bvh4: root buffer too small");
                 roots[0] = s->n_inst ? s->tlas_root4 : kEmptyRef;

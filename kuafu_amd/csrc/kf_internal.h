@@ -78,8 +78,10 @@ constexpr int kEmptyRef = 0x7fffffff;  // root ref of a BLAS without triangles
 struct SceneView {
     const Node4* blas;        // BVH4 of all geometries
     const float4* tri;        // 3 float4 per triangle slot; tri[3s].w = local prim id bits
-    const Node4* tlas;        // BVH4 over the instances
-    int tlas_root;            // BVH4 root ref of the TLAS
+    const Node4* tlas;        // BVH4 over the instances (one tree per environment)
+    int tlas_root;            // BVH4 root ref of environment 0's TLAS
+    const uint32_t* env_root; // n_env > 1: root item of each environment's TLAS (kEmptyItem: no instance)
+    uint32_t n_env;
     uint32_t n_blas_nodes, n_tri, n_tlas_nodes;  // bounds every traversal index is checked against
     const Inst* inst;
     uint32_t n_inst;
@@ -136,6 +138,13 @@ struct Scene {
     Inst* d_inst = nullptr;
     kf_instance* d_inst_src = nullptr;
     int tlas_root = kEmptyRef;
+    // Batched environments: instances [h_env_off[e], h_env_off[e+1]) form
+    // environment e, with its BVH2 nodes (and its BVH4 nodes, root first)
+    // from h_env_node_off[e].  One environment: the arrays stay empty.
+    uint32_t n_env = 1;
+    std::vector<uint32_t> h_env_off, h_env_node_off;
+    uint32_t *d_env_off = nullptr, *d_env_node_off = nullptr, *d_inst_env = nullptr, *d_env_root = nullptr;
+    uint32_t* d_env_cnt = nullptr;           // 2 collapse counters per environment
     uint8_t* d_tex = nullptr;
     uint4* d_tex_info = nullptr;
     uint32_t n_tex = 0;
@@ -169,6 +178,9 @@ void lbvh_build(uint32_t n, const float4* d_lo, const float4* d_hi, const uint32
 // through.  Returns the number of BVH4 nodes written to out (capacity cap).
 uint32_t collapse_bvh4(const Node* d_b2, uint32_t n_b2, const std::vector<int>& roots2, std::vector<int>& roots4,
                        Node4* d_out, uint32_t cap, uint32_t inner_kind, hipStream_t st);
+// Root refs of the TLAS trees of environments with instance offsets env_off
+// and BVH2 node offsets node_off (known by construction, no device read).
+void env_roots(const std::vector<uint32_t>& env_off, const std::vector<uint32_t>& node_off, std::vector<int>& roots);
 void build_blas(Scene& s);
 void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n);
 
@@ -187,6 +199,9 @@ struct Renderer {
     uint32_t *queue_a = nullptr, *queue_b = nullptr;
     uint32_t* queue_e = nullptr;               // paths k_extend traces at the next bounce
     float4 *sh_o = nullptr, *sh_d = nullptr;   // shadow queue (o.xyz, slot bits), (d.xyz, tmax)
+    uint32_t* sh_root = nullptr;               // shadow queue: TLAS root item (scenes with environments)
+    kf_camera_ubo* d_cams = nullptr;           // one camera per environment
+    uint32_t cams_cap = 0;
     float4 *nee_a = nullptr, *nee_b = nullptr; // per slot (c_a.xyz, theta), (c_b.xyz, mode|group bits)
     uint8_t* vis = nullptr;
     float* samples = nullptr;                  // spp * W * H * 3

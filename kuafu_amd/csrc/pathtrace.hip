@@ -46,10 +46,14 @@ constexpr int kCntExtQueue = 19;
 constexpr int kCntGlobal = kMaxBounces * kCntPerBounce;
 constexpr int kCntSize = kCntGlobal + 8;
 
+// The render target holds n_env images of W x He stacked vertically (one
+// image, He = H, without environments): pixel = env * WHe + (y * W + x), and
+// seeds, jitter and raster coordinates are those of the W x He image.
 struct Frame {
-    kf_camera_ubo cam;
+    const kf_camera_ubo* cams;  // n_env cameras (device)
     kf_push_constants pc;
     uint32_t W, H, WH;
+    uint32_t He, WHe, n_env;
     uint32_t s0, nsamp;   // samples [s0, s0 + nsamp) in this batch
     uint32_t S0;          // frame seed
     uint32_t slots;       // NEE slots per path
@@ -103,16 +107,19 @@ __global__ void k_zero(unsigned long long* c, int n) {
     if (i < n) c[i] = 0;
 }
 
-__global__ __launch_bounds__(TB) void k_raygen(Frame F, const uint2* __restrict__ jump, float4* __restrict__ ray_o,
+__global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2* __restrict__ jump,
+                                               float4* __restrict__ ray_o,
                                                float4* __restrict__ ray_d, float4* __restrict__ thr,
                                                float4* __restrict__ col, float4* __restrict__ rw,
                                                uint32_t* __restrict__ seed, uint32_t* __restrict__ queue,
                                                unsigned long long* __restrict__ cnt) {
     uint32_t n = F.WH * F.nsamp;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        uint32_t pixel = p % F.WH;
-        uint32_t i = F.s0 + p / F.WH;
-        uint32_t x = pixel % F.W, y = pixel / F.W;
+        const uint32_t i = F.s0 + p / F.WH;
+        const uint32_t env = (p % F.WH) / F.WHe;
+        const uint32_t pixel = (p % F.WH) - env * F.WHe;  // within the environment's image
+        const uint32_t x = pixel % F.W, y = pixel / F.W;
+        const kf_camera_ubo& cam = F.cams[env];
         // jitter stream: tea(pixel, S0) advanced by 2i draws (rgen:23, 35)
         uint32_t js = tea(pixel, F.S0);
         uint2 jm = jump[i];
@@ -121,10 +128,10 @@ __global__ __launch_bounds__(TB) void k_raygen(Frame F, const uint2* __restrict_
         float r1 = rnd(js);
         uint32_t rs = tea(pixel, F.S0 + 1u + i);
         float posx = float(x) + r0, posy = float(y) + r1;
-        float npx = posx / float(F.W), npy = posy / float(F.H);
+        float npx = posx / float(F.W), npy = posy / float(F.He);
         float dx = npx * 2.0f - 1.0f, dy = npy * 2.0f - 1.0f;
-        float aperture = F.cam.position[3];
-        float focus = F.cam.front[3];
+        float aperture = cam.position[3];
+        float focus = cam.front[3];
         float ox, oy;
         do {  // diskSampling (Sampling.glsl:124-134), always drawn (rgen:42)
             float a = rnd(rs), b = rnd(rs);
@@ -136,23 +143,24 @@ __global__ __launch_bounds__(TB) void k_raygen(Frame F, const uint2* __restrict_
         oy = half * oy;
         float tv[4] = {dx, dy, 1.0f, 1.0f};
         float target[4];
-        mat4_mul(F.cam.proj_inverse, tv, target);
+        mat4_mul(cam.proj_inverse, tv, target);
         float org[4], dir4[4];
         if (aperture > 0.0f) {
             float o4[4] = {ox, oy, 0.0f, 1.0f};
-            mat4_mul(F.cam.view_inverse, o4, org);
+            mat4_mul(cam.view_inverse, o4, org);
             v3 nd = normalize(mk(target[0], target[1], target[2]) * focus - mk(ox, oy, 0.0f));
             float d4[4] = {nd.x, nd.y, nd.z, 0.0f};
-            mat4_mul(F.cam.view_inverse, d4, dir4);
+            mat4_mul(cam.view_inverse, d4, dir4);
         } else {
             float o4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
-            mat4_mul(F.cam.view_inverse, o4, org);
+            mat4_mul(cam.view_inverse, o4, org);
             v3 nd = normalize(mk(target[0], target[1], target[2]));
             float d4[4] = {nd.x, nd.y, nd.z, 0.0f};
-            mat4_mul(F.cam.view_inverse, d4, dir4);
+            mat4_mul(cam.view_inverse, d4, dir4);
         }
+        // ray_d.w carries the path's TLAS root item through every bounce
         ray_o[p] = make_float4(org[0], org[1], org[2], 0.0f);
-        ray_d[p] = make_float4(dir4[0], dir4[1], dir4[2], 0.0f);
+        ray_d[p] = make_float4(dir4[0], dir4[1], dir4[2], __uint_as_float(env_root_item(S, env)));
         thr[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
         col[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         rw[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -171,7 +179,8 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
                                                const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
                                                const uint32_t* __restrict__ seed, float4* __restrict__ hit,
                                                uint32_t* __restrict__ hit_prim, unsigned long long* __restrict__ cnt,
-                                               uint32_t refill, uint32_t WH, uint32_t s0, uint32_t S0) {
+                                               uint32_t refill, uint32_t WH, uint32_t WHe, uint32_t s0,
+                                               uint32_t S0) {
     const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + kCntExtQueue], (unsigned long long)cap);
     uint32_t ovf = 0;
     TCount tot{0, 0, 0};
@@ -182,8 +191,9 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
         r.d = xyz(d);
         r.tmin = 0.001f;
         r.tmax = 10000.0f;
+        r.root = __float_as_uint(d.w);
         // alpha coin seed: tea(initial seed of the sample's stream, bounce)
-        r.alpha_seed = S.any_alpha ? tea(tea(p % WH, S0 + 1u + s0 + p / WH), (uint32_t)bounce) : 0u;
+        r.alpha_seed = S.any_alpha ? tea(tea((p % WH) % WHe, S0 + 1u + s0 + p / WH), (uint32_t)bounce) : 0u;
         r.payload = p;
     };
     auto store = [&](uint32_t p, THit b, bool) __attribute__((always_inline)) {
@@ -211,7 +221,8 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                                               uint32_t* __restrict__ seed, float4* __restrict__ nee_a,
                                               float4* __restrict__ nee_b, uint8_t* __restrict__ vis,
                                               float4* __restrict__ sh_o, float4* __restrict__ sh_d,
-                                              float4* __restrict__ albedo_out, float4* __restrict__ normal_out,
+                                              uint32_t* __restrict__ sh_root, float4* __restrict__ albedo_out,
+                                              float4* __restrict__ normal_out,
                                               uint32_t* __restrict__ ext_queue, unsigned long long* __restrict__ cnt) {
     uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
     uint32_t stride = gridDim.x * blockDim.x;
@@ -421,7 +432,8 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                             THit hh;
                             TCount cc{0, 0, 0};
                             n_shadow++;
-                            bool occ = traverse<true, false>(S, worldPos, Ld, 0.001f, maxd, false, 0u, hh, cc, ovf);
+                            bool occ = traverse<true, false>(S, __float_as_uint(rd.w), worldPos, Ld, 0.001f, maxd,
+                                                             false, 0u, hh, cc, ovf);
                             if (!occ) {
                                 v3 ca, cb;
                                 float th;
@@ -443,12 +455,13 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                             uint32_t at = (uint32_t)atomicAdd(&cnt[bounce * kCntPerBounce + 1], 1ull);
                             sh_o[at] = f4(worldPos, __uint_as_float((uint32_t)slot));
                             sh_d[at] = f4(Ld, maxd);
+                            if (S.n_env > 1) sh_root[at] = __float_as_uint(rd.w);
                             n_shadow++;
                         }
                     }
                     if (NEE_INLINE) nee_a[(size_t)p * K] = f4((sum_d + sum_p) + sum_a, 0.0f);
                     ray_o[p] = f4(worldPos, 0.0f);
-                    ray_d[p] = f4(L, 0.0f);
+                    ray_d[p] = f4(L, rd.w);
                     em[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
                     rw[p] = f4(weight, 0.0f);
                     ext = !all_zero(xyz(t4) * weight) && depth + 1u <= F.pc.path_depth;  // as in k_connect
@@ -468,9 +481,11 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
 }
 
 __global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t cap, const float4* __restrict__ sh_o,
-                                               const float4* __restrict__ sh_d, uint8_t* __restrict__ vis,
+                                               const float4* __restrict__ sh_d, const uint32_t* __restrict__ sh_root,
+                                               uint8_t* __restrict__ vis,
                                                unsigned long long* __restrict__ cnt, uint32_t refill) {
     const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 1], (unsigned long long)cap);
+    const uint32_t root0 = env_root_item(S, 0u);
     uint32_t ovf = 0;
     TCount tot{0, 0, 0};
     auto load = [&](uint32_t q, TRay& r) __attribute__((always_inline)) {
@@ -479,6 +494,7 @@ __global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t
         r.d = xyz(d);
         r.tmin = 0.001f;
         r.tmax = d.w;
+        r.root = S.n_env > 1 ? sh_root[q] : root0;
         r.alpha_seed = 0u;
         r.payload = __float_as_uint(o.w);
     };
@@ -642,8 +658,8 @@ Renderer::~Renderer() {
     for (auto e : ev) (void)hipEventDestroy(e);
     for (auto e : sync_ev) (void)hipEventDestroy(e);
     void* ptrs[] = {d_rgba, d_albedo, d_normal, d_bgra, ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed,
-                    queue_a, queue_b, queue_e, sh_o, sh_d, nee_a, nee_b, vis, samples, counters, d_lights, d_jump,
-                    d_acc, d_out};
+                    queue_a, queue_b, queue_e, sh_o, sh_d, sh_root, nee_a, nee_b, vis, samples, counters, d_lights,
+                    d_jump, d_acc, d_out, d_cams};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -651,7 +667,7 @@ Renderer::~Renderer() {
 void Renderer::ensure(size_t paths, uint32_t nslots, uint32_t spp) {
     if (paths > cap || nslots > slots) {
         void* ptrs[] = {ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed, queue_a, queue_b, queue_e,
-                        sh_o, sh_d, nee_a, nee_b, vis};
+                        sh_o, sh_d, sh_root, nee_a, nee_b, vis};
         for (void* p : ptrs)
             if (p) KF_CHECK(hipFree(p));
         cap = std::max(cap, paths);
@@ -670,6 +686,7 @@ void Renderer::ensure(size_t paths, uint32_t nslots, uint32_t spp) {
         queue_e = dmalloc<uint32_t>(cap);
         sh_o = dmalloc<float4>(cap * slots);
         sh_d = dmalloc<float4>(cap * slots);
+        sh_root = dmalloc<uint32_t>(cap * slots);
         nee_a = dmalloc<float4>(cap * slots);
         nee_b = dmalloc<float4>(cap * slots);
         vis = dmalloc<uint8_t>(cap * slots);
@@ -738,194 +755,231 @@ void build_light_list(const Scene& s, DevLights& L, bool& inline_nee) {
 
 using namespace kf;
 
+namespace kf {
+namespace {
+
+// One frame of n_env environments (n_env = 1: the plain kf_render).
+void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* cams, const kf_push_constants* pc,
+                  const kf_render_opts* opts, kf_render_stats* stats) {
+    if (!r || !s || !cams || !pc) throw Error("kf_render: null argument");
+    if (pc->spp == 0) throw Error("kf_render: spp must be > 0");
+    if (pc->path_depth + 2 >= (uint32_t)kMaxBounces) throw Error("kf_render: path depth too large");
+    if (n_env == 0 || n_env != s->n_env)
+        throw Error("kf_render_envs: the number of cameras must equal the scene's number of environments");
+    if (r->H % n_env) throw Error("kf_render_envs: the target height must be a multiple of the number of environments");
+    hipStream_t st = stream();
+    kf_render_opts o{};
+    if (opts) o = *opts;
+    DevLights L;
+    memset(&L, 0, sizeof(L));
+    bool inline_nee = false;
+    build_light_list(*s, L, inline_nee);
+    uint32_t WH = r->W * r->H;
+    uint32_t spb = o.samples_per_batch ? o.samples_per_batch : 0;
+    if (spb == 0) {
+        // default: up to 32M paths per batch (16 spp at 1920x1080).  Each
+        // bounce's traversal launch then holds tens of rays per lane, so
+        // the tail of the slowest rays is a small part of it; the state
+        // costs about 200 B per path (6.6 GB at 32M paths).
+        spb = std::max<uint32_t>(1, std::min<uint32_t>(pc->spp, (32u << 20) / std::max<uint32_t>(WH, 1)));
+    }
+    spb = std::min(spb, pc->spp);
+    uint32_t K = inline_nee ? 1u : (uint32_t)std::max(L.n_slots, 1);
+    r->ensure((size_t)WH * spb, K, o.keep_samples ? pc->spp : 0);
+    r->samples_spp = o.keep_samples ? pc->spp : 0;
+    // device light list (per frame, like the UBO upload)
+    if (!r->d_lights) KF_CHECK(hipMalloc(&r->d_lights, sizeof(DevLights)));
+    DevLights* d_lights = (DevLights*)r->d_lights;
+    KF_CHECK(hipMemcpyAsync(d_lights, &L, sizeof(DevLights), hipMemcpyHostToDevice, st));
+    // LCG jump-ahead for the jitter stream: sample i starts after 2i draws
+    std::vector<uint2> jump(pc->spp);
+    {
+        uint32_t a = 1, c = 0;  // identity
+        for (uint32_t i = 0; i < pc->spp; ++i) {
+            jump[i] = make_uint2(a, c);
+            for (int k = 0; k < 2; ++k) {  // compose with one LCG step
+                a = 1664525u * a;
+                c = 1664525u * c + 1013904223u;
+            }
+        }
+    }
+    if (pc->spp > r->jump_cap) {
+        if (r->d_jump) KF_CHECK(hipFree(r->d_jump));
+        KF_CHECK(hipMalloc((void**)&r->d_jump, pc->spp * sizeof(uint2)));
+        r->jump_cap = pc->spp;
+    }
+    uint2* d_jump = r->d_jump;
+    KF_CHECK(hipMemcpyAsync(d_jump, jump.data(), pc->spp * sizeof(uint2), hipMemcpyHostToDevice, st));
+    if (!r->d_acc) KF_CHECK(hipMalloc((void**)&r->d_acc, (size_t)WH * sizeof(float4)));
+    float4* d_acc = r->d_acc;
+    if ((size_t)WH * spb > r->out_cap) {
+        if (r->d_out) KF_CHECK(hipFree(r->d_out));
+        KF_CHECK(hipMalloc((void**)&r->d_out, (size_t)WH * spb * sizeof(float4)));
+        r->out_cap = (size_t)WH * spb;
+    }
+    float4* d_out = r->d_out;
+
+    if (n_env > r->cams_cap) {
+        if (r->d_cams) KF_CHECK(hipFree(r->d_cams));
+        KF_CHECK(hipMalloc((void**)&r->d_cams, n_env * sizeof(kf_camera_ubo)));
+        r->cams_cap = n_env;
+    }
+    KF_CHECK(hipMemcpyAsync(r->d_cams, cams, n_env * sizeof(kf_camera_ubo), hipMemcpyHostToDevice, st));
+
+    SceneView S = s->view();
+    Frame F;
+    F.cams = r->d_cams;
+    F.pc = *pc;
+    F.W = r->W;
+    F.H = r->H;
+    F.WH = WH;
+    F.n_env = n_env;
+    F.He = r->H / n_env;
+    F.WHe = r->W * F.He;
+    F.S0 = tea_host(o.seed, (uint32_t)pc->frame_count);
+    F.slots = K;
+    F.inline_nee = inline_nee;
+    F.count = o.count_traversal;
+
+    if (r->ev.empty()) {
+        r->ev.resize(2 + 4 * kMaxBounces);
+        for (auto& e : r->ev) KF_CHECK(hipEventCreate(&e));
+        r->sync_ev.resize(2 * kMaxBounces);
+        for (auto& e : r->sync_ev) KF_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        KF_CHECK(hipStreamCreateWithFlags(&r->st2, hipStreamNonBlocking));  // default priority
+    }
+    int dev = 0;
+    KF_CHECK(hipGetDevice(&dev));
+    int n_cu = 256;
+    KF_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t grid = (uint32_t)n_cu * 8;
+    const uint32_t refill = traversal_refill();
+    // Persistent traversal kernels: exactly the blocks that fit at once.
+    static const uint32_t grid_ext0 = resident_grid(k_extend<false>, n_cu);
+    static const uint32_t grid_ext1 = resident_grid(k_extend<true>, n_cu);
+    static const uint32_t grid_sh = resident_grid(k_shadow, n_cu);
+
+    unsigned long long totals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t n_ext = 0, n_sh = 0;
+    float ms_total = 0.f, ms_ext = 0.f, ms_sh = 0.f;
+    for (uint32_t s0 = 0; s0 < pc->spp; s0 += spb) {
+        F.s0 = s0;
+        F.nsamp = std::min(spb, pc->spp - s0);
+        uint32_t B = WH * F.nsamp;
+        k_zero<<<(kCntSize + TB - 1) / TB, TB, 0, st>>>(r->counters, kCntSize);
+        KF_CHECK(hipEventRecord(r->ev[0], st));
+        k_raygen<<<std::min<uint32_t>((B + TB - 1) / TB, 65535u), TB, 0, st>>>(
+            S, F, d_jump, r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->seed, r->queue_a, r->counters);
+        uint32_t* qa = r->queue_a;
+        uint32_t* qb = r->queue_b;
+        uint32_t nb = pc->path_depth + 1;
+        // Per bounce b:  A: shade(b) -> [B: shadow(b)] , A: extend(b+1)
+        // -> (wait B) connect(b).  k_shade lists the paths that continue
+        // unless Russian roulette ends them, so the closest-hit rays of
+        // bounce b+1 are traced while the shadow rays of bounce b are,
+        // on the second stream; k_connect then waits for both.
+        auto launch_extend = [&](uint32_t b, const uint32_t* queue) {
+            KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
+            if (o.count_traversal)
+                k_extend<true><<<grid_ext1, TB, 0, st>>>(S, (int)b, B, queue, r->ray_o, r->ray_d, r->seed, r->hit,
+                                                         r->hit_prim, r->counters, refill, WH, F.WHe, F.s0, F.S0);
+            else
+                k_extend<false><<<grid_ext0, TB, 0, st>>>(S, (int)b, B, queue, r->ray_o, r->ray_d, r->seed,
+                                                          r->hit, r->hit_prim, r->counters, refill, WH, F.WHe,
+                                                          F.s0, F.S0);
+            KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 1], st));
+            ++n_ext;
+        };
+        launch_extend(0, qa);
+        for (uint32_t b = 0; b < nb; ++b) {
+            if (inline_nee)
+                k_shade<true><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->hit,
+                                                   r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis,
+                                                   r->sh_o, r->sh_d, r->sh_root, r->d_albedo, r->d_normal, r->queue_e,
+                                                   r->counters);
+            else
+                k_shade<false><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr,
+                                                    r->hit, r->hit_prim, r->rw, r->em, r->seed, r->nee_a,
+                                                    r->nee_b, r->vis, r->sh_o, r->sh_d, r->sh_root, r->d_albedo, r->d_normal,
+                                                    r->queue_e, r->counters);
+            if (!inline_nee) {
+                KF_CHECK(hipEventRecord(r->sync_ev[2 * b], st));
+                KF_CHECK(hipStreamWaitEvent(r->st2, r->sync_ev[2 * b], 0));
+                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], r->st2));
+                k_shadow<<<grid_sh, TB, 0, r->st2>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->sh_root, r->vis,
+                                                     r->counters, refill);
+                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], r->st2));
+                KF_CHECK(hipEventRecord(r->sync_ev[2 * b + 1], r->st2));
+                ++n_sh;
+            }
+            if (b + 1 < nb) launch_extend(b + 1, r->queue_e);
+            if (!inline_nee) KF_CHECK(hipStreamWaitEvent(st, r->sync_ev[2 * b + 1], 0));
+            if (inline_nee)
+                k_connect<true><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
+                                                     r->nee_a, r->nee_b, r->vis, d_out, r->counters);
+            else
+                k_connect<false><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
+                                                      r->nee_a, r->nee_b, r->vis, d_out, r->counters);
+            k_next_count<<<1, 1, 0, st>>>((int)b, r->counters);
+            std::swap(qa, qb);
+        }
+        bool last = s0 + F.nsamp >= pc->spp;
+        k_resolve<<<(WH + TB - 1) / TB, TB, 0, st>>>(F, d_out, d_acc, r->d_rgba,
+                                                     o.keep_samples ? r->samples : nullptr, last ? 1 : 0);
+        KF_CHECK(hipEventRecord(r->ev[1], st));
+        KF_CHECK(hipGetLastError());
+        unsigned long long h[kCntSize];
+        KF_CHECK(hipMemcpyAsync(h, r->counters, sizeof(h), hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        for (int k = 0; k < 8; ++k) totals[k] += h[kCntGlobal + k];
+        float ms = 0.f;
+        KF_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
+        ms_total += ms;
+        for (uint32_t b = 0; b < nb; ++b) {
+            float a = 0.f, c = 0.f;
+            KF_CHECK(hipEventElapsedTime(&a, r->ev[2 + 4 * b + 0], r->ev[2 + 4 * b + 1]));
+            if (!inline_nee) KF_CHECK(hipEventElapsedTime(&c, r->ev[2 + 4 * b + 2], r->ev[2 + 4 * b + 3]));
+            ms_ext += a;
+            ms_sh += c;
+        }
+    }
+    if (totals[5]) throw Error("kf_render: traversal stack overflow (BVH deeper than the stack)");
+    if (stats) {
+        stats->primary_rays = totals[0];
+        stats->extension_rays = totals[1];
+        stats->shadow_rays = totals[2];
+        stats->node_visits = totals[3];
+        stats->tri_tests = totals[4];
+        stats->instance_visits = totals[6];
+        stats->speculative_rays = totals[7];
+        stats->trace_ms = ms_total;
+        stats->extend_ms = ms_ext;
+        stats->shadow_ms = ms_sh;
+        stats->other_ms = ms_total - ms_ext - ms_sh;
+        stats->extend_launches = n_ext;
+        stats->shadow_launches = n_sh;
+    }
+}
+
+}  // namespace
+}  // namespace kf
+
 extern "C" int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, const kf_push_constants* pc,
                          const kf_render_opts* opts, kf_render_stats* stats) {
     try {
-        if (!r || !s || !cam || !pc) throw Error("kf_render: null argument");
-        if (pc->spp == 0) throw Error("kf_render: spp must be > 0");
-        if (pc->path_depth + 2 >= (uint32_t)kMaxBounces) throw Error("kf_render: path depth too large");
-        hipStream_t st = stream();
-        kf_render_opts o{};
-        if (opts) o = *opts;
-        DevLights L;
-        memset(&L, 0, sizeof(L));
-        bool inline_nee = false;
-        build_light_list(*s, L, inline_nee);
-        uint32_t WH = r->W * r->H;
-        uint32_t spb = o.samples_per_batch ? o.samples_per_batch : 0;
-        if (spb == 0) {
-            // default: up to 32M paths per batch (16 spp at 1920x1080).  Each
-            // bounce's traversal launch then holds tens of rays per lane, so
-            // the tail of the slowest rays is a small part of it; the state
-            // costs about 200 B per path (6.6 GB at 32M paths).
-            spb = std::max<uint32_t>(1, std::min<uint32_t>(pc->spp, (32u << 20) / std::max<uint32_t>(WH, 1)));
-        }
-        spb = std::min(spb, pc->spp);
-        uint32_t K = inline_nee ? 1u : (uint32_t)std::max(L.n_slots, 1);
-        r->ensure((size_t)WH * spb, K, o.keep_samples ? pc->spp : 0);
-        r->samples_spp = o.keep_samples ? pc->spp : 0;
-        // device light list (per frame, like the UBO upload)
-        if (!r->d_lights) KF_CHECK(hipMalloc(&r->d_lights, sizeof(DevLights)));
-        DevLights* d_lights = (DevLights*)r->d_lights;
-        KF_CHECK(hipMemcpyAsync(d_lights, &L, sizeof(DevLights), hipMemcpyHostToDevice, st));
-        // LCG jump-ahead for the jitter stream: sample i starts after 2i draws
-        std::vector<uint2> jump(pc->spp);
-        {
-            uint32_t a = 1, c = 0;  // identity
-            for (uint32_t i = 0; i < pc->spp; ++i) {
-                jump[i] = make_uint2(a, c);
-                for (int k = 0; k < 2; ++k) {  // compose with one LCG step
-                    a = 1664525u * a;
-                    c = 1664525u * c + 1013904223u;
-                }
-            }
-        }
-        if (pc->spp > r->jump_cap) {
-            if (r->d_jump) KF_CHECK(hipFree(r->d_jump));
-            KF_CHECK(hipMalloc((void**)&r->d_jump, pc->spp * sizeof(uint2)));
-            r->jump_cap = pc->spp;
-        }
-        uint2* d_jump = r->d_jump;
-        KF_CHECK(hipMemcpyAsync(d_jump, jump.data(), pc->spp * sizeof(uint2), hipMemcpyHostToDevice, st));
-        if (!r->d_acc) KF_CHECK(hipMalloc((void**)&r->d_acc, (size_t)WH * sizeof(float4)));
-        float4* d_acc = r->d_acc;
-        if ((size_t)WH * spb > r->out_cap) {
-            if (r->d_out) KF_CHECK(hipFree(r->d_out));
-            KF_CHECK(hipMalloc((void**)&r->d_out, (size_t)WH * spb * sizeof(float4)));
-            r->out_cap = (size_t)WH * spb;
-        }
-        float4* d_out = r->d_out;
+        if (s && s->n_env > 1) throw Error("kf_render: the scene has environments; use kf_render_envs");
+        render_frame(r, s, 1u, cam, pc, opts, stats);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
 
-        SceneView S = s->view();
-        Frame F;
-        F.cam = *cam;
-        F.pc = *pc;
-        F.W = r->W;
-        F.H = r->H;
-        F.WH = WH;
-        F.S0 = tea_host(o.seed, (uint32_t)pc->frame_count);
-        F.slots = K;
-        F.inline_nee = inline_nee;
-        F.count = o.count_traversal;
-
-        if (r->ev.empty()) {
-            r->ev.resize(2 + 4 * kMaxBounces);
-            for (auto& e : r->ev) KF_CHECK(hipEventCreate(&e));
-            r->sync_ev.resize(2 * kMaxBounces);
-            for (auto& e : r->sync_ev) KF_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            KF_CHECK(hipStreamCreateWithFlags(&r->st2, hipStreamNonBlocking));  // default priority
-        }
-        int dev = 0;
-        KF_CHECK(hipGetDevice(&dev));
-        int n_cu = 256;
-        KF_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-        const uint32_t grid = (uint32_t)n_cu * 8;
-        const uint32_t refill = traversal_refill();
-        // Persistent traversal kernels: exactly the blocks that fit at once.
-        static const uint32_t grid_ext0 = resident_grid(k_extend<false>, n_cu);
-        static const uint32_t grid_ext1 = resident_grid(k_extend<true>, n_cu);
-        static const uint32_t grid_sh = resident_grid(k_shadow, n_cu);
-
-        unsigned long long totals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        uint32_t n_ext = 0, n_sh = 0;
-        float ms_total = 0.f, ms_ext = 0.f, ms_sh = 0.f;
-        for (uint32_t s0 = 0; s0 < pc->spp; s0 += spb) {
-            F.s0 = s0;
-            F.nsamp = std::min(spb, pc->spp - s0);
-            uint32_t B = WH * F.nsamp;
-            k_zero<<<(kCntSize + TB - 1) / TB, TB, 0, st>>>(r->counters, kCntSize);
-            KF_CHECK(hipEventRecord(r->ev[0], st));
-            k_raygen<<<std::min<uint32_t>((B + TB - 1) / TB, 65535u), TB, 0, st>>>(
-                F, d_jump, r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->seed, r->queue_a, r->counters);
-            uint32_t* qa = r->queue_a;
-            uint32_t* qb = r->queue_b;
-            uint32_t nb = pc->path_depth + 1;
-            // Per bounce b:  A: shade(b) -> [B: shadow(b)] , A: extend(b+1)
-            // -> (wait B) connect(b).  k_shade lists the paths that continue
-            // unless Russian roulette ends them, so the closest-hit rays of
-            // bounce b+1 are traced while the shadow rays of bounce b are,
-            // on the second stream; k_connect then waits for both.
-            auto launch_extend = [&](uint32_t b, const uint32_t* queue) {
-                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
-                if (o.count_traversal)
-                    k_extend<true><<<grid_ext1, TB, 0, st>>>(S, (int)b, B, queue, r->ray_o, r->ray_d, r->seed, r->hit,
-                                                             r->hit_prim, r->counters, refill, WH, F.s0, F.S0);
-                else
-                    k_extend<false><<<grid_ext0, TB, 0, st>>>(S, (int)b, B, queue, r->ray_o, r->ray_d, r->seed,
-                                                              r->hit, r->hit_prim, r->counters, refill, WH, F.s0,
-                                                              F.S0);
-                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 1], st));
-                ++n_ext;
-            };
-            launch_extend(0, qa);
-            for (uint32_t b = 0; b < nb; ++b) {
-                if (inline_nee)
-                    k_shade<true><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->hit,
-                                                       r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis,
-                                                       r->sh_o, r->sh_d, r->d_albedo, r->d_normal, r->queue_e,
-                                                       r->counters);
-                else
-                    k_shade<false><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr,
-                                                        r->hit, r->hit_prim, r->rw, r->em, r->seed, r->nee_a,
-                                                        r->nee_b, r->vis, r->sh_o, r->sh_d, r->d_albedo, r->d_normal,
-                                                        r->queue_e, r->counters);
-                if (!inline_nee) {
-                    KF_CHECK(hipEventRecord(r->sync_ev[2 * b], st));
-                    KF_CHECK(hipStreamWaitEvent(r->st2, r->sync_ev[2 * b], 0));
-                    KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], r->st2));
-                    k_shadow<<<grid_sh, TB, 0, r->st2>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->vis, r->counters,
-                                                         refill);
-                    KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], r->st2));
-                    KF_CHECK(hipEventRecord(r->sync_ev[2 * b + 1], r->st2));
-                    ++n_sh;
-                }
-                if (b + 1 < nb) launch_extend(b + 1, r->queue_e);
-                if (!inline_nee) KF_CHECK(hipStreamWaitEvent(st, r->sync_ev[2 * b + 1], 0));
-                if (inline_nee)
-                    k_connect<true><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
-                                                         r->nee_a, r->nee_b, r->vis, d_out, r->counters);
-                else
-                    k_connect<false><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
-                                                          r->nee_a, r->nee_b, r->vis, d_out, r->counters);
-                k_next_count<<<1, 1, 0, st>>>((int)b, r->counters);
-                std::swap(qa, qb);
-            }
-            bool last = s0 + F.nsamp >= pc->spp;
-            k_resolve<<<(WH + TB - 1) / TB, TB, 0, st>>>(F, d_out, d_acc, r->d_rgba,
-                                                         o.keep_samples ? r->samples : nullptr, last ? 1 : 0);
-            KF_CHECK(hipEventRecord(r->ev[1], st));
-            KF_CHECK(hipGetLastError());
-            unsigned long long h[kCntSize];
-            KF_CHECK(hipMemcpyAsync(h, r->counters, sizeof(h), hipMemcpyDeviceToHost, st));
-            KF_CHECK(hipStreamSynchronize(st));
-            for (int k = 0; k < 8; ++k) totals[k] += h[kCntGlobal + k];
-            float ms = 0.f;
-            KF_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
-            ms_total += ms;
-            for (uint32_t b = 0; b < nb; ++b) {
-                float a = 0.f, c = 0.f;
-                KF_CHECK(hipEventElapsedTime(&a, r->ev[2 + 4 * b + 0], r->ev[2 + 4 * b + 1]));
-                if (!inline_nee) KF_CHECK(hipEventElapsedTime(&c, r->ev[2 + 4 * b + 2], r->ev[2 + 4 * b + 3]));
-                ms_ext += a;
-                ms_sh += c;
-            }
-        }
-        if (totals[5]) throw Error("kf_render: traversal stack overflow (BVH deeper than the stack)");
-        if (stats) {
-            stats->primary_rays = totals[0];
-            stats->extension_rays = totals[1];
-            stats->shadow_rays = totals[2];
-            stats->node_visits = totals[3];
-            stats->tri_tests = totals[4];
-            stats->instance_visits = totals[6];
-            stats->speculative_rays = totals[7];
-            stats->trace_ms = ms_total;
-            stats->extend_ms = ms_ext;
-            stats->shadow_ms = ms_sh;
-            stats->other_ms = ms_total - ms_ext - ms_sh;
-            stats->extend_launches = n_ext;
-            stats->shadow_launches = n_sh;
-        }
+extern "C" int kf_render_envs(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_camera_ubo* cams,
+                              const kf_push_constants* pc, const kf_render_opts* opts, kf_render_stats* stats) {
+    try {
+        render_frame(r, s, num_envs, cams, pc, opts, stats);
         return 0;
     } catch (const std::exception& e) {
         set_error(e.what());

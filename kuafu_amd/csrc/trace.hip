@@ -18,11 +18,14 @@ static_assert(TB == (int)kRingThreads, "traversal kernels index their LDS stack 
 // overflow / malformed tree, [4..11] work-chunk counters of the persistent grid.
 constexpr int kTraceCnt = 12;
 
-KF_DEV void ray_of(const kf_ray& r, uint32_t q, TRay& t) {
+// Fixed batches trace environment 0 (the whole TLAS of a scene without
+// environments).
+KF_DEV void ray_of(const SceneView& S, const kf_ray& r, uint32_t q, TRay& t) {
     t.o = mk(r.org[0], r.org[1], r.org[2]);
     t.d = mk(r.dir[0], r.dir[1], r.dir[2]);
     t.tmin = r.tmin;
     t.tmax = r.tmax;
+    t.root = env_root_item(S, 0u);
     t.alpha_seed = 0u;
     t.payload = q;
 }
@@ -33,7 +36,7 @@ __global__ __launch_bounds__(TB) void k_closest(SceneView S, const kf_ray* __res
                                                 uint32_t refill) {
     uint32_t ovf = 0;
     TCount tot{0, 0, 0};
-    auto load = [&](uint32_t q, TRay& t) __attribute__((always_inline)) { ray_of(rays[q], q, t); };
+    auto load = [&](uint32_t q, TRay& t) __attribute__((always_inline)) { ray_of(S, rays[q], q, t); };
     auto store = [&](uint32_t q, THit b, bool) __attribute__((always_inline)) {
         kf_hit o;
         o.t = b.t;
@@ -57,7 +60,7 @@ __global__ __launch_bounds__(TB) void k_any(SceneView S, const kf_ray* __restric
                                             uint32_t refill) {
     uint32_t ovf = 0;
     TCount tot{0, 0, 0};
-    auto load = [&](uint32_t q, TRay& t) __attribute__((always_inline)) { ray_of(rays[q], q, t); };
+    auto load = [&](uint32_t q, TRay& t) __attribute__((always_inline)) { ray_of(S, rays[q], q, t); };
     auto store = [&](uint32_t q, THit, bool found) __attribute__((always_inline)) { occ[q] = found ? 1 : 0; };
     trace_persistent<true, false>(S, n, cnt + 4, refill, false, load, store, ovf, tot);
     if (ovf) atomicAdd(&cnt[2], 1ull);

@@ -158,13 +158,12 @@ __device__ const float4 kZeroRecord[8] = {};
 struct TravScene {
     const float4 *tlas, *blas, *inst, *tri;
     uint32_t n_tlas, n_blas, n_inst, n_tri;
-    int root;
     const float4* zero;  // 128 zero bytes, read in place of an out-of-range record
     const kf_material* mats;
     const uint32_t *matidx, *toff;
     KF_DEV explicit TravScene(const SceneView& S)
         : tlas((const float4*)S.tlas), blas((const float4*)S.blas), inst((const float4*)S.inst), tri(S.tri),
-          n_tlas(S.n_tlas_nodes), n_blas(S.n_blas_nodes), n_inst(S.n_inst), n_tri(S.n_tri), root(S.tlas_root),
+          n_tlas(S.n_tlas_nodes), n_blas(S.n_blas_nodes), n_inst(S.n_inst), n_tri(S.n_tri),
           zero(kZeroRecord), mats(S.mats), matidx(S.matidx), toff(S.toff) {}
 };
 
@@ -212,16 +211,24 @@ struct RingStack {
     }
 };
 
-// One query: ray, t range, alpha coin seed and an opaque payload for the
-// caller (where to write the result).
+// One query: ray, t range, the TLAS root item it starts from (its
+// environment's; kEmptyItem: nothing to hit), alpha coin seed and an opaque
+// payload for the caller (where to write the result).
 struct TRay {
     v3 o, d;
     float tmin, tmax;
-    uint32_t alpha_seed, payload;
+    uint32_t root, alpha_seed, payload;
 };
 
 KF_DEV uint32_t root_item(int root) {
     return root >= 0 ? make_item(kTlasNode, (uint32_t)root) : make_item(kInstance, (uint32_t)~root);
+}
+
+// Root item of environment e's TLAS (the whole TLAS for a scene without
+// environments); kEmptyItem when the environment has no instance.
+KF_DEV uint32_t env_root_item(const SceneView& S, uint32_t e) {
+    if (S.n_env > 1) return S.env_root[e < S.n_env ? e : 0u];
+    return S.tlas_root == kEmptyRef ? kEmptyItem : root_item(S.tlas_root);
 }
 
 // ANY: shadow-ray query (opaque, stop at the first hit in [tmin, tmax]).
@@ -466,8 +473,8 @@ struct Trav {
 
 // Whole query for one lane (used where rays are generated inside a kernel).
 template <bool ANY, bool COUNT>
-KF_DEV bool traverse(const SceneView& S, v3 o, v3 d, float tmin, float tmax, bool use_alpha, uint32_t alpha_seed,
-                     THit& best, TCount& cnt, uint32_t& overflow) {
+KF_DEV bool traverse(const SceneView& S, uint32_t root, v3 o, v3 d, float tmin, float tmax, bool use_alpha,
+                     uint32_t alpha_seed, THit& best, TCount& cnt, uint32_t& overflow) {
     const TravScene TS(S);
     Trav<ANY, COUNT> T;
     PrivStack stk;
@@ -477,10 +484,11 @@ KF_DEV bool traverse(const SceneView& S, v3 o, v3 d, float tmin, float tmax, boo
     r.d = d;
     r.tmin = tmin;
     r.tmax = tmax;
+    r.root = root;
     r.alpha_seed = alpha_seed;
     r.payload = 0u;
-    T.start(true, r, root_item(TS.root), use_alpha);
-    if (TS.root != kEmptyRef)
+    T.start(true, r, root, use_alpha);
+    if (root != kEmptyItem)
         while (T.step(TS, stk, overflow, true)) {
         }
     best = T.best;
@@ -511,20 +519,6 @@ template <bool ANY, bool COUNT, class Load, class Store>
 KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long* __restrict__ ctr, uint32_t refill,
                              bool use_alpha, Load load, Store store, uint32_t& overflow, TCount& total) {
     const TravScene TS(S);
-    if (TS.root == kEmptyRef) {  // empty scene: every query misses
-        THit miss;
-        miss.b1 = miss.b2 = 0.0f;
-        miss.inst = miss.prim = ~0u;
-        miss.back = 0u;
-        for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-            TRay r;
-            load(q, r);
-            miss.t = r.tmax;
-            store(r.payload, miss, false);
-        }
-        return;
-    }
-    const uint32_t root = root_item(TS.root);
     Trav<ANY, COUNT> T;
     T.reset();
     __shared__ uint32_t ring[kRing * kRingThreads];
@@ -544,6 +538,7 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
         TRay r;  // the ray this lane fetches in this iteration, if any
         r.o = r.d = mk(0.0f, 0.0f, 0.0f);
         r.tmin = r.tmax = 0.0f;
+        r.root = kEmptyItem;
         r.alpha_seed = r.payload = 0u;
         if ((res_n > 0u || moved < 8u) && (nidle >= refill || nidle == 64u)) {
             const uint32_t rank = (uint32_t)__popcll(idle & below);
@@ -579,11 +574,12 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
             }
         }
         if (__ballot(fresh) != 0ull) {  // wave-uniform: most iterations fetch nothing
-            T.start(fresh, r, root, use_alpha);
+            T.start(fresh, r, r.root, use_alpha);
             payload = fresh ? r.payload : payload;
-            // An empty t range (tmax < tmin, or NaN) has nothing to trace:
-            // finish it at once as a miss.
-            const bool empty = fresh && !(r.tmin <= r.tmax);
+            // An empty t range (tmax < tmin, or NaN) or an environment
+            // without instances has nothing to trace: finish it at once as a
+            // miss.
+            const bool empty = fresh && (!(r.tmin <= r.tmax) || r.root == kEmptyItem);
             if (empty) store(r.payload, T.best, false);
             live = live || (fresh && !empty);
         }

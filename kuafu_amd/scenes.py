@@ -1,4 +1,4 @@
-"""Workload scenes: meshes, the five benchmark configurations and small test scenes.
+"""Workload scenes: meshes, the benchmark configurations and small test scenes.
 
 Mesh generators follow the reference's conventions: outward winding
 (cross(v1 - v0, v2 - v0) points out of the surface, as in createSphere /
@@ -256,6 +256,74 @@ def config3(width=1280, height=720, spp=32, path_depth=6, seed=0, camera="left")
     cam = camera_ubo(eye, front, (0, 0, 1), width, height, fx=width * 0.7, fy=width * 0.7)
     pc = push_constants(spp=spp, path_depth=path_depth, clear_color=(0.0, 0.0, 0.0, 1.0))
     return Workload("config3_active_light_720p_32spp_" + camera, s, cam, pc, width, height, seed)
+
+
+def config5(n_env=128, size=512, spp=8, path_depth=6, seed=0, tri_range=(5000, 400000), n_assets=160):
+    """Batched environments: n_env SAPIEN-style rooms, one size x size camera each, 8 spp, depth 6.
+
+    A pool of n_assets geometries (icospheres of subdivision 1-5, boxes, cylinders; materials are
+    per geometry in the reference API, drawn per asset as in config2) is instanced into every
+    environment on a 10 m ground plane until the environment's triangle budget, log-uniform in
+    tri_range, is used up (larger assets are drawn in proportion to their size, so big budgets
+    take tens of instances, not thousands).  Each environment has its own TLAS and camera; the sun,
+    a point light and the gradient sky are shared.  The render target stacks the environments'
+    images vertically (width size, height size * n_env)."""
+    rng = np.random.default_rng(seed)
+    s = SceneData()
+    spheres = {k: icosphere(k) for k in (1, 2, 3, 4, 5)}  # 80 .. 20480 triangles
+    asset_tris = []
+    for a in range(n_assets):
+        kind = rng.uniform()
+        albedo = rng.uniform(0, 1, 3)
+        m = material(diffuse=tuple(albedo), roughness=rng.uniform(0.02, 1.0),
+                     metallic=1.0 if rng.uniform() < 0.3 else 0.0,
+                     transmission=1.0 if rng.uniform() < 0.15 else 0.0, ior=rng.uniform(1.3, 1.7))
+        if kind < 0.6:
+            v, t = spheres[int(rng.integers(1, 6))]
+        elif kind < 0.8:
+            v, t = box(tuple(rng.uniform(0.5, 1.5, 3)))
+        else:
+            v, t = cylinder(int(rng.integers(6, 33)), 0.5, rng.uniform(0.5, 2.0))
+        s.add_geometry(v, t, m)
+        asset_tris.append(len(t))
+    asset_tris = np.array(asset_tris)
+    gv, gt = quad(10.0)
+    grounds = [s.add_geometry(gv, gt, material(diffuse=tuple(rng.uniform(0.3, 0.8, 3)), roughness=0.9))
+               for _ in range(8)]
+    offsets = [0]
+    cams = []
+    env_tris = []
+    lo, hi = math.log(tri_range[0]), math.log(tri_range[1])
+    for e in range(n_env):
+        budget = int(math.exp(rng.uniform(lo, hi)))
+        s.add_instance(grounds[int(rng.integers(0, len(grounds)))])
+        used = 2
+        while True:
+            fit = np.nonzero(asset_tris <= budget - used)[0]
+            if len(fit) == 0:
+                break
+            w = asset_tris[fit].astype(np.float64)
+            a = int(fit[rng.choice(len(fit), p=w / w.sum())])
+            scale = rng.uniform(0.15, 0.5)
+            R = random_rotation(rng) * scale
+            t = (rng.uniform(-2.0, 2.0), rng.uniform(-2.0, 2.0), scale)
+            s.add_instance(a, mat_from_rows(R, t))
+            used += int(asset_tris[a])
+        offsets.append(len(s.instances))
+        env_tris.append(used)
+        az, el, dist = rng.uniform(0, 2 * math.pi), rng.uniform(0.35, 0.7), rng.uniform(5.0, 6.5)
+        eye = np.array([dist * math.cos(el) * math.cos(az), dist * math.cos(el) * math.sin(az), dist * math.sin(el)])
+        target = np.array([0.0, 0.0, 0.3])
+        front = (target - eye) / np.linalg.norm(target - eye)
+        cams.append(camera_ubo(eye, front, (0, 0, 1), size, size, fx=size * 0.9, fy=size * 0.9))
+    s.env_offsets = np.array(offsets, np.uint32)
+    s.set_directional_light((-0.3, -0.2, -1.0), (1.0, 0.97, 0.9), 2.5, 0.0)
+    s.add_point_light((0.0, 0.0, 4.0), (1.0, 1.0, 1.0), 20.0, radius=0.0)
+    s.set_gradient_environment((0.05, 0.05, 0.06), (0.35, 0.45, 0.65))
+    pc = push_constants(spp=spp, path_depth=path_depth, use_env_map=True)
+    w = Workload("config5_envs%d_%dx%d_%dspp" % (n_env, size, size, spp), s, cams, pc, size, size * n_env, seed)
+    w.n_env, w.env_height, w.env_tris = n_env, size, np.array(env_tris)
+    return w
 
 
 def tiny_scene(seed=0, n_spheres=3, with_lights=True, soft=False):

@@ -1,0 +1,135 @@
+"""Batched environments on the GPU (marked gpu).
+
+A scene of several environments, each with its own TLAS tree and camera, is
+rendered in one frame by kf_render_envs.  Environment e's rows of the stacked
+target must equal, bit for bit, a plain kf_render of a scene holding only
+e's instances (same seeds, same W x He raster), and match the FP32 oracle on
+that scene.  Covered root kinds: an inner BVH4 node (several instances), a
+single instance leaf, an empty environment.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from kuafu_amd import abi, scenes
+from kuafu_amd.api import GpuScene, Renderer
+
+pytestmark = pytest.mark.gpu
+
+
+def small_envs():
+    """Three generated environments plus an empty one and a one-instance one:
+    instance ranges [0,a) [a,a) [a,b) [b,c) [c,c+1)."""
+    w = scenes.config5(n_env=3, size=32, spp=2, path_depth=3, seed=5, tri_range=(300, 6000), n_assets=12)
+    d = w.data
+    a, b, c = (int(x) for x in d.env_offsets[1:])
+    d.instances.append(d.instances[0])  # a ground plane alone
+    d.env_offsets = np.array([0, a, a, b, c, c + 1], np.uint32)
+    cams = [w.cam[0], w.cam[1], w.cam[1], w.cam[2], w.cam[0]]
+    return w, d, cams
+
+
+def same(x, y):
+    return ((x == y) | (np.isnan(x) & np.isnan(y))).all()
+
+
+def test_env_bvh4_trees(gpu):
+    """Every environment's TLAS BVH4 holds exactly its own instances, its
+    nodes inside its own range of the array, each reached once."""
+    from test_gpu_parity import walk_bvh4
+    w, d, _ = small_envs()
+    gs = GpuScene(d)
+    nodes, roots = gs.read_bvh4(1)
+    off = [int(x) for x in d.env_offsets]
+    node_off = np.cumsum([0] + [max(off[e + 1] - off[e] - 1, 0) for e in range(len(off) - 1)])
+    info = gs.bvh_info()
+    assert info.tlas_nodes == node_off[-1] and info.instances == len(d.instances)
+    total = 0
+    for e in range(len(off) - 1):
+        n_e = off[e + 1] - off[e]
+        if n_e == 0:
+            assert roots[e] == 0x7fffffff
+            continue
+        if n_e == 1:
+            assert roots[e] == ~off[e]
+            continue
+        assert roots[e] == node_off[e]
+        seen, leaves = walk_bvh4(nodes, int(roots[e]), lambda i, lo, hi: None, 0)
+        used = np.nonzero(seen)[0]
+        assert (seen[used] == 1).all()
+        assert used.min() >= node_off[e] and used.max() < node_off[e + 1]
+        assert sorted(leaves) == list(range(off[e], off[e + 1]))
+        total += len(used)
+    assert info.tlas4_nodes == total
+
+
+def test_render_envs_equal_single_environment_renders(gpu):
+    w, d, cams = small_envs()
+    n_env, size = len(cams), w.env_height
+    pc = w.pc
+    pc.frame_count = 0
+    gs = GpuScene(d)
+    rd = Renderer(size, size * n_env)
+    st = rd.render_envs(gs, cams, pc, seed=3, keep_samples=True)
+    smp = rd.read_samples(pc.spp)
+    tgt = [rd.read(k) for k in range(3)]
+    sums = np.zeros(3, np.int64)
+    for e in range(n_env):
+        sub = d.environment(e)
+        r1 = Renderer(size, size)
+        st1 = r1.render(GpuScene(sub), cams[e], pc, seed=3, keep_samples=True)
+        rows = slice(e * size, (e + 1) * size)
+        assert same(smp[:, rows], r1.read_samples(pc.spp)), "environment %d samples" % e
+        for k in range(3):
+            assert same(tgt[k][rows], r1.read(k)), "environment %d target %d" % (e, k)
+        sums += (st1.primary_rays, st1.extension_rays, st1.shadow_rays)
+    assert (st.primary_rays, st.extension_rays, st.shadow_rays) == tuple(int(x) for x in sums)
+    # the environments differ (distinct cameras and content)
+    assert not same(tgt[0][0:size], tgt[0][2 * size:3 * size])
+    # the empty environment shows the sky only; environment 3 against the oracle
+    o = oracle.Scene(d.environment(3))
+    ref = o.render(cams[3], pc, size, size, seed=3, threads=8, keep_samples=True)
+    rs = ref["samples"]
+    got = smp[:, 3 * size:4 * size]
+    close = (np.abs(got - rs) <= 1e-4 * np.maximum(1.0, np.abs(rs))) | (np.isnan(got) & np.isnan(rs))
+    assert close.all(-1).mean() >= 0.999
+
+
+def test_render_envs_rejects_bad_calls(gpu):
+    w, d, cams = small_envs()
+    gs = GpuScene(d)
+    n_env, size = len(cams), w.env_height
+    rd = Renderer(size, size * n_env)
+    with pytest.raises(RuntimeError, match="environments"):
+        rd.render(gs, cams[0], w.pc)
+    with pytest.raises(RuntimeError, match="number of cameras"):
+        rd.render_envs(gs, cams[:2], w.pc)
+    with pytest.raises(RuntimeError, match="multiple"):
+        Renderer(size, size * n_env + 1).render_envs(gs, cams, w.pc)
+    inst = d.arrays()["inst"]
+    with pytest.raises(RuntimeError, match="instance count"):
+        gs.update_instances(inst[:-1])
+    # a batched scene keeps working after the rejected calls
+    rd.render_envs(gs, cams, w.pc)
+    assert np.isfinite(rd.read(0)).all()
+
+
+def test_env_instance_update(gpu):
+    """kf_scene_update_instances on a batched scene (every environment's
+    tree rebuilt in one launch) equals a scene created with the new
+    transforms."""
+    w, d, cams = small_envs()
+    gs = GpuScene(d)
+    n_env, size = len(cams), w.env_height
+    rng = np.random.default_rng(9)
+    import copy
+    moved = copy.copy(d)
+    moved.instances = [(np.asarray(t, np.float32).copy(), g) for t, g in d.instances]
+    for t, g in moved.instances[1:]:
+        t[3, :2] += rng.uniform(-0.5, 0.5, 2).astype(np.float32)
+    gs.update_instances(moved.arrays()["inst"])
+    rd = Renderer(size, size * n_env)
+    rd.render_envs(gs, cams, w.pc, seed=4)
+    rd2 = Renderer(size, size * n_env)
+    rd2.render_envs(GpuScene(moved), cams, w.pc, seed=4)
+    assert same(rd.read(0), rd2.read(0))

@@ -126,3 +126,35 @@ def test_scene_validation_errors():
     bad.add_instance(4)
     with pytest.raises(KuafuError, match="Geometry index is out of bounds"):
         GpuScene(bad)
+
+
+def test_batched_environment_workload():
+    # triangle budgets log-uniform in [lo, hi]: every environment within its
+    # range, the instance ranges cover every instance once, one camera each
+    w = scenes.config5(n_env=64, size=16, seed=2)
+    off = w.data.env_offsets
+    assert off[0] == 0 and off[-1] == len(w.data.instances) and (np.diff(off) >= 1).all()
+    assert len(w.cam) == 64 and w.height == 16 * 64
+    assert (w.env_tris >= 5000 * 0.99).all() and (w.env_tris <= 400000).all()
+    # the median of log-uniform [5k, 400k] is sqrt(5k * 400k) ~ 44.7k
+    assert 15000 < np.median(w.env_tris) < 130000
+    geoms = [g for _, g in w.data.instances]
+    tri = np.array([len(w.data.geoms[g][1]) for g in geoms])
+    per_env = np.add.reduceat(tri, off[:-1])
+    assert (per_env == w.env_tris).all()
+    sub = w.data.environment(5)
+    assert len(sub.instances) == off[6] - off[5] and sub.env_offsets is None
+
+
+def test_environment_offset_validation():
+    s = SceneData()
+    v, t = scenes.uv_sphere(8, 4)
+    g = s.add_geometry(v, t, material())
+    for _ in range(3):
+        s.add_instance(g)
+    s.env_offsets = np.array([0, 2, 1, 3], np.uint32)
+    with pytest.raises(KuafuError, match="non-decreasing"):
+        GpuScene(s)
+    s.env_offsets = np.array([0, 1, 2], np.uint32)
+    with pytest.raises(KuafuError, match="from 0 to num_instances"):
+        GpuScene(s)
