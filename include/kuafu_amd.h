@@ -193,6 +193,10 @@ typedef struct kf_render_opts {
     uint32_t samples_per_batch; /* wavefront batch size in samples per pixel; 0 = auto */
     uint32_t keep_samples;    /* keep per-sample radiance for kf_renderer_read_samples */
     uint32_t count_traversal; /* count node visits / triangle tests */
+    uint32_t use_graph;       /* capture the frame's launches in a HIP graph once and replay it while
+                                 the scene, target, options and schedule stay the same (seed, frame
+                                 count, cameras and lights may change); stats then carry the frame's
+                                 kernel time only (extend_ms / shadow_ms are 0) */
 } kf_render_opts;
 
 /* ------------------------------------------------------------------------ */
@@ -311,6 +315,23 @@ int kf_render(kf_renderer* r, kf_scene* s, const kf_camera_ubo* cam, const kf_pu
 int kf_render_envs(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_camera_ubo* cams,
                    const kf_push_constants* pc, const kf_render_opts* opts, kf_render_stats* stats);
 
+/* Autotuner of the wavefront schedule (no counterpart in the reference,
+ * whose schedule is the driver's): times frames of this scene and target
+ * over the batch size (samples per batch, i.e. the wavefront queue length),
+ * the traversal's refill threshold and its work reservation per wave (the
+ * tile a persistent wave takes from its XCD's queue), one parameter after
+ * the other, and keeps the fastest for later frames rendered with
+ * samples_per_batch = 0.  Images do not depend on the schedule.  Renders
+ * into r's targets (frame_count 0). */
+typedef struct kf_tune_result {
+    uint32_t samples_per_batch, refill, grab;
+    float frame_ms;    /* kernel time of a frame with the picked schedule */
+    float default_ms;  /* ... with the built-in default */
+    uint32_t trials;   /* frames timed */
+} kf_tune_result;
+int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_camera_ubo* cams,
+                         const kf_push_constants* pc, kf_tune_result* out);
+
 /* Read a render target: 0 = rgba, 1 = albedo, 2 = normal; W*H*4 floats. */
 int kf_renderer_read(kf_renderer* r, int target, float* out);
 /* Overwrite the rgba target (running-mean state), W*H*4 floats. */
@@ -354,10 +375,16 @@ int kf_denoiser_set_params(kf_denoiser* d, const float* in);
 
 /* The denoise stage of a frame: the renderer's rgba target is replaced by the
  * denoised image (context.cpp:650-670 writes the OptiX output back to rgba). */
-int kf_denoiser_apply(kf_denoiser* d, kf_renderer* r);
+int kf_denoiser_apply(kf_denoiser* d, kf_renderer* r);  /* after kf_render_envs: per environment */
 /* Device pointers: rgba/albedo/normal W*H float4 in HBM -> out W*H float4. */
 int kf_denoiser_infer_device(kf_denoiser* d, const float* d_rgba, const float* d_albedo, const float* d_normal,
                              uint32_t width, uint32_t height, float* d_out, float* kernel_ms);
+/* Batched denoise (no counterpart in the reference): n_images images of
+ * width x height stacked in each buffer (the layout of a kf_render_envs
+ * frame), each denoised on its own, as one launch per layer. */
+int kf_denoiser_infer_device_batch(kf_denoiser* d, const float* d_rgba, const float* d_albedo,
+                                   const float* d_normal, uint32_t width, uint32_t height, uint32_t n_images,
+                                   float* d_out, float* kernel_ms);
 
 /* Training on host batches: inputs n*h*w*9 (RGB, albedo, normal), targets
  * n*h*w*3; one forward + backward + Adam step; h and w multiples of 16. */

@@ -89,9 +89,14 @@ class RenderStats(C.Structure):
         return int(self.primary_rays + self.extension_rays + self.shadow_rays)
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("samples_per_batch", C.c_uint32), ("refill", C.c_uint32), ("grab", C.c_uint32),
+                ("frame_ms", C.c_float), ("default_ms", C.c_float), ("trials", C.c_uint32)]
+
+
 class RenderOpts(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("samples_per_batch", C.c_uint32), ("keep_samples", C.c_uint32),
-                ("count_traversal", C.c_uint32)]
+                ("count_traversal", C.c_uint32), ("use_graph", C.c_uint32)]
 
 
 class BvhInfo(C.Structure):
@@ -140,6 +145,7 @@ EXPORTS = [
     ("kf_render", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("kf_render_envs", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]),
+    ("kf_renderer_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("kf_renderer_read", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_write", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_read_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
@@ -157,6 +163,8 @@ EXPORTS = [
     ("kf_denoiser_apply", C.c_int, [C.c_void_p, C.c_void_p]),
     ("kf_denoiser_infer_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
                                            C.c_void_p, C.c_void_p]),
+    ("kf_denoiser_infer_device_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                 C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("kf_denoiser_train_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_float, C.c_void_p]),
     ("kf_denoiser_gradients", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,

@@ -374,20 +374,33 @@ class Renderer:
         except Exception:
             pass
 
-    def render(self, scene, cam, pc, seed=0, samples_per_batch=0, keep_samples=False, count_traversal=False):
-        o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal))
+    def render(self, scene, cam, pc, seed=0, samples_per_batch=0, keep_samples=False, count_traversal=False,
+               use_graph=False):
+        o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal), int(use_graph))
         st = abi.RenderStats()
         check(lib().kf_render(self.h, scene.h, C.byref(cam), C.byref(pc), C.byref(o), C.byref(st)))
         return st
 
-    def render_envs(self, scene, cams, pc, seed=0, samples_per_batch=0, keep_samples=False, count_traversal=False):
+    def render_envs(self, scene, cams, pc, seed=0, samples_per_batch=0, keep_samples=False, count_traversal=False,
+                    use_graph=False):
         """One frame of every environment of a batched scene: cams[e] views
         environment e, whose image is rows [e*H/n, (e+1)*H/n) of the targets."""
         arr = (abi.CameraUBO * len(cams))(*cams)
-        o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal))
+        o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal), int(use_graph))
         st = abi.RenderStats()
         check(lib().kf_render_envs(self.h, scene.h, len(cams), arr, C.byref(pc), C.byref(o), C.byref(st)))
         return st
+
+    def autotune(self, scene, cams, pc):
+        """Pick the wavefront schedule (samples per batch, refill threshold,
+        reservation per wave) by timing frames; later renders with
+        samples_per_batch=0 use it."""
+        cams = cams if isinstance(cams, (list, tuple)) else [cams]
+        arr = (abi.CameraUBO * len(cams))(*cams)
+        t = abi.TuneResult()
+        check(lib().kf_renderer_autotune(self.h, scene.h, len(cams), arr, C.byref(pc), C.byref(t)))
+        return dict(samples_per_batch=t.samples_per_batch, refill=t.refill, grab=t.grab, frame_ms=t.frame_ms,
+                    default_ms=t.default_ms, trials=t.trials)
 
     def read(self, target=0):
         out = np.zeros((self.h_, self.w, 4), np.float32)

@@ -315,13 +315,16 @@ __device__ inline float sane(float v) { return (v == v) ? fminf(fmaxf(v, -1e4f),
 // Network input from float4 render targets (one image, edge-replicated to
 // Hp x Wp): ch 0-2 log1p(rgb), 3-5 albedo * pi, 6-8 normal, 9-31 zero; lin
 // keeps log1p(rgb) for the residual.
+// N images of H x W stacked in the targets -> N padded Hp x Wp images
+// (edge pixels repeated into the padding).
 __global__ void k_pack_targets(const float4* __restrict__ rgba, const float4* __restrict__ alb,
-                               const float4* __restrict__ nrm, int H, int W, int Hp, int Wp, uint16_t* __restrict__ X,
-                               float* __restrict__ lin) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Hp * Wp) return;
-    int x = min(i % Wp, W - 1), y = min(i / Wp, H - 1);
-    int s = y * W + x;
+                               const float4* __restrict__ nrm, int N, int H, int W, int Hp, int Wp,
+                               uint16_t* __restrict__ X, float* __restrict__ lin) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * Hp * Wp) return;
+    const int img = (int)(i / ((size_t)Hp * Wp)), j = (int)(i % ((size_t)Hp * Wp));
+    int x = min(j % Wp, W - 1), y = min(j / Wp, H - 1);
+    size_t s = ((size_t)img * H + y) * W + x;
     float4 c = rgba[s], a = alb[s], n = nrm[s];
     float v[9] = {log1pf(fmaxf(sane(c.x), 0.f)), log1pf(fmaxf(sane(c.y), 0.f)), log1pf(fmaxf(sane(c.z), 0.f)),
                   sane(a.x) * 3.141592f, sane(a.y) * 3.141592f, sane(a.z) * 3.141592f, sane(n.x), sane(n.y),
@@ -354,12 +357,13 @@ __global__ void k_pack_records(const float* __restrict__ in, int P, uint16_t* __
 constexpr float kLogClamp = 10.0f;
 
 // denoised linear RGB = expm1(min(y + log1p(noisy), 10)), cropped to H x W
-__global__ void k_unpack(const float* __restrict__ y, const float* __restrict__ lin, int H, int W, int Hp, int Wp,
-                         float4* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= H * W) return;
-    int x = i % W, yy = i / W;
-    size_t s = (size_t)yy * Wp + x;
+__global__ void k_unpack(const float* __restrict__ y, const float* __restrict__ lin, int N, int H, int W, int Hp,
+                         int Wp, float4* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * H * W) return;
+    const int img = (int)(i / ((size_t)H * W)), j = (int)(i % ((size_t)H * W));
+    int x = j % W, yy = j / W;
+    size_t s = ((size_t)img * Hp + yy) * Wp + x;
     float r[3];
     for (int k = 0; k < 3; ++k) r[k] = expm1f(fminf(y[s * 4 + k] + lin[s * 4 + k], kLogClamp));
     out[i] = make_float4(r[0], r[1], r[2], 1.0f);
@@ -964,21 +968,30 @@ int kf_denoiser_load(kf_denoiser* d, const char* path) {
 
 int kf_denoiser_infer_device(kf_denoiser* d, const float* d_rgba, const float* d_albedo, const float* d_normal,
                              uint32_t width, uint32_t height, float* d_out, float* kernel_ms) {
+    return kf_denoiser_infer_device_batch(d, d_rgba, d_albedo, d_normal, width, height, 1, d_out, kernel_ms);
+}
+
+int kf_denoiser_infer_device_batch(kf_denoiser* d, const float* d_rgba, const float* d_albedo,
+                                   const float* d_normal, uint32_t width, uint32_t height, uint32_t n_images,
+                                   float* d_out, float* kernel_ms) {
     try {
-        if (!d || !d_rgba || !d_albedo || !d_normal || !d_out || !width || !height) throw Error("bad argument");
+        if (!d || !d_rgba || !d_albedo || !d_normal || !d_out || !width || !height || !n_images)
+            throw Error("bad argument");
         hipStream_t st = stream();
+        const int N = (int)n_images;
         int Hp = (int)((height + 15) / 16 * 16), Wp = (int)((width + 15) / 16 * 16);
-        d->act.ensure(1, Hp, Wp, false);
+        if ((uint64_t)N * Hp * Wp >= (1ull << 31)) throw Error("kf_denoiser_infer_device_batch: batch too large");
+        d->act.ensure(N, Hp, Wp, false);
         hipEvent_t e0, e1;
         KF_CHECK(hipEventCreate(&e0));
         KF_CHECK(hipEventCreate(&e1));
         KF_CHECK(hipEventRecord(e0, st));
-        k_pack_targets<<<blocks((size_t)Hp * Wp, TB), TB, 0, st>>>((const float4*)d_rgba, (const float4*)d_albedo,
-                                                                   (const float4*)d_normal, (int)height, (int)width,
-                                                                   Hp, Wp, d->act.x0, d->act.lin);
+        k_pack_targets<<<blocks((size_t)N * Hp * Wp, TB), TB, 0, st>>>(
+            (const float4*)d_rgba, (const float4*)d_albedo, (const float4*)d_normal, N, (int)height, (int)width, Hp,
+            Wp, d->act.x0, d->act.lin);
         d->forward(st);
-        k_unpack<<<blocks((size_t)width * height, TB), TB, 0, st>>>(d->act.y, d->act.lin, (int)height, (int)width, Hp,
-                                                                    Wp, (float4*)d_out);
+        k_unpack<<<blocks((size_t)N * width * height, TB), TB, 0, st>>>(d->act.y, d->act.lin, N, (int)height,
+                                                                        (int)width, Hp, Wp, (float4*)d_out);
         KF_CHECK(hipEventRecord(e1, st));
         KF_CHECK(hipGetLastError());
         KF_CHECK(hipEventSynchronize(e1));
@@ -999,9 +1012,11 @@ int kf_denoiser_apply(kf_denoiser* d, kf_renderer* r) {
         set_error("kf_denoiser_apply: null argument");
         return -1;
     }
-    // the output overwrites rgba (denoiser.cpp:225-251, context.cpp:669-670)
-    return kf_denoiser_infer_device(d, (const float*)r->d_rgba, (const float*)r->d_albedo, (const float*)r->d_normal,
-                                    r->W, r->H, (float*)r->d_rgba, nullptr);
+    // the output overwrites rgba (denoiser.cpp:225-251, context.cpp:669-670);
+    // a frame of environments is denoised as a batch of their images
+    const uint32_t n = std::max<uint32_t>(r->n_env_last, 1u);
+    return kf_denoiser_infer_device_batch(d, (const float*)r->d_rgba, (const float*)r->d_albedo,
+                                          (const float*)r->d_normal, r->W, r->H / n, n, (float*)r->d_rgba, nullptr);
 }
 
 int kf_denoiser_forward(kf_denoiser* d, const float* inputs, uint32_t n, uint32_t h, uint32_t w, float* out) {

@@ -217,6 +217,16 @@ struct Renderer {
     std::vector<hipEvent_t> ev;
     hipStream_t st2 = nullptr;                 // any-hit traversal, beside the main stream
     std::vector<hipEvent_t> sync_ev;           // per bounce: shading done, shadow rays done
+    void* d_fv = nullptr;                      // per-frame seed and count read by the kernels
+    // Frame graph (kf_render_opts.use_graph): every batch of a frame captured
+    // once and replayed while everything baked into its kernel arguments
+    // (graph_key) is unchanged.
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint8_t> graph_key;
+    uint32_t graph_ext = 0, graph_sh = 0;      // traversal launches in the graph
+    // Schedule picked by kf_renderer_autotune (0: built-in default).
+    uint32_t tune_spb = 0, tune_refill = 0, tune_grab = 0;
+    uint32_t n_env_last = 1;                   // environments of the last frame (the denoiser's batch)
     ~Renderer();
     void ensure(size_t paths, uint32_t nslots, uint32_t spp);
 };

@@ -46,6 +46,13 @@ constexpr int kCntExtQueue = 19;
 constexpr int kCntGlobal = kMaxBounces * kCntPerBounce;
 constexpr int kCntSize = kCntGlobal + 8;
 
+// Per-frame values the kernels read from device memory, so that a captured
+// frame graph replays with new seeds and running-mean weights.
+struct FrameVars {
+    uint32_t S0;          // frame seed
+    int32_t frame_count;  // running-mean position (rgen:153-163)
+};
+
 // The render target holds n_env images of W x He stacked vertically (one
 // image, He = H, without environments): pixel = env * WHe + (y * W + x), and
 // seeds, jitter and raster coordinates are those of the W x He image.
@@ -55,7 +62,7 @@ struct Frame {
     uint32_t W, H, WH;
     uint32_t He, WHe, n_env;
     uint32_t s0, nsamp;   // samples [s0, s0 + nsamp) in this batch
-    uint32_t S0;          // frame seed
+    const FrameVars* fv;  // frame seed and count (device)
     uint32_t slots;       // NEE slots per path
     int inline_nee;
     int count;            // count traversal steps
@@ -114,6 +121,7 @@ __global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2
                                                uint32_t* __restrict__ seed, uint32_t* __restrict__ queue,
                                                unsigned long long* __restrict__ cnt) {
     uint32_t n = F.WH * F.nsamp;
+    const uint32_t S0 = F.fv->S0;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         const uint32_t i = F.s0 + p / F.WH;
         const uint32_t env = (p % F.WH) / F.WHe;
@@ -121,12 +129,12 @@ __global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2
         const uint32_t x = pixel % F.W, y = pixel / F.W;
         const kf_camera_ubo& cam = F.cams[env];
         // jitter stream: tea(pixel, S0) advanced by 2i draws (rgen:23, 35)
-        uint32_t js = tea(pixel, F.S0);
+        uint32_t js = tea(pixel, S0);
         uint2 jm = jump[i];
         js = jm.x * js + jm.y;
         float r0 = rnd(js);
         float r1 = rnd(js);
-        uint32_t rs = tea(pixel, F.S0 + 1u + i);
+        uint32_t rs = tea(pixel, S0 + 1u + i);
         float posx = float(x) + r0, posy = float(y) + r1;
         float npx = posx / float(F.W), npy = posy / float(F.He);
         float dx = npx * 2.0f - 1.0f, dy = npy * 2.0f - 1.0f;
@@ -179,9 +187,10 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
                                                const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
                                                const uint32_t* __restrict__ seed, float4* __restrict__ hit,
                                                uint32_t* __restrict__ hit_prim, unsigned long long* __restrict__ cnt,
-                                               uint32_t refill, uint32_t WH, uint32_t WHe, uint32_t s0,
-                                               uint32_t S0) {
+                                               uint32_t refill, uint32_t grab, uint32_t WH, uint32_t WHe,
+                                               uint32_t s0, const FrameVars* __restrict__ fv) {
     const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + kCntExtQueue], (unsigned long long)cap);
+    const uint32_t S0 = S.any_alpha ? fv->S0 : 0u;
     uint32_t ovf = 0;
     TCount tot{0, 0, 0};
     auto load = [&](uint32_t q, TRay& r) __attribute__((always_inline)) {
@@ -200,8 +209,8 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
         hit[p] = make_float4(b.t, b.b1, b.b2, __uint_as_float(b.inst));
         hit_prim[p] = b.prim | (b.back << 31);
     };
-    trace_persistent<false, COUNT>(S, n, cnt + bounce * kCntPerBounce + kCntExtendChunks, refill, true, load, store, ovf,
-                                   tot);
+    trace_persistent<false, COUNT>(S, n, cnt + bounce * kCntPerBounce + kCntExtendChunks, refill, grab, true, load,
+                                   store, ovf, tot);
     if (COUNT) {
         wave_add(&cnt[kCntGlobal + 3], tot.nodes);
         wave_add(&cnt[kCntGlobal + 4], tot.tris);
@@ -483,7 +492,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
 __global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t cap, const float4* __restrict__ sh_o,
                                                const float4* __restrict__ sh_d, const uint32_t* __restrict__ sh_root,
                                                uint8_t* __restrict__ vis,
-                                               unsigned long long* __restrict__ cnt, uint32_t refill) {
+                                               unsigned long long* __restrict__ cnt, uint32_t refill, uint32_t grab) {
     const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 1], (unsigned long long)cap);
     const uint32_t root0 = env_root_item(S, 0u);
     uint32_t ovf = 0;
@@ -501,8 +510,8 @@ __global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t
     auto store = [&](uint32_t slot, THit, bool found) __attribute__((always_inline)) {
         if (!found) vis[slot] = 1;
     };
-    trace_persistent<true, false>(S, n, cnt + bounce * kCntPerBounce + kCntShadowChunks, refill, false, load, store, ovf,
-                                  tot);
+    trace_persistent<true, false>(S, n, cnt + bounce * kCntPerBounce + kCntShadowChunks, refill, grab, false, load,
+                                  store, ovf, tot);
     if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
 
@@ -609,10 +618,11 @@ __global__ __launch_bounds__(TB) void k_resolve(Frame F, const float4* __restric
         return;
     }
     v3 fc = c / float(F.pc.spp);
-    if (F.pc.frame_count <= 0) {
+    const int32_t frame_count = F.fv->frame_count;
+    if (frame_count <= 0) {
         rgba[px] = f4(fc, 1.0f);
     } else {
-        float a = 1.0f / float(F.pc.frame_count + 1);
+        float a = 1.0f / float(frame_count + 1);
         v3 old = xyz(rgba[px]);
         v3 m = old * (1.0f - a) + fc * a;
         rgba[px] = f4(m, 1.0f);
@@ -651,6 +661,7 @@ template <class T> T* dmalloc(size_t n) {
 }  // namespace
 
 Renderer::~Renderer() {
+    if (graph) (void)hipGraphExecDestroy(graph);
     if (st2) {
         (void)hipStreamSynchronize(st2);
         (void)hipStreamDestroy(st2);
@@ -659,7 +670,7 @@ Renderer::~Renderer() {
     for (auto e : sync_ev) (void)hipEventDestroy(e);
     void* ptrs[] = {d_rgba, d_albedo, d_normal, d_bgra, ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed,
                     queue_a, queue_b, queue_e, sh_o, sh_d, sh_root, nee_a, nee_b, vis, samples, counters, d_lights,
-                    d_jump, d_acc, d_out, d_cams};
+                    d_jump, d_acc, d_out, d_cams, d_fv};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -775,7 +786,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
     bool inline_nee = false;
     build_light_list(*s, L, inline_nee);
     uint32_t WH = r->W * r->H;
-    uint32_t spb = o.samples_per_batch ? o.samples_per_batch : 0;
+    uint32_t spb = o.samples_per_batch ? o.samples_per_batch : r->tune_spb;
     if (spb == 0) {
         // default: up to 32M paths per batch (16 spp at 1920x1080).  Each
         // bounce's traversal launch then holds tens of rays per lane, so
@@ -826,20 +837,27 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
     }
     KF_CHECK(hipMemcpyAsync(r->d_cams, cams, n_env * sizeof(kf_camera_ubo), hipMemcpyHostToDevice, st));
 
+    if (!r->d_fv) KF_CHECK(hipMalloc(&r->d_fv, sizeof(FrameVars)));
+    FrameVars fv{tea_host(o.seed, (uint32_t)pc->frame_count), pc->frame_count};
+    KF_CHECK(hipMemcpyAsync(r->d_fv, &fv, sizeof(fv), hipMemcpyHostToDevice, st));
+
     SceneView S = s->view();
     Frame F;
+    memset(&F, 0, sizeof(F));  // F is compared bytewise as part of the graph key
     F.cams = r->d_cams;
     F.pc = *pc;
+    F.pc.frame_count = 0;  // read from fv: the same graph serves every frame
+    F.fv = (const FrameVars*)r->d_fv;
     F.W = r->W;
     F.H = r->H;
     F.WH = WH;
     F.n_env = n_env;
     F.He = r->H / n_env;
     F.WHe = r->W * F.He;
-    F.S0 = tea_host(o.seed, (uint32_t)pc->frame_count);
     F.slots = K;
     F.inline_nee = inline_nee;
     F.count = o.count_traversal;
+    r->n_env_last = n_env;
 
     if (r->ev.empty()) {
         r->ev.resize(2 + 4 * kMaxBounces);
@@ -853,41 +871,47 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
     int n_cu = 256;
     KF_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     const uint32_t grid = (uint32_t)n_cu * 8;
-    const uint32_t refill = traversal_refill();
+    const uint32_t refill = std::min<uint32_t>(64u, std::max<uint32_t>(1u, r->tune_refill ? r->tune_refill
+                                                                                         : traversal_refill()));
+    const uint32_t grab = std::min<uint32_t>(4096u, std::max<uint32_t>(16u, r->tune_grab ? r->tune_grab : kGrab));
     // Persistent traversal kernels: exactly the blocks that fit at once.
     static const uint32_t grid_ext0 = resident_grid(k_extend<false>, n_cu);
     static const uint32_t grid_ext1 = resident_grid(k_extend<true>, n_cu);
     static const uint32_t grid_sh = resident_grid(k_shadow, n_cu);
+    const uint32_t nb = pc->path_depth + 1;
 
-    unsigned long long totals[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // One batch of samples [s0, s0 + nsamp).  timed: record the per-kernel
+    // events (not while capturing a graph).  The first batch clears every
+    // counter; later ones only the per-bounce ones, so the global counters
+    // add up over the frame.
     uint32_t n_ext = 0, n_sh = 0;
-    float ms_total = 0.f, ms_ext = 0.f, ms_sh = 0.f;
-    for (uint32_t s0 = 0; s0 < pc->spp; s0 += spb) {
+    auto enqueue_batch = [&](uint32_t s0, bool timed) {
         F.s0 = s0;
         F.nsamp = std::min(spb, pc->spp - s0);
-        uint32_t B = WH * F.nsamp;
-        k_zero<<<(kCntSize + TB - 1) / TB, TB, 0, st>>>(r->counters, kCntSize);
-        KF_CHECK(hipEventRecord(r->ev[0], st));
+        const uint32_t B = WH * F.nsamp;
+        const int nz = s0 == 0 ? kCntSize : kCntGlobal;
+        k_zero<<<(nz + TB - 1) / TB, TB, 0, st>>>(r->counters, nz);
+        if (timed) KF_CHECK(hipEventRecord(r->ev[0], st));
         k_raygen<<<std::min<uint32_t>((B + TB - 1) / TB, 65535u), TB, 0, st>>>(
             S, F, d_jump, r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->seed, r->queue_a, r->counters);
         uint32_t* qa = r->queue_a;
         uint32_t* qb = r->queue_b;
-        uint32_t nb = pc->path_depth + 1;
         // Per bounce b:  A: shade(b) -> [B: shadow(b)] , A: extend(b+1)
         // -> (wait B) connect(b).  k_shade lists the paths that continue
         // unless Russian roulette ends them, so the closest-hit rays of
         // bounce b+1 are traced while the shadow rays of bounce b are,
         // on the second stream; k_connect then waits for both.
         auto launch_extend = [&](uint32_t b, const uint32_t* queue) {
-            KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
+            if (timed) KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
             if (o.count_traversal)
                 k_extend<true><<<grid_ext1, TB, 0, st>>>(S, (int)b, B, queue, r->ray_o, r->ray_d, r->seed, r->hit,
-                                                         r->hit_prim, r->counters, refill, WH, F.WHe, F.s0, F.S0);
+                                                         r->hit_prim, r->counters, refill, grab, WH, F.WHe, F.s0,
+                                                         F.fv);
             else
-                k_extend<false><<<grid_ext0, TB, 0, st>>>(S, (int)b, B, queue, r->ray_o, r->ray_d, r->seed,
-                                                          r->hit, r->hit_prim, r->counters, refill, WH, F.WHe,
-                                                          F.s0, F.S0);
-            KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 1], st));
+                k_extend<false><<<grid_ext0, TB, 0, st>>>(S, (int)b, B, queue, r->ray_o, r->ray_d, r->seed, r->hit,
+                                                          r->hit_prim, r->counters, refill, grab, WH, F.WHe, F.s0,
+                                                          F.fv);
+            if (timed) KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 1], st));
             ++n_ext;
         };
         launch_extend(0, qa);
@@ -895,20 +919,20 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             if (inline_nee)
                 k_shade<true><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->hit,
                                                    r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis,
-                                                   r->sh_o, r->sh_d, r->sh_root, r->d_albedo, r->d_normal, r->queue_e,
-                                                   r->counters);
+                                                   r->sh_o, r->sh_d, r->sh_root, r->d_albedo, r->d_normal,
+                                                   r->queue_e, r->counters);
             else
-                k_shade<false><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr,
-                                                    r->hit, r->hit_prim, r->rw, r->em, r->seed, r->nee_a,
-                                                    r->nee_b, r->vis, r->sh_o, r->sh_d, r->sh_root, r->d_albedo, r->d_normal,
+                k_shade<false><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->hit,
+                                                    r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis,
+                                                    r->sh_o, r->sh_d, r->sh_root, r->d_albedo, r->d_normal,
                                                     r->queue_e, r->counters);
             if (!inline_nee) {
                 KF_CHECK(hipEventRecord(r->sync_ev[2 * b], st));
                 KF_CHECK(hipStreamWaitEvent(r->st2, r->sync_ev[2 * b], 0));
-                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], r->st2));
+                if (timed) KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], r->st2));
                 k_shadow<<<grid_sh, TB, 0, r->st2>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->sh_root, r->vis,
-                                                     r->counters, refill);
-                KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], r->st2));
+                                                     r->counters, refill, grab);
+                if (timed) KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], r->st2));
                 KF_CHECK(hipEventRecord(r->sync_ev[2 * b + 1], r->st2));
                 ++n_sh;
             }
@@ -923,26 +947,81 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             k_next_count<<<1, 1, 0, st>>>((int)b, r->counters);
             std::swap(qa, qb);
         }
-        bool last = s0 + F.nsamp >= pc->spp;
+        const bool last = s0 + F.nsamp >= pc->spp;
         k_resolve<<<(WH + TB - 1) / TB, TB, 0, st>>>(F, d_out, d_acc, r->d_rgba,
                                                      o.keep_samples ? r->samples : nullptr, last ? 1 : 0);
-        KF_CHECK(hipEventRecord(r->ev[1], st));
+        if (timed) KF_CHECK(hipEventRecord(r->ev[1], st));
         KF_CHECK(hipGetLastError());
-        unsigned long long h[kCntSize];
-        KF_CHECK(hipMemcpyAsync(h, r->counters, sizeof(h), hipMemcpyDeviceToHost, st));
-        KF_CHECK(hipStreamSynchronize(st));
-        for (int k = 0; k < 8; ++k) totals[k] += h[kCntGlobal + k];
-        float ms = 0.f;
-        KF_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
-        ms_total += ms;
-        for (uint32_t b = 0; b < nb; ++b) {
-            float a = 0.f, c = 0.f;
-            KF_CHECK(hipEventElapsedTime(&a, r->ev[2 + 4 * b + 0], r->ev[2 + 4 * b + 1]));
-            if (!inline_nee) KF_CHECK(hipEventElapsedTime(&c, r->ev[2 + 4 * b + 2], r->ev[2 + 4 * b + 3]));
-            ms_ext += a;
-            ms_sh += c;
+    };
+
+    float ms_total = 0.f, ms_ext = 0.f, ms_sh = 0.f;
+    if (o.use_graph) {
+        // Everything the captured kernel arguments hold: scene view, frame
+        // constants (seed and count excluded), schedule and buffers.
+        std::vector<uint8_t> key;
+        auto put = [&](const void* p, size_t n) {
+            const uint8_t* b = (const uint8_t*)p;
+            key.insert(key.end(), b, b + n);
+        };
+        const uint32_t sched[8] = {spb, K, nb, refill, grab, grid, (uint32_t)o.count_traversal,
+                                   (uint32_t)o.keep_samples};
+        const void* bufs[] = {r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->em, r->hit, r->hit_prim, r->seed,
+                              r->queue_a, r->queue_b, r->queue_e, r->sh_o, r->sh_d, r->sh_root, r->nee_a,
+                              r->nee_b, r->vis, r->samples, r->counters, d_lights, d_jump, d_acc, d_out,
+                              r->d_rgba, r->d_albedo, r->d_normal, (const void*)r->st2};
+        put(&S, sizeof(S));
+        put(&F, sizeof(F));
+        put(sched, sizeof(sched));
+        put(bufs, sizeof(bufs));
+        if (!r->graph || key != r->graph_key) {
+            if (r->graph) KF_CHECK(hipGraphExecDestroy(r->graph));
+            r->graph = nullptr;
+            hipGraph_t g = nullptr;
+            KF_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                for (uint32_t s0 = 0; s0 < pc->spp; s0 += spb) enqueue_batch(s0, false);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            KF_CHECK(hipStreamEndCapture(st, &g));
+            hipError_t e = hipGraphInstantiate(&r->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                r->graph = nullptr;
+                throw Error(std::string("kf_render: graph instantiation failed: ") + hipGetErrorString(e));
+            }
+            r->graph_key = key;
+            r->graph_ext = n_ext;
+            r->graph_sh = n_sh;
+        }
+        n_ext = r->graph_ext;
+        n_sh = r->graph_sh;
+        KF_CHECK(hipEventRecord(r->ev[0], st));
+        KF_CHECK(hipGraphLaunch(r->graph, st));
+        KF_CHECK(hipEventRecord(r->ev[1], st));
+        KF_CHECK(hipEventSynchronize(r->ev[1]));
+        KF_CHECK(hipEventElapsedTime(&ms_total, r->ev[0], r->ev[1]));
+    } else {
+        for (uint32_t s0 = 0; s0 < pc->spp; s0 += spb) {
+            enqueue_batch(s0, true);
+            KF_CHECK(hipEventSynchronize(r->ev[1]));
+            float ms = 0.f;
+            KF_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
+            ms_total += ms;
+            for (uint32_t b = 0; b < nb; ++b) {
+                float a = 0.f, c = 0.f;
+                KF_CHECK(hipEventElapsedTime(&a, r->ev[2 + 4 * b + 0], r->ev[2 + 4 * b + 1]));
+                if (!inline_nee) KF_CHECK(hipEventElapsedTime(&c, r->ev[2 + 4 * b + 2], r->ev[2 + 4 * b + 3]));
+                ms_ext += a;
+                ms_sh += c;
+            }
         }
     }
+    unsigned long long totals[8];
+    KF_CHECK(hipMemcpyAsync(totals, r->counters + kCntGlobal, sizeof(totals), hipMemcpyDeviceToHost, st));
+    KF_CHECK(hipStreamSynchronize(st));
     if (totals[5]) throw Error("kf_render: traversal stack overflow (BVH deeper than the stack)");
     if (stats) {
         stats->primary_rays = totals[0];
@@ -980,6 +1059,64 @@ extern "C" int kf_render_envs(kf_renderer* r, kf_scene* s, uint32_t num_envs, co
                               const kf_push_constants* pc, const kf_render_opts* opts, kf_render_stats* stats) {
     try {
         render_frame(r, s, num_envs, cams, pc, opts, stats);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_camera_ubo* cams,
+                                    const kf_push_constants* pc, kf_tune_result* out) {
+    try {
+        if (!r || !s || !cams || !pc || !out) throw Error("kf_renderer_autotune: null argument");
+        kf_push_constants p = *pc;
+        p.frame_count = 0;
+        kf_render_opts o{};
+        uint32_t trials = 0;
+        // kernel time of one frame under a schedule (0 = default); the best
+        // of three for short frames, whose times are noisier
+        auto run = [&](uint32_t spb, uint32_t refill, uint32_t grab) {
+            r->tune_spb = spb;
+            r->tune_refill = refill;
+            r->tune_grab = grab;
+            float best = INFINITY;
+            for (int k = 0; k < 3; ++k) {
+                kf_render_stats st{};
+                render_frame(r, s, num_envs, cams, &p, &o, &st);
+                ++trials;
+                best = std::min(best, st.trace_ms);
+                if (st.trace_ms > 20.0f) break;
+            }
+            return best;
+        };
+        run(0, 0, 0);  // warm-up: buffers, code objects
+        const float t_def = run(0, 0, 0);
+        const uint32_t WH = r->W * r->H;
+        uint32_t b_spb = 0, b_refill = 0, b_grab = 0;
+        float t_best = t_def;
+        for (uint32_t spb = 1; spb <= p.spp; spb *= 2) {
+            if ((uint64_t)WH * spb > (64ull << 20)) break;  // path state stays within ~13 GB
+            const float t = run(spb, b_refill, b_grab);
+            if (t < t_best) { t_best = t; b_spb = spb; }
+        }
+        for (uint32_t refill : {8u, 16u, 32u}) {
+            const float t = run(b_spb, refill, b_grab);
+            if (t < t_best) { t_best = t; b_refill = refill; }
+        }
+        for (uint32_t grab : {64u, 128u, 256u}) {
+            const float t = run(b_spb, b_refill, grab);
+            if (t < t_best) { t_best = t; b_grab = grab; }
+        }
+        r->tune_spb = b_spb;
+        r->tune_refill = b_refill;
+        r->tune_grab = b_grab;
+        out->samples_per_batch = b_spb;
+        out->refill = b_refill ? b_refill : traversal_refill();
+        out->grab = b_grab ? b_grab : kGrab;
+        out->frame_ms = t_best;
+        out->default_ms = t_def;
+        out->trials = trials;
         return 0;
     } catch (const std::exception& e) {
         set_error(e.what());

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(TB) void k_closest(SceneView S, const kf_ray* __res
         o.back = b.back;
         hits[q] = o;
     };
-    trace_persistent<false, COUNT>(S, n, cnt + 4, refill, false, load, store, ovf, tot);
+    trace_persistent<false, COUNT>(S, n, cnt + 4, refill, kGrab, false, load, store, ovf, tot);
     if (COUNT) {
         atomicAdd(&cnt[0], (unsigned long long)tot.nodes);
         atomicAdd(&cnt[1], (unsigned long long)tot.tris);
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(TB) void k_any(SceneView S, const kf_ray* __restric
     TCount tot{0, 0, 0};
     auto load = [&](uint32_t q, TRay& t) __attribute__((always_inline)) { ray_of(S, rays[q], q, t); };
     auto store = [&](uint32_t q, THit, bool found) __attribute__((always_inline)) { occ[q] = found ? 1 : 0; };
-    trace_persistent<true, false>(S, n, cnt + 4, refill, false, load, store, ovf, tot);
+    trace_persistent<true, false>(S, n, cnt + 4, refill, kGrab, false, load, store, ovf, tot);
     if (ovf) atomicAdd(&cnt[2], 1ull);
 }
 

@@ -501,7 +501,7 @@ KF_DEV bool traverse(const SceneView& S, uint32_t root, v3 o, v3 d, float tmin, 
 // ------------------------------------------------------- persistent driver
 // Queries [0, n) are split into 8 contiguous chunks, one per XCD (workgroups
 // are dispatched to the 8 XCDs round-robin, so block b runs on XCD b % 8 and
-// neighbouring queries share that XCD's L2).  A wave reserves kGrab queries
+// neighbouring queries share that XCD's L2).  A wave reserves `grab` queries
 // at a time from its chunk's counter and refills its idle lanes from that
 // reserve once at least `refill` lanes are idle (or all are), so no lane
 // waits for the slowest ray of a fixed group.  When its chunk is used up the
@@ -511,13 +511,14 @@ KF_DEV bool traverse(const SceneView& S, uint32_t root, v3 o, v3 d, float tmin, 
 //   ctr[8]: per-chunk counters, zero before the launch.
 //   load(q, TRay&) fills the ray of query q and its payload;
 //   store(payload, best, found) writes a finished query.
-constexpr uint32_t kGrab = 128;
+constexpr uint32_t kGrab = 128;  // default reservation
 
 KF_DEV uint64_t chunk_begin(uint32_t n, uint32_t c) { return (uint64_t)n * c / 8u; }
 
 template <bool ANY, bool COUNT, class Load, class Store>
 KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long* __restrict__ ctr, uint32_t refill,
-                             bool use_alpha, Load load, Store store, uint32_t& overflow, TCount& total) {
+                             uint32_t grab, bool use_alpha, Load load, Store store, uint32_t& overflow,
+                             TCount& total) {
     const TravScene TS(S);
     Trav<ANY, COUNT> T;
     T.reset();
@@ -550,14 +551,14 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
                     unsigned long long base = 0;
                     if (lane == 0) {
                         base = __hip_atomic_load(&ctr[chunk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        if (base < len) base = atomicAdd(&ctr[chunk], (unsigned long long)kGrab);
+                        if (base < len) base = atomicAdd(&ctr[chunk], (unsigned long long)grab);
                     }
                     base = __shfl(base, 0);
                     if (base < len) {
                         res_q = (uint32_t)(c0 + base);
-                        res_n = (uint32_t)min((unsigned long long)kGrab, len - base);
+                        res_n = (uint32_t)min((unsigned long long)grab, len - base);
                     }
-                    if (base + kGrab >= len) {
+                    if (base + grab >= len) {
                         chunk = (chunk + 1u) & 7u;
                         ++moved;
                     }

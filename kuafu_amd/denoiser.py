@@ -87,9 +87,11 @@ class Denoiser:
         """Denoise a renderer's rgba target in place (uses its albedo / normal)."""
         check(lib().kf_denoiser_apply(self.h, renderer.h))
 
-    def infer_device(self, d_rgba, d_albedo, d_normal, width, height, d_out):
+    def infer_device(self, d_rgba, d_albedo, d_normal, width, height, d_out, n_images=1):
+        """Denoise n_images stacked width x height images (device pointers); returns the kernel ms."""
         ms = C.c_float()
-        check(lib().kf_denoiser_infer_device(self.h, d_rgba, d_albedo, d_normal, width, height, d_out, C.byref(ms)))
+        check(lib().kf_denoiser_infer_device_batch(self.h, d_rgba, d_albedo, d_normal, width, height, n_images, d_out,
+                                                   C.byref(ms)))
         return ms.value
 
 

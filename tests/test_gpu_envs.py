@@ -133,3 +133,83 @@ def test_env_instance_update(gpu):
     rd2 = Renderer(size, size * n_env)
     rd2.render_envs(GpuScene(moved), cams, w.pc, seed=4)
     assert same(rd.read(0), rd2.read(0))
+
+
+def test_frame_graph_replays_match_eager_frames(gpu):
+    """A captured frame graph (three batches of one sample, two streams)
+    replayed over frames with new seeds, a moved camera and the running
+    mean, interleaved with eager work, equals eager rendering bit for bit;
+    a changed sample count captures a new graph."""
+    w = scenes.config2(width=48, height=27, spp=3, path_depth=4, n_inst=24, subdiv=2)
+    gs = GpuScene(w.data)
+    rg, re = Renderer(w.width, w.height), Renderer(w.width, w.height)
+    from kuafu_amd.api import camera_ubo
+    cam2 = camera_ubo((2.0, -28.0, 7.0), (-0.05, 1.0, -0.2), (0, 0, 1), w.width, w.height, fx=30.0, fy=30.0)
+    probe = np.zeros(4, abi.RAY_DTYPE)
+    probe["dir"] = (0.0, 1.0, 0.0)
+    probe["org"] = (0.0, -30.0, 0.0)
+    probe["tmax"] = 100.0
+    pc = w.pc
+    for f in range(6):
+        pc.frame_count = f
+        cam = cam2 if f in (2, 3) else w.cam
+        if f == 5:
+            pc.spp = 2
+        sg = rg.render(gs, cam, pc, seed=10 + f, samples_per_batch=1, use_graph=True)
+        se = re.render(gs, cam, pc, seed=10 + f, samples_per_batch=1)
+        gs.trace_closest(probe)  # eager work between replays
+        assert same(rg.read(0), re.read(0)), "frame %d" % f
+        assert (sg.primary_rays, sg.extension_rays, sg.shadow_rays) == (se.primary_rays, se.extension_rays,
+                                                                        se.shadow_rays)
+        assert sg.extend_launches == se.extend_launches and sg.trace_ms > 0
+
+
+def test_frame_graph_batched_environments(gpu):
+    w, d, cams = small_envs()
+    gs = GpuScene(d)
+    n_env, size = len(cams), w.env_height
+    rg, re = Renderer(size, size * n_env), Renderer(size, size * n_env)
+    pc = w.pc
+    for f in range(3):
+        pc.frame_count = f
+        cs = cams[::-1] if f == 1 else cams  # other cameras, same graph
+        rg.render_envs(gs, cs, pc, seed=f, use_graph=True)
+        re.render_envs(gs, cs, pc, seed=f)
+        assert same(rg.read(0), re.read(0)), "frame %d" % f
+
+
+def test_autotune_keeps_images(gpu):
+    w = scenes.config2(width=64, height=36, spp=4, path_depth=4, n_inst=24, subdiv=2)
+    gs = GpuScene(w.data)
+    rd = Renderer(w.width, w.height)
+    pc = w.pc
+    pc.frame_count = 0
+    rd.render(gs, w.cam, pc, seed=2)
+    before = rd.read(0).copy()
+    t = rd.autotune(gs, w.cam, pc)
+    print(t)
+    assert t["samples_per_batch"] in (0, 1, 2, 4) and t["refill"] in (8, 16, 32) and t["grab"] in (64, 128, 256)
+    assert 0 < t["frame_ms"] <= t["default_ms"] and t["trials"] >= 11
+    rd.render(gs, w.cam, pc, seed=2)
+    assert same(rd.read(0), before)
+
+
+def test_batched_denoise_equals_per_environment(gpu):
+    from kuafu_amd.denoiser import Denoiser
+    w, d, cams = small_envs()
+    gs = GpuScene(d)
+    n_env, size = len(cams), w.env_height
+    rd = Renderer(size, size * n_env)
+    rd.render_envs(gs, cams, w.pc, seed=1)
+    tg = [rd.read(k).copy() for k in range(3)]
+    den = Denoiser(random_seed=7)
+    den.apply(rd)  # one batch of n_env images
+    out = rd.read(0)
+    for e in range(n_env):
+        r1 = Renderer(size, size)
+        rows = slice(e * size, (e + 1) * size)
+        for k in range(3):
+            r1.write(k, tg[k][rows])
+        den.apply(r1)
+        assert same(out[rows], r1.read(0)), "environment %d" % e
+    assert not same(out, tg[0])
