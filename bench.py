@@ -116,6 +116,56 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     return res
 
 
+def batched_environments(frames=8):
+    """128 environments of 512x512 (8 spp, depth 6, 5k-400k triangles each):
+    autotuned schedule, every frame one replay of a captured HIP graph, then
+    one batched denoise of the 128 images.  Frame latency = render + denoise,
+    host wall time per frame."""
+    import numpy as np
+    from kuafu_amd import GpuScene, Renderer, scenes
+    from kuafu_amd.denoiser import Denoiser
+    w = scenes.config5()
+    gs = GpuScene(w.data)
+    info = gs.bvh_info()
+    rd = Renderer(w.width, w.height)
+    pc = w.pc
+    pc.frame_count = 0
+    tune = rd.autotune(gs, w.cam, pc)
+    den = Denoiser()
+    rd.render_envs(gs, w.cam, pc, seed=1, use_graph=True)  # capture
+    den.apply(rd)
+    lat, rend, dms = [], [], []
+    rays = 0
+    kms = 0.0
+    for k in range(frames):
+        t0 = time.perf_counter()
+        st = rd.render_envs(gs, w.cam, pc, seed=2 + k, use_graph=True)
+        t1 = time.perf_counter()
+        den.apply(rd)
+        t2 = time.perf_counter()
+        lat.append((t2 - t0) * 1e3)
+        rend.append((t1 - t0) * 1e3)
+        dms.append((t2 - t1) * 1e3)
+        rays += st.rays()
+        kms += st.trace_ms
+    res = {"environments": w.n_env, "width": w.env_height, "height": w.env_height, "spp": pc.spp,
+           "path_depth": pc.path_depth, "triangles_instanced": int(w.env_tris.sum()),
+           "triangles_per_env_min_max": [int(w.env_tris.min()), int(w.env_tris.max())],
+           "unique_triangles": int(info.triangles), "instances": int(info.instances),
+           "bvh_build_ms": round(float(info.build_ms), 2), "rays_per_frame": rays // frames,
+           "mrays_s": round(rays / (sum(rend) * 1e-3) / 1e6, 1),
+           "kernel_mrays_s": round(rays / (kms * 1e-3) / 1e6, 1),
+           "frame_ms_p50": round(float(np.percentile(lat, 50)), 2),
+           "frame_ms_p99": round(float(np.percentile(lat, 99)), 2),
+           "render_ms_p50": round(float(np.percentile(rend, 50)), 2),
+           "denoise_ms_p50": round(float(np.percentile(dms, 50)), 2), "frames": frames,
+           "schedule": {k: tune[k] for k in ("samples_per_batch", "refill", "grab")},
+           "autotune_ms": [round(tune["default_ms"], 1), round(tune["frame_ms"], 1)]}
+    gs.close()
+    rd.close()
+    return res
+
+
 def other_workloads(steps=3):
     """Mrays/s of the other scene recipes on the same build (not the headline)."""
     from kuafu_amd import GpuScene, Renderer, scenes
@@ -125,6 +175,7 @@ def other_workloads(steps=3):
         rd = Renderer(w.width, w.height)
         pc = w.pc
         pc.frame_count = 0
+        rd.autotune(gs, w.cam, pc)
         rd.render(gs, w.cam, pc, seed=1)
         rays = 0
         t = time.perf_counter()
@@ -146,6 +197,8 @@ def main():
     ap.add_argument("--samples-per-batch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-relmse", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--schedule", default="", help="samples_per_batch,refill,grab instead of autotuning")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     args = ap.parse_args()
@@ -166,6 +219,16 @@ def main():
     pc = w.pc
     pc.frame_count = 0
 
+    # schedule (batch size, refill threshold, reservation per wave) picked by
+    # timing frames of this scene, before anything is measured
+    tune = None
+    if args.schedule:
+        spb_, refill_, grab_ = (int(v) for v in args.schedule.split(","))
+        rd.set_schedule(spb_, refill_, grab_)
+        tune = {"samples_per_batch": spb_, "refill": refill_, "grab": grab_}
+    elif not args.no_autotune and not args.samples_per_batch:
+        print("bench: autotune", file=sys.stderr, flush=True)
+        tune = rd.autotune(gs, w.cam, pc)
     # instrumented frame (untimed): node visits / triangle tests of k_extend
     cst = rd.render(gs, w.cam, pc, seed=1, samples_per_batch=args.samples_per_batch, count_traversal=True)
     for _ in range(args.warmup):
@@ -212,6 +275,7 @@ def main():
         "config": {"workload": w.name, "width": w.width, "height": w.height, "spp": pc.spp,
                    "path_depth": pc.path_depth, "triangles": int(info.triangles), "instances": int(info.instances),
                    "rays_per_frame": rays // args.steps,
+                   "schedule": None if tune is None else {k: tune[k] for k in ("samples_per_batch", "refill", "grab")},
                    "kernel_mrays_s": round(rays / (trace_ms * 1e-3) / 1e6, 2)},
         "roofline": roof,
     }
@@ -225,6 +289,8 @@ def main():
         out["time_to_relmse"] = time_to_relmse(w, gs, rd)
         print("bench: other scenes", file=sys.stderr, flush=True)
         out["other_workloads"] = other_workloads()
+        print("bench: batched environments", file=sys.stderr, flush=True)
+        out["other_workloads"]["config5_envs128_512x512_8spp"] = batched_environments()
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w)

@@ -331,6 +331,10 @@ typedef struct kf_tune_result {
 } kf_tune_result;
 int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_camera_ubo* cams,
                          const kf_push_constants* pc, kf_tune_result* out);
+/* Set the schedule directly (e.g. one kf_renderer_autotune picked earlier);
+ * 0 keeps the built-in default of that parameter.  refill is clamped to
+ * [1, 64], grab to [16, 4096]. */
+int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_batch, uint32_t refill, uint32_t grab);
 
 /* Read a render target: 0 = rgba, 1 = albedo, 2 = normal; W*H*4 floats. */
 int kf_renderer_read(kf_renderer* r, int target, float* out);

@@ -402,6 +402,9 @@ class Renderer:
         return dict(samples_per_batch=t.samples_per_batch, refill=t.refill, grab=t.grab, frame_ms=t.frame_ms,
                     default_ms=t.default_ms, trials=t.trials)
 
+    def set_schedule(self, samples_per_batch=0, refill=0, grab=0):
+        check(lib().kf_renderer_set_schedule(self.h, samples_per_batch, refill, grab))
+
     def read(self, target=0):
         out = np.zeros((self.h_, self.w, 4), np.float32)
         check(lib().kf_renderer_read(self.h, target, ptr(out)))

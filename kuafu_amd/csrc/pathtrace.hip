@@ -1100,11 +1100,11 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
             const float t = run(spb, b_refill, b_grab);
             if (t < t_best) { t_best = t; b_spb = spb; }
         }
-        for (uint32_t refill : {8u, 16u, 32u}) {
+        for (uint32_t refill : {8u, 16u, 32u, 48u}) {
             const float t = run(b_spb, refill, b_grab);
             if (t < t_best) { t_best = t; b_refill = refill; }
         }
-        for (uint32_t grab : {64u, 128u, 256u}) {
+        for (uint32_t grab : {64u, 128u, 256u, 512u, 1024u}) {
             const float t = run(b_spb, b_refill, grab);
             if (t < t_best) { t_best = t; b_grab = grab; }
         }
@@ -1122,6 +1122,17 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
         set_error(e.what());
         return -1;
     }
+}
+
+extern "C" int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_batch, uint32_t refill, uint32_t grab) {
+    if (!r) {
+        set_error("kf_renderer_set_schedule: null renderer");
+        return -1;
+    }
+    r->tune_spb = samples_per_batch;
+    r->tune_refill = refill;
+    r->tune_grab = grab;
+    return 0;
 }
 
 extern "C" int kf_renderer_download_bgra8(kf_renderer* r, uint8_t* out) {

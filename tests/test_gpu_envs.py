@@ -188,8 +188,12 @@ def test_autotune_keeps_images(gpu):
     before = rd.read(0).copy()
     t = rd.autotune(gs, w.cam, pc)
     print(t)
-    assert t["samples_per_batch"] in (0, 1, 2, 4) and t["refill"] in (8, 16, 32) and t["grab"] in (64, 128, 256)
-    assert 0 < t["frame_ms"] <= t["default_ms"] and t["trials"] >= 11
+    assert t["samples_per_batch"] in (0, 1, 2, 4) and t["refill"] in (8, 16, 32, 48)
+    assert t["grab"] in (64, 128, 256, 512, 1024)
+    assert 0 < t["frame_ms"] <= t["default_ms"] and t["trials"] >= 14
+    rd.render(gs, w.cam, pc, seed=2)
+    assert same(rd.read(0), before)
+    rd.set_schedule(1, 48, 1024)  # an explicit schedule: the same image
     rd.render(gs, w.cam, pc, seed=2)
     assert same(rd.read(0), before)
 
