@@ -319,10 +319,11 @@ int kf_render_envs(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_came
  * whose schedule is the driver's): times frames of this scene and target
  * over the batch size (samples per batch, i.e. the wavefront queue length),
  * the traversal's refill threshold and its work reservation per wave (the
- * tile a persistent wave takes from its XCD's queue), one parameter after
- * the other, and keeps the fastest for later frames rendered with
- * samples_per_batch = 0.  Images do not depend on the schedule.  Renders
- * into r's targets (frame_count 0). */
+ * tile a persistent wave takes from its XCD's queue): batch size, then
+ * reservation, refill threshold and reservation again, each candidate timed
+ * in a forward and a backward sweep; keeps the fastest for later frames
+ * rendered with samples_per_batch = 0.  Images do not depend on the
+ * schedule.  Renders into r's targets (frame_count 0). */
 typedef struct kf_tune_result {
     uint32_t samples_per_batch, refill, grab;
     float frame_ms;    /* kernel time of a frame with the picked schedule */

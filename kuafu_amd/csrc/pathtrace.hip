@@ -173,7 +173,15 @@ __global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2
         col[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         rw[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         seed[p] = rs;
-        queue[p] = p;
+        // Camera rays are traced in 8-row bands, column by column, so a
+        // wave's 64 rays cover an 8x8 pixel tile (paths stay stored by
+        // sample and pixel; only the order of the queue changes).
+        {
+            const uint32_t r = p % F.WH, band = r / (8u * F.W);
+            const uint32_t rows = min(8u, F.H - band * 8u), off = r - band * 8u * F.W;
+            const uint32_t x = off / rows, yy = off - x * rows;
+            queue[p] = (p - r) + (band * 8u + yy) * F.W + x;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         cnt[0 * kCntPerBounce + 0] = n;  // active paths at bounce 0
@@ -724,6 +732,14 @@ uint32_t resident_grid(Kernel kernel, int n_cu) {
     return (uint32_t)std::max(nb, 1) * (uint32_t)n_cu;
 }
 
+// Default batch: up to 32M paths (16 spp at 1920x1080).  Each bounce's
+// traversal launch then holds tens of rays per lane, so the tail of the
+// slowest rays is a small part of it; the state costs about 200 B per path
+// (6.6 GB at 32M paths).
+uint32_t default_batch(uint32_t spp, uint32_t WH) {
+    return std::max<uint32_t>(1, std::min<uint32_t>(spp, (32u << 20) / std::max<uint32_t>(WH, 1)));
+}
+
 void build_light_list(const Scene& s, DevLights& L, bool& inline_nee) {
     L.dl = s.dl;
     L.pl = s.pl;
@@ -787,13 +803,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
     build_light_list(*s, L, inline_nee);
     uint32_t WH = r->W * r->H;
     uint32_t spb = o.samples_per_batch ? o.samples_per_batch : r->tune_spb;
-    if (spb == 0) {
-        // default: up to 32M paths per batch (16 spp at 1920x1080).  Each
-        // bounce's traversal launch then holds tens of rays per lane, so
-        // the tail of the slowest rays is a small part of it; the state
-        // costs about 200 B per path (6.6 GB at 32M paths).
-        spb = std::max<uint32_t>(1, std::min<uint32_t>(pc->spp, (32u << 20) / std::max<uint32_t>(WH, 1)));
-    }
+    if (spb == 0) spb = default_batch(pc->spp, WH);
     spb = std::min(spb, pc->spp);
     uint32_t K = inline_nee ? 1u : (uint32_t)std::max(L.n_slots, 1);
     r->ensure((size_t)WH * spb, K, o.keep_samples ? pc->spp : 0);
@@ -1090,30 +1100,57 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
             }
             return best;
         };
-        run(0, 0, 0);  // warm-up: buffers, code objects
+        run(0, 0, 0);  // warm-up: buffers, code objects, clocks
+        run(0, 0, 0);
         const float t_def = run(0, 0, 0);
         const uint32_t WH = r->W * r->H;
         uint32_t b_spb = 0, b_refill = 0, b_grab = 0;
         float t_best = t_def;
+        // One parameter at a time.  Every candidate is timed in a forward
+        // and a backward sweep and keeps its faster time, so a drift of the
+        // clocks over the sweep does not favour the candidates timed last.
+        auto sweep = [&](const std::vector<uint32_t>& cand, uint32_t& best, int which) {
+            std::vector<float> t(cand.size(), INFINITY);
+            auto one = [&](size_t k) {
+                const uint32_t v = cand[k];
+                const float m = which == 0 ? run(v, b_refill, b_grab)
+                                           : (which == 1 ? run(b_spb, v, b_grab) : run(b_spb, b_refill, v));
+                t[k] = std::min(t[k], m);
+            };
+            for (size_t k = 0; k < cand.size(); ++k) one(k);
+            for (size_t k = cand.size(); k-- > 0;) one(k);
+            // the candidates include the current value, so the sweep's
+            // fastest is chosen among times taken side by side
+            size_t kb = 0;
+            for (size_t k = 1; k < cand.size(); ++k)
+                if (t[k] < t[kb]) kb = k;
+            best = cand[kb];
+            t_best = t[kb];
+        };
+        std::vector<uint32_t> spbs;
         for (uint32_t spb = 1; spb <= p.spp; spb *= 2) {
             if ((uint64_t)WH * spb > (64ull << 20)) break;  // path state stays within ~13 GB
-            const float t = run(spb, b_refill, b_grab);
-            if (t < t_best) { t_best = t; b_spb = spb; }
+            spbs.push_back(spb);
         }
-        for (uint32_t refill : {8u, 16u, 32u, 48u}) {
-            const float t = run(b_spb, refill, b_grab);
-            if (t < t_best) { t_best = t; b_refill = refill; }
-        }
-        for (uint32_t grab : {64u, 128u, 256u, 512u, 1024u}) {
-            const float t = run(b_spb, b_refill, grab);
-            if (t < t_best) { t_best = t; b_grab = grab; }
-        }
+        const uint32_t spb_def = default_batch(p.spp, WH);
+        if (std::find(spbs.begin(), spbs.end(), spb_def) == spbs.end()) spbs.push_back(spb_def);
+        b_refill = std::min<uint32_t>(64u, std::max<uint32_t>(1u, traversal_refill()));
+        b_grab = kGrab;
+        // the best refill threshold depends on the reservation size: the
+        // reservation is swept before and after it
+        const std::vector<uint32_t> grabs = {64u, 128u, 256u, 512u, 1024u};
+        std::vector<uint32_t> refills = {8u, 16u, 32u, 48u};
+        if (std::find(refills.begin(), refills.end(), b_refill) == refills.end()) refills.push_back(b_refill);
+        sweep(spbs, b_spb, 0);
+        sweep(grabs, b_grab, 2);
+        sweep(refills, b_refill, 1);
+        sweep(grabs, b_grab, 2);
         r->tune_spb = b_spb;
         r->tune_refill = b_refill;
         r->tune_grab = b_grab;
         out->samples_per_batch = b_spb;
-        out->refill = b_refill ? b_refill : traversal_refill();
-        out->grab = b_grab ? b_grab : kGrab;
+        out->refill = b_refill;
+        out->grab = b_grab;
         out->frame_ms = t_best;
         out->default_ms = t_def;
         out->trials = trials;

@@ -188,9 +188,9 @@ def test_autotune_keeps_images(gpu):
     before = rd.read(0).copy()
     t = rd.autotune(gs, w.cam, pc)
     print(t)
-    assert t["samples_per_batch"] in (0, 1, 2, 4) and t["refill"] in (8, 16, 32, 48)
+    assert t["samples_per_batch"] in (1, 2, 4) and t["refill"] in (8, 16, 32, 48)
     assert t["grab"] in (64, 128, 256, 512, 1024)
-    assert 0 < t["frame_ms"] <= t["default_ms"] and t["trials"] >= 14
+    assert 0 < t["frame_ms"] <= 1.25 * t["default_ms"] and t["trials"] >= 38
     rd.render(gs, w.cam, pc, seed=2)
     assert same(rd.read(0), before)
     rd.set_schedule(1, 48, 1024)  # an explicit schedule: the same image
