@@ -78,12 +78,35 @@ public:
     /// ActiveLight texture ids.  The reference only loads textures from files.
     int addTexture(const std::vector<uint8_t>& rgba8, uint32_t width, uint32_t height);
 
+    /// The device scene (acceleration structures and buffers) behind this
+    /// Scene; null before the first run().  No counterpart in the reference.
+    const kf_scene* getDeviceScene() const { return mDevice; }
+    /// Device scenes built from scratch so far (full uploads and BLAS
+    /// builds); incremental updates do not count.  No counterpart in the
+    /// reference, which rebuilds every BLAS on any change.
+    size_t getDeviceBuilds() const { return mDeviceBuilds; }
+
     void init();
 
 private:
     explicit Scene(std::shared_ptr<Config> pConfig);
     void buildDevice();   // upload + BLAS/TLAS build (Context::update)
     void updateDevice();  // per-frame TLAS rebuild + lights
+    void syncDevice();    // the cheapest of the two that brings the device up to date
+    bool refitDevice();   // geometries changed in place only: refit their BLAS, re-upload materials
+
+  public:
+    // what the device scene was built from (to tell in-place changes apart)
+    struct DeviceGeometry {
+        const Geometry* geometry;
+        size_t vertices, indices;
+        uint64_t topology;  // hash of indices and material indices
+        bool opaque, hidden;
+    };
+
+  private:
+    std::vector<DeviceGeometry> mDeviceGeometries;
+    size_t mDeviceMaterials = 0, mDeviceTextures = 0, mDeviceBuilds = 0;
 
     bool initialized = false;
     glm::vec4 mClearColor = glm::vec4(0.F, 0.F, 0.F, 1.F);

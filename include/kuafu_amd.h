@@ -243,6 +243,19 @@ void kf_scene_destroy(kf_scene* s);
  * RayTracer::updateTlas, rt.cpp:372-404, context.cpp:337-341). */
 int kf_scene_update_instances(kf_scene* s, const kf_instance* inst, uint32_t n);
 
+/* Move the vertices of geometry g (same vertex count and triangles) and
+ * refit its BLAS on the GPU: triangle slots, BVH2 and BVH4 boxes of that
+ * geometry only, then the TLAS.  The reference rebuilds every BLAS when
+ * anything changes (context.cpp:328-336, rt.cpp:142-161); this is the
+ * incremental path for dirty geometry.  The trees keep their shape, so
+ * traversal cost can grow with large deformations (rebuild the scene then);
+ * hits do not depend on the tree. */
+int kf_scene_update_geometry(kf_scene* s, uint32_t g, const kf_vertex* vertices, uint32_t n);
+
+/* Replace the material parameters (same count): the material upload of
+ * Scene::uploadGeometries (scene.cpp:414-415) without a rebuild. */
+int kf_scene_update_materials(kf_scene* s, const kf_material* materials, uint32_t n);
+
 /* Replace the lights (Scene::uploadLightBuffers, scene.cpp:252-292). */
 int kf_scene_set_lights(kf_scene* s, const kf_dir_light* d, const kf_point_lights* p,
                         const kf_active_lights* a);

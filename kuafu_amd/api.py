@@ -319,6 +319,11 @@ class GpuScene:
     def update_instances(self, inst):
         check(lib().kf_scene_update_instances(self.h, ptr(inst), len(inst)))
 
+    def update_geometry(self, g, verts):
+        """New vertices for geometry g (same count): refit its BLAS, rebuild the TLAS."""
+        verts = np.ascontiguousarray(verts, abi.VERTEX_DTYPE)
+        check(lib().kf_scene_update_geometry(self.h, g, ptr(verts), len(verts)))
+
     def trace_closest(self, rays):
         rays = np.ascontiguousarray(rays, abi.RAY_DTYPE)
         hits = np.zeros(len(rays), abi.HIT_DTYPE)

@@ -510,7 +510,10 @@ KF_DEV_INLINE void collapse_item(const Node* __restrict__ b2, uint32_t n_b2, int
     W.loz = make_float4(blo[2][0], blo[2][1], blo[2][2], blo[2][3]);
     W.hiz = make_float4(bhi[2][0], bhi[2][1], bhi[2][2], bhi[2][3]);
     W.ref = make_int4(wref[0], wref[1], wref[2], wref[3]);
-    W.pad = make_int4(0, 0, 0, 0);
+    // the BVH2 ref each child was taken from (a refit recomputes the child
+    // boxes from these); the traversal never reads this line
+    W.pad = make_int4(0 < cnt ? ref[0] : kEmptyRef, 1 < cnt ? ref[1] : kEmptyRef, 2 < cnt ? ref[2] : kEmptyRef,
+                      3 < cnt ? ref[3] : kEmptyRef);
     wide[it.y] = W;
 }
 
@@ -715,16 +718,166 @@ void build_blas(Scene& s) {
         KF_CHECK(hipMemcpyAsync(s.h_blas_root.data(), s.d_blas_root, s.n_geom * sizeof(int), hipMemcpyDeviceToHost, st));
     dfree(d_node_off); dfree(lo); dfree(hi); dfree(order);
     KF_CHECK(hipStreamSynchronize(st));
-    // BVH4 for the traversal
+    // BVH4 for the traversal: geometry g's nodes in [node_off[g], node_off[g]
+    // + n_g - 1), root first, so roots are known by construction and a refit
+    // of one geometry touches only its own range
     dfree(s.d_blas4);
     dfree(s.d_blas_root4);
+    dfree(s.d_node_off);
+    dfree(s.d_blas_cnt);
     const uint32_t cap4 = std::max<uint32_t>(acc, 1);
     s.d_blas4 = dalloc<Node4>(cap4);
-    s.n_blas4 = collapse_bvh4(s.d_blas, acc, s.h_blas_root, s.h_blas_root4, s.d_blas4, cap4, 1u, st);
+    s.d_node_off = dalloc<uint32_t>(std::max<uint32_t>(s.n_geom, 1));
+    s.d_blas_cnt = dalloc<uint32_t>(2 * (size_t)std::max<uint32_t>(s.n_geom, 1));
+    if (s.n_geom)
+        KF_CHECK(hipMemcpyAsync(s.d_node_off, s.h_node_off.data(), s.n_geom * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                st));
+    int* err = dalloc<int>(1);
+    KF_CHECK(hipMemsetAsync(err, 0, sizeof(int), st));
+    if (acc) {
+        int2* qa = dalloc<int2>(acc);
+        int2* qb = dalloc<int2>(acc);
+        k_collapse_forest<<<s.n_geom, kTreeThreads, 0, st>>>(s.d_blas, acc, s.d_toff, s.d_node_off, qa, qb, s.d_blas4,
+                                                               1u, s.d_blas_cnt, err);
+        KF_CHECK(hipGetLastError());
+        dfree(qa);
+        dfree(qb);
+    }
+    s.h_blas_root4.assign(s.n_geom, kEmptyRef);
+    for (uint32_t g = 0; g < s.n_geom; ++g) {
+        const uint32_t ng = s.h_toff[g + 1] - s.h_toff[g];
+        s.h_blas_root4[g] = ng >= 2 ? (int)s.h_node_off[g] : (ng == 1 ? ~(int)s.h_toff[g] : kEmptyRef);
+    }
+    s.n_blas4 = cap4;  // index bound; used nodes per geometry are counted in d_blas_cnt
     s.d_blas_root4 = dalloc<int>(std::max<uint32_t>(s.n_geom, 1));
     if (s.n_geom)
         KF_CHECK(hipMemcpyAsync(s.d_blas_root4, s.h_blas_root4.data(), s.n_geom * sizeof(int), hipMemcpyHostToDevice, st));
+    int h_err = 0;
+    KF_CHECK(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
     KF_CHECK(hipStreamSynchronize(st));
+    dfree(err);
+    if (h_err) throw Error("BLAS build: malformed BVH2");
+}
+
+// ---------------------------------------------------------------- refit
+// Refit of one geometry after its vertices moved (same triangles): the
+// triangle slots take the new positions, the BVH2 boxes are recomputed
+// bottom-up (the second child to arrive at a node carries on, as in the
+// build), then every BVH4 child box is the box of the BVH2 ref it was
+// collapsed from.  The trees keep their shape.
+__global__ void k_geom_slots(uint32_t g, const float4* __restrict__ verts, const uint32_t* __restrict__ idx,
+                             const uint32_t* __restrict__ voff, const uint32_t* __restrict__ toff, float4* tri,
+                             float4* __restrict__ lo, float4* __restrict__ hi) {
+    const uint32_t t0 = toff[g], n = toff[g + 1] - t0;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const size_t slot = t0 + j;
+    const uint32_t prim = __float_as_uint(tri[3 * slot].w);
+    if (prim >= n) return;  // not a slot of this geometry (malformed)
+    const size_t i = t0 + prim;
+    float3 p[3];
+    for (int k = 0; k < 3; ++k) {
+        const float4 v = verts[3 * (size_t)(voff[g] + idx[3 * i + k])];
+        p[k] = make_float3(v.x, v.y, v.z);
+        tri[3 * slot + k] = make_float4(v.x, v.y, v.z, k == 0 ? __uint_as_float(prim) : 0.f);
+    }
+    lo[j] = make_float4(fminf(p[0].x, fminf(p[1].x, p[2].x)), fminf(p[0].y, fminf(p[1].y, p[2].y)),
+                        fminf(p[0].z, fminf(p[1].z, p[2].z)), 0.f);
+    hi[j] = make_float4(fmaxf(p[0].x, fmaxf(p[1].x, p[2].x)), fmaxf(p[0].y, fmaxf(p[1].y, p[2].y)),
+                        fmaxf(p[0].z, fmaxf(p[1].z, p[2].z)), 0.f);
+}
+
+__global__ void k_geom_parents(uint32_t g, const uint32_t* __restrict__ toff, const uint32_t* __restrict__ node_off,
+                               const Node* __restrict__ nodes, uint32_t n_nodes, int* __restrict__ parent_node,
+                               int* __restrict__ parent_leaf, int* __restrict__ flags) {
+    const uint32_t t0 = toff[g], n = toff[g + 1] - t0, nb = node_off[g];
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < 2 || j >= n - 1 || nb + j >= n_nodes) return;
+    const int me = (int)(nb + j);
+    const int4 d = nodes[me].d;
+    for (int c = 0; c < 2; ++c) {
+        const int r = c ? d.y : d.x;
+        if (r >= 0) {
+            if ((uint32_t)r >= nb && (uint32_t)r < nb + n - 1) parent_node[r - nb] = me;
+        } else if ((uint32_t)~r >= t0 && (uint32_t)~r < t0 + n) {
+            parent_leaf[~r - t0] = me;
+        }
+    }
+    flags[j] = 0;
+}
+
+__global__ void k_geom_refit2(uint32_t g, const uint32_t* __restrict__ toff, const uint32_t* __restrict__ node_off,
+                              uint32_t n_nodes, const float4* __restrict__ plo, const float4* __restrict__ phi,
+                              const int* __restrict__ parent_node, const int* __restrict__ parent_leaf, Node* nodes,
+                              int* flags) {
+    const uint32_t t0 = toff[g], n = toff[g + 1] - t0, nb = node_off[g];
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < 2 || j >= n) return;
+    float4 a = plo[j], b = phi[j];
+    float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+    int child = ~(int)(t0 + j);
+    int p = parent_leaf[j];
+    for (int step = 0; p >= (int)nb && (uint32_t)p < nb + n - 1 && (uint32_t)p < n_nodes && step < 4096; ++step) {
+        Node* nd = &nodes[p];
+        const bool left = nd->d.x == child;
+        write_child(nd, left, lo, hi);
+        const int old = __hip_atomic_fetch_add(&flags[p - nb], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        const float4 na = nd->a, nbx = nd->b, nc = nd->c;
+        lo[0] = fminf(na.x, nbx.x);
+        hi[0] = fmaxf(na.y, nbx.y);
+        lo[1] = fminf(na.z, nbx.z);
+        hi[1] = fmaxf(na.w, nbx.w);
+        lo[2] = fminf(nc.x, nc.z);
+        hi[2] = fmaxf(nc.y, nc.w);
+        child = p;
+        p = parent_node[p - nb];
+    }
+}
+
+__global__ void k_geom_refit4(uint32_t g, const uint32_t* __restrict__ toff, const uint32_t* __restrict__ node_off,
+                              const uint32_t* __restrict__ cnt4, const Node* __restrict__ nodes, uint32_t n_nodes,
+                              const float4* __restrict__ tri, Node4* wide) {
+    const uint32_t t0 = toff[g], n = toff[g + 1] - t0, nb = node_off[g];
+    if (n < 2) return;
+    const uint32_t used = min(cnt4[2 * g] - nb, n - 1);
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= used) return;
+    Node4& W = wide[nb + j];
+    const int4 pr = W.pad;
+    float blo[3][4], bhi[3][4];
+    for (int c = 0; c < 4; ++c) {
+        const int r = c == 0 ? pr.x : (c == 1 ? pr.y : (c == 2 ? pr.z : pr.w));
+        float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
+        if (r == kEmptyRef) {
+        } else if (r >= 0) {
+            if ((uint32_t)r < n_nodes) {
+                const Node m = nodes[r];
+                lo[0] = fminf(m.a.x, m.b.x); hi[0] = fmaxf(m.a.y, m.b.y);
+                lo[1] = fminf(m.a.z, m.b.z); hi[1] = fmaxf(m.a.w, m.b.w);
+                lo[2] = fminf(m.c.x, m.c.z); hi[2] = fmaxf(m.c.y, m.c.w);
+            }
+        } else if ((uint32_t)~r >= t0 && (uint32_t)~r < t0 + n) {
+            const size_t sl = (size_t)~r;
+            for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
+            for (int k = 0; k < 3; ++k) {
+                const float4 v = tri[3 * sl + k];
+                lo[0] = fminf(lo[0], v.x); hi[0] = fmaxf(hi[0], v.x);
+                lo[1] = fminf(lo[1], v.y); hi[1] = fmaxf(hi[1], v.y);
+                lo[2] = fminf(lo[2], v.z); hi[2] = fmaxf(hi[2], v.z);
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            blo[a][c] = lo[a];
+            bhi[a][c] = hi[a];
+        }
+    }
+    W.lox = make_float4(blo[0][0], blo[0][1], blo[0][2], blo[0][3]);
+    W.hix = make_float4(bhi[0][0], bhi[0][1], bhi[0][2], bhi[0][3]);
+    W.loy = make_float4(blo[1][0], blo[1][1], blo[1][2], blo[1][3]);
+    W.hiy = make_float4(bhi[1][0], bhi[1][1], bhi[1][2], bhi[1][3]);
+    W.loz = make_float4(blo[2][0], blo[2][1], blo[2][2], blo[2][3]);
+    W.hiz = make_float4(bhi[2][0], bhi[2][1], bhi[2][2], bhi[2][3]);
 }
 
 __global__ void k_tlas_aux(uint32_t n, uint32_t* aux) {
@@ -759,6 +912,7 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     const bool multi = s.n_env > 1;
     if (multi && n != s.h_env_off.back())
         throw Error("kf_scene_update_instances: a scene with environments keeps its instance count");
+    if (h_inst != s.h_inst.data()) s.h_inst.assign(h_inst, h_inst + n);
     if (n != s.n_inst || !s.d_inst) {
         dfree(s.d_inst); dfree(s.d_tlas); dfree(s.d_inst_src);
         s.d_inst = dalloc<Inst>(std::max<uint32_t>(n, 1));
@@ -839,6 +993,37 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     s.n_tlas4 = n_tlas2 ? s.tlas4_cap : 0;  // index bound; the node count stays on the device
     dfree(d_w2o); dfree(glo); dfree(ghi); dfree(ilo); dfree(ihi); dfree(iseg);
     dfree(order); dfree(d_root); dfree(qa); dfree(qb);
+}
+
+// Geometry g's vertices were replaced on the device (same count, same
+// triangles): refit its BLAS on the GPU, no host round trip.
+void refit_blas(Scene& s, uint32_t g) {
+    hipStream_t st = stream();
+    const uint32_t TB = 256;
+    const uint32_t n = s.h_toff[g + 1] - s.h_toff[g];
+    if (n == 0) return;
+    float4* lo = dalloc<float4>(n);
+    float4* hi = dalloc<float4>(n);
+    k_geom_slots<<<blocks(n, TB), TB, 0, st>>>(g, s.d_verts, s.d_idx, s.d_voff, s.d_toff, s.d_tri, lo, hi);
+    if (n >= 2) {
+        int* parent_node = dalloc<int>(n - 1);
+        int* parent_leaf = dalloc<int>(n);
+        int* flags = dalloc<int>(n - 1);
+        k_fill_int<<<blocks(n - 1, TB), TB, 0, st>>>(parent_node, -1, n - 1);
+        k_fill_int<<<blocks(n, TB), TB, 0, st>>>(parent_leaf, -1, n);
+        k_geom_parents<<<blocks(n - 1, TB), TB, 0, st>>>(g, s.d_toff, s.d_node_off, s.d_blas, s.n_blas_nodes, parent_node,
+                                                        parent_leaf, flags);
+        k_geom_refit2<<<blocks(n, TB), TB, 0, st>>>(g, s.d_toff, s.d_node_off, s.n_blas_nodes, lo, hi, parent_node,
+                                                   parent_leaf, s.d_blas, flags);
+        k_geom_refit4<<<blocks(n - 1, TB), TB, 0, st>>>(g, s.d_toff, s.d_node_off, s.d_blas_cnt, s.d_blas,
+                                                       s.n_blas_nodes, s.d_tri, s.d_blas4);
+        dfree(parent_node);
+        dfree(parent_leaf);
+        dfree(flags);
+    }
+    KF_CHECK(hipGetLastError());
+    dfree(lo);
+    dfree(hi);
 }
 
 }  // namespace kf

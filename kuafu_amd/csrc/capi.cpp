@@ -103,6 +103,9 @@ Scene::~Scene() {
     release(d_blas4); release(d_blas_root4); release(d_tlas4); release(d_gflags); release(d_tlas_aux);
     release(d_tlas_err);
     release(d_env_off); release(d_env_node_off); release(d_inst_env); release(d_env_root); release(d_env_cnt);
+    release(d_node_off); release(d_blas_cnt);
+    if (h_vstage) (void)hipHostFree(h_vstage);
+    if (vstage_ev) (void)hipEventDestroy(vstage_ev);
     if (h_stage) (void)hipHostFree(h_stage);
     if (stage_ev) (void)hipEventDestroy(stage_ev);
     release(d_inst_src); release(d_tex); release(d_tex_info); release(d_env_info); release(d_srgb);
@@ -451,6 +454,58 @@ int kf_scene_update_instances(kf_scene* s, const kf_instance* inst, uint32_t n) 
     }
 }
 
+int kf_scene_update_geometry(kf_scene* s, uint32_t g, const kf_vertex* vertices, uint32_t n) {
+    try {
+        if (!s || (n && !vertices)) throw Error("kf_scene_update_geometry: null argument");
+        if (g >= s->n_geom) throw Error("kf_scene_update_geometry: no such geometry");
+        if (n != s->h_voff[g + 1] - s->h_voff[g])
+            throw Error("kf_scene_update_geometry: the vertex count must stay the same (rebuild the scene otherwise)");
+        hipStream_t st = stream();
+        const size_t bytes = (size_t)n * sizeof(kf_vertex);
+        if (bytes) {
+            // host vertices -> pinned staging -> device, no wait for the GPU
+            // unless the previous update's copy is still pending
+            if (!s->vstage_ev) KF_CHECK(hipEventCreateWithFlags(&s->vstage_ev, hipEventDisableTiming));
+            if (s->vstage_pending) KF_CHECK(hipEventSynchronize(s->vstage_ev));
+            s->vstage_pending = false;
+            if (bytes > s->vstage_cap) {
+                if (s->h_vstage) KF_CHECK(hipHostFree(s->h_vstage));
+                s->h_vstage = nullptr;
+                KF_CHECK(hipHostMalloc((void**)&s->h_vstage, bytes, hipHostMallocDefault));
+                s->vstage_cap = bytes;
+            }
+            memcpy(s->h_vstage, vertices, bytes);
+            KF_CHECK(hipMemcpyAsync(s->d_verts + 3 * (size_t)s->h_voff[g], s->h_vstage, bytes, hipMemcpyHostToDevice, st));
+            KF_CHECK(hipEventRecord(s->vstage_ev, st));
+            s->vstage_pending = true;
+        }
+        refit_blas(*s, g);
+        // instance boxes follow the refit BLAS
+        std::vector<kf_instance> inst = s->h_inst;
+        build_tlas(*s, inst.data(), (uint32_t)inst.size());
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int kf_scene_update_materials(kf_scene* s, const kf_material* m, uint32_t n) {
+    try {
+        if (!s || (n && !m)) throw Error("kf_scene_update_materials: null argument");
+        if (n != s->n_mat) throw Error("kf_scene_update_materials: the material count must stay the same");
+        if (n) {
+            hipStream_t st = stream();
+            KF_CHECK(hipMemcpyAsync(s->d_mats, m, n * sizeof(kf_material), hipMemcpyHostToDevice, st));
+            KF_CHECK(hipStreamSynchronize(st));
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
 int kf_scene_set_lights(kf_scene* s, const kf_dir_light* d, const kf_point_lights* p, const kf_active_lights* a) {
     if (!s) {
         set_error("kf_scene_set_lights: null scene");
@@ -482,6 +537,19 @@ static uint32_t subtree_depth(const std::vector<Node>& nodes, int ref) {
 
 // BVH4 node count of the TLAS (kept on the device by the asynchronous
 // rebuild); also reports a malformed tree.
+// BVH4 nodes of all BLAS: geometry g's sit at the start of its range
+// [h_node_off[g], h_node_off[g] + n_g - 1); the counts stay on the device.
+static uint32_t blas4_count(kf_scene* s) {
+    if (!s->d_blas_cnt || !s->n_geom) return 0;
+    std::vector<uint32_t> c(2 * s->n_geom);
+    KF_CHECK(hipStreamSynchronize(stream()));
+    KF_CHECK(hipMemcpy(c.data(), s->d_blas_cnt, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint32_t tot = 0;
+    for (uint32_t g = 0; g < s->n_geom; ++g)
+        if (s->h_toff[g + 1] - s->h_toff[g] >= 2) tot += c[2 * g] - s->h_node_off[g];
+    return tot;
+}
+
 // With environments the count is the sum over their trees, whose nodes sit
 // in per-environment ranges of the array (not packed).
 static uint32_t tlas4_count(kf_scene* s) {
@@ -529,7 +597,7 @@ int kf_scene_bvh_info(kf_scene* s, kf_bvh_info* out) {
         } else if (s->n_inst > 1) {
             out->max_tlas_depth = subtree_depth(tn, s->tlas_root);
         }
-        out->blas4_nodes = s->n_blas4;
+        out->blas4_nodes = blas4_count(s);
         out->tlas4_nodes = tlas4_count(s);
         return 0;
     } catch (const std::exception& e) {

@@ -119,8 +119,15 @@ struct Scene {
     uint32_t n_blas_nodes = 0;
     Node* d_tlas = nullptr;
     // BVH4 copies read by the traversal
-    Node4* d_blas4 = nullptr;
+    Node4* d_blas4 = nullptr;                // geometry g's BVH4 nodes in [h_node_off[g], ... + n_g - 1)
     int* d_blas_root4 = nullptr;
+    uint32_t* d_node_off = nullptr;          // h_node_off on the device
+    uint32_t* d_blas_cnt = nullptr;          // 2 collapse counters per geometry: [2g] = next free BVH4 slot
+    std::vector<kf_instance> h_inst;         // instances of the last TLAS build (for rebuilds after a refit)
+    uint8_t* h_vstage = nullptr;             // pinned staging of geometry vertex updates
+    size_t vstage_cap = 0;
+    hipEvent_t vstage_ev = nullptr;
+    bool vstage_pending = false;
     std::vector<int> h_blas_root4;
     uint32_t n_blas4 = 0;
     Node4* d_tlas4 = nullptr;
@@ -182,6 +189,9 @@ uint32_t collapse_bvh4(const Node* d_b2, uint32_t n_b2, const std::vector<int>& 
 // and BVH2 node offsets node_off (known by construction, no device read).
 void env_roots(const std::vector<uint32_t>& env_off, const std::vector<uint32_t>& node_off, std::vector<int>& roots);
 void build_blas(Scene& s);
+// Refit geometry g's BLAS (BVH2 + BVH4 + triangle slots) after its
+// vertices changed in place; the TLAS must be rebuilt afterwards.
+void refit_blas(Scene& s, uint32_t g);
 void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n);
 
 // ------------------------------------------------------------------ renderer

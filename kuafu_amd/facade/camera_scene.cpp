@@ -5,7 +5,12 @@
 // Scene: bookkeeping and limits as scene.cpp:46-173; buildDevice() is
 // uploadGeometries + uploadGeometryInstances + createBottomLevelAS + buildTlas
 // (context.cpp:323-336), updateDevice() is the per-frame updateTlas +
-// uploadLightBuffers (context.cpp:337-343, scene.cpp:252-292).
+// uploadLightBuffers (context.cpp:337-343, scene.cpp:252-292).  Unlike the
+// reference, which rebuilds every BLAS whenever geometries or instances
+// change, syncDevice() rebuilds from scratch only when the set of
+// geometries, their triangles, the material count or the textures changed:
+// instance changes rebuild the TLAS alone, and geometries re-marked
+// uninitialized with the same triangles have their BLAS refit.
 #include <cstring>
 #include <stdexcept>
 
@@ -368,24 +373,27 @@ static void fill_lights(const Scene& s, std::shared_ptr<DirectionalLight> dl,
     }
 }
 
-void Scene::buildDevice() {
-    // flatten geometries (reference layouts) and the global material list
-    std::vector<uint32_t> voff{0}, toff{0}, flags;
-    std::vector<kf_vertex> verts;
-    std::vector<uint32_t> idx, mi;
-    for (auto& g : mGeometries) {
-        size_t nt = g->indices.size() / 3;
-        verts.resize(verts.size() + g->vertices.size());
-        memcpy(&verts[verts.size() - g->vertices.size()], g->vertices.data(), g->vertices.size() * sizeof(Vertex));
-        idx.insert(idx.end(), g->indices.begin(), g->indices.begin() + 3 * nt);
-        if (g->matIndex.size() < nt) throw std::runtime_error("Geometry without a material per triangle");
-        mi.insert(mi.end(), g->matIndex.begin(), g->matIndex.begin() + nt);
-        voff.push_back((uint32_t)verts.size());
-        toff.push_back((uint32_t)(idx.size() / 3));
-        flags.push_back((g->isOpaque ? KF_GEOM_OPAQUE : 0u) | (g->hideRender ? KF_GEOM_HIDDEN : 0u));
-        g->initialized = true;
-    }
-    size_t n_tex = mTextures.size();
+// 64-bit FNV-1a over the triangles and material indices of a geometry.
+static uint64_t topology_hash(const Geometry& g) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const std::vector<uint32_t>& v) {
+        for (uint32_t x : v)
+            for (int k = 0; k < 4; ++k) {
+                h ^= (x >> (8 * k)) & 0xffu;
+                h *= 1099511628211ull;
+            }
+    };
+    mix(g.indices);
+    mix(g.matIndex);
+    return h;
+}
+
+static Scene::DeviceGeometry device_geometry(const Geometry& g) {
+    return {&g, g.vertices.size(), g.indices.size(), topology_hash(g), g.isOpaque, g.hideRender};
+}
+
+// The global material list in the NiceMaterialSSBO layout (scene.cpp:354-400).
+static std::vector<kf_material> device_materials(size_t n_tex) {
     std::vector<kf_material> mats;
     for (auto& m : global::materials) {
         kf_material s{};
@@ -405,13 +413,40 @@ void Scene::buildDevice() {
         s.transmission_tex = texture_ref(m.transmissionTexPath, n_tex, "transmission");
         mats.push_back(s);
     }
+    return mats;
+}
+
+static std::vector<kf_instance> device_instances(const std::vector<std::shared_ptr<GeometryInstance>>& instances) {
     std::vector<kf_instance> inst;
-    for (auto& gi : mGeometryInstances) {
+    for (auto& gi : instances) {
         kf_instance k{};
         memcpy(k.transform, glm::value_ptr(gi->transform), sizeof(k.transform));
         k.geometry_index = (uint32_t)gi->geometryIndex;
         inst.push_back(k);
     }
+    return inst;
+}
+
+void Scene::buildDevice() {
+    // flatten geometries (reference layouts) and the global material list
+    std::vector<uint32_t> voff{0}, toff{0}, flags;
+    std::vector<kf_vertex> verts;
+    std::vector<uint32_t> idx, mi;
+    for (auto& g : mGeometries) {
+        size_t nt = g->indices.size() / 3;
+        verts.resize(verts.size() + g->vertices.size());
+        memcpy(&verts[verts.size() - g->vertices.size()], g->vertices.data(), g->vertices.size() * sizeof(Vertex));
+        idx.insert(idx.end(), g->indices.begin(), g->indices.begin() + 3 * nt);
+        if (g->matIndex.size() < nt) throw std::runtime_error("Geometry without a material per triangle");
+        mi.insert(mi.end(), g->matIndex.begin(), g->matIndex.begin() + nt);
+        voff.push_back((uint32_t)verts.size());
+        toff.push_back((uint32_t)(idx.size() / 3));
+        flags.push_back((g->isOpaque ? KF_GEOM_OPAQUE : 0u) | (g->hideRender ? KF_GEOM_HIDDEN : 0u));
+        g->initialized = true;
+    }
+    size_t n_tex = mTextures.size();
+    std::vector<kf_material> mats = device_materials(n_tex);
+    std::vector<kf_instance> inst = device_instances(mGeometryInstances);
     std::vector<kf_texture_desc> tex;
     for (size_t i = 0; i < n_tex; ++i) tex.push_back({mTextureSizes[i].first, mTextureSizes[i].second, mTextures[i].data()});
     std::vector<kf_texture_desc> faces;
@@ -447,16 +482,47 @@ void Scene::buildDevice() {
     mDevice = s;
     mUploadGeometries = false;
     mUploadGeometryInstancesToBuffer = false;
+    ++mDeviceBuilds;
+    mDeviceGeometries.clear();
+    for (auto& g : mGeometries) mDeviceGeometries.push_back(device_geometry(*g));
+    mDeviceMaterials = global::materials.size();
+    mDeviceTextures = mTextures.size();
+}
+
+bool Scene::refitDevice() {
+    if (mGeometries.size() != mDeviceGeometries.size() || global::materials.size() != mDeviceMaterials ||
+        mTextures.size() != mDeviceTextures)
+        return false;
+    for (size_t i = 0; i < mGeometries.size(); ++i) {
+        const DeviceGeometry now = device_geometry(*mGeometries[i]), was = mDeviceGeometries[i];
+        if (now.geometry != was.geometry || now.vertices != was.vertices || now.indices != was.indices ||
+            now.topology != was.topology || now.opaque != was.opaque || now.hidden != was.hidden)
+            return false;
+    }
+    for (size_t i = 0; i < mGeometries.size(); ++i) {
+        Geometry& g = *mGeometries[i];
+        if (g.initialized) continue;
+        kf_ok(kf_scene_update_geometry(mDevice, (uint32_t)i, reinterpret_cast<const kf_vertex*>(g.vertices.data()),
+                                       (uint32_t)g.vertices.size()));
+        g.initialized = true;
+    }
+    std::vector<kf_material> mats = device_materials(mTextures.size());
+    kf_ok(kf_scene_update_materials(mDevice, mats.data(), (uint32_t)mats.size()));
+    mUploadGeometries = false;
+    return true;
+}
+
+void Scene::syncDevice() {
+    if (!mDevice || (mUploadGeometries && !refitDevice())) {
+        buildDevice();
+        return;
+    }
+    updateDevice();  // the TLAS (instances may have changed) and the lights
+    mUploadGeometryInstancesToBuffer = false;
 }
 
 void Scene::updateDevice() {
-    std::vector<kf_instance> inst;
-    for (auto& gi : mGeometryInstances) {
-        kf_instance k{};
-        memcpy(k.transform, glm::value_ptr(gi->transform), sizeof(k.transform));
-        k.geometry_index = (uint32_t)gi->geometryIndex;
-        inst.push_back(k);
-    }
+    std::vector<kf_instance> inst = device_instances(mGeometryInstances);
     kf_ok(kf_scene_update_instances(mDevice, inst.data(), (uint32_t)inst.size()));
     kf_dir_light d;
     kf_point_lights p;

@@ -30,11 +30,8 @@ void Context::render() {
     KF_ASSERT(scene, "No scene to render!");
     Camera* cam = scene->getCamera();
     KF_ASSERT(cam, "Trying to render with an invalid camera!");
-    // Context::update (context.cpp:298-350)
-    if (!scene->mDevice || scene->mUploadGeometries || scene->mUploadGeometryInstancesToBuffer)
-        scene->buildDevice();
-    else
-        scene->updateDevice();
+    // Context::update (context.cpp:298-350), incrementally
+    scene->syncDevice();
     if (pConfig->mAccumulateFrames)
         ++global::frameCount;
     else

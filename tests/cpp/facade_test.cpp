@@ -124,6 +124,73 @@ static void json_vec(std::ostream& o, const float* v, int n) {
     o << "]";
 }
 
+// The scene, read back through public fields only, as JSON for the Python
+// side to rebuild over the C ABI.
+static void dump_scene(const std::string& dir, Scene* s, const DirectionalLight& dl, const PointLight& pl,
+                       const ActiveLight& al, Camera* cam, int frames, int first_frame = 0) {
+    std::ofstream j(dir + "/scene.json");
+    j.precision(9);
+    j << "{\"geometries\":[";
+    for (size_t g = 0; g < s->getGeometries().size(); ++g) {
+        auto& G = *s->getGeometries()[g];
+        j << (g ? "," : "") << "{\"pos\":[";
+        for (size_t v = 0; v < G.vertices.size(); ++v) {
+            j << (v ? "," : "");
+            json_vec(j, &G.vertices[v].pos.x, 3);
+        }
+        j << "],\"normal\":[";
+        for (size_t v = 0; v < G.vertices.size(); ++v) {
+            j << (v ? "," : "");
+            json_vec(j, &G.vertices[v].normal.x, 3);
+        }
+        j << "],\"indices\":[";
+        for (size_t i = 0; i < G.indices.size(); ++i) j << (i ? "," : "") << G.indices[i];
+        j << "],\"material\":" << G.matIndex[0] << "}";
+    }
+    j << "],\"instances\":[";
+    for (size_t i = 0; i < s->getGeometryInstances().size(); ++i) {
+        auto& I = *s->getGeometryInstances()[i];
+        j << (i ? "," : "") << "{\"geometry\":" << I.geometryIndex << ",\"transform\":";
+        json_vec(j, glm::value_ptr(I.transform), 16);
+        j << "}";
+    }
+    j << "],\"materials\":[";
+    for (size_t m = 0; m < global::materials.size(); ++m) {
+        auto& M = global::materials[m];
+        j << (m ? "," : "") << "{\"diffuse\":";
+        json_vec(j, &M.diffuseColor.x, 3);
+        j << ",\"metallic\":" << M.metallic << ",\"specular\":" << M.specular << ",\"roughness\":" << M.roughness
+          << ",\"ior\":" << M.ior << ",\"transmission\":" << M.transmission << ",\"alpha\":" << M.alpha
+          << ",\"emission\":";
+        json_vec(j, &M.emission.x, 3);
+        j << ",\"emissionStrength\":" << M.emissionStrength << "}";
+    }
+    j << "],\"dir_light\":{\"direction\":";
+    json_vec(j, &dl.direction.x, 3);
+    j << ",\"color\":";
+    json_vec(j, &dl.color.x, 3);
+    j << ",\"strength\":" << dl.strength << ",\"softness\":" << dl.softness << "},\"point_light\":{\"position\":";
+    json_vec(j, &pl.position.x, 3);
+    j << ",\"color\":";
+    json_vec(j, &pl.color.x, 3);
+    j << ",\"strength\":" << pl.strength << ",\"radius\":" << pl.radius << "},\"active_light\":{\"view\":";
+    json_vec(j, glm::value_ptr(al.viewMat), 16);
+    j << ",\"color\":";
+    json_vec(j, &al.color.x, 3);
+    j << ",\"fov\":" << al.fov << ",\"strength\":" << al.strength << "},\"clear_color\":";
+    glm::vec4 cc = s->getClearColor();
+    json_vec(j, &cc.x, 4);
+    j << ",\"camera\":{\"position\":";
+    json_vec(j, &cam->getPosition().x, 3);
+    j << ",\"front\":";
+    json_vec(j, &cam->getFront().x, 3);
+    j << ",\"up\":[0,0,1],\"width\":" << cam->getWidth() << ",\"height\":" << cam->getHeight()
+      << ",\"fx\":" << cam->getFocalX() << ",\"fy\":" << cam->getFocalY() << ",\"cx\":" << cam->getPrincipalPointX()
+      << ",\"cy\":" << cam->getPrincipalPointY() << "},\"spp\":4,\"path_depth\":4,\"frames\":" << frames
+      << ",\"first_frame\":" << first_frame << "}";
+    j.close();
+}
+
 static int gpu_run(const std::string& dir) {
     Config c;
     c.setPresent(false);
@@ -187,67 +254,7 @@ static int gpu_run(const std::string& dir) {
         std::ofstream g(dir + "/bgra.bin", std::ios::binary);
         g.write((const char*)bgra.data(), bgra.size());
     }
-    // the scene, read back through public fields only
-    std::ofstream j(dir + "/scene.json");
-    j.precision(9);
-    j << "{\"geometries\":[";
-    for (size_t g = 0; g < s->getGeometries().size(); ++g) {
-        auto& G = *s->getGeometries()[g];
-        j << (g ? "," : "") << "{\"pos\":[";
-        for (size_t v = 0; v < G.vertices.size(); ++v) {
-            j << (v ? "," : "");
-            json_vec(j, &G.vertices[v].pos.x, 3);
-        }
-        j << "],\"normal\":[";
-        for (size_t v = 0; v < G.vertices.size(); ++v) {
-            j << (v ? "," : "");
-            json_vec(j, &G.vertices[v].normal.x, 3);
-        }
-        j << "],\"indices\":[";
-        for (size_t i = 0; i < G.indices.size(); ++i) j << (i ? "," : "") << G.indices[i];
-        j << "],\"material\":" << G.matIndex[0] << "}";
-    }
-    j << "],\"instances\":[";
-    for (size_t i = 0; i < s->getGeometryInstances().size(); ++i) {
-        auto& I = *s->getGeometryInstances()[i];
-        j << (i ? "," : "") << "{\"geometry\":" << I.geometryIndex << ",\"transform\":";
-        json_vec(j, glm::value_ptr(I.transform), 16);
-        j << "}";
-    }
-    j << "],\"materials\":[";
-    for (size_t m = 0; m < global::materials.size(); ++m) {
-        auto& M = global::materials[m];
-        j << (m ? "," : "") << "{\"diffuse\":";
-        json_vec(j, &M.diffuseColor.x, 3);
-        j << ",\"metallic\":" << M.metallic << ",\"specular\":" << M.specular << ",\"roughness\":" << M.roughness
-          << ",\"ior\":" << M.ior << ",\"transmission\":" << M.transmission << ",\"alpha\":" << M.alpha
-          << ",\"emission\":";
-        json_vec(j, &M.emission.x, 3);
-        j << ",\"emissionStrength\":" << M.emissionStrength << "}";
-    }
-    j << "],\"dir_light\":{\"direction\":";
-    json_vec(j, &dl->direction.x, 3);
-    j << ",\"color\":";
-    json_vec(j, &dl->color.x, 3);
-    j << ",\"strength\":" << dl->strength << ",\"softness\":" << dl->softness << "},\"point_light\":{\"position\":";
-    json_vec(j, &pl->position.x, 3);
-    j << ",\"color\":";
-    json_vec(j, &pl->color.x, 3);
-    j << ",\"strength\":" << pl->strength << ",\"radius\":" << pl->radius << "},\"active_light\":{\"view\":";
-    json_vec(j, glm::value_ptr(al->viewMat), 16);
-    j << ",\"color\":";
-    json_vec(j, &al->color.x, 3);
-    j << ",\"fov\":" << al->fov << ",\"strength\":" << al->strength << "},\"clear_color\":";
-    glm::vec4 cc = s->getClearColor();
-    json_vec(j, &cc.x, 4);
-    j << ",\"camera\":{\"position\":";
-    json_vec(j, &cam->getPosition().x, 3);
-    j << ",\"front\":";
-    json_vec(j, &cam->getFront().x, 3);
-    j << ",\"up\":[0,0,1],\"width\":" << cam->getWidth() << ",\"height\":" << cam->getHeight()
-      << ",\"fx\":" << cam->getFocalX() << ",\"fy\":" << cam->getFocalY() << ",\"cx\":" << cam->getPrincipalPointX()
-      << ",\"cy\":" << cam->getPrincipalPointY() << "},\"spp\":4,\"path_depth\":4,\"frames\":2}";
-    j.close();
+    dump_scene(dir, s, *dl, *pl, *al, cam, 2);
     // the denoise stage through the facade (random weights unless a weight file ships)
     k->getConfig().setUseDenoiser(true);
     k->run();
@@ -257,11 +264,100 @@ static int gpu_run(const std::string& dir) {
     return 0;
 }
 
+// Incremental device updates: instance changes rebuild the TLAS only, a
+// geometry re-marked uninitialized with the same triangles has its BLAS refit
+// and materials are re-uploaded, all on the device scene built first; a new
+// geometry makes a full build.  The final frame and scene are dumped for the
+// Python side to compare with a scene built from scratch over the C ABI.
+static int gpu_update(const std::string& dir) {
+    Config c;
+    c.setPresent(false);
+    c.setPerPixelSampleRate(4);
+    c.setPathDepth(4);
+    c.setGeometryLimit(64);
+    c.setAccumulatingFrames(false);
+    auto k = std::make_unique<Kuafu>(std::make_shared<Config>(c));
+    Scene* s = k->getScene();
+    NiceMaterial floorM;
+    floorM.diffuseColor = {0.7f, 0.7f, 0.7f};
+    floorM.roughness = 0.8f;
+    auto floor = createYZPlane(true, floorM);
+    NiceMaterial red;
+    red.diffuseColor = {0.8f, 0.2f, 0.2f};
+    auto cube = createCube(true, red);
+    NiceMaterial metal;
+    metal.metallic = 1.0f;
+    metal.roughness = 0.3f;
+    auto sphere = createSphere(true, metal);
+    s->setGeometries({floor, cube, sphere});
+    glm::mat4 ft = glm::translate(glm::mat4(1.0f), {0, 0, -0.5f});
+    ft = glm::rotate(ft, glm::radians(90.f), {0., -1., 0.});
+    ft = glm::scale(ft, glm::vec3(6.0f));
+    s->setGeometryInstances({instance(floor, ft), instance(cube, glm::scale(glm::translate(glm::mat4(1.0f), {0, -1.5f, 0}), glm::vec3(0.4f))),
+                             instance(sphere, glm::scale(glm::translate(glm::mat4(1.0f), {0.2f, 0.2f, 0.1f}), glm::vec3(0.5f)))});
+    auto dl = std::make_shared<DirectionalLight>();
+    dl->direction = {-0.3f, 0.2f, -1.0f};
+    dl->strength = 2.0f;
+    s->setDirectionalLight(dl);
+    auto pl = std::make_shared<PointLight>();
+    pl->position = {1, -1, 2};
+    pl->strength = 6;
+    pl->radius = 0;
+    s->addPointLight(pl);
+    auto al = std::make_shared<ActiveLight>();
+    al->viewMat = glm::lookAt(glm::vec3(3, 0, 3), glm::vec3(0, 0, 0), glm::vec3(0, 0, 1));
+    al->fov = 0.6f;
+    al->strength = 20.0f;
+    s->addActiveLight(al);
+    s->setClearColor({0.6f, 0.7f, 0.8f, 0.5f});
+    Camera* cam = s->createCamera(96, 64);
+    s->setCamera(cam);
+    cam->setFullPerspective(96, 64, 80, 80, 48, 32, 0);
+    cam->setPosition({-4.0f, 0.0f, 1.5f});
+    cam->setFront(glm::normalize(glm::vec3(1.0f, 0.0f, -0.3f)));
+    cam->setUp({0, 0, 1});
+    k->run();
+    CHECK(s->getDeviceBuilds() == 1);
+    const kf_scene* dev = s->getDeviceScene();
+    // instances: one moved, one added -> TLAS only
+    s->getGeometryInstance(1)->setTransform(glm::scale(glm::translate(glm::mat4(1.0f), {0.3f, -1.2f, 0.1f}), glm::vec3(0.5f)));
+    s->submitGeometryInstance(instance(cube, glm::scale(glm::translate(glm::mat4(1.0f), {0.4f, 1.2f, 0.0f}), glm::vec3(0.3f))));
+    k->run();
+    CHECK(s->getDeviceBuilds() == 1 && s->getDeviceScene() == dev);
+    // the sphere deformed in place and a material edited -> refit + material upload
+    for (auto& v : sphere->vertices) {
+        v.pos.x *= 1.4f;
+        v.pos.z += 0.2f * v.pos.y * v.pos.y;
+    }
+    sphere->initialized = false;
+    global::materials[cube->matIndex[0]].diffuseColor = {0.1f, 0.6f, 0.9f};
+    s->markGeometriesChanged();
+    k->run();
+    CHECK(s->getDeviceBuilds() == 1 && s->getDeviceScene() == dev && sphere->initialized);
+    auto rgba = cam->downloadTarget(0);
+    {
+        std::ofstream f(dir + "/rgba.bin", std::ios::binary);
+        f.write((const char*)rgba.data(), rgba.size() * 4);
+    }
+    dump_scene(dir, s, *dl, *pl, *al, cam, 1, global::frameCount);  // -1: frames are not accumulated
+    // a new geometry -> full build
+    NiceMaterial green;
+    green.diffuseColor = {0.2f, 0.8f, 0.2f};
+    auto ball = createSphere(true, green);
+    s->submitGeometry(ball);
+    s->submitGeometryInstance(instance(ball, glm::scale(glm::translate(glm::mat4(1.0f), {-0.5f, 0.5f, 0.0f}), glm::vec3(0.3f))));
+    k->run();
+    CHECK(s->getDeviceBuilds() == 2);
+    std::cout << "OK" << std::endl;
+    return 0;
+}
+
 int main(int argc, char** argv) {
     try {
         if (argc >= 2 && std::string(argv[1]) == "cpu") return cpu_checks();
         if (argc >= 3 && std::string(argv[1]) == "gpu") return gpu_run(argv[2]);
-        std::fprintf(stderr, "usage: facade_test cpu | gpu <dir>\n");
+        if (argc >= 3 && std::string(argv[1]) == "gpu_update") return gpu_update(argv[2]);
+        std::fprintf(stderr, "usage: facade_test cpu | gpu <dir> | gpu_update <dir>\n");
         return 2;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "exception: %s\n", e.what());

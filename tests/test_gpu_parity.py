@@ -122,10 +122,45 @@ def walk_bvh4(nodes, root, leaf_box_check, inner_kind):
     return seen, leaves
 
 
+def check_blas4(gs, s):
+    """Every triangle slot of a geometry in exactly one BVH4 leaf whose box
+    holds the triangle (at its current vertex positions), every node
+    reached once, geometry g's nodes packed at the start of its range
+    [node_off[g], node_off[g] + n_g - 1), fewer nodes than the BVH2."""
+    nodes4, roots4 = gs.read_bvh4(0)
+    info = gs.bvh_info()
+    assert len(nodes4) == info.blas_nodes and info.blas4_nodes < info.blas_nodes
+    toff = np.cumsum([0] + [len(g[1]) for g in s.geoms])
+    node_off = np.cumsum([0] + [max(len(g[1]) - 1, 0) for g in s.geoms])
+    all_seen = np.zeros(len(nodes4), int)
+    for g, (verts, tris, _, _) in enumerate(s.geoms):
+        nt = len(tris)
+        _, _, order = gs.read_blas(g)
+        pos = verts["pos"].astype(np.float64)
+        if nt == 1:
+            assert roots4[g] == ~int(toff[g])
+            continue
+
+        def tri_in_box(slot, lo, hi):
+            local = slot - toff[g]
+            assert 0 <= local < nt
+            p = pos[tris[order[local]]]
+            assert (p.min(0) >= lo).all() and (p.max(0) <= hi).all()
+
+        assert roots4[g] == node_off[g]
+        seen, leaves = walk_bvh4(nodes4, int(roots4[g]), tri_in_box, 1)
+        used = np.nonzero(seen)[0]
+        assert used.min() >= node_off[g] and used.max() < node_off[g + 1]
+        assert used.max() - used.min() + 1 == len(used)  # packed from the start of the range
+        all_seen += seen
+        assert sorted(leaves) == list(range(toff[g], toff[g] + nt))
+    assert all_seen.max() == 1 and all_seen.sum() == info.blas4_nodes
+    return info
+
+
 def test_bvh4_invariants(gpu):
-    """The BVH4 the traversal reads: every triangle slot of a geometry in
-    exactly one leaf whose box holds the triangle, every node reached once,
-    fewer nodes than the BVH2; the TLAS BVH4 holds every instance once."""
+    """The BVH4 the traversal reads (check_blas4); the TLAS BVH4 holds every
+    instance once."""
     s = scenes.SceneData()
     v, t = scenes.icosphere(4)
     s.add_geometry(v, t, scenes.material())
@@ -140,29 +175,7 @@ def test_bvh4_invariants(gpu):
         m[:3, 3] = rng.uniform(-5, 5, 3)
         s.add_instance(i % 4, m)
     gs = GpuScene(s)
-    nodes4, roots4 = gs.read_bvh4(0)
-    info = gs.bvh_info()
-    assert info.blas4_nodes == len(nodes4) and info.blas4_nodes < info.blas_nodes
-    toff = np.cumsum([0] + [len(g[1]) for g in s.geoms])
-    all_seen = np.zeros(len(nodes4), int)
-    for g, (verts, tris, _, _) in enumerate(s.geoms):
-        nt = len(tris)
-        _, _, order = gs.read_blas(g)
-        pos = verts["pos"].astype(np.float64)
-        if nt == 1:
-            assert roots4[g] == ~int(toff[g])
-            continue
-
-        def tri_in_box(slot, lo, hi):
-            local = slot - toff[g]
-            assert 0 <= local < nt
-            p = pos[tris[order[local]]]
-            assert (p.min(0) >= lo - 1e-7).all() and (p.max(0) <= hi + 1e-7).all()
-
-        seen, leaves = walk_bvh4(nodes4, int(roots4[g]), tri_in_box, 1)
-        all_seen += seen
-        assert sorted(leaves) == list(range(toff[g], toff[g] + nt))
-    assert (all_seen == 1).all()
+    info = check_blas4(gs, s)
     # TLAS
     tl, troot = gs.read_bvh4(1)
     assert info.tlas4_nodes == len(tl) and len(tl) < 36
@@ -334,6 +347,49 @@ def test_instance_update_matches_fresh_scene(gpu):
     ref = o.render(w.cam, pc, w.width, w.height, seed=11, threads=8)
     close = np.abs(a - ref["rgba"]) <= 1e-4 * np.maximum(1.0, np.abs(ref["rgba"]))
     assert close.mean() > 0.999
+
+
+def test_geometry_refit_matches_fresh_scene(gpu):
+    """kf_scene_update_geometry (refit of one geometry's BLAS on the GPU,
+    then the TLAS) after two geometries were deformed: BVH2 and BVH4
+    invariants hold at the new positions, closest hits equal the oracle's,
+    and the frame equals, bit for bit, the frame of a scene created with the
+    deformed vertices (whose BVHs are built from scratch)."""
+    import copy
+    w = scenes.config2(width=48, height=27, spp=2, path_depth=4, n_inst=12, subdiv=3)
+    gs = GpuScene(w.data)
+    rd = Renderer(w.width, w.height)
+    pc = w.pc
+    pc.frame_count = 0
+    rd.render(gs, w.cam, pc, seed=3)
+    moved = copy.deepcopy(w.data)
+    rng = np.random.default_rng(17)
+    for g in (3, 7):
+        v, t, mi, fl = moved.geoms[g]
+        v = v.copy()
+        p = v["pos"].astype(np.float64)
+        # anisotropic stretch and a twist about z: boxes change everywhere
+        p *= rng.uniform(0.6, 1.8, 3)
+        a = 0.8 * p[:, 2]
+        p[:, 0], p[:, 1] = p[:, 0] * np.cos(a) - p[:, 1] * np.sin(a), p[:, 0] * np.sin(a) + p[:, 1] * np.cos(a)
+        v["pos"] = p.astype(np.float32)
+        moved.geoms[g] = (v, t, mi, fl)
+        gs.update_geometry(g, v)
+    check_blas4(gs, moved)
+    view = type("View", (), {})()  # the refit BVH2 against the moved vertices
+    view.data, view.read_blas = moved, gs.read_blas
+    for g in (3, 7):
+        check_blas(view, g)
+    rng2 = np.random.default_rng(5)
+    rays = rays_toward(rng2, 4000, np.zeros(3), 10.0, 25.0)
+    assert_hits_equal(gs.trace_closest(rays), oracle.Scene(moved).trace_closest(rays))
+    rd.render(gs, w.cam, pc, seed=11)
+    a = rd.read(0).copy()
+    rd2 = Renderer(w.width, w.height)
+    rd2.render(GpuScene(moved), w.cam, pc, seed=11)
+    assert (a == rd2.read(0)).all()
+    with pytest.raises(RuntimeError, match="vertex count"):
+        gs.update_geometry(3, moved.geoms[3][0][:-1])
 
 
 @pytest.mark.parametrize("camera", ["left", "right"])
