@@ -107,6 +107,7 @@ private:
   private:
     std::vector<DeviceGeometry> mDeviceGeometries;
     size_t mDeviceMaterials = 0, mDeviceTextures = 0, mDeviceBuilds = 0;
+    uint32_t mEnvVersion = 0, mDeviceEnvVersion = 0;  // environment maps set so far / in the device scene
 
     bool initialized = false;
     glm::vec4 mClearColor = glm::vec4(0.F, 0.F, 0.F, 1.F);

@@ -223,6 +223,15 @@ int kf_camera_ubo_compute(const float position[3], const float front[3], const f
  * traversal applies to rays (gl_WorldToObjectEXT). */
 int kf_world_to_object(const float transform[16], float out_rows[12]);
 
+/* Environment cube map from a KTX file (KTX 1 or 2; uncompressed 8-bit
+ * RGBA, 6 square faces, base level), as vkCore::Cubemap::init loads it for
+ * Scene::setEnvironmentMap (vkCore.hpp:1852-1970, scene.cpp:301-303): *size =
+ * face width; faces (may be NULL to query the size) receives 6 * size * size
+ * * 4 bytes in the order +X, -X, +Y, -Y, +Z, -Z, to be used as sRGB (the
+ * reference uploads them as R8G8B8A8_SRGB whatever the file's format).
+ * Host only. */
+int kf_ktx_cubemap_read(const char* path, uint32_t* size, uint8_t* faces, uint64_t max_bytes);
+
 /* Active-light UBO slot for a light, as scene.cpp:266-281 fills it. */
 int kf_active_light_fill(kf_active_lights* ubo, int slot, const float view[16], const float color[3],
                          float fov, float strength, float softness, int tex_id);

@@ -132,6 +132,7 @@ EXPORTS = [
     ("kf_scene_destroy", None, [C.c_void_p]),
     ("kf_scene_update_instances", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     ("kf_scene_update_geometry", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
+    ("kf_ktx_cubemap_read", C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("kf_scene_update_materials", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     ("kf_scene_set_lights", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("kf_scene_bvh_info", C.c_int, [C.c_void_p, C.c_void_p]),

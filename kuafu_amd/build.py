@@ -18,7 +18,7 @@ FACADE_LIB = os.path.join(HERE, "libkuafu.so")
 FACADE = ["glm_impl.cpp", "config_global.cpp", "geometry.cpp", "camera_scene.cpp", "kuafu.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["capi.cpp", "bvh.hip", "trace.hip", "pathtrace.hip", "denoise.hip"]
+SOURCES = ["capi.cpp", "ktx.cpp", "bvh.hip", "trace.hip", "pathtrace.hip", "denoise.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

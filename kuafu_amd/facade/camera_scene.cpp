@@ -238,10 +238,26 @@ auto Scene::findGeometry(std::string_view path) const -> std::shared_ptr<Geometr
 }
 
 void Scene::setEnvironmentMap(std::string_view path) {
-    // KTX cube maps need libktx (absent); an empty path selects the clear colour
+    // KTX cube maps (scene.cpp:301-303, vkCore.hpp:1852-1970) are read here,
+    // when set, instead of at the next upload; an empty path selects the
+    // clear colour
     mEnvironmentMapTexturePath = path;
-    if (!path.empty()) throw std::runtime_error("cubemap format not supported: " + std::string(path));
-    mUseEnvironmentMap = false;
+    if (path.empty()) {
+        mUseEnvironmentMap = false;
+        return;
+    }
+    const std::string p(path);
+    const bool ktx = (p.size() >= 4 && p.compare(p.size() - 4, 4, ".ktx") == 0) ||
+                     (p.size() >= 5 && p.compare(p.size() - 5, 5, ".ktx2") == 0);
+    if (!ktx) throw std::runtime_error("cubemap format not supported: " + p);
+    uint32_t size = 0;
+    kf_ok(kf_ktx_cubemap_read(p.c_str(), &size, nullptr, 0));
+    std::vector<uint8_t> all(6ull * 4 * size * size);
+    kf_ok(kf_ktx_cubemap_read(p.c_str(), &size, all.data(), all.size()));
+    std::vector<std::vector<uint8_t>> faces(6);
+    const size_t face = 4ull * size * size;
+    for (int f = 0; f < 6; ++f) faces[f].assign(all.begin() + f * face, all.begin() + (f + 1) * face);
+    setEnvironmentCubemap(faces, size);
 }
 void Scene::setEnvironmentCubemap(const std::vector<std::vector<uint8_t>>& faces, uint32_t size) {
     if (faces.size() != 6) throw std::runtime_error("setEnvironmentCubemap: 6 faces expected");
@@ -250,6 +266,7 @@ void Scene::setEnvironmentCubemap(const std::vector<std::vector<uint8_t>>& faces
     mEnvFaces = faces;
     mEnvSize = size;
     mEnvKind = KF_ENV_CUBEMAP;
+    ++mEnvVersion;
     mUseEnvironmentMap = true;
     markGeometriesChanged();
 }
@@ -257,6 +274,7 @@ void Scene::setEnvironmentGradient(const glm::vec3& bottom, const glm::vec3& top
     float g[8] = {bottom.x, bottom.y, bottom.z, 0, top.x, top.y, top.z, 0};
     memcpy(mEnvGradient, g, sizeof(g));
     mEnvKind = KF_ENV_GRADIENT;
+    ++mEnvVersion;
     mUseEnvironmentMap = true;
     markGeometriesChanged();
 }
@@ -487,11 +505,12 @@ void Scene::buildDevice() {
     for (auto& g : mGeometries) mDeviceGeometries.push_back(device_geometry(*g));
     mDeviceMaterials = global::materials.size();
     mDeviceTextures = mTextures.size();
+    mDeviceEnvVersion = mEnvVersion;
 }
 
 bool Scene::refitDevice() {
     if (mGeometries.size() != mDeviceGeometries.size() || global::materials.size() != mDeviceMaterials ||
-        mTextures.size() != mDeviceTextures)
+        mTextures.size() != mDeviceTextures || mEnvVersion != mDeviceEnvVersion)
         return false;
     for (size_t i = 0; i < mGeometries.size(); ++i) {
         const DeviceGeometry now = device_geometry(*mGeometries[i]), was = mDeviceGeometries[i];

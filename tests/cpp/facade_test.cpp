@@ -114,6 +114,25 @@ static int cpu_checks() {
     CHECK(obj->indices.size() == 6 && obj->vertices.size() == 4);
     CHECK(std::fabs(obj->vertices[0].normal.z - 1.0f) < 1e-6f && obj->vertices[2].texCoord.y == 0.0f);
     CHECK(throws([&] { loadObj("/tmp/does_not_exist.obj"); }));
+
+    // KTX environment map (KTX 1: identifier, 13 u32 header fields, no
+    // key/value data, image size, 6 faces of 1x1 RGBA8)
+    {
+        const uint8_t id[12] = {0xAB, 'K', 'T', 'X', ' ', '1', '1', 0xBB, '\r', '\n', 0x1A, '\n'};
+        const uint32_t hdr[13] = {0x04030201u, 0x1401u, 1u, 0x1908u, 0x8C43u, 0x1908u, 1u, 1u, 0u, 0u, 6u, 1u, 0u};
+        const uint32_t image = 4;
+        std::ofstream f("/tmp/kf_facade_test.ktx", std::ios::binary);
+        f.write((const char*)id, 12);
+        f.write((const char*)hdr, sizeof(hdr));
+        f.write((const char*)&image, 4);
+        for (int face = 0; face < 6; ++face) {
+            const uint8_t px[4] = {(uint8_t)(40 * face), 0, 255, 255};
+            f.write((const char*)px, 4);
+        }
+    }
+    s->setEnvironmentMap("/tmp/kf_facade_test.ktx");
+    CHECK(throws([&] { s->setEnvironmentMap("/tmp/kf_facade_test.png"); }));  // scene.cpp:306-308
+    CHECK(throws([&] { s->setEnvironmentMap("/tmp/does_not_exist.ktx"); }));
     std::cout << "OK" << std::endl;
     return 0;
 }

@@ -158,3 +158,58 @@ def test_environment_offset_validation():
     s.env_offsets = np.array([0, 1, 2], np.uint32)
     with pytest.raises(KuafuError, match="from 0 to num_instances"):
         GpuScene(s)
+
+
+def _ktx1(size, faces, internal=0x8C43, big_endian=False, nfaces=6, kvd=b"KTXorientation\x00S=r,T=d\x00\x00"):
+    # KTX 1.1 container (Khronos): identifier, 13 u32 header fields, key/value
+    # data, then per mip level a u32 image size and the faces (4-byte aligned)
+    import struct
+    e = ">" if big_endian else "<"
+    kv = struct.pack(e + "I", len(kvd)) + kvd
+    kv += b"\x00" * ((-len(kv)) % 4)
+    hdr = struct.pack(e + "13I", 0x04030201, 0x1401, 1, 0x1908, internal, 0x1908, size, size, 0, 0, nfaces, 1,
+                      len(kv))
+    body = struct.pack(e + "I", size * size * 4) + b"".join(f.tobytes() for f in faces)
+    return b"\xabKTX 11\xbb\r\n\x1a\n" + hdr + kv + body
+
+
+def _ktx2(size, faces, vk_format=43, scheme=0):
+    # KTX 2.0 container: identifier, 9 u32 header fields, the index
+    # (DFD/KVD offsets and lengths, SGD u64s), one level index entry, data
+    import struct
+    data = b"".join(f.tobytes() for f in faces)
+    off = 12 + 36 + 32 + 24
+    hdr = struct.pack("<9I", vk_format, 1, size, size, 0, 0, 6, 1, scheme)
+    index = struct.pack("<4I2Q", 0, 0, 0, 0, 0, 0)
+    level = struct.pack("<3Q", off, len(data), len(data))
+    return b"\xabKTX 20\xbb\r\n\x1a\n" + hdr + index + level + data
+
+
+def test_ktx_cubemap_reader(tmp_path):
+    import ctypes as C
+    from kuafu_amd import lib
+    rng = np.random.default_rng(4)
+    size = 5
+    faces = [rng.integers(0, 256, (size, size, 4), dtype=np.uint8) for _ in range(6)]
+    cases = {"a.ktx": _ktx1(size, faces), "b.ktx": _ktx1(size, faces, internal=0x8058, big_endian=True),
+             "c.ktx2": _ktx2(size, faces), "d.ktx2": _ktx2(size, faces, vk_format=37)}
+    for name, blob in cases.items():
+        p = tmp_path / name
+        p.write_bytes(blob)
+        n = C.c_uint32()
+        assert lib().kf_ktx_cubemap_read(str(p).encode(), C.byref(n), None, 0) == 0, name
+        assert n.value == size
+        out = np.zeros((6, size, size, 4), np.uint8)
+        assert lib().kf_ktx_cubemap_read(str(p).encode(), C.byref(n), out.ctypes.data, out.nbytes) == 0
+        assert (out == np.stack(faces)).all(), name
+    bad = {"five.ktx": (_ktx1(size, faces[:5], nfaces=5), "6 square faces"),
+           "bc1.ktx": (_ktx1(size, faces, internal=0x83F1), "uncompressed"),
+           "zstd.ktx2": (_ktx2(size, faces, scheme=2), "supercompressed"),
+           "cut.ktx": (_ktx1(size, faces)[:-7], "truncated"),
+           "png.ktx": (b"\x89PNG\r\n\x1a\n" + b"\x00" * 64, "not a KTX")}
+    for name, (blob, msg) in bad.items():
+        p = tmp_path / name
+        p.write_bytes(blob)
+        n = C.c_uint32()
+        assert lib().kf_ktx_cubemap_read(str(p).encode(), C.byref(n), None, 0) != 0, name
+        assert msg in lib().kf_last_error().decode(), name
