@@ -906,11 +906,12 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             S, F, d_jump, r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->seed, r->queue_a, r->counters);
         uint32_t* qa = r->queue_a;
         uint32_t* qb = r->queue_b;
-        // Per bounce b:  A: shade(b) -> [B: shadow(b)] , A: extend(b+1)
-        // -> (wait B) connect(b).  k_shade lists the paths that continue
-        // unless Russian roulette ends them, so the closest-hit rays of
-        // bounce b+1 are traced while the shadow rays of bounce b are,
-        // on the second stream; k_connect then waits for both.
+        // Per bounce b:  A: shade(b) -> B: [shadow(b)], connect(b) ; A:
+        // extend(b+1) -> (wait B) shade(b+1).  k_shade lists the paths that
+        // continue unless Russian roulette ends them, so the closest-hit
+        // rays of bounce b+1 are traced on the main stream while the second
+        // stream traces the shadow rays of bounce b and connects the paths
+        // (k_connect touches neither the rays nor the hits k_extend uses).
         auto launch_extend = [&](uint32_t b, const uint32_t* queue) {
             if (timed) KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 0], st));
             if (o.count_traversal)
@@ -936,25 +937,25 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
                                                     r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis,
                                                     r->sh_o, r->sh_d, r->sh_root, r->d_albedo, r->d_normal,
                                                     r->queue_e, r->counters);
+            KF_CHECK(hipEventRecord(r->sync_ev[2 * b], st));
+            KF_CHECK(hipStreamWaitEvent(r->st2, r->sync_ev[2 * b], 0));
             if (!inline_nee) {
-                KF_CHECK(hipEventRecord(r->sync_ev[2 * b], st));
-                KF_CHECK(hipStreamWaitEvent(r->st2, r->sync_ev[2 * b], 0));
                 if (timed) KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], r->st2));
                 k_shadow<<<grid_sh, TB, 0, r->st2>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->sh_root, r->vis,
                                                      r->counters, refill, grab);
                 if (timed) KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], r->st2));
-                KF_CHECK(hipEventRecord(r->sync_ev[2 * b + 1], r->st2));
                 ++n_sh;
             }
-            if (b + 1 < nb) launch_extend(b + 1, r->queue_e);
-            if (!inline_nee) KF_CHECK(hipStreamWaitEvent(st, r->sync_ev[2 * b + 1], 0));
             if (inline_nee)
-                k_connect<true><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
-                                                     r->nee_a, r->nee_b, r->vis, d_out, r->counters);
+                k_connect<true><<<grid, TB, 0, r->st2>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
+                                                         r->nee_a, r->nee_b, r->vis, d_out, r->counters);
             else
-                k_connect<false><<<grid, TB, 0, st>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
-                                                      r->nee_a, r->nee_b, r->vis, d_out, r->counters);
-            k_next_count<<<1, 1, 0, st>>>((int)b, r->counters);
+                k_connect<false><<<grid, TB, 0, r->st2>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
+                                                          r->nee_a, r->nee_b, r->vis, d_out, r->counters);
+            k_next_count<<<1, 1, 0, r->st2>>>((int)b, r->counters);
+            KF_CHECK(hipEventRecord(r->sync_ev[2 * b + 1], r->st2));
+            if (b + 1 < nb) launch_extend(b + 1, r->queue_e);
+            KF_CHECK(hipStreamWaitEvent(st, r->sync_ev[2 * b + 1], 0));
             std::swap(qa, qb);
         }
         const bool last = s0 + F.nsamp >= pc->spp;

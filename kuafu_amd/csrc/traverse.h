@@ -535,13 +535,13 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
     while (true) {
         const unsigned long long idle = __ballot(!live);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        bool fresh = false;
-        TRay r;  // the ray this lane fetches in this iteration, if any
-        r.o = r.d = mk(0.0f, 0.0f, 0.0f);
-        r.tmin = r.tmax = 0.0f;
-        r.root = kEmptyItem;
-        r.alpha_seed = r.payload = 0u;
         if ((res_n > 0u || moved < 8u) && (nidle >= refill || nidle == 64u)) {
+            bool fresh = false;
+            TRay r;  // the ray this lane fetches, if any
+            r.o = r.d = mk(0.0f, 0.0f, 0.0f);
+            r.tmin = r.tmax = 0.0f;
+            r.root = kEmptyItem;
+            r.alpha_seed = r.payload = 0u;
             const uint32_t rank = (uint32_t)__popcll(idle & below);
             const bool wants = !live;
             uint32_t got = 0;
@@ -573,21 +573,22 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
                 res_q += take;
                 res_n -= take;
             }
+            if (__ballot(fresh) != 0ull) {  // wave-uniform
+                T.start(fresh, r, r.root, use_alpha);
+                payload = fresh ? r.payload : payload;
+                // An empty t range (tmax < tmin, or NaN) or an environment
+                // without instances has nothing to trace: finish it at once
+                // as a miss.
+                const bool empty = fresh && (!(r.tmin <= r.tmax) || r.root == kEmptyItem);
+                if (empty) store(r.payload, T.best, false);
+                live = live || (fresh && !empty);
+            }
         }
-        if (__ballot(fresh) != 0ull) {  // wave-uniform: most iterations fetch nothing
-            T.start(fresh, r, r.root, use_alpha);
-            payload = fresh ? r.payload : payload;
-            // An empty t range (tmax < tmin, or NaN) or an environment
-            // without instances has nothing to trace: finish it at once as a
-            // miss.
-            const bool empty = fresh && (!(r.tmin <= r.tmax) || r.root == kEmptyItem);
-            if (empty) store(r.payload, T.best, false);
-            live = live || (fresh && !empty);
-        }
-        if (__ballot(live) == 0ull) {
-            if (moved >= 8u && res_n == 0u) break;
-            continue;
-        }
+        if (__ballot(live) == 0ull && moved >= 8u && res_n == 0u) break;
+        // The step runs even when no lane holds a query (idle lanes read the
+        // zero record; this happens only while a wave runs out of work):
+        // a path that skipped it would make every step pay for merging the
+        // two paths' registers, and the refill state lives only in its branch.
         const bool alive = T.step(TS, stk, overflow, live);
         if (live && !alive) {
             live = false;
