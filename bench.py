@@ -159,7 +159,7 @@ def batched_environments(frames=8):
            "frame_ms_p99": round(float(np.percentile(lat, 99)), 2),
            "render_ms_p50": round(float(np.percentile(rend, 50)), 2),
            "denoise_ms_p50": round(float(np.percentile(dms, 50)), 2), "frames": frames,
-           "schedule": {k: tune[k] for k in ("samples_per_batch", "refill", "grab")},
+           "schedule": {k: tune[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce")},
            "autotune_ms": [round(tune["default_ms"], 1), round(tune["frame_ms"], 1)]}
     gs.close()
     rd.close()
@@ -198,7 +198,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-relmse", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--schedule", default="", help="samples_per_batch,refill,grab instead of autotuning")
+    ap.add_argument("--schedule", default="", help="samples_per_batch,refill,grab[,tail_bounce] instead of autotuning")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     args = ap.parse_args()
@@ -223,9 +223,9 @@ def main():
     # timing frames of this scene, before anything is measured
     tune = None
     if args.schedule:
-        spb_, refill_, grab_ = (int(v) for v in args.schedule.split(","))
-        rd.set_schedule(spb_, refill_, grab_)
-        tune = {"samples_per_batch": spb_, "refill": refill_, "grab": grab_}
+        v = [int(x) for x in args.schedule.split(",")] + [0]
+        rd.set_schedule(v[0], v[1], v[2], v[3])
+        tune = {"samples_per_batch": v[0], "refill": v[1], "grab": v[2], "tail_bounce": v[3]}
     elif not args.no_autotune and not args.samples_per_batch:
         print("bench: autotune", file=sys.stderr, flush=True)
         tune = rd.autotune(gs, w.cam, pc)
@@ -243,7 +243,8 @@ def main():
     for k in range(args.steps):
         st = rd.render(gs, w.cam, pc, seed=100 + k, samples_per_batch=args.samples_per_batch)
         rays += st.rays()
-        ext_rays += st.primary_rays + st.extension_rays + st.speculative_rays
+        # rays k_extend traced (the tail kernel's are not in extend_ms)
+        ext_rays += st.primary_rays + st.extension_rays + st.speculative_rays - st.tail_rays
         ext_ms += st.extend_ms
         trace_ms += st.trace_ms
         launches += st.extend_launches
@@ -267,6 +268,7 @@ def main():
             "kernel": "k_extend (closest-hit traversal)",
             "bytes_per_ray": round(bytes_per_ray, 1), "node_visits_per_ray": round(nodes_per_ray, 2),
             "instance_visits_per_ray": round(insts_per_ray, 2), "tri_tests_per_ray": round(tris_per_ray, 2),
+            "tlas_node_visits_per_ray": round(cst.tlas_node_visits / max(closest, 1), 2),
             "avg_launch_ms": round(ext_ms / launches, 4)}
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -275,7 +277,8 @@ def main():
         "config": {"workload": w.name, "width": w.width, "height": w.height, "spp": pc.spp,
                    "path_depth": pc.path_depth, "triangles": int(info.triangles), "instances": int(info.instances),
                    "rays_per_frame": rays // args.steps,
-                   "schedule": None if tune is None else {k: tune[k] for k in ("samples_per_batch", "refill", "grab")},
+                   "schedule": None if tune is None else {k: tune[k] for k in ("samples_per_batch", "refill", "grab",
+                                                                                 "tail_bounce")},
                    "kernel_mrays_s": round(rays / (trace_ms * 1e-3) / 1e6, 2)},
         "roofline": roof,
     }

@@ -185,6 +185,8 @@ typedef struct kf_render_stats {
     uint64_t instance_visits; /* instance records (64 B) */
     uint64_t speculative_rays; /* next-bounce rays traced for paths Russian roulette
                                   then ended (not in extension_rays) */
+    uint64_t tlas_node_visits; /* of node_visits, the TLAS ones */
+    uint64_t tail_rays;        /* closest-hit rays traced by the tail kernel (in primary/extension_rays) */
 } kf_render_stats;
 
 /* Render options that the reference takes from Config / push constants. */
@@ -341,8 +343,10 @@ int kf_render_envs(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_came
  * whose schedule is the driver's): times frames of this scene and target
  * over the batch size (samples per batch, i.e. the wavefront queue length),
  * the traversal's refill threshold and its work reservation per wave (the
- * tile a persistent wave takes from its XCD's queue): batch size, then
- * reservation, refill threshold and reservation again, each candidate timed
+ * tile a persistent wave takes from its XCD's queue) and the bounce from
+ * which one thread carries each remaining path to its end instead of the
+ * wavefront: batch size, then reservation, refill threshold, reservation
+ * again and that bounce, each candidate timed
  * in a forward and a backward sweep; keeps the fastest for later frames
  * rendered with samples_per_batch = 0.  Images do not depend on the
  * schedule.  Renders into r's targets (frame_count 0). */
@@ -351,13 +355,16 @@ typedef struct kf_tune_result {
     float frame_ms;    /* kernel time of a frame with the picked schedule */
     float default_ms;  /* ... with the built-in default */
     uint32_t trials;   /* frames timed */
+    uint32_t tail_bounce; /* bounce from which one thread carries each path to its end (0: never) */
 } kf_tune_result;
 int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_camera_ubo* cams,
                          const kf_push_constants* pc, kf_tune_result* out);
 /* Set the schedule directly (e.g. one kf_renderer_autotune picked earlier);
- * 0 keeps the built-in default of that parameter.  refill is clamped to
- * [1, 64], grab to [16, 4096]. */
-int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_batch, uint32_t refill, uint32_t grab);
+ * 0 keeps the built-in default of that parameter (for tail_bounce: the
+ * wavefront runs every bounce).  refill is clamped to [1, 64], grab to
+ * [16, 4096]. */
+int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_batch, uint32_t refill, uint32_t grab,
+                             uint32_t tail_bounce);
 
 /* Read a render target: 0 = rgba, 1 = albedo, 2 = normal; W*H*4 floats. */
 int kf_renderer_read(kf_renderer* r, int target, float* out);

@@ -83,7 +83,8 @@ class RenderStats(C.Structure):
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("trace_ms", C.c_float),
                 ("extend_ms", C.c_float), ("shadow_ms", C.c_float), ("other_ms", C.c_float),
                 ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
-                ("instance_visits", C.c_uint64), ("speculative_rays", C.c_uint64)]
+                ("instance_visits", C.c_uint64), ("speculative_rays", C.c_uint64),
+                ("tlas_node_visits", C.c_uint64), ("tail_rays", C.c_uint64)]
 
     def rays(self):
         return int(self.primary_rays + self.extension_rays + self.shadow_rays)
@@ -91,7 +92,8 @@ class RenderStats(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("samples_per_batch", C.c_uint32), ("refill", C.c_uint32), ("grab", C.c_uint32),
-                ("frame_ms", C.c_float), ("default_ms", C.c_float), ("trials", C.c_uint32)]
+                ("frame_ms", C.c_float), ("default_ms", C.c_float), ("trials", C.c_uint32),
+                ("tail_bounce", C.c_uint32)]
 
 
 class RenderOpts(C.Structure):
@@ -149,7 +151,7 @@ EXPORTS = [
     ("kf_render_envs", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]),
     ("kf_renderer_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
-    ("kf_renderer_set_schedule", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("kf_renderer_set_schedule", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("kf_renderer_read", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_write", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_read_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),

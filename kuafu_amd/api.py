@@ -404,11 +404,11 @@ class Renderer:
         arr = (abi.CameraUBO * len(cams))(*cams)
         t = abi.TuneResult()
         check(lib().kf_renderer_autotune(self.h, scene.h, len(cams), arr, C.byref(pc), C.byref(t)))
-        return dict(samples_per_batch=t.samples_per_batch, refill=t.refill, grab=t.grab, frame_ms=t.frame_ms,
-                    default_ms=t.default_ms, trials=t.trials)
+        return dict(samples_per_batch=t.samples_per_batch, refill=t.refill, grab=t.grab, tail_bounce=t.tail_bounce,
+                    frame_ms=t.frame_ms, default_ms=t.default_ms, trials=t.trials)
 
-    def set_schedule(self, samples_per_batch=0, refill=0, grab=0):
-        check(lib().kf_renderer_set_schedule(self.h, samples_per_batch, refill, grab))
+    def set_schedule(self, samples_per_batch=0, refill=0, grab=0, tail_bounce=0):
+        check(lib().kf_renderer_set_schedule(self.h, samples_per_batch, refill, grab, tail_bounce))
 
     def read(self, target=0):
         out = np.zeros((self.h_, self.w, 4), np.float32)

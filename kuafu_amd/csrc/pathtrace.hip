@@ -37,14 +37,15 @@ static_assert(TB == (int)kRingThreads, "traversal kernels index their LDS stack 
 // of k_shadow, [19] rays k_extend traces at bounce b.  Global, at the end:
 // primary / extension / shadow rays, BVH4 node records / triangle tests
 // (closest-hit traversal), stack overflow, instance records, speculative
-// rays.
+// rays, TLAS node records (part of the node records), closest-hit rays
+// traced by k_tail (part of the extension rays).
 constexpr int kMaxBounces = 64;
 constexpr int kCntPerBounce = 20;
 constexpr int kCntExtendChunks = 3;
 constexpr int kCntShadowChunks = 11;
 constexpr int kCntExtQueue = 19;
 constexpr int kCntGlobal = kMaxBounces * kCntPerBounce;
-constexpr int kCntSize = kCntGlobal + 8;
+constexpr int kCntSize = kCntGlobal + 10;
 
 // Per-frame values the kernels read from device memory, so that a captured
 // frame graph replays with new seeds and running-mean weights.
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
     const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + kCntExtQueue], (unsigned long long)cap);
     const uint32_t S0 = S.any_alpha ? fv->S0 : 0u;
     uint32_t ovf = 0;
-    TCount tot{0, 0, 0};
+    TCount tot{0, 0, 0, 0};
     auto load = [&](uint32_t q, TRay& r) __attribute__((always_inline)) {
         const uint32_t p = queue[q];
         const float4 o = ray_o[p], d = ray_d[p];
@@ -223,12 +224,263 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
         wave_add(&cnt[kCntGlobal + 3], tot.nodes);
         wave_add(&cnt[kCntGlobal + 4], tot.tris);
         wave_add(&cnt[kCntGlobal + 6], tot.insts);
+        wave_add(&cnt[kCntGlobal + 8], tot.tlas);
     }
     if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
 
 // Closest-hit / miss for one path; writes next ray, emission, BSDF weight and
 // NEE slots (deferred) or the NEE sum (inline).
+// Closest hit / miss of path p at this bounce (rchit:322-469, rmiss:12-31):
+// emission, BSDF sample and next ray, NEE (shadow-ray records, or the sum
+// with the shadow rays traced here when NEE_INLINE), AOVs.  Returns whether
+// the path is traced at the next bounce unless Russian roulette ends it.
+template <bool NEE_INLINE>
+KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __restrict__ LT, int bounce, uint32_t p,
+                       float4 t4, bool aov, float4* __restrict__ ray_o, float4* __restrict__ ray_d,
+                       const float4* __restrict__ hit, const uint32_t* __restrict__ hit_prim, float4* __restrict__ rw,
+                       float4* __restrict__ em, uint32_t* __restrict__ seed, float4* __restrict__ nee_a,
+                       float4* __restrict__ nee_b, uint8_t* __restrict__ vis, float4* __restrict__ sh_o,
+                       float4* __restrict__ sh_d, uint32_t* __restrict__ sh_root, float4* __restrict__ albedo_out,
+                       float4* __restrict__ normal_out, unsigned long long* __restrict__ cnt, uint32_t& n_shadow,
+                       uint32_t& ovf) {
+    const uint32_t K = F.slots;
+    const uint32_t depth = __float_as_uint(t4.w);
+    bool ext = false;
+    float4 h = hit[p];
+    float4 ro = ray_o[p], rd = ray_d[p];
+    v3 rdir = xyz(rd);
+    uint32_t rs = seed[p];
+    uint32_t inst = __float_as_uint(h.w);
+    for (uint32_t j = 0; j < K; ++j) vis[(size_t)p * K + j] = 0;
+    if (inst == ~0u) {
+        // miss (rmiss:12-31)
+        v3 e = miss_emission(S, F.pc, rdir);
+        em[p] = f4(e, __uint_as_float(1u));
+        if (aov) {
+            albedo_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+            normal_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+        }
+        if (NEE_INLINE) nee_a[(size_t)p * K] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        uint32_t hp = hit_prim[p];
+        uint32_t prim = hp & 0x7fffffffu;
+        bool inside = (hp >> 31) != 0;
+        Inst I = S.inst[inst];
+        uint32_t g = (uint32_t)I.info.y;
+        size_t tri = (size_t)S.toff[g] + prim;
+        uint32_t vb = S.voff[g];
+        uint32_t i0 = S.idx[3 * tri + 0], i1 = S.idx[3 * tri + 1], i2 = S.idx[3 * tri + 2];
+        const float4* V0 = S.verts + 3 * (size_t)(vb + i0);
+        const float4* V1 = S.verts + 3 * (size_t)(vb + i1);
+        const float4* V2 = S.verts + 3 * (size_t)(vb + i2);
+        float4 a0 = V0[0], a1 = V0[1], a2v = V0[2];
+        float4 b0 = V1[0], b1 = V1[1], b2v = V1[2];
+        float4 c0 = V2[0], c1 = V2[1], c2v = V2[2];
+        float bx = 1.0f - h.y - h.z, by = h.y, bz = h.z;
+        v3 n0 = mk(a0.w, a1.x, a1.y), n1 = mk(b0.w, b1.x, b1.y), n2 = mk(c0.w, c1.x, c1.y);
+        v3 ln = n0 * bx + n1 * by + n2 * bz;
+        v3 wn = mk(ln.x * I.w2o[0].x + ln.y * I.w2o[1].x + ln.z * I.w2o[2].x,
+                   ln.x * I.w2o[0].y + ln.y * I.w2o[1].y + ln.z * I.w2o[2].y,
+                   ln.x * I.w2o[0].z + ln.y * I.w2o[1].z + ln.z * I.w2o[2].z);
+        v3 N = normalize(wn);
+        v3 worldPos = xyz(ro) + rdir * h.x;
+        float uvx = a2v.y * bx + b2v.y * by + c2v.y * bz;
+        float uvy = a2v.z * bx + b2v.z * by + c2v.z * bz;
+        const kf_material& m = S.mats[S.matidx[tri]];
+        v3 emission = mk(m.emission[0], m.emission[1], m.emission[2]) * m.emission[3];
+        if (!all_zero(emission)) {
+            em[p] = f4(emission, __uint_as_float(1u));
+            if (aov) {
+                albedo_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+                normal_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
+            }
+            if (NEE_INLINE) nee_a[(size_t)p * K] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            auto tex = [&](int id) -> float4 {
+                return tex_sample(S, S.tex_data, S.tex_info[id], uvx, uvy, true);
+            };
+            auto valid_tex = [&](int id) { return id >= 0 && (uint32_t)id < S.n_tex; };
+            v3 base = mk(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+            if (valid_tex(m.diffuse_tex)) base = xyz(tex(m.diffuse_tex));
+            base = base / kPi;
+            float metallic = m.metallic;
+            if (valid_tex(m.metallic_tex)) metallic = tex(m.metallic_tex).x;
+            float roughness = m.roughness;
+            if (valid_tex(m.roughness_tex)) roughness = tex(m.roughness_tex).x;
+            float a2 = roughness * roughness;
+            float transmission = m.transmission;
+            if (valid_tex(m.transmission_tex)) transmission = tex(m.transmission_tex).x;
+            float f = gmax(m.ior, 1e-5f);
+            float diffuse_weight = (1.0f - gclamp(metallic, 0.0f, 1.0f)) * (1.0f - gclamp(transmission, 0.0f, 1.0f));
+            float final_transmission = gclamp(transmission, 0.0f, 1.0f) * (1.0f - gclamp(metallic, 0.0f, 1.0f));
+            float specular_weight = 1.0f - final_transmission;
+            v3 weight = splat(0.0f);
+            v3 V = normalize(-rdir);
+            N = inside ? -N : N;
+            v3 L = splat(0.0f);
+            float NdotV = dot(N, V);
+            v3 diffuseColor = diffuse_weight * base;
+            v3 specularColor = specular_weight * (base * metallic + splat(m.specular * 0.08f) * (1.0f - metallic));
+            v3 transmissionColor = transmission * base;
+            float dl = length(diffuseColor), sl = length(specularColor);
+            float pD = dl / (dl + sl);
+            if (dl == 0.0f && sl == 0.0f) {
+                if (all_zero(base)) {
+                    if (diffuse_weight == 1.0f) pD = 1.0f;
+                    else if (diffuse_weight == 0.0f) pD = 0.0f;
+                    else pD = 0.5f;
+                } else {
+                    pD = 0.0f;
+                }
+            } else {
+                pD *= specular_weight;
+            }
+            bool chooseDiffuse = rnd(rs) < pD;
+            if (chooseDiffuse) {
+                L = cosine_hemisphere(rs, N);
+                float NdotL = gclamp(dot(N, L), 0.0f, 1.0f);
+                weight = kPi * diffuseColor * NdotL / pD;
+            } else if (final_transmission == 0.0f) {
+                v3 H = sample_ggx(rs, a2, N);
+                float HdotV = dot(H, V);
+                L = 2.0f * HdotV * H - V;
+                float NoV = gmax(dot(N, V), 1e-7f);
+                float NoL = dot(N, L);
+                float NoH = gmax(dot(N, H), 1e-7f);
+                float VoH = gmax(dot(V, H), 1e-7f);
+                if (NoL >= 0.0f) {
+                    float G = geom_shadow(NoV, NoL, a2);
+                    v3 F = schlick_fresnel(specularColor, VoH);
+                    weight = kPi * F * G * VoH / (NoH * NoV * (1.0f - pD));
+                } else {
+                    weight = splat(0.0f);
+                }
+            } else {
+                float ior = inside ? 1.0f / f : f;
+                float dd = inside ? NdotV * ior : NdotV;
+                v3 refr = refract(-V, N, 1.0f / f);
+                float reflectProb = !all_zero(refr) ? schlick(dd, f) : 1.0f;
+                if (rnd(rs) >= reflectProb) {
+                    L = refr;
+                    weight = kPi * transmissionColor / (1.0f - pD);
+                } else {
+                    L = reflect(-V, N);
+                    weight = kPi * transmissionColor / (1.0f - pD);
+                }
+            }
+            ShadeCtx sh;
+            sh.N = N; sh.V = V; sh.worldPos = worldPos; sh.f = f; sh.a2 = a2;
+            sh.diffuseColor = diffuseColor; sh.specularColor = specularColor;
+            sh.transmissionColor = transmissionColor; sh.inside = inside;
+            // NEE slots in evaluation order (rchit:457-460)
+            v3 sum_d = splat(0.0f), sum_p = splat(0.0f), sum_a = splat(0.0f);
+            const int nl = LT->n_slots;
+            for (int j = 0; j < nl; ++j) {
+                int kind = LT->kind[j], li = LT->index[j];
+                v3 Ld, lem;
+                float maxd;
+                bool ok = true;
+                if (kind == 0) {  // traceDirectionalLight (rchit:204-223)
+                    lem = mk(LT->dl.rgbs[0], LT->dl.rgbs[1], LT->dl.rgbs[2]) * LT->dl.rgbs[3];
+                    Ld = -mk(LT->dl.direction[0], LT->dl.direction[1], LT->dl.direction[2]);
+                    if (NEE_INLINE && LT->dl.direction[3] != 0.0f) {
+                        float a = rnd(rs), b = rnd(rs), c = rnd(rs);
+                        Ld = normalize(Ld + LT->dl.direction[3] * mk(a, b, c));
+                    }
+                    maxd = 1e6f;
+                } else if (kind == 1) {  // tracePointLights (rchit:227-258)
+                    v3 rgb = mk(LT->pl.rgbs[li][0], LT->pl.rgbs[li][1], LT->pl.rgbs[li][2]);
+                    v3 lpos = mk(LT->pl.posr[li][0], LT->pl.posr[li][1], LT->pl.posr[li][2]);
+                    if (NEE_INLINE && LT->pl.posr[li][3] != 0.0f) {
+                        v3 pert = uniform_sphere(rs);
+                        lpos = lpos + LT->pl.posr[li][3] * normalize(pert);
+                    }
+                    v3 Lv = lpos - worldPos;
+                    float dd2 = length(Lv);
+                    Ld = normalize(Lv);
+                    lem = rgb * LT->pl.rgbs[li][3] / dd2 / dd2;
+                    maxd = dd2;
+                } else {  // traceActiveLights (rchit:262-316)
+                    v3 rgb = mk(LT->al.rgbs[li][0], LT->al.rgbs[li][1], LT->al.rgbs[li][2]);
+                    float softness = LT->al.sftp[li][0];
+                    v3 lpos = mk(LT->al.position[li][0], LT->al.position[li][1], LT->al.position[li][2]);
+                    if (NEE_INLINE && softness != 0.0f) {
+                        v3 pert = uniform_sphere(rs);
+                        lpos = lpos + softness * normalize(pert);
+                    }
+                    v3 Lv = lpos - worldPos;
+                    float dd2 = length(Lv);
+                    Ld = normalize(Lv);
+                    float fov = LT->al.sftp[li][1];
+                    v3 ldir = normalize(mk(LT->al.front[li][0], LT->al.front[li][1], LT->al.front[li][2]));
+                    float halfA = gclamp(fov, 0.0f, kPi) / 2.0f;
+                    float cs = dot(ldir, -Ld);
+                    ok = cs > cos_k(halfA);
+                    v3 color = rgb;
+                    int texID = (int)LT->al.sftp[li][2];
+                    if (ok && texID >= 0 && (uint32_t)texID < S.n_tex) {
+                        float v4[4] = {worldPos.x, worldPos.y, worldPos.z, 1.0f};
+                        float tc[4];
+                        mat4_mul(LT->pv[li], v4, tc);
+                        float u = (tc[0] / tc[3]) * 0.5f + 0.5f;
+                        float v = (tc[1] / tc[3]) * 0.5f + 0.5f;
+                        float4 tx = tex_sample(S, S.tex_data, S.tex_info[texID], u, v, true);
+                        color = color * xyz(tx);
+                    }
+                    lem = color * LT->al.rgbs[li][3] / dd2 / dd2;
+                    maxd = dd2;
+                }
+                if (!ok) continue;
+                float NdotL = dot(N, Ld);
+                if (!(NdotL > 0.0f)) continue;
+                if (NEE_INLINE) {
+                    THit hh;
+                    TCount cc{0, 0, 0, 0};
+                    n_shadow++;
+                    bool occ = traverse<true, false>(S, __float_as_uint(rd.w), worldPos, Ld, 0.001f, maxd,
+                                                     false, 0u, hh, cc, ovf);
+                    if (!occ) {
+                        v3 ca, cb;
+                        float th;
+                        int mode;
+                        direct_split(sh, Ld, lem, ca, cb, th, mode);
+                        v3 cj = direct_resolve(ca, cb, th, mode, rs);
+                        if (kind == 0) sum_d = sum_d + cj;
+                        else if (kind == 1) sum_p = sum_p + cj;
+                        else sum_a = sum_a + cj;
+                    }
+                } else {
+                    v3 ca, cb;
+                    float th;
+                    int mode;
+                    direct_split(sh, Ld, lem, ca, cb, th, mode);
+                    size_t slot = (size_t)p * K + j;
+                    nee_a[slot] = f4(ca, th);
+                    nee_b[slot] = f4(cb, __int_as_float(mode | (kind << 4)));
+                    uint32_t at = (uint32_t)atomicAdd(&cnt[bounce * kCntPerBounce + 1], 1ull);
+                    sh_o[at] = f4(worldPos, __uint_as_float((uint32_t)slot));
+                    sh_d[at] = f4(Ld, maxd);
+                    if (S.n_env > 1) sh_root[at] = __float_as_uint(rd.w);
+                    n_shadow++;
+                }
+            }
+            if (NEE_INLINE) nee_a[(size_t)p * K] = f4((sum_d + sum_p) + sum_a, 0.0f);
+            ray_o[p] = f4(worldPos, 0.0f);
+            ray_d[p] = f4(L, rd.w);
+            em[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
+            rw[p] = f4(weight, 0.0f);
+            ext = !all_zero(xyz(t4) * weight) && depth + 1u <= F.pc.path_depth;  // as in k_connect
+            if (aov) {
+                albedo_out[p] = f4(base, 1.0f);
+                normal_out[p] = f4(N, 1.0f);
+            }
+        }
+    }
+    seed[p] = rs;
+    return ext;
+}
+
 template <bool NEE_INLINE>
 __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLights* __restrict__ LT, int bounce,
                                               const uint32_t* __restrict__ queue, float4* __restrict__ ray_o,
@@ -244,7 +496,6 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
     uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t iters = (n + stride - 1) / stride;
-    const uint32_t K = F.slots;
     uint32_t n_shadow = 0, ovf = 0;  // shadow rays of this thread
     for (uint32_t it = 0; it < iters; ++it) {
         uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,239 +508,9 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
         // only Russian roulette, decided in k_connect, can still end it.
         bool ext = false;
         bool aov = active && depth == 0 && F.s0 == 0 && p < F.WH;  // sample 0, first segment
-        if (active) {
-            float4 h = hit[p];
-            float4 ro = ray_o[p], rd = ray_d[p];
-            v3 rdir = xyz(rd);
-            uint32_t rs = seed[p];
-            uint32_t inst = __float_as_uint(h.w);
-            for (uint32_t j = 0; j < K; ++j) vis[(size_t)p * K + j] = 0;
-            if (inst == ~0u) {
-                // miss (rmiss:12-31)
-                v3 e = miss_emission(S, F.pc, rdir);
-                em[p] = f4(e, __uint_as_float(1u));
-                if (aov) {
-                    albedo_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
-                    normal_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
-                }
-                if (NEE_INLINE) nee_a[(size_t)p * K] = make_float4(0.f, 0.f, 0.f, 0.f);
-            } else {
-                uint32_t hp = hit_prim[p];
-                uint32_t prim = hp & 0x7fffffffu;
-                bool inside = (hp >> 31) != 0;
-                Inst I = S.inst[inst];
-                uint32_t g = (uint32_t)I.info.y;
-                size_t tri = (size_t)S.toff[g] + prim;
-                uint32_t vb = S.voff[g];
-                uint32_t i0 = S.idx[3 * tri + 0], i1 = S.idx[3 * tri + 1], i2 = S.idx[3 * tri + 2];
-                const float4* V0 = S.verts + 3 * (size_t)(vb + i0);
-                const float4* V1 = S.verts + 3 * (size_t)(vb + i1);
-                const float4* V2 = S.verts + 3 * (size_t)(vb + i2);
-                float4 a0 = V0[0], a1 = V0[1], a2v = V0[2];
-                float4 b0 = V1[0], b1 = V1[1], b2v = V1[2];
-                float4 c0 = V2[0], c1 = V2[1], c2v = V2[2];
-                float bx = 1.0f - h.y - h.z, by = h.y, bz = h.z;
-                v3 n0 = mk(a0.w, a1.x, a1.y), n1 = mk(b0.w, b1.x, b1.y), n2 = mk(c0.w, c1.x, c1.y);
-                v3 ln = n0 * bx + n1 * by + n2 * bz;
-                v3 wn = mk(ln.x * I.w2o[0].x + ln.y * I.w2o[1].x + ln.z * I.w2o[2].x,
-                           ln.x * I.w2o[0].y + ln.y * I.w2o[1].y + ln.z * I.w2o[2].y,
-                           ln.x * I.w2o[0].z + ln.y * I.w2o[1].z + ln.z * I.w2o[2].z);
-                v3 N = normalize(wn);
-                v3 worldPos = xyz(ro) + rdir * h.x;
-                float uvx = a2v.y * bx + b2v.y * by + c2v.y * bz;
-                float uvy = a2v.z * bx + b2v.z * by + c2v.z * bz;
-                const kf_material& m = S.mats[S.matidx[tri]];
-                v3 emission = mk(m.emission[0], m.emission[1], m.emission[2]) * m.emission[3];
-                if (!all_zero(emission)) {
-                    em[p] = f4(emission, __uint_as_float(1u));
-                    if (aov) {
-                        albedo_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
-                        normal_out[p] = make_float4(0.f, 0.f, 0.f, 1.f);
-                    }
-                    if (NEE_INLINE) nee_a[(size_t)p * K] = make_float4(0.f, 0.f, 0.f, 0.f);
-                } else {
-                    auto tex = [&](int id) -> float4 {
-                        return tex_sample(S, S.tex_data, S.tex_info[id], uvx, uvy, true);
-                    };
-                    auto valid_tex = [&](int id) { return id >= 0 && (uint32_t)id < S.n_tex; };
-                    v3 base = mk(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-                    if (valid_tex(m.diffuse_tex)) base = xyz(tex(m.diffuse_tex));
-                    base = base / kPi;
-                    float metallic = m.metallic;
-                    if (valid_tex(m.metallic_tex)) metallic = tex(m.metallic_tex).x;
-                    float roughness = m.roughness;
-                    if (valid_tex(m.roughness_tex)) roughness = tex(m.roughness_tex).x;
-                    float a2 = roughness * roughness;
-                    float transmission = m.transmission;
-                    if (valid_tex(m.transmission_tex)) transmission = tex(m.transmission_tex).x;
-                    float f = gmax(m.ior, 1e-5f);
-                    float diffuse_weight = (1.0f - gclamp(metallic, 0.0f, 1.0f)) * (1.0f - gclamp(transmission, 0.0f, 1.0f));
-                    float final_transmission = gclamp(transmission, 0.0f, 1.0f) * (1.0f - gclamp(metallic, 0.0f, 1.0f));
-                    float specular_weight = 1.0f - final_transmission;
-                    v3 weight = splat(0.0f);
-                    v3 V = normalize(-rdir);
-                    N = inside ? -N : N;
-                    v3 L = splat(0.0f);
-                    float NdotV = dot(N, V);
-                    v3 diffuseColor = diffuse_weight * base;
-                    v3 specularColor = specular_weight * (base * metallic + splat(m.specular * 0.08f) * (1.0f - metallic));
-                    v3 transmissionColor = transmission * base;
-                    float dl = length(diffuseColor), sl = length(specularColor);
-                    float pD = dl / (dl + sl);
-                    if (dl == 0.0f && sl == 0.0f) {
-                        if (all_zero(base)) {
-                            if (diffuse_weight == 1.0f) pD = 1.0f;
-                            else if (diffuse_weight == 0.0f) pD = 0.0f;
-                            else pD = 0.5f;
-                        } else {
-                            pD = 0.0f;
-                        }
-                    } else {
-                        pD *= specular_weight;
-                    }
-                    bool chooseDiffuse = rnd(rs) < pD;
-                    if (chooseDiffuse) {
-                        L = cosine_hemisphere(rs, N);
-                        float NdotL = gclamp(dot(N, L), 0.0f, 1.0f);
-                        weight = kPi * diffuseColor * NdotL / pD;
-                    } else if (final_transmission == 0.0f) {
-                        v3 H = sample_ggx(rs, a2, N);
-                        float HdotV = dot(H, V);
-                        L = 2.0f * HdotV * H - V;
-                        float NoV = gmax(dot(N, V), 1e-7f);
-                        float NoL = dot(N, L);
-                        float NoH = gmax(dot(N, H), 1e-7f);
-                        float VoH = gmax(dot(V, H), 1e-7f);
-                        if (NoL >= 0.0f) {
-                            float G = geom_shadow(NoV, NoL, a2);
-                            v3 F = schlick_fresnel(specularColor, VoH);
-                            weight = kPi * F * G * VoH / (NoH * NoV * (1.0f - pD));
-                        } else {
-                            weight = splat(0.0f);
-                        }
-                    } else {
-                        float ior = inside ? 1.0f / f : f;
-                        float dd = inside ? NdotV * ior : NdotV;
-                        v3 refr = refract(-V, N, 1.0f / f);
-                        float reflectProb = !all_zero(refr) ? schlick(dd, f) : 1.0f;
-                        if (rnd(rs) >= reflectProb) {
-                            L = refr;
-                            weight = kPi * transmissionColor / (1.0f - pD);
-                        } else {
-                            L = reflect(-V, N);
-                            weight = kPi * transmissionColor / (1.0f - pD);
-                        }
-                    }
-                    ShadeCtx sh;
-                    sh.N = N; sh.V = V; sh.worldPos = worldPos; sh.f = f; sh.a2 = a2;
-                    sh.diffuseColor = diffuseColor; sh.specularColor = specularColor;
-                    sh.transmissionColor = transmissionColor; sh.inside = inside;
-                    // NEE slots in evaluation order (rchit:457-460)
-                    v3 sum_d = splat(0.0f), sum_p = splat(0.0f), sum_a = splat(0.0f);
-                    const int nl = LT->n_slots;
-                    for (int j = 0; j < nl; ++j) {
-                        int kind = LT->kind[j], li = LT->index[j];
-                        v3 Ld, lem;
-                        float maxd;
-                        bool ok = true;
-                        if (kind == 0) {  // traceDirectionalLight (rchit:204-223)
-                            lem = mk(LT->dl.rgbs[0], LT->dl.rgbs[1], LT->dl.rgbs[2]) * LT->dl.rgbs[3];
-                            Ld = -mk(LT->dl.direction[0], LT->dl.direction[1], LT->dl.direction[2]);
-                            if (NEE_INLINE && LT->dl.direction[3] != 0.0f) {
-                                float a = rnd(rs), b = rnd(rs), c = rnd(rs);
-                                Ld = normalize(Ld + LT->dl.direction[3] * mk(a, b, c));
-                            }
-                            maxd = 1e6f;
-                        } else if (kind == 1) {  // tracePointLights (rchit:227-258)
-                            v3 rgb = mk(LT->pl.rgbs[li][0], LT->pl.rgbs[li][1], LT->pl.rgbs[li][2]);
-                            v3 lpos = mk(LT->pl.posr[li][0], LT->pl.posr[li][1], LT->pl.posr[li][2]);
-                            if (NEE_INLINE && LT->pl.posr[li][3] != 0.0f) {
-                                v3 pert = uniform_sphere(rs);
-                                lpos = lpos + LT->pl.posr[li][3] * normalize(pert);
-                            }
-                            v3 Lv = lpos - worldPos;
-                            float dd2 = length(Lv);
-                            Ld = normalize(Lv);
-                            lem = rgb * LT->pl.rgbs[li][3] / dd2 / dd2;
-                            maxd = dd2;
-                        } else {  // traceActiveLights (rchit:262-316)
-                            v3 rgb = mk(LT->al.rgbs[li][0], LT->al.rgbs[li][1], LT->al.rgbs[li][2]);
-                            float softness = LT->al.sftp[li][0];
-                            v3 lpos = mk(LT->al.position[li][0], LT->al.position[li][1], LT->al.position[li][2]);
-                            if (NEE_INLINE && softness != 0.0f) {
-                                v3 pert = uniform_sphere(rs);
-                                lpos = lpos + softness * normalize(pert);
-                            }
-                            v3 Lv = lpos - worldPos;
-                            float dd2 = length(Lv);
-                            Ld = normalize(Lv);
-                            float fov = LT->al.sftp[li][1];
-                            v3 ldir = normalize(mk(LT->al.front[li][0], LT->al.front[li][1], LT->al.front[li][2]));
-                            float halfA = gclamp(fov, 0.0f, kPi) / 2.0f;
-                            float cs = dot(ldir, -Ld);
-                            ok = cs > cos_k(halfA);
-                            v3 color = rgb;
-                            int texID = (int)LT->al.sftp[li][2];
-                            if (ok && texID >= 0 && (uint32_t)texID < S.n_tex) {
-                                float v4[4] = {worldPos.x, worldPos.y, worldPos.z, 1.0f};
-                                float tc[4];
-                                mat4_mul(LT->pv[li], v4, tc);
-                                float u = (tc[0] / tc[3]) * 0.5f + 0.5f;
-                                float v = (tc[1] / tc[3]) * 0.5f + 0.5f;
-                                float4 tx = tex_sample(S, S.tex_data, S.tex_info[texID], u, v, true);
-                                color = color * xyz(tx);
-                            }
-                            lem = color * LT->al.rgbs[li][3] / dd2 / dd2;
-                            maxd = dd2;
-                        }
-                        if (!ok) continue;
-                        float NdotL = dot(N, Ld);
-                        if (!(NdotL > 0.0f)) continue;
-                        if (NEE_INLINE) {
-                            THit hh;
-                            TCount cc{0, 0, 0};
-                            n_shadow++;
-                            bool occ = traverse<true, false>(S, __float_as_uint(rd.w), worldPos, Ld, 0.001f, maxd,
-                                                             false, 0u, hh, cc, ovf);
-                            if (!occ) {
-                                v3 ca, cb;
-                                float th;
-                                int mode;
-                                direct_split(sh, Ld, lem, ca, cb, th, mode);
-                                v3 cj = direct_resolve(ca, cb, th, mode, rs);
-                                if (kind == 0) sum_d = sum_d + cj;
-                                else if (kind == 1) sum_p = sum_p + cj;
-                                else sum_a = sum_a + cj;
-                            }
-                        } else {
-                            v3 ca, cb;
-                            float th;
-                            int mode;
-                            direct_split(sh, Ld, lem, ca, cb, th, mode);
-                            size_t slot = (size_t)p * K + j;
-                            nee_a[slot] = f4(ca, th);
-                            nee_b[slot] = f4(cb, __int_as_float(mode | (kind << 4)));
-                            uint32_t at = (uint32_t)atomicAdd(&cnt[bounce * kCntPerBounce + 1], 1ull);
-                            sh_o[at] = f4(worldPos, __uint_as_float((uint32_t)slot));
-                            sh_d[at] = f4(Ld, maxd);
-                            if (S.n_env > 1) sh_root[at] = __float_as_uint(rd.w);
-                            n_shadow++;
-                        }
-                    }
-                    if (NEE_INLINE) nee_a[(size_t)p * K] = f4((sum_d + sum_p) + sum_a, 0.0f);
-                    ray_o[p] = f4(worldPos, 0.0f);
-                    ray_d[p] = f4(L, rd.w);
-                    em[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
-                    rw[p] = f4(weight, 0.0f);
-                    ext = !all_zero(xyz(t4) * weight) && depth + 1u <= F.pc.path_depth;  // as in k_connect
-                    if (aov) {
-                        albedo_out[p] = f4(base, 1.0f);
-                        normal_out[p] = f4(N, 1.0f);
-                    }
-                }
-            }
-            seed[p] = rs;
-        }
+        if (active)
+            ext = shade_path<NEE_INLINE>(S, F, LT, bounce, p, t4, aov, ray_o, ray_d, hit, hit_prim, rw, em, seed, nee_a,
+                                         nee_b, vis, sh_o, sh_d, sh_root, albedo_out, normal_out, cnt, n_shadow, ovf);
         const uint32_t at = block_append(&cnt[(bounce + 1) * kCntPerBounce + kCntExtQueue], ext);
         if (ext) ext_queue[at] = p;
     }
@@ -504,7 +525,7 @@ __global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t
     const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 1], (unsigned long long)cap);
     const uint32_t root0 = env_root_item(S, 0u);
     uint32_t ovf = 0;
-    TCount tot{0, 0, 0};
+    TCount tot{0, 0, 0, 0};
     auto load = [&](uint32_t q, TRay& r) __attribute__((always_inline)) {
         const float4 o = sh_o[q], d = sh_d[q];
         r.o = xyz(o);
@@ -523,6 +544,70 @@ __global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t
     if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
 
+// Path update of path p after this bounce (rgen:109-138): emission, light
+// contributions in light order, throughput, Russian roulette.  Returns
+// whether the path continues; a finished path writes its radiance to out.
+template <bool NEE_INLINE>
+KF_DEV bool connect_path(const Frame& F, int bounce, uint32_t p, float4* __restrict__ thr, float4* __restrict__ col,
+                         const float4* __restrict__ rw, const float4* __restrict__ em, uint32_t* __restrict__ seed,
+                         const float4* __restrict__ nee_a, const float4* __restrict__ nee_b,
+                         const uint8_t* __restrict__ vis, float4* __restrict__ out) {
+    const uint32_t K = F.slots;
+    const uint32_t maxDepth = F.pc.path_depth;
+    bool cont = false;
+    float4 t4 = thr[p];
+    v3 weight = xyz(t4);
+    uint32_t depth = __float_as_uint(t4.w);
+    v3 color = xyz(col[p]);
+    float4 e4 = em[p];
+    bool term = __float_as_uint(e4.w) != 0u;
+    uint32_t rs = seed[p];
+    v3 shadow = splat(0.0f);
+    if (!term) {
+        if (NEE_INLINE) {
+            shadow = xyz(nee_a[(size_t)p * K]);
+        } else {
+            v3 sd = splat(0.0f), sp = splat(0.0f), sa = splat(0.0f);
+            for (uint32_t j = 0; j < K; ++j) {
+                size_t slot = (size_t)p * K + j;
+                if (!vis[slot]) continue;
+                float4 a = nee_a[slot], b = nee_b[slot];
+                int mb = __float_as_int(b.w);
+                v3 cj = direct_resolve(xyz(a), xyz(b), a.w, mb & 15, rs);
+                int kind = mb >> 4;
+                if (kind == 0) sd = sd + cj;
+                else if (kind == 1) sp = sp + cj;
+                else sa = sa + cj;
+            }
+            shadow = (sd + sp) + sa;
+        }
+    }
+    // rgen:109-138
+    color = color + xyz(e4) * weight;
+    weight = weight * xyz(rw[p]);
+    color = color + shadow * weight;
+    uint32_t d2 = term ? maxDepth + 1 : depth;
+    cont = true;
+    if (all_zero(weight)) {
+        cont = false;
+    } else if (F.pc.russian_roulette && d2 >= F.pc.rr_min_bounces) {
+        float pr = gmax(weight.x, gmax(weight.y, weight.z));
+        float rr = rnd(rs);
+        if (rr > pr) cont = false;
+        else weight = weight * (1.0f / pr);
+    }
+    uint32_t nd = d2 + 1;
+    if (nd > maxDepth) cont = false;
+    if (cont) {
+        thr[p] = f4(weight, __uint_as_float(nd));
+        col[p] = f4(color, 0.0f);
+        seed[p] = rs;
+    } else {
+        out[p] = f4(color, 0.0f);
+    }
+    return cont;
+}
+
 template <bool NEE_INLINE>
 __global__ __launch_bounds__(TB) void k_connect(Frame F, int bounce, const uint32_t* __restrict__ queue,
                                                 uint32_t* __restrict__ next, float4* __restrict__ thr,
@@ -534,68 +619,62 @@ __global__ __launch_bounds__(TB) void k_connect(Frame F, int bounce, const uint3
     uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t iters = (n + stride - 1) / stride;
-    const uint32_t K = F.slots;
-    const uint32_t maxDepth = F.pc.path_depth;
     for (uint32_t it = 0; it < iters; ++it) {
         uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool cont = false;
         uint32_t p = 0;
         if (q < n) {
             p = queue[q];
-            float4 t4 = thr[p];
-            v3 weight = xyz(t4);
-            uint32_t depth = __float_as_uint(t4.w);
-            v3 color = xyz(col[p]);
-            float4 e4 = em[p];
-            bool term = __float_as_uint(e4.w) != 0u;
-            uint32_t rs = seed[p];
-            v3 shadow = splat(0.0f);
-            if (!term) {
-                if (NEE_INLINE) {
-                    shadow = xyz(nee_a[(size_t)p * K]);
-                } else {
-                    v3 sd = splat(0.0f), sp = splat(0.0f), sa = splat(0.0f);
-                    for (uint32_t j = 0; j < K; ++j) {
-                        size_t slot = (size_t)p * K + j;
-                        if (!vis[slot]) continue;
-                        float4 a = nee_a[slot], b = nee_b[slot];
-                        int mb = __float_as_int(b.w);
-                        v3 cj = direct_resolve(xyz(a), xyz(b), a.w, mb & 15, rs);
-                        int kind = mb >> 4;
-                        if (kind == 0) sd = sd + cj;
-                        else if (kind == 1) sp = sp + cj;
-                        else sa = sa + cj;
-                    }
-                    shadow = (sd + sp) + sa;
-                }
-            }
-            // rgen:109-138
-            color = color + xyz(e4) * weight;
-            weight = weight * xyz(rw[p]);
-            color = color + shadow * weight;
-            uint32_t d2 = term ? maxDepth + 1 : depth;
-            cont = true;
-            if (all_zero(weight)) {
-                cont = false;
-            } else if (F.pc.russian_roulette && d2 >= F.pc.rr_min_bounces) {
-                float pr = gmax(weight.x, gmax(weight.y, weight.z));
-                float rr = rnd(rs);
-                if (rr > pr) cont = false;
-                else weight = weight * (1.0f / pr);
-            }
-            uint32_t nd = d2 + 1;
-            if (nd > maxDepth) cont = false;
-            if (cont) {
-                thr[p] = f4(weight, __uint_as_float(nd));
-                col[p] = f4(color, 0.0f);
-                seed[p] = rs;
-            } else {
-                out[p] = f4(color, 0.0f);
-            }
+            cont = connect_path<NEE_INLINE>(F, bounce, p, thr, col, rw, em, seed, nee_a, nee_b, vis, out);
         }
         uint32_t at = block_append(&cnt[bounce * kCntPerBounce + 2], cont);
         if (cont) next[at] = p;
     }
+}
+
+// The last bounces of the paths still active at bounce b0, each path carried
+// to its end by one thread: closest hit (as k_extend traces it), shading with
+// the shadow rays traced inline, path update, and again.  Few paths survive
+// Russian roulette, and the wavefront would then wait, bounce after bounce,
+// for the slowest ray of each launch; here a launch lasts as long as the
+// longest remaining path.  The random streams and light order are those of
+// the wavefront kernels, so the radiance is the same.
+__global__ __launch_bounds__(TB) void k_tail(SceneView S, Frame F, const DevLights* __restrict__ LT, int b0,
+                                             const uint32_t* __restrict__ queue, float4* __restrict__ ray_o,
+                                             float4* __restrict__ ray_d, float4* __restrict__ thr,
+                                             float4* __restrict__ col, float4* __restrict__ hit,
+                                             uint32_t* __restrict__ hit_prim, float4* __restrict__ rw,
+                                             float4* __restrict__ em, uint32_t* __restrict__ seed,
+                                             float4* __restrict__ nee_a, float4* __restrict__ nee_b,
+                                             uint8_t* __restrict__ vis, float4* __restrict__ out,
+                                             unsigned long long* __restrict__ cnt) {
+    const uint32_t n = (uint32_t)min(cnt[b0 * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
+    const uint32_t S0 = F.fv->S0;
+    uint32_t n_shadow = 0, n_ext = 0, n_traced = 0, ovf = 0;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const uint32_t p = queue[q];
+        for (int b = b0; b < kMaxBounces; ++b) {
+            // closest hit (rgen:97-107), alpha coin seeded as in k_extend
+            const float4 o = ray_o[p], d = ray_d[p];
+            const uint32_t aseed =
+                S.any_alpha ? tea(tea((p % F.WH) % F.WHe, S0 + 1u + F.s0 + p / F.WH), (uint32_t)b) : 0u;
+            THit h;
+            TCount c{0, 0, 0, 0};
+            traverse<false, false>(S, __float_as_uint(d.w), xyz(o), xyz(d), 0.001f, 10000.0f, true, aseed, h, c,
+                                   ovf);
+            ++n_traced;
+            hit[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.inst));
+            hit_prim[p] = h.prim | (h.back << 31);
+            shade_path<true>(S, F, LT, b, p, thr[p], false, ray_o, ray_d, hit, hit_prim, rw, em, seed, nee_a, nee_b,
+                             vis, nullptr, nullptr, nullptr, nullptr, nullptr, cnt, n_shadow, ovf);
+            if (!connect_path<true>(F, b, p, thr, col, rw, em, seed, nee_a, nee_b, vis, out)) break;
+            ++n_ext;  // traced at the next bounce
+        }
+    }
+    wave_add(&cnt[kCntGlobal + 1], n_ext);
+    wave_add(&cnt[kCntGlobal + 2], n_shadow);
+    wave_add(&cnt[kCntGlobal + 9], n_traced);
+    if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
 
 __global__ void k_next_count(int bounce, unsigned long long* cnt) {
@@ -881,6 +960,8 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
     int n_cu = 256;
     KF_CHECK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     const uint32_t grid = (uint32_t)n_cu * 8;
+    // k_connect runs beside k_extend: a small grid leaves it most of the CUs
+    const uint32_t grid_conn = (uint32_t)n_cu * 2;
     const uint32_t refill = std::min<uint32_t>(64u, std::max<uint32_t>(1u, r->tune_refill ? r->tune_refill
                                                                                          : traversal_refill()));
     const uint32_t grab = std::min<uint32_t>(4096u, std::max<uint32_t>(16u, r->tune_grab ? r->tune_grab : kGrab));
@@ -889,6 +970,9 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
     static const uint32_t grid_ext1 = resident_grid(k_extend<true>, n_cu);
     static const uint32_t grid_sh = resident_grid(k_shadow, n_cu);
     const uint32_t nb = pc->path_depth + 1;
+    // bounces from b_tail on run in k_tail (not in counting frames, whose
+    // node counts come from the wavefront kernels)
+    const uint32_t b_tail = (o.count_traversal || r->tune_tail == 0 || r->tune_tail >= nb) ? nb : r->tune_tail;
 
     // One batch of samples [s0, s0 + nsamp).  timed: record the per-kernel
     // events (not while capturing a graph).  The first batch clears every
@@ -927,6 +1011,12 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         };
         launch_extend(0, qa);
         for (uint32_t b = 0; b < nb; ++b) {
+            if (b == b_tail) {
+                k_tail<<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->col, r->hit,
+                                            r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis, d_out,
+                                            r->counters);
+                break;
+            }
             if (inline_nee)
                 k_shade<true><<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->hit,
                                                    r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis,
@@ -947,14 +1037,14 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
                 ++n_sh;
             }
             if (inline_nee)
-                k_connect<true><<<grid, TB, 0, r->st2>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
-                                                         r->nee_a, r->nee_b, r->vis, d_out, r->counters);
+                k_connect<true><<<grid_conn, TB, 0, r->st2>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em,
+                                                              r->seed, r->nee_a, r->nee_b, r->vis, d_out, r->counters);
             else
-                k_connect<false><<<grid, TB, 0, r->st2>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em, r->seed,
-                                                          r->nee_a, r->nee_b, r->vis, d_out, r->counters);
+                k_connect<false><<<grid_conn, TB, 0, r->st2>>>(F, (int)b, qa, qb, r->thr, r->col, r->rw, r->em,
+                                                               r->seed, r->nee_a, r->nee_b, r->vis, d_out, r->counters);
             k_next_count<<<1, 1, 0, r->st2>>>((int)b, r->counters);
             KF_CHECK(hipEventRecord(r->sync_ev[2 * b + 1], r->st2));
-            if (b + 1 < nb) launch_extend(b + 1, r->queue_e);
+            if (b + 1 < b_tail) launch_extend(b + 1, r->queue_e);
             KF_CHECK(hipStreamWaitEvent(st, r->sync_ev[2 * b + 1], 0));
             std::swap(qa, qb);
         }
@@ -974,8 +1064,8 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             const uint8_t* b = (const uint8_t*)p;
             key.insert(key.end(), b, b + n);
         };
-        const uint32_t sched[8] = {spb, K, nb, refill, grab, grid, (uint32_t)o.count_traversal,
-                                   (uint32_t)o.keep_samples};
+        const uint32_t sched[9] = {spb, K, nb, refill, grab, grid, (uint32_t)o.count_traversal,
+                                   (uint32_t)o.keep_samples, b_tail};
         const void* bufs[] = {r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->em, r->hit, r->hit_prim, r->seed,
                               r->queue_a, r->queue_b, r->queue_e, r->sh_o, r->sh_d, r->sh_root, r->nee_a,
                               r->nee_b, r->vis, r->samples, r->counters, d_lights, d_jump, d_acc, d_out,
@@ -1021,7 +1111,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             float ms = 0.f;
             KF_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
             ms_total += ms;
-            for (uint32_t b = 0; b < nb; ++b) {
+            for (uint32_t b = 0; b < b_tail; ++b) {  // the tail kernel counts as other time
                 float a = 0.f, c = 0.f;
                 KF_CHECK(hipEventElapsedTime(&a, r->ev[2 + 4 * b + 0], r->ev[2 + 4 * b + 1]));
                 if (!inline_nee) KF_CHECK(hipEventElapsedTime(&c, r->ev[2 + 4 * b + 2], r->ev[2 + 4 * b + 3]));
@@ -1030,7 +1120,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             }
         }
     }
-    unsigned long long totals[8];
+    unsigned long long totals[10];
     KF_CHECK(hipMemcpyAsync(totals, r->counters + kCntGlobal, sizeof(totals), hipMemcpyDeviceToHost, st));
     KF_CHECK(hipStreamSynchronize(st));
     if (totals[5]) throw Error("kf_render: traversal stack overflow (BVH deeper than the stack)");
@@ -1042,6 +1132,8 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         stats->tri_tests = totals[4];
         stats->instance_visits = totals[6];
         stats->speculative_rays = totals[7];
+        stats->tlas_node_visits = totals[8];
+        stats->tail_rays = totals[9];
         stats->trace_ms = ms_total;
         stats->extend_ms = ms_ext;
         stats->shadow_ms = ms_sh;
@@ -1087,10 +1179,12 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
         uint32_t trials = 0;
         // kernel time of one frame under a schedule (0 = default); the best
         // of three for short frames, whose times are noisier
+        uint32_t b_tail = 0;
         auto run = [&](uint32_t spb, uint32_t refill, uint32_t grab) {
             r->tune_spb = spb;
             r->tune_refill = refill;
             r->tune_grab = grab;
+            r->tune_tail = b_tail;
             float best = INFINITY;
             for (int k = 0; k < 3; ++k) {
                 kf_render_stats st{};
@@ -1101,6 +1195,7 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
             }
             return best;
         };
+        r->tune_tail = 0;
         run(0, 0, 0);  // warm-up: buffers, code objects, clocks
         run(0, 0, 0);
         const float t_def = run(0, 0, 0);
@@ -1114,8 +1209,10 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
             std::vector<float> t(cand.size(), INFINITY);
             auto one = [&](size_t k) {
                 const uint32_t v = cand[k];
+                if (which == 3) b_tail = v;
                 const float m = which == 0 ? run(v, b_refill, b_grab)
-                                           : (which == 1 ? run(b_spb, v, b_grab) : run(b_spb, b_refill, v));
+                                           : (which == 1 ? run(b_spb, v, b_grab) : run(b_spb, b_refill,
+                                                                                       which == 2 ? v : b_grab));
                 t[k] = std::min(t[k], m);
             };
             for (size_t k = 0; k < cand.size(); ++k) one(k);
@@ -1142,13 +1239,25 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
         const std::vector<uint32_t> grabs = {64u, 128u, 256u, 512u, 1024u};
         std::vector<uint32_t> refills = {8u, 16u, 32u, 48u};
         if (std::find(refills.begin(), refills.end(), b_refill) == refills.end()) refills.push_back(b_refill);
+        // the bounce from which k_tail finishes the paths: never (0), or
+        // from the first bounces after Russian roulette starts
+        std::vector<uint32_t> tails = {0u};
+        for (uint32_t tb = std::max<uint32_t>(1u, p.rr_min_bounces); tb < std::min<uint32_t>(p.path_depth + 1u,
+                                                                                                 p.rr_min_bounces + 4u);
+             ++tb)
+            tails.push_back(tb);
         sweep(spbs, b_spb, 0);
         sweep(grabs, b_grab, 2);
         sweep(refills, b_refill, 1);
         sweep(grabs, b_grab, 2);
+        uint32_t best_tail = 0;
+        sweep(tails, best_tail, 3);
+        b_tail = best_tail;
         r->tune_spb = b_spb;
         r->tune_refill = b_refill;
         r->tune_grab = b_grab;
+        r->tune_tail = b_tail;
+        out->tail_bounce = b_tail;
         out->samples_per_batch = b_spb;
         out->refill = b_refill;
         out->grab = b_grab;
@@ -1162,7 +1271,8 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
     }
 }
 
-extern "C" int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_batch, uint32_t refill, uint32_t grab) {
+extern "C" int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_batch, uint32_t refill, uint32_t grab,
+                                        uint32_t tail_bounce) {
     if (!r) {
         set_error("kf_renderer_set_schedule: null renderer");
         return -1;
@@ -1170,6 +1280,7 @@ extern "C" int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_bat
     r->tune_spb = samples_per_batch;
     r->tune_refill = refill;
     r->tune_grab = grab;
+    r->tune_tail = tail_bounce;
     return 0;
 }
 

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(TB) void k_closest(SceneView S, const kf_ray* __res
                                                 kf_hit* __restrict__ hits, unsigned long long* __restrict__ cnt,
                                                 uint32_t refill) {
     uint32_t ovf = 0;
-    TCount tot{0, 0, 0};
+    TCount tot{0, 0, 0, 0};
     auto load = [&](uint32_t q, TRay& t) __attribute__((always_inline)) { ray_of(S, rays[q], q, t); };
     auto store = [&](uint32_t q, THit b, bool) __attribute__((always_inline)) {
         kf_hit o;
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(TB) void k_any(SceneView S, const kf_ray* __restric
                                             uint8_t* __restrict__ occ, unsigned long long* __restrict__ cnt,
                                             uint32_t refill) {
     uint32_t ovf = 0;
-    TCount tot{0, 0, 0};
+    TCount tot{0, 0, 0, 0};
     auto load = [&](uint32_t q, TRay& t) __attribute__((always_inline)) { ray_of(S, rays[q], q, t); };
     auto store = [&](uint32_t q, THit, bool found) __attribute__((always_inline)) { occ[q] = found ? 1 : 0; };
     trace_persistent<true, false>(S, n, cnt + 4, refill, kGrab, false, load, store, ovf, tot);

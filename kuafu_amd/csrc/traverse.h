@@ -114,6 +114,7 @@ struct THit {
 
 struct TCount {
     uint32_t nodes, tris, insts;  // BVH4 node records, triangle tests, instance records
+    uint32_t tlas;                // of the node records, TLAS ones
 };
 
 KF_DEV v3 xform_point(const Inst& I, v3 v) {
@@ -268,7 +269,7 @@ struct Trav {
         has_top = found = false;
         use_alpha = false;
         alpha_seed = steps = 0u;
-        cnt.nodes = cnt.tris = cnt.insts = 0u;
+        cnt.nodes = cnt.tris = cnt.insts = cnt.tlas = 0u;
     }
 
     // Lanes with f take a new query r starting at item root (the scene is
@@ -292,6 +293,7 @@ struct Trav {
         alpha_seed = f ? r.alpha_seed : alpha_seed;
         steps = f ? 0u : steps;
         cnt.nodes = f ? 0u : cnt.nodes;
+        cnt.tlas = f ? 0u : cnt.tlas;
         cnt.tris = f ? 0u : cnt.tris;
         cnt.insts = f ? 0u : cnt.insts;
     }
@@ -435,6 +437,7 @@ struct Trav {
 
         if (COUNT) {
             cnt.nodes += is_node ? 1u : 0u;
+            cnt.tlas += (is_node && kind == kTlasNode) ? 1u : 0u;
             cnt.tris += is_tri ? 1u : 0u;
             cnt.insts += is_inst ? 1u : 0u;
         }
@@ -493,6 +496,7 @@ KF_DEV bool traverse(const SceneView& S, uint32_t root, v3 o, v3 d, float tmin, 
         }
     best = T.best;
     cnt.nodes += T.cnt.nodes;
+    cnt.tlas += T.cnt.tlas;
     cnt.tris += T.cnt.tris;
     cnt.insts += T.cnt.insts;
     return T.found;
@@ -595,6 +599,7 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
             store(payload, T.best, T.found);
             if (COUNT) {
                 total.nodes += T.cnt.nodes;
+                total.tlas += T.cnt.tlas;
                 total.tris += T.cnt.tris;
                 total.insts += T.cnt.insts;
             }

@@ -217,3 +217,43 @@ def test_batched_denoise_equals_per_environment(gpu):
         den.apply(r1)
         assert same(out[rows], r1.read(0)), "environment %d" % e
     assert not same(out, tg[0])
+
+
+def _tail_workloads():
+    from kuafu_amd.api import camera_ubo, push_constants
+    out = [scenes.config1(48, 40, spp=3), scenes.config2(width=48, height=27, spp=2, path_depth=8, n_inst=24, subdiv=2)]
+    soft = scenes.tiny_scene(5, n_spheres=5, soft=True)  # inline shadow rays in the wavefront too
+    cam = camera_ubo((0.0, 2.5, 5.0), (0, -0.4, -1.0), (0, 1, 0), 48, 40, fx=50.0, fy=50.0)
+    out.append(scenes.Workload("soft", soft, cam, push_constants(spp=3, path_depth=6), 48, 40))
+    alpha = scenes.tiny_scene(2, n_spheres=2)  # any-hit alpha test on path rays
+    sv, si = scenes.uv_sphere(12, 8, 0.6)
+    for a, pos in ((0.5, (-0.6, 0.7, 0.8)), (0.2, (0.7, 0.6, 1.2))):
+        g = alpha.add_geometry(sv, si, scenes.material(diffuse=(0.3, 0.8, 0.4), alpha=a), opaque=False)
+        alpha.add_instance(g, scenes.mat_from_rows(np.eye(3), pos))
+    cam = camera_ubo((0.0, 2.0, 5.0), (0, -0.3, -1.0), (0, 1, 0), 48, 40, fx=50.0, fy=50.0)
+    out.append(scenes.Workload("alpha", alpha, cam, push_constants(spp=4, path_depth=5), 48, 40))
+    return out
+
+
+def test_tail_kernel_equals_wavefront(gpu):
+    """Bounces finished by k_tail (one thread per remaining path, shadow rays
+    inline) give the wavefront's per-sample radiance, targets and ray counts
+    bit for bit, whichever bounce it takes over from."""
+    for w in _tail_workloads():
+        gs = GpuScene(w.data)
+        pc = w.pc
+        pc.frame_count = 0
+        ref = Renderer(w.width, w.height)
+        st0 = ref.render(gs, w.cam, pc, seed=7, keep_samples=True)
+        smp0 = ref.read_samples(pc.spp)
+        tg0 = [ref.read(k) for k in range(3)]
+        for tail in range(1, pc.path_depth + 1):
+            rd = Renderer(w.width, w.height)
+            rd.set_schedule(0, 0, 0, tail)
+            st = rd.render(gs, w.cam, pc, seed=7, keep_samples=True)
+            assert same(rd.read_samples(pc.spp), smp0), "%s tail %d" % (w.name, tail)
+            for k in range(3):
+                assert same(rd.read(k), tg0[k]), "%s tail %d target %d" % (w.name, tail, k)
+            assert st.primary_rays == st0.primary_rays and st.shadow_rays == st0.shadow_rays
+            # the wavefront also traces rays that Russian roulette then drops
+            assert st.extension_rays == st0.extension_rays
