@@ -116,7 +116,7 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     return res
 
 
-def batched_environments(frames=8):
+def batched_environments(frames=8, schedule=None):
     """128 environments of 512x512 (8 spp, depth 6, 5k-400k triangles each):
     autotuned schedule, every frame one replay of a captured HIP graph, then
     one batched denoise of the 128 images.  Frame latency = render + denoise,
@@ -130,7 +130,12 @@ def batched_environments(frames=8):
     rd = Renderer(w.width, w.height)
     pc = w.pc
     pc.frame_count = 0
-    tune = rd.autotune(gs, w.cam, pc)
+    if schedule:
+        rd.set_schedule(*schedule)
+        tune = dict(zip(("samples_per_batch", "refill", "grab", "tail_bounce"), schedule), default_ms=0.0,
+                    frame_ms=0.0)
+    else:
+        tune = rd.autotune(gs, w.cam, pc)
     den = Denoiser()
     rd.render_envs(gs, w.cam, pc, seed=1, use_graph=True)  # capture
     den.apply(rd)
@@ -199,6 +204,8 @@ def main():
     ap.add_argument("--no-relmse", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--schedule", default="", help="samples_per_batch,refill,grab[,tail_bounce] instead of autotuning")
+    ap.add_argument("--envs-only", action="store_true",
+                    help="only the 128-environment workload (frames = --steps), e.g. for profiling")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     args = ap.parse_args()
@@ -211,6 +218,10 @@ def main():
     from kuafu_amd.abi import check
 
     check(lib().kf_init(0))
+    if args.envs_only:
+        sched = [int(x) for x in args.schedule.split(",")] + [0] * 4 if args.schedule else None
+        print(json.dumps(batched_environments(frames=args.steps, schedule=sched[:4] if sched else None)))
+        return
     w = scenes.config2(width=args.width, height=args.height)
     print("bench: building scene (%d triangles)" % w.data.n_triangles(), file=sys.stderr, flush=True)
     gs = GpuScene(w.data)

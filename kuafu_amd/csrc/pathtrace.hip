@@ -231,6 +231,125 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
 
 // Closest-hit / miss for one path; writes next ray, emission, BSDF weight and
 // NEE slots (deferred) or the NEE sum (inline).
+// Light j of the frame's light list seen from worldPos: direction Ld,
+// emitted radiance lem, shadow-ray length maxd; false when the point is
+// outside an active light's cone.  PERTURB draws the light's random
+// perturbation from rs first (soft directional light, point-light radius,
+// active-light softness), as the reference does before tracing.
+template <bool PERTURB>
+KF_DEV bool light_eval(const SceneView& S, const DevLights* __restrict__ LT, int kind, int li, v3 worldPos, uint32_t& rs,
+                       v3& Ld_out, v3& lem_out, float& maxd_out) {
+    bool ok = true;
+    float ldx, ldy, ldz, lex, ley, lez, maxd;  // scalars: per-branch vector writes would go through memory
+    if (kind == 0) {  // traceDirectionalLight (rchit:204-223)
+        const v3 lem = mk(LT->dl.rgbs[0], LT->dl.rgbs[1], LT->dl.rgbs[2]) * LT->dl.rgbs[3];
+        v3 Ld = -mk(LT->dl.direction[0], LT->dl.direction[1], LT->dl.direction[2]);
+        if (PERTURB && LT->dl.direction[3] != 0.0f) {
+            float a = rnd(rs), b = rnd(rs), c = rnd(rs);
+            Ld = normalize(Ld + LT->dl.direction[3] * mk(a, b, c));
+        }
+        ldx = Ld.x; ldy = Ld.y; ldz = Ld.z;
+        lex = lem.x; ley = lem.y; lez = lem.z;
+        maxd = 1e6f;
+    } else if (kind == 1) {  // tracePointLights (rchit:227-258)
+        v3 rgb = mk(LT->pl.rgbs[li][0], LT->pl.rgbs[li][1], LT->pl.rgbs[li][2]);
+        v3 lpos = mk(LT->pl.posr[li][0], LT->pl.posr[li][1], LT->pl.posr[li][2]);
+        if (PERTURB && LT->pl.posr[li][3] != 0.0f) {
+            v3 pert = uniform_sphere(rs);
+            lpos = lpos + LT->pl.posr[li][3] * normalize(pert);
+        }
+        v3 Lv = lpos - worldPos;
+        float dd2 = length(Lv);
+        const v3 Ld = normalize(Lv);
+        const v3 lem = rgb * LT->pl.rgbs[li][3] / dd2 / dd2;
+        ldx = Ld.x; ldy = Ld.y; ldz = Ld.z;
+        lex = lem.x; ley = lem.y; lez = lem.z;
+        maxd = dd2;
+    } else {  // traceActiveLights (rchit:262-316)
+        v3 rgb = mk(LT->al.rgbs[li][0], LT->al.rgbs[li][1], LT->al.rgbs[li][2]);
+        float softness = LT->al.sftp[li][0];
+        v3 lpos = mk(LT->al.position[li][0], LT->al.position[li][1], LT->al.position[li][2]);
+        if (PERTURB && softness != 0.0f) {
+            v3 pert = uniform_sphere(rs);
+            lpos = lpos + softness * normalize(pert);
+        }
+        v3 Lv = lpos - worldPos;
+        float dd2 = length(Lv);
+        const v3 Ld = normalize(Lv);
+        float fov = LT->al.sftp[li][1];
+        v3 ldir = normalize(mk(LT->al.front[li][0], LT->al.front[li][1], LT->al.front[li][2]));
+        float halfA = gclamp(fov, 0.0f, kPi) / 2.0f;
+        float cs = dot(ldir, -Ld);
+        ok = cs > cos_k(halfA);
+        v3 color = rgb;
+        int texID = (int)LT->al.sftp[li][2];
+        if (ok && texID >= 0 && (uint32_t)texID < S.n_tex) {
+            float v4[4] = {worldPos.x, worldPos.y, worldPos.z, 1.0f};
+            float tc[4];
+            mat4_mul(LT->pv[li], v4, tc);
+            float u = (tc[0] / tc[3]) * 0.5f + 0.5f;
+            float v = (tc[1] / tc[3]) * 0.5f + 0.5f;
+            float4 tx = tex_sample(S, S.tex_data, S.tex_info[texID], u, v, true);
+            color = color * xyz(tx);
+        }
+        const v3 lem = color * LT->al.rgbs[li][3] / dd2 / dd2;
+        ldx = Ld.x; ldy = Ld.y; ldz = Ld.z;
+        lex = lem.x; ley = lem.y; lez = lem.z;
+        maxd = dd2;
+    }
+    Ld_out = mk(ldx, ldy, ldz);
+    lem_out = mk(lex, ley, lez);
+    maxd_out = maxd;
+    return ok;
+}
+
+// Shading point of a path whose NEE is deferred to the shadow-ray kernel.
+struct NeeCtx {
+    ShadeCtx sh;
+    uint32_t root;  // the path's TLAS root item
+    bool on;
+};
+
+// The deferred light loop for the paths of a block (rchit:457-460 order):
+// per light, the (ca, theta, cb, mode) record of the contribution and a
+// shadow ray, appended with one atomic per block.  Every thread of the
+// block calls it; threads without a shading point (nc.on false) append
+// nothing.
+KF_DEV void nee_deferred(const SceneView& S, const DevLights* __restrict__ LT, const NeeCtx& nc, uint32_t p,
+                         uint32_t K, float4* __restrict__ nee_a, float4* __restrict__ nee_b,
+                         float4* __restrict__ sh_o, float4* __restrict__ sh_d, uint32_t* __restrict__ sh_root,
+                         unsigned long long* __restrict__ shadow_ctr, uint32_t& n_shadow) {
+    const int nl = LT->n_slots;
+    uint32_t unused_rs = 0u;
+    for (int j = 0; j < nl; ++j) {
+        bool emit = false;
+        v3 Ld = splat(0.0f);
+        float maxd = 0.0f;
+        if (nc.on) {
+            const int kind = LT->kind[j], li = LT->index[j];
+            v3 lem;
+            const bool ok = light_eval<false>(S, LT, kind, li, nc.sh.worldPos, unused_rs, Ld, lem, maxd);
+            if (ok && dot(nc.sh.N, Ld) > 0.0f) {
+                v3 ca, cb;
+                float th;
+                int mode;
+                direct_split(nc.sh, Ld, lem, ca, cb, th, mode);
+                const size_t slot = (size_t)p * K + j;
+                nee_a[slot] = f4(ca, th);
+                nee_b[slot] = f4(cb, __int_as_float(mode | (kind << 4)));
+                emit = true;
+            }
+        }
+        const uint32_t at = block_append(shadow_ctr, emit);
+        if (emit) {
+            sh_o[at] = f4(nc.sh.worldPos, __uint_as_float((uint32_t)((size_t)p * K + j)));
+            sh_d[at] = f4(Ld, maxd);
+            if (S.n_env > 1) sh_root[at] = nc.root;
+            n_shadow++;
+        }
+    }
+}
+
 // Closest hit / miss of path p at this bounce (rchit:322-469, rmiss:12-31):
 // emission, BSDF sample and next ray, NEE (shadow-ray records, or the sum
 // with the shadow rays traced here when NEE_INLINE), AOVs.  Returns whether
@@ -243,10 +362,11 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
                        float4* __restrict__ nee_b, uint8_t* __restrict__ vis, float4* __restrict__ sh_o,
                        float4* __restrict__ sh_d, uint32_t* __restrict__ sh_root, float4* __restrict__ albedo_out,
                        float4* __restrict__ normal_out, unsigned long long* __restrict__ cnt, uint32_t& n_shadow,
-                       uint32_t& ovf) {
+                       uint32_t& ovf, NeeCtx& nc) {
     const uint32_t K = F.slots;
     const uint32_t depth = __float_as_uint(t4.w);
     bool ext = false;
+    nc.on = false;
     float4 h = hit[p];
     float4 ro = ray_o[p], rd = ray_d[p];
     v3 rdir = xyz(rd);
@@ -373,96 +493,38 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
             sh.N = N; sh.V = V; sh.worldPos = worldPos; sh.f = f; sh.a2 = a2;
             sh.diffuseColor = diffuseColor; sh.specularColor = specularColor;
             sh.transmissionColor = transmissionColor; sh.inside = inside;
-            // NEE slots in evaluation order (rchit:457-460)
+            // NEE slots in evaluation order (rchit:457-460).  Deferred NEE
+            // hands the shading point to the caller, which runs the light loop
+            // with the whole block (nee_deferred).
             v3 sum_d = splat(0.0f), sum_p = splat(0.0f), sum_a = splat(0.0f);
-            const int nl = LT->n_slots;
+            if (!NEE_INLINE) {
+                nc.sh = sh;
+                nc.root = __float_as_uint(rd.w);
+                nc.on = true;
+            }
+            const int nl = NEE_INLINE ? LT->n_slots : 0;
             for (int j = 0; j < nl; ++j) {
-                int kind = LT->kind[j], li = LT->index[j];
+                const int kind = LT->kind[j], li = LT->index[j];
                 v3 Ld, lem;
                 float maxd;
-                bool ok = true;
-                if (kind == 0) {  // traceDirectionalLight (rchit:204-223)
-                    lem = mk(LT->dl.rgbs[0], LT->dl.rgbs[1], LT->dl.rgbs[2]) * LT->dl.rgbs[3];
-                    Ld = -mk(LT->dl.direction[0], LT->dl.direction[1], LT->dl.direction[2]);
-                    if (NEE_INLINE && LT->dl.direction[3] != 0.0f) {
-                        float a = rnd(rs), b = rnd(rs), c = rnd(rs);
-                        Ld = normalize(Ld + LT->dl.direction[3] * mk(a, b, c));
-                    }
-                    maxd = 1e6f;
-                } else if (kind == 1) {  // tracePointLights (rchit:227-258)
-                    v3 rgb = mk(LT->pl.rgbs[li][0], LT->pl.rgbs[li][1], LT->pl.rgbs[li][2]);
-                    v3 lpos = mk(LT->pl.posr[li][0], LT->pl.posr[li][1], LT->pl.posr[li][2]);
-                    if (NEE_INLINE && LT->pl.posr[li][3] != 0.0f) {
-                        v3 pert = uniform_sphere(rs);
-                        lpos = lpos + LT->pl.posr[li][3] * normalize(pert);
-                    }
-                    v3 Lv = lpos - worldPos;
-                    float dd2 = length(Lv);
-                    Ld = normalize(Lv);
-                    lem = rgb * LT->pl.rgbs[li][3] / dd2 / dd2;
-                    maxd = dd2;
-                } else {  // traceActiveLights (rchit:262-316)
-                    v3 rgb = mk(LT->al.rgbs[li][0], LT->al.rgbs[li][1], LT->al.rgbs[li][2]);
-                    float softness = LT->al.sftp[li][0];
-                    v3 lpos = mk(LT->al.position[li][0], LT->al.position[li][1], LT->al.position[li][2]);
-                    if (NEE_INLINE && softness != 0.0f) {
-                        v3 pert = uniform_sphere(rs);
-                        lpos = lpos + softness * normalize(pert);
-                    }
-                    v3 Lv = lpos - worldPos;
-                    float dd2 = length(Lv);
-                    Ld = normalize(Lv);
-                    float fov = LT->al.sftp[li][1];
-                    v3 ldir = normalize(mk(LT->al.front[li][0], LT->al.front[li][1], LT->al.front[li][2]));
-                    float halfA = gclamp(fov, 0.0f, kPi) / 2.0f;
-                    float cs = dot(ldir, -Ld);
-                    ok = cs > cos_k(halfA);
-                    v3 color = rgb;
-                    int texID = (int)LT->al.sftp[li][2];
-                    if (ok && texID >= 0 && (uint32_t)texID < S.n_tex) {
-                        float v4[4] = {worldPos.x, worldPos.y, worldPos.z, 1.0f};
-                        float tc[4];
-                        mat4_mul(LT->pv[li], v4, tc);
-                        float u = (tc[0] / tc[3]) * 0.5f + 0.5f;
-                        float v = (tc[1] / tc[3]) * 0.5f + 0.5f;
-                        float4 tx = tex_sample(S, S.tex_data, S.tex_info[texID], u, v, true);
-                        color = color * xyz(tx);
-                    }
-                    lem = color * LT->al.rgbs[li][3] / dd2 / dd2;
-                    maxd = dd2;
-                }
+                const bool ok = light_eval<true>(S, LT, kind, li, worldPos, rs, Ld, lem, maxd);
                 if (!ok) continue;
                 float NdotL = dot(N, Ld);
                 if (!(NdotL > 0.0f)) continue;
-                if (NEE_INLINE) {
-                    THit hh;
-                    TCount cc{0, 0, 0, 0};
-                    n_shadow++;
-                    bool occ = traverse<true, false>(S, __float_as_uint(rd.w), worldPos, Ld, 0.001f, maxd,
-                                                     false, 0u, hh, cc, ovf);
-                    if (!occ) {
-                        v3 ca, cb;
-                        float th;
-                        int mode;
-                        direct_split(sh, Ld, lem, ca, cb, th, mode);
-                        v3 cj = direct_resolve(ca, cb, th, mode, rs);
-                        if (kind == 0) sum_d = sum_d + cj;
-                        else if (kind == 1) sum_p = sum_p + cj;
-                        else sum_a = sum_a + cj;
-                    }
-                } else {
+                THit hh;
+                TCount cc{0, 0, 0, 0};
+                n_shadow++;
+                bool occ = traverse<true, false>(S, __float_as_uint(rd.w), worldPos, Ld, 0.001f, maxd, false, 0u, hh,
+                                                 cc, ovf);
+                if (!occ) {
                     v3 ca, cb;
                     float th;
                     int mode;
                     direct_split(sh, Ld, lem, ca, cb, th, mode);
-                    size_t slot = (size_t)p * K + j;
-                    nee_a[slot] = f4(ca, th);
-                    nee_b[slot] = f4(cb, __int_as_float(mode | (kind << 4)));
-                    uint32_t at = (uint32_t)atomicAdd(&cnt[bounce * kCntPerBounce + 1], 1ull);
-                    sh_o[at] = f4(worldPos, __uint_as_float((uint32_t)slot));
-                    sh_d[at] = f4(Ld, maxd);
-                    if (S.n_env > 1) sh_root[at] = __float_as_uint(rd.w);
-                    n_shadow++;
+                    v3 cj = direct_resolve(ca, cb, th, mode, rs);
+                    if (kind == 0) sum_d = sum_d + cj;
+                    else if (kind == 1) sum_p = sum_p + cj;
+                    else sum_a = sum_a + cj;
                 }
             }
             if (NEE_INLINE) nee_a[(size_t)p * K] = f4((sum_d + sum_p) + sum_a, 0.0f);
@@ -508,9 +570,15 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
         // only Russian roulette, decided in k_connect, can still end it.
         bool ext = false;
         bool aov = active && depth == 0 && F.s0 == 0 && p < F.WH;  // sample 0, first segment
+        NeeCtx nc;
+        nc.on = false;
         if (active)
             ext = shade_path<NEE_INLINE>(S, F, LT, bounce, p, t4, aov, ray_o, ray_d, hit, hit_prim, rw, em, seed, nee_a,
-                                         nee_b, vis, sh_o, sh_d, sh_root, albedo_out, normal_out, cnt, n_shadow, ovf);
+                                         nee_b, vis, sh_o, sh_d, sh_root, albedo_out, normal_out, cnt, n_shadow, ovf,
+                                         nc);
+        if (!NEE_INLINE)
+            nee_deferred(S, LT, nc, p, F.slots, nee_a, nee_b, sh_o, sh_d, sh_root, &cnt[bounce * kCntPerBounce + 1],
+                         n_shadow);
         const uint32_t at = block_append(&cnt[(bounce + 1) * kCntPerBounce + kCntExtQueue], ext);
         if (ext) ext_queue[at] = p;
     }
@@ -665,8 +733,9 @@ __global__ __launch_bounds__(TB) void k_tail(SceneView S, Frame F, const DevLigh
             ++n_traced;
             hit[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.inst));
             hit_prim[p] = h.prim | (h.back << 31);
+            NeeCtx nc;
             shade_path<true>(S, F, LT, b, p, thr[p], false, ray_o, ray_d, hit, hit_prim, rw, em, seed, nee_a, nee_b,
-                             vis, nullptr, nullptr, nullptr, nullptr, nullptr, cnt, n_shadow, ovf);
+                             vis, nullptr, nullptr, nullptr, nullptr, nullptr, cnt, n_shadow, ovf, nc);
             if (!connect_path<true>(F, b, p, thr, col, rw, em, seed, nee_a, nee_b, vis, out)) break;
             ++n_ext;  // traced at the next bounce
         }
