@@ -49,114 +49,206 @@ __host__ __device__ inline float bf2f(uint16_t b) {
 
 // ---------------------------------------------------------------- convolution
 // y[p, co] = sum_{tap, ci} x[p + off(tap), ci] * w[tap, co, ci]   (zero padding)
-// Block tile: 32 x 8 pixels of one image x BN output channels (BN = 16 or
-// 32, chosen from the channels actually stored).  Per 32-channel input chunk
-// the block stages in LDS the 34 x 10 input halo of its tile and the weights
-// of all 9 taps, so one barrier pair covers 9 taps x 4 x BN/16 MFMAs per
-// wave.  Wave w owns pixel rows 2w and 2w + 1 (four 16-pixel MFMA tiles).
+// Work item: a tile of 32 x 8 pixels of one image x BN output channels (BN =
+// 16 or 32, chosen from the channels actually stored); step: an item and
+// one 32-channel chunk of its input.  Per step the block holds in LDS the
+// 34 x 10 input halo of its tile and the weights of all 9 taps, so one
+// barrier pair covers 9 taps x 4 x BN/16 MFMAs per wave; wave w owns a
+// 16 x 4 block of pixels (four 16-pixel MFMA tiles, one per row).
+// The grid is persistent: block b takes items b, b + gridDim.x, ... and,
+// while its waves run the MFMAs of one step, holds the global loads of the
+// next step in registers (written to LDS after the next barrier), so memory
+// latency hides behind matrix work instead of behind other blocks.  Weights
+// stay in LDS while consecutive steps need the same ones (one input chunk
+// and the same output channels).
 // Cin must be a multiple of 32; ldx, ldy multiples of 8.
 constexpr int CBK = 32, CLD = CBK + 8;
 constexpr int TX = 32, TY = 8, HXX = TX + 2, HYY = TY + 2, NHALO = HXX * HYY;
+
+template <int BN>
+struct ConvShape {
+    static constexpr int kHaloU4 = NHALO * 4;               // 16-byte groups of the halo
+    static constexpr int kWU4 = 9 * BN * 4;                 // ... of one chunk of weights
+    static constexpr int kHR = (kHaloU4 + TB - 1) / TB;     // per thread
+    static constexpr int kWR = (kWU4 + TB - 1) / TB;
+    static constexpr int kLds = NHALO * CLD + 9 * BN * CLD;  // halves
+};
 
 template <int BN>
 __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int ldx, int Cin,
                                              const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
                                              int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
                                              float* __restrict__ Yf, int ldyf, int cstore,
-                                             const uint16_t* __restrict__ M, int ldm, int relu) {
-    __shared__ __attribute__((aligned(16))) uint16_t Hs[NHALO * CLD];
-    __shared__ __attribute__((aligned(16))) uint16_t Ws[9 * BN * CLD];
+                                             const uint16_t* __restrict__ M, int ldm, int relu, int n_items) {
+    using SH = ConvShape<BN>;
+    __shared__ __attribute__((aligned(16))) uint16_t smem[SH::kLds];
+    uint16_t* Hs = smem;
+    uint16_t* Ws = smem + NHALO * CLD;
     const int tiles_x = (W + TX - 1) / TX, tiles_y = (H + TY - 1) / TY;
-    int bi = blockIdx.x;
-    const int x0 = (bi % tiles_x) * TX;
-    bi /= tiles_x;
-    const int y0 = (bi % tiles_y) * TY;
-    const int n_ = bi / tiles_y;
-    const int co0 = blockIdx.y * BN;
+    const int n_tiles = N * tiles_x * tiles_y;
+    const int nchunk = Cin / CBK;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
     const uint4 zero = make_uint4(0, 0, 0, 0);
-    f32x4 acc[4][BN / 16];  // [row * 2 + 16-pixel half][channel tile]
+    struct Item {
+        int n, y0, x0, co0;
+    };
+    auto decode = [&](int i) {
+        Item it;
+        const int tile = i % n_tiles;
+        it.co0 = (i / n_tiles) * BN;
+        it.x0 = (tile % tiles_x) * TX;
+        const int r = tile / tiles_x;
+        it.y0 = (r % tiles_y) * TY;
+        it.n = r / tiles_y;
+        return it;
+    };
+    uint4 hreg[SH::kHR], wreg[SH::kWR];
+    auto load_halo = [&](const Item& it, int ci0) {
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-        for (int nb = 0; nb < BN / 16; ++nb) acc[mt][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int ci0 = 0; ci0 < Cin; ci0 += CBK) {
-        __syncthreads();  // the previous chunk's LDS reads are done
-        for (int i = t; i < NHALO * 4; i += TB) {
-            const int hp = i >> 2, q = i & 3;
-            const int hy = hp / HXX, hx = hp - hy * HXX;
-            const int yy = y0 + hy - 1, xx = x0 + hx - 1;
+        for (int r = 0; r < SH::kHR; ++r) {
+            const int i = t + r * TB;
             uint4 v = zero;
-            if (n_ < N && yy >= 0 && yy < H && xx >= 0 && xx < W)
-                v = *(const uint4*)(X + ((size_t)(n_ * H + yy) * W + xx) * ldx + ci0 + q * 8);
-            *(uint4*)&Hs[hp * CLD + q * 8] = v;
+            if (i < SH::kHaloU4) {
+                const int hp = i >> 2, q = i & 3;
+                const int hy = hp / HXX, hx = hp - hy * HXX;
+                const int yy = it.y0 + hy - 1, xx = it.x0 + hx - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    v = *(const uint4*)(X + ((size_t)(it.n * H + yy) * W + xx) * ldx + ci0 + q * 8);
+            }
+            hreg[r] = v;
         }
-        for (int i = t; i < 9 * BN * 4; i += TB) {
-            const int q = i & 3, r = (i >> 2) % BN, tap = (i >> 2) / BN;
-            const int co = co0 + r;
+    };
+    auto load_w = [&](int co0, int ci0) {
+#pragma unroll
+        for (int r = 0; r < SH::kWR; ++r) {
+            const int i = t + r * TB;
             uint4 v = zero;
-            if (co < Cout) v = *(const uint4*)(Wt + ((size_t)tap * Cout + co) * Cin + ci0 + q * 8);
-            *(uint4*)&Ws[(tap * BN + r) * CLD + q * 8] = v;
+            if (i < SH::kWU4) {
+                const int q = i & 3, rr = (i >> 2) % BN, tap = (i >> 2) / BN;
+                const int co = co0 + rr;
+                if (co < Cout) v = *(const uint4*)(Wt + ((size_t)tap * Cout + co) * Cin + ci0 + q * 8);
+            }
+            wreg[r] = v;
+        }
+    };
+    int i = blockIdx.x, c = 0;
+    if (i >= n_items) return;
+    Item cur = decode(i);
+    load_halo(cur, 0);
+    load_w(cur.co0, 0);
+    bool need_w = true;
+    f32x4 acc[4][BN / 16];
+    while (true) {
+        __syncthreads();  // the previous step's LDS reads (and epilogue) are done
+#pragma unroll
+        for (int r = 0; r < SH::kHR; ++r) {
+            const int k = t + r * TB;
+            if (k < SH::kHaloU4) *(uint4*)&Hs[(k >> 2) * CLD + (k & 3) * 8] = hreg[r];
+        }
+        if (need_w) {
+#pragma unroll
+            for (int r = 0; r < SH::kWR; ++r) {
+                const int k = t + r * TB;
+                if (k < SH::kWU4) *(uint4*)&Ws[(k >> 2) * CLD + (k & 3) * 8] = wreg[r];
+            }
         }
         __syncthreads();
-#pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int dy = tap / 3, dx = tap % 3;  // offsets inside the halo
-            bf16x8 a[4];
+        // the next step's loads go out now and land during the MFMAs below
+        int ni = i, nc = c + 1;
+        if (nc == nchunk) {
+            nc = 0;
+            ni = i + gridDim.x;
+        }
+        const bool has_next = ni < n_items;
+        Item nxt = cur;
+        bool next_w = false;
+        if (has_next) {
+            nxt = decode(ni);
+            load_halo(nxt, nc * CBK);
+            next_w = nxt.co0 != cur.co0 || nc != c;
+            if (next_w) load_w(nxt.co0, nc * CBK);
+        }
+        if (c == 0) {
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
-                a[mt] = __builtin_bit_cast(
-                    bf16x8, *(const uint4*)&Hs[((2 * wv + (mt >> 1) + dy) * HXX + (mt & 1) * 16 + fr + dx) * CLD + fk]);
 #pragma unroll
-            for (int nb = 0; nb < BN / 16; ++nb) {
-                const bf16x8 b = __builtin_bit_cast(bf16x8, *(const uint4*)&Ws[(tap * BN + nb * 16 + fr) * CLD + fk]);
-#pragma unroll
-                for (int mt = 0; mt < 4; ++mt)
-                    acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b, acc[mt][nb], 0, 0, 0);
-            }
+                for (int nb = 0; nb < BN / 16; ++nb) acc[mt][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
-    }
-    // Epilogue.  BF16 outputs go through LDS so that each lane stores whole
-    // 16-byte channel groups of a pixel; FP32 outputs (the RGB layer) are
-    // stored directly.
-    constexpr int OLD = BN + 8;  // LDS row of one output pixel
-    uint16_t* Os = Hs;           // TX * TY * OLD halves fit in the halo buffer
-    static_assert(TX * TY * OLD <= NHALO * CLD, "output tile must fit in the halo buffer");
-    if (!Yf) __syncthreads();  // every wave is done reading the halo
+        // Wave w owns pixel rows 4 (w / 2) .. + 3 of the 16 columns (w % 2) * 16 ..;
+        // per horizontal offset dx it reads the 6 halo rows it spans once and
+        // uses each for up to 3 vertical taps.  The weights are the MFMA's A
+        // operand and the pixels its B operand, so each lane ends up holding 4
+        // consecutive output channels of one pixel.
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+        for (int dx = 0; dx < 3; ++dx) {
+            bf16x8 a[6];
 #pragma unroll
-        for (int nb = 0; nb < BN / 16; ++nb)
+            for (int j = 0; j < 6; ++j)
+                a[j] = __builtin_bit_cast(
+                    bf16x8, *(const uint4*)&Hs[((4 * (wv >> 1) + j) * HXX + (wv & 1) * 16 + fr + dx) * CLD + fk]);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int ry = 2 * wv + (mt >> 1), rx = (mt & 1) * 16 + (lane >> 4) * 4 + i;
-                const int yo = y0 + ry, xo = x0 + rx;
-                const int cl = nb * 16 + (lane & 15), c = co0 + cl;
-                const bool ok = n_ < N && yo < H && xo < W && c < cstore;
-                const size_t pp = (size_t)(n_ * H + yo) * W + xo;
-                float v = acc[mt][nb][i] + ((bias && c < Cout) ? bias[c] : 0.0f);
-                if (relu) v = fmaxf(v, 0.0f);
-                if (ok && M && !(bf2f(M[pp * ldm + c]) > 0.0f)) v = 0.0f;
-                if (Yf) {
-                    if (ok) Yf[pp * ldyf + c] = v;
-                } else {
-                    Os[(ry * TX + rx) * OLD + cl] = f2bf(v);
+            for (int dy = 0; dy < 3; ++dy) {
+                const int tap = dy * 3 + dx;
+#pragma unroll
+                for (int nb = 0; nb < BN / 16; ++nb) {
+                    const bf16x8 b =
+                        __builtin_bit_cast(bf16x8, *(const uint4*)&Ws[(tap * BN + nb * 16 + fr) * CLD + fk]);
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt)
+                        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[mt + dy], acc[mt][nb], 0, 0, 0);
                 }
             }
-    if (Yf) return;
-    __syncthreads();
-    for (int i = t; i < TX * TY * (BN / 8); i += TB) {
-        const int pix = i / (BN / 8), q = i % (BN / 8);
-        const int yo = y0 + pix / TX, xo = x0 + pix % TX;
-        const int c = co0 + q * 8;
-        if (n_ >= N || yo >= H || xo >= W || c >= cstore) continue;
-        const size_t pp = (size_t)(n_ * H + yo) * W + xo;
-        if (c + 8 <= cstore) {
-            *(uint4*)(Y + pp * ldy + c) = *(const uint4*)&Os[pix * OLD + q * 8];
-        } else {
-            for (int j = 0; c + j < cstore; ++j) Y[pp * ldy + c + j] = Os[pix * OLD + q * 8 + j];
         }
+        if (c == nchunk - 1) {
+            // Epilogue straight from the accumulators: lane holds channels
+            // co0 + nb * 16 + 4 (lane / 16) .. + 3 of pixel (lane % 16) of row mt.
+            const int xo = cur.x0 + (wv & 1) * 16 + fr;
+#pragma unroll
+            for (int nb = 0; nb < BN / 16; ++nb) {
+                const int ch0 = cur.co0 + nb * 16 + (lane >> 4) * 4;
+                float bv[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) bv[e] = (bias && ch0 + e < Cout) ? bias[ch0 + e] : 0.0f;
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    const int yo = cur.y0 + 4 * (wv >> 1) + mt;
+                    if (yo >= H || xo >= W || ch0 >= cstore) continue;
+                    const size_t pp = (size_t)(cur.n * H + yo) * W + xo;
+                    float v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        v[e] = acc[mt][nb][e] + bv[e];
+                        if (relu) v[e] = fmaxf(v[e], 0.0f);
+                        if (M && ch0 + e < cstore && !(bf2f(M[pp * ldm + ch0 + e]) > 0.0f)) v[e] = 0.0f;
+                    }
+                    if (Yf) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (ch0 + e < cstore) Yf[pp * ldyf + ch0 + e] = v[e];
+                    } else {
+                        // v_cvt_pk_bf16_f32: round to nearest even, as f2bf
+                        typedef float f2 __attribute__((ext_vector_type(2)));
+                        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+                        const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[0], v[1]}, b2));
+                        const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[2], v[3]}, b2));
+                        if (ch0 + 4 <= cstore) {
+                            *(uint2*)(Y + pp * ldy + ch0) = make_uint2(lo, hi);
+                        } else {
+                            const uint32_t h4[2] = {lo, hi};
+#pragma unroll
+                            for (int e = 0; e < 4; ++e)
+                                if (ch0 + e < cstore) Y[pp * ldy + ch0 + e] = (uint16_t)(h4[e >> 1] >> (16 * (e & 1)));
+                        }
+                    }
+                }
+            }
+        }
+        if (!has_next) break;
+        i = ni;
+        c = nc;
+        cur = nxt;
+        need_w = next_w;
     }
 }
 
@@ -452,20 +544,39 @@ template <class T> T* dalloc(size_t n) {
 // ---------------------------------------------------------------- launchers
 // One 3x3 convolution: output tiles of 16 channels when at most 16 are
 // stored (the RGB output layer), else 32.
+// Persistent grid of k_conv<BN>: the blocks that are resident at once (from
+// its LDS and registers), or fewer when there are fewer items.
+template <int BN>
+unsigned conv_grid(unsigned n_items) {
+    static unsigned resident = 0;
+    if (!resident) {
+        int dev = 0, cus = 256, nb = 0;
+        KF_CHECK(hipGetDevice(&dev));
+        KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_conv<BN>, TB, 0));
+        resident = (unsigned)std::max(nb, 1) * (unsigned)cus;
+    }
+    return std::min(n_items, resident);
+}
+
 void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const float* bias, int Cout, int N, int H,
                  int W, uint16_t* Y, int ldy, float* Yf, int ldyf, int cstore, const uint16_t* M, int ldm, int relu,
                  hipStream_t st) {
     if (Cin % CBK != 0) throw Error("conv: input channels must be a multiple of 32");
     const unsigned tiles = (unsigned)N * blocks(H, TY) * blocks(W, TX);
+    if ((uint64_t)tiles * blocks(cstore, 16) >= (1u << 31)) throw Error("conv: too many tiles");
     if (cstore <= 16) {
-        k_conv<16><<<dim3(tiles, blocks(cstore, 16)), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf,
-                                                                   ldyf, cstore, M, ldm, relu);
+        const unsigned n = tiles;
+        k_conv<16><<<conv_grid<16>(n), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore, M,
+                                                   ldm, relu, (int)n);
     } else {
-        k_conv<32><<<dim3(tiles, blocks(cstore, 32)), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf,
-                                                                   ldyf, cstore, M, ldm, relu);
+        const unsigned n = tiles * blocks(cstore, 32);
+        k_conv<32><<<conv_grid<32>(n), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore, M,
+                                                   ldm, relu, (int)n);
     }
     KF_CHECK(hipGetLastError());
 }
+
 void conv_fwd(const uint16_t* X, int ldx, int Cin, const uint16_t* Wb, const float* bias, int Cout, int N, int H,
               int W, uint16_t* Y, int ldy, bool relu, hipStream_t st, float* Yf = nullptr, int ldyf = 0,
               int cstore = -1) {
