@@ -79,7 +79,9 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                                              const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
                                              int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
                                              float* __restrict__ Yf, int ldyf, int cstore,
-                                             const uint16_t* __restrict__ M, int ldm, int relu, int n_items) {
+                                             const uint16_t* __restrict__ M, int ldm, int relu, int n_items,
+                                             const uint16_t* __restrict__ X2, int ldx2, int csplit,
+                                             uint16_t* __restrict__ P) {
     using SH = ConvShape<BN>;
     __shared__ __attribute__((aligned(16))) uint16_t smem[SH::kLds];
     uint16_t* Hs = smem;
@@ -105,6 +107,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
     };
     uint4 hreg[SH::kHR], wreg[SH::kWR];
     auto load_halo = [&](const Item& it, int ci0) {
+        const bool up = ci0 >= csplit;  // channels of the upsampled half-resolution tensor
 #pragma unroll
         for (int r = 0; r < SH::kHR; ++r) {
             const int i = t + r * TB;
@@ -113,8 +116,13 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                 const int hp = i >> 2, q = i & 3;
                 const int hy = hp / HXX, hx = hp - hy * HXX;
                 const int yy = it.y0 + hy - 1, xx = it.x0 + hx - 1;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                    v = *(const uint4*)(X + ((size_t)(it.n * H + yy) * W + xx) * ldx + ci0 + q * 8);
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                    if (up)
+                        v = *(const uint4*)(X2 + ((size_t)(it.n * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1)) * ldx2 +
+                                            (ci0 - csplit) + q * 8);
+                    else
+                        v = *(const uint4*)(X + ((size_t)(it.n * H + yy) * W + xx) * ldx + ci0 + q * 8);
+                }
             }
             hreg[r] = v;
         }
@@ -210,6 +218,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                 float bv[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) bv[e] = (bias && ch0 + e < Cout) ? bias[ch0 + e] : 0.0f;
+                uint32_t pool[4][2] = {};
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt) {
                     const int yo = cur.y0 + 4 * (wv >> 1) + mt;
@@ -234,12 +243,51 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                         const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[2], v[3]}, b2));
                         if (ch0 + 4 <= cstore) {
                             *(uint2*)(Y + pp * ldy + ch0) = make_uint2(lo, hi);
+                            if (P) {
+                                // 2x2 average pool of the rounded outputs: rows mt, mt + 1
+                                // here, columns xo, xo + 1 in this lane and the next
+                                pool[mt][0] = lo;
+                                pool[mt][1] = hi;
+                            }
                         } else {
                             const uint32_t h4[2] = {lo, hi};
 #pragma unroll
                             for (int e = 0; e < 4; ++e)
                                 if (ch0 + e < cstore) Y[pp * ldy + ch0 + e] = (uint16_t)(h4[e >> 1] >> (16 * (e & 1)));
                         }
+                    }
+                }
+                if (P) {
+                    // pooled pixel (yo / 2, xo / 2) from rows 2 mp, 2 mp + 1 of the lanes
+                    // of columns xo (even) and xo + 1 (the next lane); summed in the
+                    // order (2y, 2x), (2y, 2x + 1), (2y + 1, 2x), (2y + 1, 2x + 1)
+#pragma unroll
+                    for (int mp = 0; mp < 2; ++mp) {
+                        const int yo = cur.y0 + 4 * (wv >> 1) + 2 * mp;
+                        uint32_t q4[4][2];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            q4[0][h] = pool[2 * mp][h];
+                            q4[1][h] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pool[2 * mp][h], 0xB1, 0xF, 0xF, false);
+                            q4[2][h] = pool[2 * mp + 1][h];
+                            q4[3][h] =
+                                (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pool[2 * mp + 1][h], 0xB1, 0xF, 0xF, false);
+                        }
+                        if ((fr & 1) || yo >= H || xo >= W || ch0 + 4 > cstore) continue;
+                        float m[4];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            float sum = 0.0f;
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) sum += bf2f((uint16_t)(q4[k][e >> 1] >> (16 * (e & 1))));
+                            m[e] = 0.25f * sum;
+                        }
+                        typedef float f2 __attribute__((ext_vector_type(2)));
+                        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+                        const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{m[0], m[1]}, b2));
+                        const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{m[2], m[3]}, b2));
+                        *(uint2*)(P + ((size_t)(cur.n * (H >> 1) + (yo >> 1)) * (W >> 1) + (xo >> 1)) * cstore + ch0) =
+                            make_uint2(lo, hi);
                     }
                 }
             }
@@ -319,29 +367,6 @@ __global__ void k_bias_grad(const uint16_t* __restrict__ dY, int ldy, int Cout, 
 }
 
 // ---------------------------------------------------------------- glue kernels
-// 2x2 average pooling: Y[n, y, x, c] = mean of X[n, 2y..2y+1, 2x..2x+1, c].
-__global__ void k_pool(const uint16_t* __restrict__ X, int ldx, int C, int N, int H, int W, uint16_t* __restrict__ Y) {
-    int Ho = H / 2, Wo = W / 2, C8 = C / 8;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * Ho * Wo * C8) return;
-    int c8 = (int)(i % C8);
-    size_t q = i / C8;
-    int x = (int)(q % Wo), y = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
-    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < 4; ++k) {
-        size_t src = ((size_t)(n * H + 2 * y + (k >> 1)) * W + 2 * x + (k & 1)) * ldx + c8 * 8;
-        uint4 v = *(const uint4*)(X + src);
-        uint16_t e[8];
-        memcpy(e, &v, 16);
-        for (int j = 0; j < 8; ++j) s[j] += bf2f(e[j]);
-    }
-    uint16_t o[8];
-    for (int j = 0; j < 8; ++j) o[j] = f2bf(0.25f * s[j]);
-    uint4 ov;
-    memcpy(&ov, o, 16);
-    *(uint4*)(Y + q * C + c8 * 8) = ov;
-}
-
 // dS[hi] += (S[hi] > 0) * dP[lo] / 4   (gradient of the pooled skip tensor)
 __global__ void k_pool_bwd(const uint16_t* __restrict__ dPl, int C, const uint16_t* __restrict__ S, int lds,
                            uint16_t* __restrict__ dS, int ldd, int N, int H, int W) {
@@ -559,29 +584,42 @@ unsigned conv_grid(unsigned n_items) {
     return std::min(n_items, resident);
 }
 
+// Optional fusions: input channels csplit.. come from X2 (ldx2), an H/2 x W/2
+// tensor upsampled by nearest neighbour; P receives the 2x2 average pool of
+// the bf16 outputs (H/2 x W/2 x cstore).
+struct ConvFuse {
+    const uint16_t* X2 = nullptr;
+    int ldx2 = 0, csplit = 1 << 30;
+    uint16_t* P = nullptr;
+};
+
 void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const float* bias, int Cout, int N, int H,
                  int W, uint16_t* Y, int ldy, float* Yf, int ldyf, int cstore, const uint16_t* M, int ldm, int relu,
-                 hipStream_t st) {
+                 hipStream_t st, const ConvFuse& fz = ConvFuse()) {
     if (Cin % CBK != 0) throw Error("conv: input channels must be a multiple of 32");
+    if (fz.X2 && (H % 2 || W % 2 || fz.csplit % CBK || fz.ldx2 % 8 || Cin - fz.csplit > fz.ldx2))
+        throw Error("conv: bad upsampled input");
+    if (fz.P && (H % 2 || W % 2 || !Y || Yf || cstore != Cout || cstore % 16))
+        throw Error("conv: bad pooled output");
     const unsigned tiles = (unsigned)N * blocks(H, TY) * blocks(W, TX);
     if ((uint64_t)tiles * blocks(cstore, 16) >= (1u << 31)) throw Error("conv: too many tiles");
     if (cstore <= 16) {
         const unsigned n = tiles;
         k_conv<16><<<conv_grid<16>(n), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore, M,
-                                                   ldm, relu, (int)n);
+                                                   ldm, relu, (int)n, fz.X2, fz.ldx2, fz.csplit, fz.P);
     } else {
         const unsigned n = tiles * blocks(cstore, 32);
         k_conv<32><<<conv_grid<32>(n), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore, M,
-                                                   ldm, relu, (int)n);
+                                                   ldm, relu, (int)n, fz.X2, fz.ldx2, fz.csplit, fz.P);
     }
     KF_CHECK(hipGetLastError());
 }
 
 void conv_fwd(const uint16_t* X, int ldx, int Cin, const uint16_t* Wb, const float* bias, int Cout, int N, int H,
               int W, uint16_t* Y, int ldy, bool relu, hipStream_t st, float* Yf = nullptr, int ldyf = 0,
-              int cstore = -1) {
+              int cstore = -1, const ConvFuse& fz = ConvFuse()) {
     conv_launch(X, ldx, Cin, Wb, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore < 0 ? Cout : std::min(cstore, Cout),
-                nullptr, 0, relu ? 1 : 0, st);
+                nullptr, 0, relu ? 1 : 0, st, fz);
 }
 // dX = conv(dY, Wd) masked by (Xf > 0) when Xf is given
 void conv_dgrad(const uint16_t* dY, int ldy, int Cout, const uint16_t* Wd, int Cin, int N, int H, int W, uint16_t* dX,
@@ -791,11 +829,11 @@ struct Unet {
             ++k;
             if (l < kLevels - 1) {
                 int ldc = kCh[l] + kCh[l + 1];
+                ConvFuse fz;
+                fz.P = act.pl[l];  // the 2x2 pool comes out of this convolution's epilogue
                 conv_fwd(act.t[l], kCh[l], kCh[l], wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.cat[l], ldc,
-                         true, st);
+                         true, st, nullptr, 0, -1, fz);
                 ++k;
-                size_t n8 = (size_t)N * H(l + 1) * W(l + 1) * kCh[l] / 8;
-                k_pool<<<blocks(n8, TB), TB, 0, st>>>(act.cat[l], ldc, kCh[l], N, H(l), W(l), act.pl[l]);
                 in = act.pl[l];
                 ldin = cin = kCh[l];
             } else {
@@ -807,9 +845,17 @@ struct Unet {
         const uint16_t* low = act.bt;
         for (int l = kLevels - 2; l >= 0; --l) {
             int ldc = kCh[l] + kCh[l + 1];
-            size_t n8 = (size_t)N * H(l) * W(l) * kCh[l + 1] / 8;
-            k_up<<<blocks(n8, TB), TB, 0, st>>>(low, kCh[l + 1], N, H(l + 1), W(l + 1), act.cat[l] + kCh[l], ldc);
-            conv_fwd(act.cat[l], ldc, ldc, wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.da[l], kCh[l], true, st);
+            if (act.grads) {  // the backward pass reads the whole concatenation
+                size_t n8 = (size_t)N * H(l) * W(l) * kCh[l + 1] / 8;
+                k_up<<<blocks(n8, TB), TB, 0, st>>>(low, kCh[l + 1], N, H(l + 1), W(l + 1), act.cat[l] + kCh[l], ldc);
+            }
+            // skip channels from act.cat[l], the upsampled ones straight from low
+            ConvFuse fz;
+            fz.X2 = low;
+            fz.ldx2 = kCh[l + 1];
+            fz.csplit = kCh[l];
+            conv_fwd(act.cat[l], ldc, ldc, wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.da[l], kCh[l], true, st,
+                     nullptr, 0, -1, fz);
             ++k;
             conv_fwd(act.da[l], kCh[l], kCh[l], wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.u[l], kCh[l], true,
                      st);
