@@ -23,6 +23,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace kf {
@@ -46,6 +47,31 @@ __host__ __device__ inline float bf2f(uint16_t b) {
     memcpy(&f, &u, 4);
     return f;
 }
+
+__device__ inline float sane(float v) { return (v == v) ? fminf(fmaxf(v, -1e4f), 1e4f) : 0.0f; }
+
+// The 9 network input channels of one pixel: log1p(rgb), albedo * pi, normal.
+__device__ inline float log_radiance(float v) { return log1pf(fmaxf(sane(v), 0.f)); }
+__device__ inline void pack9(float4 c, float4 a, float4 n, float v[9]) {
+    v[0] = log_radiance(c.x);
+    v[1] = log_radiance(c.y);
+    v[2] = log_radiance(c.z);
+    v[3] = sane(a.x) * 3.141592f;
+    v[4] = sane(a.y) * 3.141592f;
+    v[5] = sane(a.z) * 3.141592f;
+    v[6] = sane(n.x);
+    v[7] = sane(n.y);
+    v[8] = sane(n.z);
+}
+
+// float4 render targets of N images of h x w stacked vertically, read as the
+// network input padded to the convolution's H x W by repeating edge pixels.
+struct PackSrc {
+    const float4* rgba = nullptr;
+    const float4* alb = nullptr;
+    const float4* nrm = nullptr;
+    int h = 0, w = 0;
+};
 
 // ---------------------------------------------------------------- convolution
 // y[p, co] = sum_{tap, ci} x[p + off(tap), ci] * w[tap, co, ci]   (zero padding)
@@ -74,14 +100,18 @@ struct ConvShape {
     static constexpr int kLds = NHALO * CLD + 9 * BN * CLD;  // halves
 };
 
-template <int BN>
+// FUSE: 0 plain, kFuseUp upsampled input channels, kFusePool pooled output,
+// kFusePack the 32-channel network input computed from the render targets.
+constexpr int kFuseUp = 1, kFusePool = 2, kFusePack = 4;
+
+template <int BN, int FUSE>
 __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int ldx, int Cin,
                                              const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
                                              int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
                                              float* __restrict__ Yf, int ldyf, int cstore,
                                              const uint16_t* __restrict__ M, int ldm, int relu, int n_items,
                                              const uint16_t* __restrict__ X2, int ldx2, int csplit,
-                                             uint16_t* __restrict__ P) {
+                                             uint16_t* __restrict__ P, PackSrc pk) {
     using SH = ConvShape<BN>;
     __shared__ __attribute__((aligned(16))) uint16_t smem[SH::kLds];
     uint16_t* Hs = smem;
@@ -107,7 +137,30 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
     };
     uint4 hreg[SH::kHR], wreg[SH::kWR];
     auto load_halo = [&](const Item& it, int ci0) {
-        const bool up = ci0 >= csplit;  // channels of the upsampled half-resolution tensor
+        if constexpr (FUSE == kFusePack) {
+            static_assert(SH::kHR >= 6, "the packed input stages 6 registers");
+            // two halo pixels per thread, 3 float4 each (zero outside the image)
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int hp = t + r * TB;
+                float4 c = make_float4(0.f, 0.f, 0.f, 0.f), a = c, nm = c;
+                if (hp < NHALO) {
+                    const int hy = hp / HXX, hx = hp - hy * HXX;
+                    const int yy = it.y0 + hy - 1, xx = it.x0 + hx - 1;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                        const size_t src = ((size_t)it.n * pk.h + min(yy, pk.h - 1)) * pk.w + min(xx, pk.w - 1);
+                        c = pk.rgba[src];
+                        a = pk.alb[src];
+                        nm = pk.nrm[src];
+                    }
+                }
+                hreg[3 * r] = __builtin_bit_cast(uint4, c);
+                hreg[3 * r + 1] = __builtin_bit_cast(uint4, a);
+                hreg[3 * r + 2] = __builtin_bit_cast(uint4, nm);
+            }
+            return;
+        }
+        const bool up = FUSE == kFuseUp && ci0 >= csplit;  // channels of the upsampled half-resolution tensor
 #pragma unroll
         for (int r = 0; r < SH::kHR; ++r) {
             const int i = t + r * TB;
@@ -149,10 +202,28 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
     f32x4 acc[4][BN / 16];
     while (true) {
         __syncthreads();  // the previous step's LDS reads (and epilogue) are done
+        if constexpr (FUSE == kFusePack) {
 #pragma unroll
-        for (int r = 0; r < SH::kHR; ++r) {
-            const int k = t + r * TB;
-            if (k < SH::kHaloU4) *(uint4*)&Hs[(k >> 2) * CLD + (k & 3) * 8] = hreg[r];
+            for (int r = 0; r < 2; ++r) {
+                const int hp = t + r * TB;
+                if (hp >= NHALO) continue;
+                float v[9];
+                pack9(__builtin_bit_cast(float4, hreg[3 * r]), __builtin_bit_cast(float4, hreg[3 * r + 1]),
+                      __builtin_bit_cast(float4, hreg[3 * r + 2]), v);
+                uint32_t o[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) o[j] = (uint32_t)f2bf(v[2 * j]) | ((uint32_t)f2bf(v[2 * j + 1]) << 16);
+                *(uint4*)&Hs[hp * CLD] = make_uint4(o[0], o[1], o[2], o[3]);
+                *(uint4*)&Hs[hp * CLD + 8] = make_uint4(f2bf(v[8]), 0u, 0u, 0u);
+                *(uint4*)&Hs[hp * CLD + 16] = make_uint4(0u, 0u, 0u, 0u);
+                *(uint4*)&Hs[hp * CLD + 24] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < SH::kHR; ++r) {
+                const int k = t + r * TB;
+                if (k < SH::kHaloU4) *(uint4*)&Hs[(k >> 2) * CLD + (k & 3) * 8] = hreg[r];
+            }
         }
         if (need_w) {
 #pragma unroll
@@ -243,7 +314,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                         const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[2], v[3]}, b2));
                         if (ch0 + 4 <= cstore) {
                             *(uint2*)(Y + pp * ldy + ch0) = make_uint2(lo, hi);
-                            if (P) {
+                            if (FUSE == kFusePool) {
                                 // 2x2 average pool of the rounded outputs: rows mt, mt + 1
                                 // here, columns xo, xo + 1 in this lane and the next
                                 pool[mt][0] = lo;
@@ -257,7 +328,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                         }
                     }
                 }
-                if (P) {
+                if (FUSE == kFusePool) {
                     // pooled pixel (yo / 2, xo / 2) from rows 2 mp, 2 mp + 1 of the lanes
                     // of columns xo (even) and xo + 1 (the next lane); summed in the
                     // order (2y, 2x), (2y, 2x + 1), (2y + 1, 2x), (2y + 1, 2x + 1)
@@ -427,43 +498,15 @@ __global__ void k_up_bwd(const uint16_t* __restrict__ dY, int ldy, int C, const 
     *(uint4*)(dX + p * C + c8 * 8) = ov;
 }
 
-__device__ inline float sane(float v) { return (v == v) ? fminf(fmaxf(v, -1e4f), 1e4f) : 0.0f; }
-
-// Network input from float4 render targets (one image, edge-replicated to
-// Hp x Wp): ch 0-2 log1p(rgb), 3-5 albedo * pi, 6-8 normal, 9-31 zero; lin
-// keeps log1p(rgb) for the residual.
-// N images of H x W stacked in the targets -> N padded Hp x Wp images
-// (edge pixels repeated into the padding).
-__global__ void k_pack_targets(const float4* __restrict__ rgba, const float4* __restrict__ alb,
-                               const float4* __restrict__ nrm, int N, int H, int W, int Hp, int Wp,
-                               uint16_t* __restrict__ X, float* __restrict__ lin) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * Hp * Wp) return;
-    const int img = (int)(i / ((size_t)Hp * Wp)), j = (int)(i % ((size_t)Hp * Wp));
-    int x = min(j % Wp, W - 1), y = min(j / Wp, H - 1);
-    size_t s = ((size_t)img * H + y) * W + x;
-    float4 c = rgba[s], a = alb[s], n = nrm[s];
-    float v[9] = {log1pf(fmaxf(sane(c.x), 0.f)), log1pf(fmaxf(sane(c.y), 0.f)), log1pf(fmaxf(sane(c.z), 0.f)),
-                  sane(a.x) * 3.141592f, sane(a.y) * 3.141592f, sane(a.z) * 3.141592f, sane(n.x), sane(n.y),
-                  sane(n.z)};
-    uint16_t o[32];
-    for (int k = 0; k < 32; ++k) o[k] = k < 9 ? f2bf(v[k]) : (uint16_t)0;
-    for (int k = 0; k < 4; ++k) *(uint4*)(X + (size_t)i * 32 + 8 * k) = *(const uint4*)&o[8 * k];
-    lin[(size_t)i * 4 + 0] = v[0];
-    lin[(size_t)i * 4 + 1] = v[1];
-    lin[(size_t)i * 4 + 2] = v[2];
-    lin[(size_t)i * 4 + 3] = 0.f;
-}
-
 // Network input from packed records (n, h, w, 9): rgb radiance, albedo (as
 // stored in the albedo target, baseColor / pi), normal.
 __global__ void k_pack_records(const float* __restrict__ in, int P, uint16_t* __restrict__ X, float* __restrict__ lin) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     const float* r = in + (size_t)i * 9;
-    float v[9] = {log1pf(fmaxf(sane(r[0]), 0.f)), log1pf(fmaxf(sane(r[1]), 0.f)), log1pf(fmaxf(sane(r[2]), 0.f)),
-                  sane(r[3]) * 3.141592f, sane(r[4]) * 3.141592f, sane(r[5]) * 3.141592f, sane(r[6]), sane(r[7]),
-                  sane(r[8])};
+    float v[9];
+    pack9(make_float4(r[0], r[1], r[2], 0.f), make_float4(r[3], r[4], r[5], 0.f), make_float4(r[6], r[7], r[8], 0.f),
+          v);
     uint16_t o[32];
     for (int k = 0; k < 32; ++k) o[k] = k < 9 ? f2bf(v[k]) : (uint16_t)0;
     for (int k = 0; k < 4; ++k) *(uint4*)(X + (size_t)i * 32 + 8 * k) = *(const uint4*)&o[8 * k];
@@ -473,16 +516,20 @@ __global__ void k_pack_records(const float* __restrict__ in, int P, uint16_t* __
 
 constexpr float kLogClamp = 10.0f;
 
-// denoised linear RGB = expm1(min(y + log1p(noisy), 10)), cropped to H x W
-__global__ void k_unpack(const float* __restrict__ y, const float* __restrict__ lin, int N, int H, int W, int Hp,
-                         int Wp, float4* __restrict__ out) {
+// denoised linear RGB = expm1(min(y + log1p(noisy), 10)), cropped to H x W;
+// out may be rgba itself (each pixel is read before it is written, by the
+// same thread)
+__global__ void k_unpack(const float* __restrict__ y, const float4* rgba, int N, int H, int W, int Hp, int Wp,
+                         float4* out) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)N * H * W) return;
     const int img = (int)(i / ((size_t)H * W)), j = (int)(i % ((size_t)H * W));
     int x = j % W, yy = j / W;
     size_t s = ((size_t)img * Hp + yy) * Wp + x;
+    const float4 c = rgba[i];
+    const float lin[3] = {log_radiance(c.x), log_radiance(c.y), log_radiance(c.z)};
     float r[3];
-    for (int k = 0; k < 3; ++k) r[k] = expm1f(fminf(y[s * 4 + k] + lin[s * 4 + k], kLogClamp));
+    for (int k = 0; k < 3; ++k) r[k] = expm1f(fminf(y[s * 4 + k] + lin[k], kLogClamp));
     out[i] = make_float4(r[0], r[1], r[2], 1.0f);
 }
 
@@ -571,14 +618,14 @@ template <class T> T* dalloc(size_t n) {
 // stored (the RGB output layer), else 32.
 // Persistent grid of k_conv<BN>: the blocks that are resident at once (from
 // its LDS and registers), or fewer when there are fewer items.
-template <int BN>
+template <int BN, int FUSE>
 unsigned conv_grid(unsigned n_items) {
     static unsigned resident = 0;
     if (!resident) {
         int dev = 0, cus = 256, nb = 0;
         KF_CHECK(hipGetDevice(&dev));
         KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_conv<BN>, TB, 0));
+        KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_conv<BN, FUSE>, TB, 0));
         resident = (unsigned)std::max(nb, 1) * (unsigned)cus;
     }
     return std::min(n_items, resident);
@@ -591,6 +638,7 @@ struct ConvFuse {
     const uint16_t* X2 = nullptr;
     int ldx2 = 0, csplit = 1 << 30;
     uint16_t* P = nullptr;
+    PackSrc pk;  // when pk.rgba is set, the input is packed from the render targets
 };
 
 void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const float* bias, int Cout, int N, int H,
@@ -603,14 +651,29 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
         throw Error("conv: bad pooled output");
     const unsigned tiles = (unsigned)N * blocks(H, TY) * blocks(W, TX);
     if ((uint64_t)tiles * blocks(cstore, 16) >= (1u << 31)) throw Error("conv: too many tiles");
+    if ((fz.X2 != nullptr) + (fz.P != nullptr) + (fz.pk.rgba != nullptr) > 1) throw Error("conv: one fusion at a time");
+    if (fz.pk.rgba && (Cin != CBK || cstore <= 16 || fz.pk.h > H || fz.pk.w > W || fz.pk.h < 1 || fz.pk.w < 1))
+        throw Error("conv: bad packed input");
+    auto go = [&](auto bn, auto fuse) {
+        constexpr int BN = decltype(bn)::value, F = decltype(fuse)::value;
+        const unsigned n = tiles * blocks(cstore, BN);
+        k_conv<BN, F><<<conv_grid<BN, F>(n), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf,
+                                                         cstore, M, ldm, relu, (int)n, fz.X2, fz.ldx2, fz.csplit,
+                                                         fz.P, fz.pk);
+    };
+    using I16 = std::integral_constant<int, 16>;
+    using I32 = std::integral_constant<int, 32>;
     if (cstore <= 16) {
-        const unsigned n = tiles;
-        k_conv<16><<<conv_grid<16>(n), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore, M,
-                                                   ldm, relu, (int)n, fz.X2, fz.ldx2, fz.csplit, fz.P);
+        if (fz.X2 || fz.P || fz.pk.rgba) throw Error("conv: fusions need more than 16 output channels");
+        go(I16(), std::integral_constant<int, 0>());
+    } else if (fz.X2) {
+        go(I32(), std::integral_constant<int, kFuseUp>());
+    } else if (fz.P) {
+        go(I32(), std::integral_constant<int, kFusePool>());
+    } else if (fz.pk.rgba) {
+        go(I32(), std::integral_constant<int, kFusePack>());
     } else {
-        const unsigned n = tiles * blocks(cstore, 32);
-        k_conv<32><<<conv_grid<32>(n), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf, cstore, M,
-                                                   ldm, relu, (int)n, fz.X2, fz.ldx2, fz.csplit, fz.P);
+        go(I32(), std::integral_constant<int, 0>());
     }
     KF_CHECK(hipGetLastError());
 }
@@ -816,8 +879,8 @@ struct Unet {
         set_params_host(h);
     }
 
-    // forward pass on act.x0 / act.lin -> act.y
-    void forward(hipStream_t st) {
+    // forward pass on act.x0 (or the render targets pk) -> act.y
+    void forward(hipStream_t st, const PackSrc* pk = nullptr) {
         const int N = act.N;
         auto H = [&](int l) { return act.H >> l; };
         auto W = [&](int l) { return act.W >> l; };
@@ -825,7 +888,10 @@ struct Unet {
         const uint16_t* in = act.x0;
         int ldin = kInCh, cin = kInCh;
         for (int l = 0; l < kLevels; ++l) {
-            conv_fwd(in, ldin, cin, wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.t[l], kCh[l], true, st);
+            ConvFuse fin;
+            if (l == 0 && pk) fin.pk = *pk;
+            conv_fwd(in, ldin, cin, wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.t[l], kCh[l], true, st, nullptr,
+                     0, -1, fin);
             ++k;
             if (l < kLevels - 1) {
                 int ldc = kCh[l] + kCh[l + 1];
@@ -1143,12 +1209,17 @@ int kf_denoiser_infer_device_batch(kf_denoiser* d, const float* d_rgba, const fl
         KF_CHECK(hipEventCreate(&e0));
         KF_CHECK(hipEventCreate(&e1));
         KF_CHECK(hipEventRecord(e0, st));
-        k_pack_targets<<<blocks((size_t)N * Hp * Wp, TB), TB, 0, st>>>(
-            (const float4*)d_rgba, (const float4*)d_albedo, (const float4*)d_normal, N, (int)height, (int)width, Hp,
-            Wp, d->act.x0, d->act.lin);
-        d->forward(st);
-        k_unpack<<<blocks((size_t)N * width * height, TB), TB, 0, st>>>(d->act.y, d->act.lin, N, (int)height,
-                                                                        (int)width, Hp, Wp, (float4*)d_out);
+        // the first convolution reads the render targets itself
+        PackSrc pk;
+        pk.rgba = (const float4*)d_rgba;
+        pk.alb = (const float4*)d_albedo;
+        pk.nrm = (const float4*)d_normal;
+        pk.h = (int)height;
+        pk.w = (int)width;
+        d->forward(st, &pk);
+        k_unpack<<<blocks((size_t)N * width * height, TB), TB, 0, st>>>(d->act.y, (const float4*)d_rgba, N,
+                                                                        (int)height, (int)width, Hp, Wp,
+                                                                        (float4*)d_out);
         KF_CHECK(hipEventRecord(e1, st));
         KF_CHECK(hipGetLastError());
         KF_CHECK(hipEventSynchronize(e1));
