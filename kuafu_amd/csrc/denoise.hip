@@ -210,11 +210,16 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                 float v[9];
                 pack9(__builtin_bit_cast(float4, hreg[3 * r]), __builtin_bit_cast(float4, hreg[3 * r + 1]),
                       __builtin_bit_cast(float4, hreg[3 * r + 2]), v);
-                uint32_t o[4];
+                // v_cvt_pk_bf16_f32 rounds to nearest even as f2bf (no NaN after sane)
+                typedef float f2 __attribute__((ext_vector_type(2)));
+                typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+                uint32_t o[5];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) o[j] = (uint32_t)f2bf(v[2 * j]) | ((uint32_t)f2bf(v[2 * j + 1]) << 16);
+                for (int j = 0; j < 5; ++j)
+                    o[j] = __builtin_bit_cast(uint32_t,
+                                              __builtin_convertvector(f2{v[2 * j], j < 4 ? v[2 * j + 1] : 0.f}, b2));
                 *(uint4*)&Hs[hp * CLD] = make_uint4(o[0], o[1], o[2], o[3]);
-                *(uint4*)&Hs[hp * CLD + 8] = make_uint4(f2bf(v[8]), 0u, 0u, 0u);
+                *(uint4*)&Hs[hp * CLD + 8] = make_uint4(o[4], 0u, 0u, 0u);
                 *(uint4*)&Hs[hp * CLD + 16] = make_uint4(0u, 0u, 0u, 0u);
                 *(uint4*)&Hs[hp * CLD + 24] = make_uint4(0u, 0u, 0u, 0u);
             }
@@ -879,6 +884,11 @@ struct Unet {
         set_params_host(h);
     }
 
+    // Pixel stride of the skip tensor in act.cat[l]: the concatenation's when
+    // the backward pass needs it whole, else the skip channels alone (dense
+    // rows, no partly used cache lines).
+    int skip_stride(int l) const { return act.grads ? kCh[l] + kCh[l + 1] : kCh[l]; }
+
     // forward pass on act.x0 (or the render targets pk) -> act.y
     void forward(hipStream_t st, const PackSrc* pk = nullptr) {
         const int N = act.N;
@@ -894,11 +904,10 @@ struct Unet {
                      0, -1, fin);
             ++k;
             if (l < kLevels - 1) {
-                int ldc = kCh[l] + kCh[l + 1];
                 ConvFuse fz;
                 fz.P = act.pl[l];  // the 2x2 pool comes out of this convolution's epilogue
-                conv_fwd(act.t[l], kCh[l], kCh[l], wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.cat[l], ldc,
-                         true, st, nullptr, 0, -1, fz);
+                conv_fwd(act.t[l], kCh[l], kCh[l], wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.cat[l],
+                         skip_stride(l), true, st, nullptr, 0, -1, fz);
                 ++k;
                 in = act.pl[l];
                 ldin = cin = kCh[l];
@@ -920,7 +929,7 @@ struct Unet {
             fz.X2 = low;
             fz.ldx2 = kCh[l + 1];
             fz.csplit = kCh[l];
-            conv_fwd(act.cat[l], ldc, ldc, wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.da[l], kCh[l], true, st,
+            conv_fwd(act.cat[l], skip_stride(l), ldc, wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.da[l], kCh[l], true, st,
                      nullptr, 0, -1, fz);
             ++k;
             conv_fwd(act.da[l], kCh[l], kCh[l], wb[k], w32 + boff[k], kCh[l], N, H(l), W(l), act.u[l], kCh[l], true,
