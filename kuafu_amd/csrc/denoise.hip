@@ -97,12 +97,13 @@ struct ConvShape {
     static constexpr int kWU4 = 9 * BN * 4;                 // ... of one chunk of weights
     static constexpr int kHR = (kHaloU4 + TB - 1) / TB;     // per thread
     static constexpr int kWR = (kWU4 + TB - 1) / TB;
-    static constexpr int kLds = NHALO * CLD + 9 * BN * CLD;  // halves
+    static constexpr int kLds = NHALO * CLD + 9 * BN * CLD + 2 * BN;  // halves (+ BN FP32 biases)
 };
 
-// FUSE: 0 plain, kFuseUp upsampled input channels, kFusePool pooled output,
-// kFusePack the 32-channel network input computed from the render targets.
-constexpr int kFuseUp = 1, kFusePool = 2, kFusePack = 4;
+// FUSE bits: kFuseUp upsampled input channels, kFusePool pooled output,
+// kFusePack the 32-channel network input computed from the render targets,
+// kOutF32 FP32 output (Yf) instead of BF16 (Y), kMask output masked by M > 0.
+constexpr int kFuseUp = 1, kFusePool = 2, kFusePack = 4, kOutF32 = 8, kMask = 16;
 
 template <int BN, int FUSE>
 __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int ldx, int Cin,
@@ -116,6 +117,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
     __shared__ __attribute__((aligned(16))) uint16_t smem[SH::kLds];
     uint16_t* Hs = smem;
     uint16_t* Ws = smem + NHALO * CLD;
+    float* Bsm = (float*)(smem + NHALO * CLD + 9 * BN * CLD);  // biases of the LDS weights' channels
     const int tiles_x = (W + TX - 1) / TX, tiles_y = (H + TY - 1) / TY;
     const int n_tiles = N * tiles_x * tiles_y;
     const int nchunk = Cin / CBK;
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
     };
     uint4 hreg[SH::kHR], wreg[SH::kWR];
     auto load_halo = [&](const Item& it, int ci0) {
-        if constexpr (FUSE == kFusePack) {
+        if constexpr ((FUSE & kFusePack) != 0) {
             static_assert(SH::kHR >= 6, "the packed input stages 6 registers");
             // two halo pixels per thread, 3 float4 each (zero outside the image)
 #pragma unroll
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
             }
             return;
         }
-        const bool up = FUSE == kFuseUp && ci0 >= csplit;  // channels of the upsampled half-resolution tensor
+        const bool up = (FUSE & kFuseUp) != 0 && ci0 >= csplit;  // channels of the upsampled half-resolution tensor
 #pragma unroll
         for (int r = 0; r < SH::kHR; ++r) {
             const int i = t + r * TB;
@@ -180,7 +182,9 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
             hreg[r] = v;
         }
     };
+    float breg = 0.0f;
     auto load_w = [&](int co0, int ci0) {
+        if (t < BN) breg = (bias && co0 + t < Cout) ? bias[co0 + t] : 0.0f;
 #pragma unroll
         for (int r = 0; r < SH::kWR; ++r) {
             const int i = t + r * TB;
@@ -202,7 +206,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
     f32x4 acc[4][BN / 16];
     while (true) {
         __syncthreads();  // the previous step's LDS reads (and epilogue) are done
-        if constexpr (FUSE == kFusePack) {
+        if constexpr ((FUSE & kFusePack) != 0) {
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 const int hp = t + r * TB;
@@ -231,6 +235,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
             }
         }
         if (need_w) {
+            if (t < BN) Bsm[t] = breg;
 #pragma unroll
             for (int r = 0; r < SH::kWR; ++r) {
                 const int k = t + r * TB;
@@ -291,9 +296,8 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
 #pragma unroll
             for (int nb = 0; nb < BN / 16; ++nb) {
                 const int ch0 = cur.co0 + nb * 16 + (lane >> 4) * 4;
-                float bv[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) bv[e] = (bias && ch0 + e < Cout) ? bias[ch0 + e] : 0.0f;
+                const float4 b4 = *(const float4*)&Bsm[nb * 16 + (lane >> 4) * 4];
+                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
                 uint32_t pool[4][2] = {};
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt) {
@@ -305,9 +309,10 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                     for (int e = 0; e < 4; ++e) {
                         v[e] = acc[mt][nb][e] + bv[e];
                         if (relu) v[e] = fmaxf(v[e], 0.0f);
-                        if (M && ch0 + e < cstore && !(bf2f(M[pp * ldm + ch0 + e]) > 0.0f)) v[e] = 0.0f;
+                        if constexpr ((FUSE & kMask) != 0)
+                            if (ch0 + e < cstore && !(bf2f(M[pp * ldm + ch0 + e]) > 0.0f)) v[e] = 0.0f;
                     }
-                    if (Yf) {
+                    if constexpr ((FUSE & kOutF32) != 0) {
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
                             if (ch0 + e < cstore) Yf[pp * ldyf + ch0 + e] = v[e];
@@ -319,7 +324,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                         const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[2], v[3]}, b2));
                         if (ch0 + 4 <= cstore) {
                             *(uint2*)(Y + pp * ldy + ch0) = make_uint2(lo, hi);
-                            if (FUSE == kFusePool) {
+                            if constexpr ((FUSE & kFusePool) != 0) {
                                 // 2x2 average pool of the rounded outputs: rows mt, mt + 1
                                 // here, columns xo, xo + 1 in this lane and the next
                                 pool[mt][0] = lo;
@@ -333,7 +338,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                         }
                     }
                 }
-                if (FUSE == kFusePool) {
+                if constexpr ((FUSE & kFusePool) != 0) {
                     // pooled pixel (yo / 2, xo / 2) from rows 2 mp, 2 mp + 1 of the lanes
                     // of columns xo (even) and xo + 1 (the next lane); summed in the
                     // order (2y, 2x), (2y, 2x + 1), (2y + 1, 2x), (2y + 1, 2x + 1)
@@ -656,11 +661,14 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
         throw Error("conv: bad pooled output");
     const unsigned tiles = (unsigned)N * blocks(H, TY) * blocks(W, TX);
     if ((uint64_t)tiles * blocks(cstore, 16) >= (1u << 31)) throw Error("conv: too many tiles");
-    if ((fz.X2 != nullptr) + (fz.P != nullptr) + (fz.pk.rgba != nullptr) > 1) throw Error("conv: one fusion at a time");
-    if (fz.pk.rgba && (Cin != CBK || cstore <= 16 || fz.pk.h > H || fz.pk.w > W || fz.pk.h < 1 || fz.pk.w < 1))
+    const int fuse = (fz.X2 ? kFuseUp : 0) | (fz.P ? kFusePool : 0) | (fz.pk.rgba ? kFusePack : 0);
+    if (fuse & (fuse - 1)) throw Error("conv: one fusion at a time");
+    if (fuse && (M || Yf || cstore <= 16)) throw Error("conv: fusions write plain BF16 outputs of > 16 channels");
+    if (fz.pk.rgba && (Cin != CBK || fz.pk.h > H || fz.pk.w > W || fz.pk.h < 1 || fz.pk.w < 1))
         throw Error("conv: bad packed input");
-    auto go = [&](auto bn, auto fuse) {
-        constexpr int BN = decltype(bn)::value, F = decltype(fuse)::value;
+    const int flags = fuse | (Yf ? kOutF32 : 0) | (M ? kMask : 0);
+    auto go = [&](auto bn, auto fl) {
+        constexpr int BN = decltype(bn)::value, F = decltype(fl)::value;
         const unsigned n = tiles * blocks(cstore, BN);
         k_conv<BN, F><<<conv_grid<BN, F>(n), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf,
                                                          cstore, M, ldm, relu, (int)n, fz.X2, fz.ldx2, fz.csplit,
@@ -668,17 +676,24 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
     };
     using I16 = std::integral_constant<int, 16>;
     using I32 = std::integral_constant<int, 32>;
+    auto outs = [&](auto bn) {  // the output kinds of an unfused convolution
+        switch (flags) {
+            case 0: go(bn, std::integral_constant<int, 0>()); break;
+            case kOutF32: go(bn, std::integral_constant<int, kOutF32>()); break;
+            case kMask: go(bn, std::integral_constant<int, kMask>()); break;
+            default: go(bn, std::integral_constant<int, kOutF32 | kMask>()); break;
+        }
+    };
     if (cstore <= 16) {
-        if (fz.X2 || fz.P || fz.pk.rgba) throw Error("conv: fusions need more than 16 output channels");
-        go(I16(), std::integral_constant<int, 0>());
-    } else if (fz.X2) {
+        outs(I16());
+    } else if (fuse == kFuseUp) {
         go(I32(), std::integral_constant<int, kFuseUp>());
-    } else if (fz.P) {
+    } else if (fuse == kFusePool) {
         go(I32(), std::integral_constant<int, kFusePool>());
-    } else if (fz.pk.rgba) {
+    } else if (fuse == kFusePack) {
         go(I32(), std::integral_constant<int, kFusePack>());
     } else {
-        go(I32(), std::integral_constant<int, 0>());
+        outs(I32());
     }
     KF_CHECK(hipGetLastError());
 }
