@@ -89,6 +89,13 @@ struct PackSrc {
 // and the same output channels).
 // Cin must be a multiple of 32; ldx, ldy multiples of 8.
 constexpr int CBK = 32, CLD = CBK + 8;
+// LDS rows of the convolution tiles: 32 halves (64 bytes) per pixel or
+// weight row, 16-byte group q of row r stored at group q ^ ((r / 2) % 4), so
+// that the 16-byte reads of an MFMA operand (16 consecutive rows, one group)
+// hit distinct banks (measured: no bank conflicts, against ~half the LDS
+// cycles with rows padded to 80 bytes).
+constexpr int CCLD = CBK;
+__device__ inline int cofs(int row, int q) { return row * CCLD + ((q ^ ((row >> 1) & 3)) << 3); }
 constexpr int TX = 32, TY = 8, HXX = TX + 2, HYY = TY + 2, NHALO = HXX * HYY;
 
 template <int BN>
@@ -97,7 +104,7 @@ struct ConvShape {
     static constexpr int kWU4 = 9 * BN * 4;                 // ... of one chunk of weights
     static constexpr int kHR = (kHaloU4 + TB - 1) / TB;     // per thread
     static constexpr int kWR = (kWU4 + TB - 1) / TB;
-    static constexpr int kLds = NHALO * CLD + 9 * BN * CLD + 2 * BN;  // halves (+ BN FP32 biases)
+    static constexpr int kLds = NHALO * CCLD + 9 * BN * CCLD + 2 * BN;  // halves (+ BN FP32 biases)
 };
 
 // FUSE bits: kFuseUp upsampled input channels, kFusePool pooled output,
@@ -116,25 +123,25 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
     using SH = ConvShape<BN>;
     __shared__ __attribute__((aligned(16))) uint16_t smem[SH::kLds];
     uint16_t* Hs = smem;
-    uint16_t* Ws = smem + NHALO * CLD;
-    float* Bsm = (float*)(smem + NHALO * CLD + 9 * BN * CLD);  // biases of the LDS weights' channels
+    uint16_t* Ws = smem + NHALO * CCLD;
+    float* Bsm = (float*)(smem + NHALO * CCLD + 9 * BN * CCLD);  // biases of the LDS weights' channels
     const int tiles_x = (W + TX - 1) / TX, tiles_y = (H + TY - 1) / TY;
     const int n_tiles = N * tiles_x * tiles_y;
     const int nchunk = Cin / CBK;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    const int fr = lane & 15;
     const uint4 zero = make_uint4(0, 0, 0, 0);
     struct Item {
         int n, y0, x0, co0;
     };
-    auto decode = [&](int i) {
+    auto decode = [&](int i) {  // i is uniform: the fields live in scalar registers
         Item it;
         const int tile = i % n_tiles;
-        it.co0 = (i / n_tiles) * BN;
-        it.x0 = (tile % tiles_x) * TX;
+        it.co0 = __builtin_amdgcn_readfirstlane((i / n_tiles) * BN);
+        it.x0 = __builtin_amdgcn_readfirstlane((tile % tiles_x) * TX);
         const int r = tile / tiles_x;
-        it.y0 = (r % tiles_y) * TY;
-        it.n = r / tiles_y;
+        it.y0 = __builtin_amdgcn_readfirstlane((r % tiles_y) * TY);
+        it.n = __builtin_amdgcn_readfirstlane(r / tiles_y);
         return it;
     };
     uint4 hreg[SH::kHR], wreg[SH::kWR];
@@ -162,7 +169,32 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
             }
             return;
         }
-        const bool up = (FUSE & kFuseUp) != 0 && ci0 >= csplit;  // channels of the upsampled half-resolution tensor
+        // Halo slot r of this thread: pixel hp = (t + r TB) / 4 of the (TX + 2) x
+        // (TY + 2) halo, 16-byte channel group q.  Pixel indices are 32-bit
+        // (N H W < 2^31); a tile whose halo lies inside the image skips the
+        // bounds tests.
+        const bool inside = it.y0 >= 1 && it.y0 + TY < H && it.x0 >= 1 && it.x0 + TX < W;
+        const int y0 = it.y0 - 1, x0 = it.x0 - 1;
+        if ((FUSE & kFuseUp) != 0 && ci0 >= csplit) {  // channels of the upsampled half-resolution tensor
+            const uint16_t* src = X2 + (ci0 - csplit);
+            const int Wh = W >> 1, row0 = it.n * (H >> 1);
+#pragma unroll
+            for (int r = 0; r < SH::kHR; ++r) {
+                const int i = t + r * TB;
+                uint4 v = zero;
+                if (i < SH::kHaloU4) {
+                    const int hp = i >> 2, q = i & 3;
+                    const int hy = hp / HXX, hx = hp - hy * HXX;
+                    const int yy = y0 + hy, xx = x0 + hx;
+                    if (inside || ((unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W))
+                        v = *(const uint4*)(src + (size_t)(unsigned)((row0 + (yy >> 1)) * Wh + (xx >> 1)) * ldx2 + q * 8);
+                }
+                hreg[r] = v;
+            }
+            return;
+        }
+        const uint16_t* src = X + ci0;
+        const int pix0 = (it.n * H + y0) * W + x0;  // may be negative off the image; used only inside
 #pragma unroll
         for (int r = 0; r < SH::kHR; ++r) {
             const int i = t + r * TB;
@@ -170,14 +202,8 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
             if (i < SH::kHaloU4) {
                 const int hp = i >> 2, q = i & 3;
                 const int hy = hp / HXX, hx = hp - hy * HXX;
-                const int yy = it.y0 + hy - 1, xx = it.x0 + hx - 1;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                    if (up)
-                        v = *(const uint4*)(X2 + ((size_t)(it.n * (H >> 1) + (yy >> 1)) * (W >> 1) + (xx >> 1)) * ldx2 +
-                                            (ci0 - csplit) + q * 8);
-                    else
-                        v = *(const uint4*)(X + ((size_t)(it.n * H + yy) * W + xx) * ldx + ci0 + q * 8);
-                }
+                if (inside || ((unsigned)(y0 + hy) < (unsigned)H && (unsigned)(x0 + hx) < (unsigned)W))
+                    v = *(const uint4*)(src + (size_t)(unsigned)(pix0 + hy * W + hx) * ldx + q * 8);
             }
             hreg[r] = v;
         }
@@ -189,10 +215,10 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
         for (int r = 0; r < SH::kWR; ++r) {
             const int i = t + r * TB;
             uint4 v = zero;
-            if (i < SH::kWU4) {
+            if (i < SH::kWU4) {  // 9 Cout Cin < 2^31
                 const int q = i & 3, rr = (i >> 2) % BN, tap = (i >> 2) / BN;
                 const int co = co0 + rr;
-                if (co < Cout) v = *(const uint4*)(Wt + ((size_t)tap * Cout + co) * Cin + ci0 + q * 8);
+                if (co < Cout) v = *(const uint4*)(Wt + (unsigned)((tap * Cout + co) * Cin + ci0 + q * 8));
             }
             wreg[r] = v;
         }
@@ -222,16 +248,16 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                 for (int j = 0; j < 5; ++j)
                     o[j] = __builtin_bit_cast(uint32_t,
                                               __builtin_convertvector(f2{v[2 * j], j < 4 ? v[2 * j + 1] : 0.f}, b2));
-                *(uint4*)&Hs[hp * CLD] = make_uint4(o[0], o[1], o[2], o[3]);
-                *(uint4*)&Hs[hp * CLD + 8] = make_uint4(o[4], 0u, 0u, 0u);
-                *(uint4*)&Hs[hp * CLD + 16] = make_uint4(0u, 0u, 0u, 0u);
-                *(uint4*)&Hs[hp * CLD + 24] = make_uint4(0u, 0u, 0u, 0u);
+                *(uint4*)&Hs[cofs(hp, 0)] = make_uint4(o[0], o[1], o[2], o[3]);
+                *(uint4*)&Hs[cofs(hp, 1)] = make_uint4(o[4], 0u, 0u, 0u);
+                *(uint4*)&Hs[cofs(hp, 2)] = make_uint4(0u, 0u, 0u, 0u);
+                *(uint4*)&Hs[cofs(hp, 3)] = make_uint4(0u, 0u, 0u, 0u);
             }
         } else {
 #pragma unroll
             for (int r = 0; r < SH::kHR; ++r) {
                 const int k = t + r * TB;
-                if (k < SH::kHaloU4) *(uint4*)&Hs[(k >> 2) * CLD + (k & 3) * 8] = hreg[r];
+                if (k < SH::kHaloU4) *(uint4*)&Hs[cofs(k >> 2, k & 3)] = hreg[r];
             }
         }
         if (need_w) {
@@ -239,7 +265,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
 #pragma unroll
             for (int r = 0; r < SH::kWR; ++r) {
                 const int k = t + r * TB;
-                if (k < SH::kWU4) *(uint4*)&Ws[(k >> 2) * CLD + (k & 3) * 8] = wreg[r];
+                if (k < SH::kWU4) *(uint4*)&Ws[cofs(k >> 2, k & 3)] = wreg[r];
             }
         }
         __syncthreads();
@@ -275,14 +301,14 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
 #pragma unroll
             for (int j = 0; j < 6; ++j)
                 a[j] = __builtin_bit_cast(
-                    bf16x8, *(const uint4*)&Hs[((4 * (wv >> 1) + j) * HXX + (wv & 1) * 16 + fr + dx) * CLD + fk]);
+                    bf16x8, *(const uint4*)&Hs[cofs((4 * (wv >> 1) + j) * HXX + (wv & 1) * 16 + fr + dx, lane >> 4)]);
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy) {
                 const int tap = dy * 3 + dx;
 #pragma unroll
                 for (int nb = 0; nb < BN / 16; ++nb) {
                     const bf16x8 b =
-                        __builtin_bit_cast(bf16x8, *(const uint4*)&Ws[(tap * BN + nb * 16 + fr) * CLD + fk]);
+                        __builtin_bit_cast(bf16x8, *(const uint4*)&Ws[cofs(tap * BN + nb * 16 + fr, lane >> 4)]);
 #pragma unroll
                     for (int mt = 0; mt < 4; ++mt)
                         acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[mt + dy], acc[mt][nb], 0, 0, 0);
@@ -293,6 +319,9 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
             // Epilogue straight from the accumulators: lane holds channels
             // co0 + nb * 16 + 4 (lane / 16) .. + 3 of pixel (lane % 16) of row mt.
             const int xo = cur.x0 + (wv & 1) * 16 + fr;
+            // per-tile base (uniform, 64-bit) + per-lane 32-bit offsets
+            const size_t pbase = (size_t)(cur.n * H + cur.y0) * W + cur.x0;
+            const int xl = (wv & 1) * 16 + fr;
 #pragma unroll
             for (int nb = 0; nb < BN / 16; ++nb) {
                 const int ch0 = cur.co0 + nb * 16 + (lane >> 4) * 4;
@@ -303,27 +332,29 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                 for (int mt = 0; mt < 4; ++mt) {
                     const int yo = cur.y0 + 4 * (wv >> 1) + mt;
                     if (yo >= H || xo >= W || ch0 >= cstore) continue;
-                    const size_t pp = (size_t)(cur.n * H + yo) * W + xo;
+                    const int loc = (4 * (wv >> 1) + mt) * W + xl;  // pixel - pbase
                     float v[4];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         v[e] = acc[mt][nb][e] + bv[e];
                         if (relu) v[e] = fmaxf(v[e], 0.0f);
                         if constexpr ((FUSE & kMask) != 0)
-                            if (ch0 + e < cstore && !(bf2f(M[pp * ldm + ch0 + e]) > 0.0f)) v[e] = 0.0f;
+                            if (ch0 + e < cstore && !(bf2f(M[pbase * ldm + (unsigned)(loc * ldm + ch0 + e)]) > 0.0f))
+                                v[e] = 0.0f;
                     }
                     if constexpr ((FUSE & kOutF32) != 0) {
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
-                            if (ch0 + e < cstore) Yf[pp * ldyf + ch0 + e] = v[e];
+                            if (ch0 + e < cstore) Yf[pbase * ldyf + (unsigned)(loc * ldyf + ch0 + e)] = v[e];
                     } else {
                         // v_cvt_pk_bf16_f32: round to nearest even, as f2bf
                         typedef float f2 __attribute__((ext_vector_type(2)));
                         typedef __bf16 b2 __attribute__((ext_vector_type(2)));
                         const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[0], v[1]}, b2));
                         const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[2], v[3]}, b2));
+                        uint16_t* yp = Y + pbase * ldy + (unsigned)(loc * ldy + ch0);
                         if (ch0 + 4 <= cstore) {
-                            *(uint2*)(Y + pp * ldy + ch0) = make_uint2(lo, hi);
+                            *(uint2*)yp = make_uint2(lo, hi);
                             if constexpr ((FUSE & kFusePool) != 0) {
                                 // 2x2 average pool of the rounded outputs: rows mt, mt + 1
                                 // here, columns xo, xo + 1 in this lane and the next
@@ -334,7 +365,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                             const uint32_t h4[2] = {lo, hi};
 #pragma unroll
                             for (int e = 0; e < 4; ++e)
-                                if (ch0 + e < cstore) Y[pp * ldy + ch0 + e] = (uint16_t)(h4[e >> 1] >> (16 * (e & 1)));
+                                if (ch0 + e < cstore) yp[e] = (uint16_t)(h4[e >> 1] >> (16 * (e & 1)));
                         }
                     }
                 }
@@ -367,8 +398,9 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                         typedef __bf16 b2 __attribute__((ext_vector_type(2)));
                         const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{m[0], m[1]}, b2));
                         const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{m[2], m[3]}, b2));
-                        *(uint2*)(P + ((size_t)(cur.n * (H >> 1) + (yo >> 1)) * (W >> 1) + (xo >> 1)) * cstore + ch0) =
-                            make_uint2(lo, hi);
+                        const size_t qbase = (size_t)(cur.n * (H >> 1) + (cur.y0 >> 1)) * (W >> 1) + (cur.x0 >> 1);
+                        const int qloc = (2 * (wv >> 1) + mp) * (W >> 1) + (xl >> 1);
+                        *(uint2*)(P + qbase * cstore + (unsigned)(qloc * cstore + ch0)) = make_uint2(lo, hi);
                     }
                 }
             }
@@ -661,6 +693,10 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
         throw Error("conv: bad pooled output");
     const unsigned tiles = (unsigned)N * blocks(H, TY) * blocks(W, TX);
     if ((uint64_t)tiles * blocks(cstore, 16) >= (1u << 31)) throw Error("conv: too many tiles");
+    // the kernel indexes pixels, weights and rows of a tile with 32-bit integers
+    const uint64_t ld_max = std::max<uint64_t>(std::max(ldy, ldyf), std::max(ldm, ldx));
+    if ((uint64_t)N * H * W >= (1u << 31) || 9ull * Cout * Cin >= (1u << 31) || (uint64_t)TY * W * ld_max >= (1u << 31))
+        throw Error("conv: tensor too large for 32-bit indexing");
     const int fuse = (fz.X2 ? kFuseUp : 0) | (fz.P ? kFusePool : 0) | (fz.pk.rgba ? kFusePack : 0);
     if (fuse & (fuse - 1)) throw Error("conv: one fusion at a time");
     if (fuse && (M || Yf || cstore <= 16)) throw Error("conv: fusions write plain BF16 outputs of > 16 channels");
