@@ -96,6 +96,13 @@ constexpr int CBK = 32, CLD = CBK + 8;
 // cycles with rows padded to 80 bytes).
 constexpr int CCLD = CBK;
 __device__ inline int cofs(int row, int q) { return row * CCLD + ((q ^ ((row >> 1) & 3)) << 3); }
+
+// Raw buffer resource over `bytes` bytes at p (stride 0, 32-bit data format):
+// loads at offsets >= bytes return zeros and touch no memory.
+constexpr uint32_t kOOB = 0x80000000u;  // an offset past every resource built here (< 2^31 bytes)
+__device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
 constexpr int TX = 32, TY = 8, HXX = TX + 2, HYY = TY + 2, NHALO = HXX * HYY;
 
 template <int BN>
@@ -112,8 +119,12 @@ struct ConvShape {
 // kOutF32 FP32 output (Yf) instead of BF16 (Y), kMask output masked by M > 0.
 constexpr int kFuseUp = 1, kFusePool = 2, kFusePack = 4, kOutF32 = 8, kMask = 16;
 
+// Waves per SIMD the register budget is set for: 3 (168 registers) where that
+// needs no spills, else 2.
+constexpr int conv_waves(int fuse) { return (fuse & (kFusePack | kMask)) ? 2 : 3; }
+
 template <int BN, int FUSE>
-__global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int ldx, int Cin,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(FUSE), conv_waves(FUSE)))) void k_conv(const uint16_t* __restrict__ X, int ldx, int Cin,
                                              const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
                                              int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
                                              float* __restrict__ Yf, int ldyf, int cstore,
@@ -145,7 +156,7 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
         return it;
     };
     uint4 hreg[SH::kHR], wreg[SH::kWR];
-    auto load_halo = [&](const Item& it, int ci0) {
+    auto load_halo = [&](const Item& it, int ci0, bool valid) {
         if constexpr ((FUSE & kFusePack) != 0) {
             static_assert(SH::kHR >= 6, "the packed input stages 6 registers");
             // two halo pixels per thread, 3 float4 each (zero outside the image)
@@ -170,63 +181,47 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
             return;
         }
         // Halo slot r of this thread: pixel hp = (t + r TB) / 4 of the (TX + 2) x
-        // (TY + 2) halo, 16-byte channel group q.  Pixel indices are 32-bit
-        // (N H W < 2^31); a tile whose halo lies inside the image skips the
-        // bounds tests.
-        const bool inside = it.y0 >= 1 && it.y0 + TY < H && it.x0 >= 1 && it.x0 + TX < W;
+        // (TY + 2) halo, 16-byte channel group q, read through a buffer
+        // resource over the item's image: offsets past its end (pixels off the
+        // image, unused slots, or every slot when there is no next step) read
+        // zeros without a branch.
         const int y0 = it.y0 - 1, x0 = it.x0 - 1;
-        if ((FUSE & kFuseUp) != 0 && ci0 >= csplit) {  // channels of the upsampled half-resolution tensor
-            const uint16_t* src = X2 + (ci0 - csplit);
-            const int Wh = W >> 1, row0 = it.n * (H >> 1);
-#pragma unroll
-            for (int r = 0; r < SH::kHR; ++r) {
-                const int i = t + r * TB;
-                uint4 v = zero;
-                if (i < SH::kHaloU4) {
-                    const int hp = i >> 2, q = i & 3;
-                    const int hy = hp / HXX, hx = hp - hy * HXX;
-                    const int yy = y0 + hy, xx = x0 + hx;
-                    if (inside || ((unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W))
-                        v = *(const uint4*)(src + (size_t)(unsigned)((row0 + (yy >> 1)) * Wh + (xx >> 1)) * ldx2 + q * 8);
-                }
-                hreg[r] = v;
-            }
-            return;
-        }
-        const uint16_t* src = X + ci0;
-        const int pix0 = (it.n * H + y0) * W + x0;  // may be negative off the image; used only inside
+        const bool up = (FUSE & kFuseUp) != 0 && ci0 >= csplit;  // the upsampled half-resolution tensor
+        const int sh = up ? 1 : 0, Wl = W >> sh, ld = up ? ldx2 : ldx, c0 = up ? ci0 - csplit : ci0;
+        const uint32_t img = (uint32_t)(H >> sh) * (uint32_t)Wl * (uint32_t)ld;  // elements of one image
+        const uint16_t* base = (up ? X2 : X) + (size_t)it.n * img;
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(base, valid ? 2 * img : 0);
 #pragma unroll
         for (int r = 0; r < SH::kHR; ++r) {
             const int i = t + r * TB;
-            uint4 v = zero;
-            if (i < SH::kHaloU4) {
-                const int hp = i >> 2, q = i & 3;
-                const int hy = hp / HXX, hx = hp - hy * HXX;
-                if (inside || ((unsigned)(y0 + hy) < (unsigned)H && (unsigned)(x0 + hx) < (unsigned)W))
-                    v = *(const uint4*)(src + (size_t)(unsigned)(pix0 + hy * W + hx) * ldx + q * 8);
-            }
-            hreg[r] = v;
+            const int hp = i >> 2, q = i & 3;
+            const int hy = hp / HXX, hx = hp - hy * HXX;
+            const int yy = y0 + hy, xx = x0 + hx;
+            const bool ok = i < SH::kHaloU4 && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W;
+            const uint32_t off = ok ? 2u * (uint32_t)(((yy >> sh) * Wl + (xx >> sh)) * ld + c0 + q * 8) : kOOB;
+            hreg[r] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
         }
     };
     float breg = 0.0f;
     auto load_w = [&](int co0, int ci0) {
-        if (t < BN) breg = (bias && co0 + t < Cout) ? bias[co0 + t] : 0.0f;
+        const __amdgpu_buffer_rsrc_t rb = buffer_rsrc(bias, bias ? 4u * Cout : 0u);
+        const bool okb = t < BN && co0 + t < Cout;
+        breg = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rb, okb ? 4u * (co0 + t) : kOOB, 0, 0));
+        const __amdgpu_buffer_rsrc_t rw = buffer_rsrc(Wt, 18u * Cout * Cin);
 #pragma unroll
         for (int r = 0; r < SH::kWR; ++r) {
             const int i = t + r * TB;
-            uint4 v = zero;
-            if (i < SH::kWU4) {  // 9 Cout Cin < 2^31
-                const int q = i & 3, rr = (i >> 2) % BN, tap = (i >> 2) / BN;
-                const int co = co0 + rr;
-                if (co < Cout) v = *(const uint4*)(Wt + (unsigned)((tap * Cout + co) * Cin + ci0 + q * 8));
-            }
-            wreg[r] = v;
+            const int q = i & 3, rr = (i >> 2) % BN, tap = (i >> 2) / BN;
+            const int co = co0 + rr;
+            const bool ok = i < SH::kWU4 && co < Cout;
+            const uint32_t off = ok ? 2u * (uint32_t)((tap * Cout + co) * Cin + ci0 + q * 8) : kOOB;
+            wreg[r] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rw, off, 0, 0));
         }
     };
     int i = blockIdx.x, c = 0;
     if (i >= n_items) return;
     Item cur = decode(i);
-    load_halo(cur, 0);
+    load_halo(cur, 0, true);
     load_w(cur.co0, 0);
     bool need_w = true;
     f32x4 acc[4][BN / 16];
@@ -269,21 +264,15 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
             }
         }
         __syncthreads();
-        // the next step's loads go out now and land during the MFMAs below
+        // the next step; its loads go out between the MFMAs below
         int ni = i, nc = c + 1;
         if (nc == nchunk) {
             nc = 0;
             ni = i + gridDim.x;
         }
         const bool has_next = ni < n_items;
-        Item nxt = cur;
-        bool next_w = false;
-        if (has_next) {
-            nxt = decode(ni);
-            load_halo(nxt, nc * CBK);
-            next_w = nxt.co0 != cur.co0 || nc != c;
-            if (next_w) load_w(nxt.co0, nc * CBK);
-        }
+        const Item nxt = has_next ? decode(ni) : cur;
+        const bool next_w = has_next && (nxt.co0 != cur.co0 || nc != c);
         if (c == 0) {
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
@@ -314,6 +303,14 @@ __global__ __launch_bounds__(TB) void k_conv(const uint16_t* __restrict__ X, int
                         acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[mt + dy], acc[mt][nb], 0, 0, 0);
                 }
             }
+            if (dx == 0) {
+                if constexpr ((FUSE & kFusePack) != 0) {
+                    if (has_next) load_halo(nxt, nc * CBK, true);
+                } else {
+                    load_halo(nxt, nc * CBK, has_next);
+                }
+            }
+            if (dx == 1 && next_w) load_w(nxt.co0, nc * CBK);
         }
         if (c == nchunk - 1) {
             // Epilogue straight from the accumulators: lane holds channels
@@ -695,8 +692,9 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
     if ((uint64_t)tiles * blocks(cstore, 16) >= (1u << 31)) throw Error("conv: too many tiles");
     // the kernel indexes pixels, weights and rows of a tile with 32-bit integers
     const uint64_t ld_max = std::max<uint64_t>(std::max(ldy, ldyf), std::max(ldm, ldx));
-    if ((uint64_t)N * H * W >= (1u << 31) || 9ull * Cout * Cin >= (1u << 31) || (uint64_t)TY * W * ld_max >= (1u << 31))
-        throw Error("conv: tensor too large for 32-bit indexing");
+    if ((uint64_t)N * H * W >= (1u << 31) || 18ull * Cout * Cin >= (1u << 31) || (uint64_t)TY * W * ld_max >= (1u << 31) ||
+        2ull * H * W * ldx >= (1u << 31) || (fz.X2 && 2ull * (H / 2) * (W / 2) * fz.ldx2 >= (1u << 31)))
+        throw Error("conv: tensor too large for 32-bit indexing (2 GiB per image)");
     const int fuse = (fz.X2 ? kFuseUp : 0) | (fz.P ? kFusePool : 0) | (fz.pk.rgba ? kFusePack : 0);
     if (fuse & (fuse - 1)) throw Error("conv: one fusion at a time");
     if (fuse && (M || Yf || cstore <= 16)) throw Error("conv: fusions write plain BF16 outputs of > 16 channels");
