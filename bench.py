@@ -56,6 +56,37 @@ def relmse(x, y):
     return float(np.mean((x.astype(np.float64) - y) ** 2 / (y.astype(np.float64) ** 2 + 1e-2)))
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+
+
+def unet_flops(height, width, n_images=1):
+    """Network FLOPs of one denoise (true channel counts: 9 inputs, 3 outputs;
+    the kernels pad to 32 / 16 channels): 2 x 9 x Cin x Cout per pixel and
+    3x3 convolution, levels at the padded size (multiple of 16) >> level."""
+    ch = (32, 64, 96, 128, 160)
+    hp, wp = -(-height // 16) * 16, -(-width // 16) * 16
+    px = [(hp >> l) * (wp >> l) for l in range(5)]
+    macs, cin = 0, 9
+    for l, c in enumerate(ch):  # encoder: cin -> c, c -> c
+        macs += px[l] * (cin * c + c * c)
+        cin = c
+    for l in (3, 2, 1, 0):  # decoder: concat(skip, up) -> c, c -> c
+        macs += px[l] * ((ch[l] + ch[l + 1]) * ch[l] + ch[l] * ch[l])
+    macs += px[0] * ch[0] * 3  # output layer
+    return 2 * 9 * macs * n_images
+
+
+def unet_rate(den, rd, width, height, n_images, reps=5):
+    """Kernel time (HIP events) of one batched denoise of the renderer's
+    targets and the MFMA throughput it implies."""
+    import numpy as np
+    ptr = [rd.device_target(k) for k in range(3)]
+    ms = [den.infer_device(ptr[0], ptr[1], ptr[2], width, height, ptr[0], n_images=n_images) for _ in range(reps)]
+    k = float(np.median(ms))
+    tf = unet_flops(height, width, n_images) / (k * 1e-3) / 1e12
+    return {"kernel_ms": round(k, 3), "tflops": round(tf, 1), "frac_of_bf16_peak": round(tf / MFMA_BF16_PEAK_TFLOPS, 3)}
+
+
 def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     """Wall time to reach relMSE <= target against a high-spp reference of the
     same frame: plain tracer (spp doubling) vs tracer + denoiser."""
@@ -110,7 +141,8 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     hit_p = [p for p in plain if p["relmse"] <= target]
     hit_d = [d for d in dn if d["relmse"] <= target]
     res = {"target_relmse": target, "reference": "GPU FP32 tracer, %d spp, same frame" % ref_spp,
-           "plain": plain, "denoised": dn, "denoiser_by_spp": table}
+           "plain": plain, "denoised": dn, "denoiser_by_spp": table,
+           "unet": unet_rate(den, rd, w.width, w.height, 1)}
     if hit_p and hit_d:
         res["speedup"] = round(hit_p[0]["ms"] / hit_d[0]["ms"], 2)
     return res
@@ -165,7 +197,8 @@ def batched_environments(frames=8, schedule=None):
            "render_ms_p50": round(float(np.percentile(rend, 50)), 2),
            "denoise_ms_p50": round(float(np.percentile(dms, 50)), 2), "frames": frames,
            "schedule": {k: tune[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce")},
-           "autotune_ms": [round(tune["default_ms"], 1), round(tune["frame_ms"], 1)]}
+           "autotune_ms": [round(tune["default_ms"], 1), round(tune["frame_ms"], 1)],
+           "unet": unet_rate(den, rd, w.width, w.env_height, w.n_env)}
     gs.close()
     rd.close()
     return res
