@@ -89,7 +89,8 @@ def unet_rate(den, rd, width, height, n_images, reps=5):
 
 def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     """Wall time to reach relMSE <= target against a high-spp reference of the
-    same frame: plain tracer (spp doubling) vs tracer + denoiser."""
+    same frame: plain tracer (spp doubling) vs tracer + denoiser; each point
+    the median of 3 frames (the same seed: the last frame is the one scored)."""
     import numpy as np
     from kuafu_amd.denoiser import Denoiser
     pc = w.pc
@@ -100,27 +101,43 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
         rd.render(gs, w.cam, pc, seed=900000 + f)
     ref = rd.read(0)[..., :3].astype(np.float64).copy()
     pc.frame_count = 0
+
+    def timed(fn, reps=3):  # median wall time of reps calls (the last call's result stays)
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
     plain = []
     spp = 1
     while spp <= 1024:
         pc.spp = spp
-        t = time.perf_counter()
-        rd.render(gs, w.cam, pc, seed=31 * spp + 1)
-        dt = time.perf_counter() - t
+        dt = timed(lambda: rd.render(gs, w.cam, pc, seed=31 * spp + 1))
         e = relmse(rd.read(0)[..., :3], ref)
         plain.append({"spp": spp, "ms": round(dt * 1e3, 2), "relmse": round(e, 5)})
         if e <= target:
             break
         spp *= 2
+    # the denoised frames are few-sample frames: the schedule is tuned for them
+    # (the plain sweep keeps the schedule tuned at 16 spp, whose batches it
+    # reaches its target with)
+    pc.spp = 1
+    tune1 = rd.autotune(gs, w.cam, pc)
     den = Denoiser()
+    rd.render(gs, w.cam, pc, seed=5)
+    den.apply(rd)  # warm-up: the first call allocates the activations
     dn = []
     spp = 1
     while spp <= 64:
         pc.spp = spp
-        t = time.perf_counter()
-        rd.render(gs, w.cam, pc, seed=37 * spp + 3)
-        den.apply(rd)
-        dt = time.perf_counter() - t
+
+        def frame():
+            rd.render(gs, w.cam, pc, seed=37 * spp + 3)
+            den.apply(rd)
+
+        dt = timed(frame)
         e = relmse(rd.read(0)[..., :3], ref)
         dn.append({"spp": spp, "ms": round(dt * 1e3, 2), "relmse": round(e, 5)})
         if e <= target:
@@ -142,7 +159,8 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     hit_d = [d for d in dn if d["relmse"] <= target]
     res = {"target_relmse": target, "reference": "GPU FP32 tracer, %d spp, same frame" % ref_spp,
            "plain": plain, "denoised": dn, "denoiser_by_spp": table,
-           "unet": unet_rate(den, rd, w.width, w.height, 1)}
+           "unet": unet_rate(den, rd, w.width, w.height, 1),
+           "denoised_schedule": {k: tune1[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce")}}
     if hit_p and hit_d:
         res["speedup"] = round(hit_p[0]["ms"] / hit_d[0]["ms"], 2)
     return res

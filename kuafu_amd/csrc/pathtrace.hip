@@ -1308,12 +1308,12 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
         const std::vector<uint32_t> grabs = {64u, 128u, 256u, 512u, 1024u};
         std::vector<uint32_t> refills = {8u, 16u, 32u, 48u};
         if (std::find(refills.begin(), refills.end(), b_refill) == refills.end()) refills.push_back(b_refill);
-        // the bounce from which k_tail finishes the paths: never (0), or
-        // from the first bounces after Russian roulette starts
+        // the bounce from which k_tail finishes the paths: never (0), or from
+        // bounce 2 up to the first bounces after Russian roulette starts (few
+        // paths per frame favour an early tail: at 1 spp of the 1080p scene
+        // bounce 3 beats bounce 5 by 8 %)
         std::vector<uint32_t> tails = {0u};
-        for (uint32_t tb = std::max<uint32_t>(1u, p.rr_min_bounces); tb < std::min<uint32_t>(p.path_depth + 1u,
-                                                                                                 p.rr_min_bounces + 4u);
-             ++tb)
+        for (uint32_t tb = 2u; tb < std::min<uint32_t>(p.path_depth + 1u, p.rr_min_bounces + 4u); ++tb)
             tails.push_back(tb);
         sweep(spbs, b_spb, 0);
         sweep(grabs, b_grab, 2);
