@@ -191,13 +191,19 @@ struct PrivStack {  // everything in private memory
 // slots its lanes use); older entries spill to private memory.  Private
 // stores dirty whole cache lines that end up written back to HBM, so the
 // ring keeps the common shallow stack off the memory system entirely.
+// The ring pointer is typed as an LDS (address space 3) pointer and the spill
+// array lives outside the struct: with both in one struct the whole struct
+// stayed in scratch, so every step reloaded the ring pointer from scratch and
+// reached the ring through flat loads and stores (which wait on every
+// outstanding global load as well).
 constexpr int kRing = 16;                 // power of two
 constexpr uint32_t kRingThreads = 256;    // block size of kernels using RingStack
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 struct RingStack {
     static constexpr int kCap = kRing + kStack;
-    uint32_t* ring;  // this thread's column: slot k at ring[k * kRingThreads]
-    uint32_t spill[kStack];
-    KF_DEV uint32_t& slot(int i) const { return ring[(uint32_t)(i & (kRing - 1)) * kRingThreads]; }
+    lds_u32* ring;    // this thread's column: slot k at ring[k * kRingThreads]
+    uint32_t* spill;  // kStack entries in private memory
+    KF_DEV lds_u32& slot(int i) const { return ring[(uint32_t)(i & (kRing - 1)) * kRingThreads]; }
     KF_DEV uint32_t peek(int sp) const { return slot(sp - 1); }
     KF_DEV void push(int sp, uint32_t v, uint32_t& overflow) {
         if (sp >= kCap) {
@@ -527,8 +533,10 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
     Trav<ANY, COUNT> T;
     T.reset();
     __shared__ uint32_t ring[kRing * kRingThreads];
+    uint32_t spill[kStack];
     RingStack stk;
-    stk.ring = ring + threadIdx.x;
+    stk.ring = (lds_u32*)(ring + threadIdx.x);
+    stk.spill = spill;
     bool live = false;
     uint32_t payload = 0;
     const uint32_t lane = threadIdx.x & 63u;
