@@ -21,8 +21,12 @@ def bf(x):
     return from_bf16_bits(to_bf16_bits(x)).astype(np.float64)
 
 
+# The kernel works on 32 x 8-pixel tiles; a tile whose 34 x 10 halo lies
+# inside the image takes a path without bounds tests, so the larger shapes
+# (24 x 72, 20 x 100) include such interior tiles besides edge tiles.
 @pytest.mark.parametrize("cin,cout,h,w", [(32, 32, 8, 8), (32, 64, 6, 10), (96, 32, 5, 7), (288, 128, 4, 4),
-                                          (64, 96, 9, 3), (32, 3, 7, 5), (160, 160, 3, 3)])
+                                          (64, 96, 9, 3), (32, 3, 7, 5), (160, 160, 3, 3), (32, 32, 24, 72),
+                                          (64, 32, 20, 100)])
 def test_conv_forward(gpu, cin, cout, h, w):
     rng = np.random.default_rng(cin + cout)
     x = bf(rng.normal(0, 1, (2, h, w, cin)))
@@ -36,7 +40,7 @@ def test_conv_forward(gpu, cin, cout, h, w):
 
 
 @pytest.mark.parametrize("cin,cout,h,w", [(32, 32, 8, 8), (64, 32, 6, 10), (96, 64, 5, 7), (288, 128, 4, 4),
-                                          (32, 160, 3, 5)])
+                                          (32, 160, 3, 5), (64, 32, 24, 72)])
 def test_conv_backward(gpu, cin, cout, h, w):
     rng = np.random.default_rng(3 * cin + cout)
     x = bf(rng.normal(0, 1, (2, h, w, cin)))
@@ -112,6 +116,19 @@ def test_network_gradients_match_reference(gpu):
     assert np.abs(out - ro).max() <= 1e-2 * np.abs(ro).max()
 
 
+def test_network_forward_interior_tiles(gpu):
+    # 96 x 160: interior tiles at levels 0 and 1, where the fused pooling
+    # (encoder) and upsampling (decoder) run
+    d = Denoiser(random_seed=0)
+    p = he_params(5)
+    d.set_params(p)
+    inp, _ = batch(6, n=1, h=96, w=160)
+    net = U.Net(p, bf16=True)
+    ro = net.output(net.forward(inp))
+    out = d.forward(inp)
+    assert np.abs(out - ro).max() <= 1e-2 * np.abs(ro).max()
+
+
 def test_training_reduces_loss_and_save_load(gpu, tmp_path):
     d = Denoiser(random_seed=3)
     inp, tgt = batch(5, n=4, h=32, w=48)
@@ -126,9 +143,12 @@ def test_training_reduces_loss_and_save_load(gpu, tmp_path):
     assert np.allclose(e.forward(inp), Denoiser(weights=f).forward(inp))
 
 
-def test_denoise_stage_on_render_targets(gpu):
+@pytest.mark.parametrize("width,height", [(64, 48), (160, 96)])
+def test_denoise_stage_on_render_targets(gpu, width, height):
+    # the render-target path (first convolution reading the float4 targets)
+    # against the record path; 160 x 96 has interior tiles at levels 0 and 1
     from kuafu_amd import GpuScene, Renderer, scenes
-    w = scenes.config1(64, 48, spp=1)
+    w = scenes.config1(width, height, spp=1)
     r = Renderer(w.width, w.height)
     r.render(GpuScene(w.data), w.cam, w.pc)
     before = r.read(0)
@@ -137,6 +157,7 @@ def test_denoise_stage_on_render_targets(gpu):
     after = r.read(0)
     assert np.isfinite(after).all() and (after[..., 3] == 1).all()
     # against the record path on the same inputs (16-aligned crop of the frame)
-    rec = np.concatenate([before[..., :3], r.read(1)[..., :3], r.read(2)[..., :3]], -1)[None, :48, :64]
+    hc, wc = height // 16 * 16, width // 16 * 16
+    rec = np.concatenate([before[..., :3], r.read(1)[..., :3], r.read(2)[..., :3]], -1)[None, :hc, :wc]
     ref = d.forward(np.ascontiguousarray(rec))
-    assert np.abs(ref[0] - after[:48, :64, :3]).max() <= 1e-3 * max(1.0, np.abs(ref).max())
+    assert np.abs(ref[0] - after[:hc, :wc, :3]).max() <= 1e-3 * max(1.0, np.abs(ref).max())
