@@ -407,7 +407,19 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
         v3 worldPos = xyz(ro) + rdir * h.x;
         float uvx = a2v.y * bx + b2v.y * by + c2v.y * bz;
         float uvy = a2v.z * bx + b2v.z * by + c2v.z * bz;
-        const kf_material& m = S.mats[S.matidx[tri]];
+        // the whole 80-byte material in one round trip (five 16-byte loads
+        // issued together), not field by field behind the emission test
+        kf_material m;
+        {
+            const gf4* mp = (const gf4*)(S.mats + S.matidx[tri]);
+            const vf4 m0 = mp[0], m1 = mp[1], m2 = mp[2], m3 = mp[3], m4 = mp[4];
+            // the empty asm needs all five: the loads are not sunk into the
+            // branches below (each sunk load is another round trip)
+            asm volatile("" ::"v"(m0), "v"(m1), "v"(m2), "v"(m3), "v"(m4));
+            const vf4 mv[5] = {m0, m1, m2, m3, m4};
+            static_assert(sizeof(kf_material) == sizeof(mv), "material record is 80 bytes");
+            __builtin_memcpy(&m, mv, sizeof(m));
+        }
         v3 emission = mk(m.emission[0], m.emission[1], m.emission[2]) * m.emission[3];
         if (!all_zero(emission)) {
             em[p] = f4(emission, __uint_as_float(1u));
