@@ -261,7 +261,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this path runs on one GPU; --gpus %d is not supported (see DESIGN.md for sharding)"
+        print("bench.py: this path runs on one GPU; --gpus %d is not supported (rendering would shard by image "
+              "rows or environments with the scene replicated, denoiser training by data parallelism)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
