@@ -405,10 +405,11 @@ class Renderer:
         t = abi.TuneResult()
         check(lib().kf_renderer_autotune(self.h, scene.h, len(cams), arr, C.byref(pc), C.byref(t)))
         return dict(samples_per_batch=t.samples_per_batch, refill=t.refill, grab=t.grab, tail_bounce=t.tail_bounce,
-                    frame_ms=t.frame_ms, default_ms=t.default_ms, trials=t.trials)
+                    grab0=t.grab0, frame_ms=t.frame_ms, default_ms=t.default_ms, trials=t.trials)
 
-    def set_schedule(self, samples_per_batch=0, refill=0, grab=0, tail_bounce=0):
-        check(lib().kf_renderer_set_schedule(self.h, samples_per_batch, refill, grab, tail_bounce))
+    def set_schedule(self, samples_per_batch=0, refill=0, grab=0, tail_bounce=0, grab0=0):
+        """grab0: reservation per wave for the camera rays (0: grab)."""
+        check(lib().kf_renderer_set_schedule(self.h, samples_per_batch, refill, grab, tail_bounce, grab0))
 
     def read(self, target=0):
         out = np.zeros((self.h_, self.w, 4), np.float32)
