@@ -3,8 +3,11 @@
 // normal +X, the [-1,1]^3 cube, the 50x50 latitude-longitude sphere about the
 // X axis, the capsule along X), written as parametric generators.  Every
 // triangle is wound so that cross(v1 - v0, v2 - v0) points out of the surface.
+#include <cctype>
 #include <cmath>
+#include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <sstream>
 #include <stdexcept>
@@ -184,13 +187,107 @@ std::shared_ptr<Geometry> createCapsule(float halfLength, float radius, bool dyn
     return finish(g, dynamic, std::move(mat));
 }
 
+// STL, binary (80-byte header, u32 facet count, 50 bytes per facet: normal,
+// three vertices, u16 attribute) or ASCII (solid / facet normal / outer loop /
+// vertex x3 / endloop / endfacet / endsolid).  As assimp imports it for the
+// reference's loadScene (geometry.cpp:46-232): one mesh per solid, three
+// vertices of its own per facet carrying the facet normal (the import flags do
+// not join identical vertices, and GenNormals keeps normals a file has), no
+// UVs, and one default material whose only property is a diffuse colour of
+// 0.6 grey; the reference turns it into diffuse 0.6, specular 0.5, IOR 1.4,
+// roughness 0 (no shininess and not an OBJ file), opaque.  Colours that some
+// binary exporters pack into the attribute bytes are not read.
+static std::vector<std::shared_ptr<Geometry>> loadStl(const std::string& path, bool dynamic) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Failed to load scene: cannot open " + path);
+    std::string data((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    std::vector<std::vector<glm::vec3>> solids;  // per solid: normal, v0, v1, v2 per facet
+    uint32_t n = 0;
+    if (data.size() >= 84) std::memcpy(&n, data.data() + 80, 4);
+    if (data.size() >= 84 && data.size() == 84 + 50ull * n) {
+        solids.emplace_back();
+        for (uint32_t f = 0; f < n; ++f) {
+            float v[12];
+            std::memcpy(v, data.data() + 84 + 50ull * f, sizeof(v));
+            for (int k = 0; k < 4; ++k) solids.back().push_back(glm::vec3(v[3 * k], v[3 * k + 1], v[3 * k + 2]));
+        }
+    } else {
+        std::istringstream ss(data);
+        std::string tok;
+        if (!(ss >> tok) || tok != "solid") throw std::runtime_error("Failed to load scene: not an STL file, " + path);
+        solids.emplace_back();
+        glm::vec3 nrm(0.0f);
+        int nv = 0;
+        while (ss >> tok) {
+            if (tok == "solid") {
+                solids.emplace_back();
+            } else if (tok == "facet") {
+                ss >> tok;  // "normal"
+                if (tok != "normal" || !(ss >> nrm.x >> nrm.y >> nrm.z))
+                    throw std::runtime_error("Failed to load scene: bad facet in " + path);
+                solids.back().push_back(nrm);
+                nv = 0;
+            } else if (tok == "vertex") {
+                glm::vec3 p;
+                if (solids.back().empty() || !(ss >> p.x >> p.y >> p.z) || nv >= 3)
+                    throw std::runtime_error("Failed to load scene: bad vertex in " + path);
+                solids.back().push_back(p);
+                ++nv;
+            } else if (tok == "endfacet") {
+                if (nv != 3) throw std::runtime_error("Failed to load scene: facet without 3 vertices in " + path);
+            }
+        }
+        for (auto& sd : solids)  // a facet cut off by the end of the file
+            if (sd.size() % 4) throw std::runtime_error("Failed to load scene: truncated facet in " + path);
+    }
+    NiceMaterial m;
+    m.diffuseColor = glm::vec3(0.6f);
+    m.specular = 0.5f;
+    m.ior = 1.4f;
+    m.roughness = 0.0f;
+    m.metallic = 0.0f;
+    m.transmission = 0.0f;
+    m.alpha = 1.0f;
+    m.emission = glm::vec3(0.0f);
+    m.emissionStrength = 1.0f;
+    std::vector<std::shared_ptr<Geometry>> out;
+    uint32_t mat = ~0u;
+    for (auto& sd : solids) {
+        if (sd.size() < 4) continue;  // a solid without facets has no mesh
+        auto g = std::make_shared<Geometry>();
+        for (size_t f = 0; f + 4 <= sd.size(); f += 4)
+            for (int k = 1; k <= 3; ++k) {
+                Vertex vx{};
+                vx.pos = sd[f + k];
+                vx.normal = sd[f];
+                g->indices.push_back((uint32_t)g->vertices.size());
+                g->vertices.push_back(vx);
+            }
+        g->initialized = false;
+        g->dynamic = dynamic;
+        g->isOpaque = true;
+        if (mat == ~0u) {  // one material for the file, shared by its solids
+            mat = global::materialIndex++;
+            global::materials.push_back(m);
+        }
+        g->matIndex.assign(g->indices.size() / 3, mat);
+        g->path = path;
+        out.push_back(std::move(g));
+    }
+    if (out.empty()) throw std::runtime_error("A mesh in the file has no triangles: " + path);
+    return out;
+}
+
 // Wavefront OBJ: v / vt / vn / f (polygons fanned), UVs flipped vertically and
 // missing normals generated, like the reference's assimp flags
-// (Triangulate | GenNormals | FlipUVs, geometry.cpp:52-54).
+// (Triangulate | GenNormals | FlipUVs, geometry.cpp:52-54).  STL: loadStl.
 std::vector<std::shared_ptr<Geometry>> loadScene(std::string_view fname, bool dynamic) {
     std::string path(fname);
-    if (path.size() < 4 || path.substr(path.size() - 4) != ".obj")
-        throw std::runtime_error("Failed to load scene: only .obj is supported in this build, " + path);
+    std::string ext = path.size() >= 4 ? path.substr(path.size() - 4) : std::string();
+    for (auto& ch : ext) ch = (char)std::tolower((unsigned char)ch);
+    if (ext == ".stl") return loadStl(path, dynamic);
+    if (ext != ".obj")
+        throw std::runtime_error("Failed to load scene: only .obj and .stl are supported in this build, " + path);
     std::ifstream in(path);
     if (!in) throw std::runtime_error("Failed to load scene: cannot open " + path);
     std::vector<glm::vec3> P, N;

@@ -115,6 +115,52 @@ static int cpu_checks() {
     CHECK(std::fabs(obj->vertices[0].normal.z - 1.0f) < 1e-6f && obj->vertices[2].texCoord.y == 0.0f);
     CHECK(throws([&] { loadObj("/tmp/does_not_exist.obj"); }));
 
+    // STL import: ASCII (two solids -> two meshes, one shared material) and
+    // binary (80-byte header, facet count, 50-byte facets); three vertices of
+    // its own per facet carrying the facet normal; assimp's default material
+    {
+        std::ofstream f("/tmp/kf_facade_test.stl");
+        f << "solid a\n facet normal 0 0 1\n  outer loop\n   vertex 0 0 0\n   vertex 1 0 0\n   vertex 1 1 0\n"
+             "  endloop\n endfacet\n facet normal 0 0 1\n  outer loop\n   vertex 0 0 0\n   vertex 1 1 0\n"
+             "   vertex 0 1 0\n  endloop\n endfacet\nendsolid a\nsolid b\n facet normal 1 0 0\n  outer loop\n"
+             "   vertex 2 0 0\n   vertex 2 1 0\n   vertex 2 0 1\n  endloop\n endfacet\nendsolid b\n";
+    }
+    auto stl = loadScene("/tmp/kf_facade_test.stl", false);
+    CHECK(stl.size() == 2 && stl[0]->vertices.size() == 6 && stl[0]->indices.size() == 6);
+    CHECK(stl[1]->vertices.size() == 3 && stl[1]->vertices[2].pos.z == 1.0f && stl[1]->vertices[0].normal.x == 1.0f);
+    CHECK(stl[0]->vertices[4].pos.x == 1.0f && stl[0]->vertices[4].pos.y == 1.0f && stl[0]->vertices[5].normal.z == 1.0f);
+    CHECK(stl[0]->matIndex.size() == 2 && stl[0]->matIndex[0] == stl[1]->matIndex[0] && stl[0]->isOpaque);
+    {
+        const NiceMaterial& m = global::materials[stl[0]->matIndex[0]];
+        CHECK(m.diffuseColor == glm::vec3(0.6f) && m.roughness == 0.0f && m.specular == 0.5f && m.ior == 1.4f &&
+              m.emissionStrength == 1.0f && m.emission == glm::vec3(0.0f));
+    }
+    {
+        std::ofstream f("/tmp/kf_facade_test_bin.STL", std::ios::binary);
+        char header[80] = "solid binary files may start with solid too";
+        f.write(header, 80);
+        const uint32_t n = 1;
+        f.write((const char*)&n, 4);
+        const float v[12] = {0, 1, 0, 0, 0, 0, 0, 0, 1, 1, 0, 0};  // normal, then three vertices
+        const uint16_t attr = 0;
+        f.write((const char*)v, sizeof(v));
+        f.write((const char*)&attr, 2);
+    }
+    auto bin = loadObj("/tmp/kf_facade_test_bin.STL");
+    CHECK(bin->vertices.size() == 3 && bin->indices.size() == 3 && bin->vertices[0].normal.y == 1.0f);
+    CHECK(bin->vertices[1].pos.z == 1.0f && bin->vertices[2].pos.x == 1.0f);
+    {
+        std::ofstream f("/tmp/kf_facade_test_bad.stl");
+        f << "solid c\n facet normal 0 0 1\n  outer loop\n   vertex 0 0 0\n   vertex 1 0 0\n  endloop\n endfacet\n";
+    }
+    CHECK(throws([&] { loadScene("/tmp/kf_facade_test_bad.stl", false); }));
+    {
+        std::ofstream f("/tmp/kf_facade_test_cut.stl");
+        f << "solid d\n facet normal 0 0 1\n  outer loop\n   vertex 0 0 0\n   vertex 1 0 0\n";
+    }
+    CHECK(throws([&] { loadScene("/tmp/kf_facade_test_cut.stl", false); }));
+    CHECK(throws([&] { loadScene("/tmp/kf_facade_test.ply", false); }));
+
     // KTX environment map (KTX 1: identifier, 13 u32 header fields, no
     // key/value data, image size, 6 faces of 1x1 RGBA8)
     {
