@@ -160,7 +160,7 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     res = {"target_relmse": target, "reference": "GPU FP32 tracer, %d spp, same frame" % ref_spp,
            "plain": plain, "denoised": dn, "denoiser_by_spp": table,
            "unet": unet_rate(den, rd, w.width, w.height, 1),
-           "denoised_schedule": {k: tune1[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0")}}
+           "denoised_schedule": {k: tune1[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow")}}
     if hit_p and hit_d:
         res["speedup"] = round(hit_p[0]["ms"] / hit_d[0]["ms"], 2)
     return res
@@ -182,7 +182,7 @@ def batched_environments(frames=8, schedule=None):
     pc.frame_count = 0
     if schedule:
         rd.set_schedule(*schedule)
-        tune = dict(zip(("samples_per_batch", "refill", "grab", "tail_bounce", "grab0"), schedule), default_ms=0.0,
+        tune = dict(zip(("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow"), schedule), default_ms=0.0,
                     frame_ms=0.0)
     else:
         tune = rd.autotune(gs, w.cam, pc)
@@ -214,7 +214,7 @@ def batched_environments(frames=8, schedule=None):
            "frame_ms_p99": round(float(np.percentile(lat, 99)), 2),
            "render_ms_p50": round(float(np.percentile(rend, 50)), 2),
            "denoise_ms_p50": round(float(np.percentile(dms, 50)), 2), "frames": frames,
-           "schedule": {k: tune[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0")},
+           "schedule": {k: tune[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow")},
            "autotune_ms": [round(tune["default_ms"], 1), round(tune["frame_ms"], 1)],
            "unet": unet_rate(den, rd, w.width, w.env_height, w.n_env)}
     gs.close()
@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--no-relmse", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--schedule", default="",
-                    help="samples_per_batch,refill,grab[,tail_bounce[,grab0]] instead of autotuning")
+                    help="samples_per_batch,refill,grab[,tail_bounce[,grab0[,grab_shadow]]] instead of autotuning")
     ap.add_argument("--envs-only", action="store_true",
                     help="only the 128-environment workload (frames = --steps), e.g. for profiling")
     ap.add_argument("--width", type=int, default=1920)
@@ -272,8 +272,8 @@ def main():
 
     check(lib().kf_init(0))
     if args.envs_only:
-        sched = [int(x) for x in args.schedule.split(",")] + [0] * 5 if args.schedule else None
-        print(json.dumps(batched_environments(frames=args.steps, schedule=sched[:5] if sched else None)))
+        sched = [int(x) for x in args.schedule.split(",")] + [0] * 6 if args.schedule else None
+        print(json.dumps(batched_environments(frames=args.steps, schedule=sched[:6] if sched else None)))
         return
     w = scenes.config2(width=args.width, height=args.height)
     print("bench: building scene (%d triangles)" % w.data.n_triangles(), file=sys.stderr, flush=True)
@@ -287,9 +287,9 @@ def main():
     # timing frames of this scene, before anything is measured
     tune = None
     if args.schedule:
-        v = [int(x) for x in args.schedule.split(",")] + [0, 0]
-        rd.set_schedule(v[0], v[1], v[2], v[3], v[4])
-        tune = {"samples_per_batch": v[0], "refill": v[1], "grab": v[2], "tail_bounce": v[3], "grab0": v[4]}
+        v = [int(x) for x in args.schedule.split(",")] + [0, 0, 0]
+        rd.set_schedule(*v[:6])
+        tune = dict(zip(("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow"), v[:6]))
     elif not args.no_autotune and not args.samples_per_batch:
         print("bench: autotune", file=sys.stderr, flush=True)
         tune = rd.autotune(gs, w.cam, pc)
@@ -342,7 +342,8 @@ def main():
                    "path_depth": pc.path_depth, "triangles": int(info.triangles), "instances": int(info.instances),
                    "rays_per_frame": rays // args.steps,
                    "schedule": None if tune is None else {k: tune[k] for k in ("samples_per_batch", "refill", "grab",
-                                                                                 "tail_bounce", "grab0")},
+                                                                                 "tail_bounce", "grab0",
+                                                                                 "grab_shadow")},
                    "kernel_mrays_s": round(rays / (trace_ms * 1e-3) / 1e6, 2)},
         "roofline": roof,
     }

@@ -357,15 +357,17 @@ typedef struct kf_tune_result {
     uint32_t trials;   /* frames timed */
     uint32_t tail_bounce; /* bounce from which one thread carries each path to its end (0: never) */
     uint32_t grab0;       /* queries reserved per wave for the camera rays (0: grab) */
+    uint32_t grab_shadow; /* ... for the shadow rays (0: grab) */
 } kf_tune_result;
 int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_camera_ubo* cams,
                          const kf_push_constants* pc, kf_tune_result* out);
 /* Set the schedule directly (e.g. one kf_renderer_autotune picked earlier);
  * 0 keeps the built-in default of that parameter (for tail_bounce: the
- * wavefront runs every bounce; for grab0, the camera rays' reservation:
- * grab).  refill is clamped to [1, 64], grab and grab0 to [16, 4096]. */
+ * wavefront runs every bounce; for grab0 and grab_shadow, the camera rays'
+ * and the shadow rays' reservations: grab).  refill is clamped to [1, 64],
+ * the reservations to [16, 4096]. */
 int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_batch, uint32_t refill, uint32_t grab,
-                             uint32_t tail_bounce, uint32_t grab0);
+                             uint32_t tail_bounce, uint32_t grab0, uint32_t grab_shadow);
 
 /* Read a render target: 0 = rgba, 1 = albedo, 2 = normal; W*H*4 floats. */
 int kf_renderer_read(kf_renderer* r, int target, float* out);

@@ -93,7 +93,7 @@ class RenderStats(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("samples_per_batch", C.c_uint32), ("refill", C.c_uint32), ("grab", C.c_uint32),
                 ("frame_ms", C.c_float), ("default_ms", C.c_float), ("trials", C.c_uint32),
-                ("tail_bounce", C.c_uint32), ("grab0", C.c_uint32)]
+                ("tail_bounce", C.c_uint32), ("grab0", C.c_uint32), ("grab_shadow", C.c_uint32)]
 
 
 class RenderOpts(C.Structure):
@@ -151,7 +151,7 @@ EXPORTS = [
     ("kf_render_envs", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p]),
     ("kf_renderer_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
-    ("kf_renderer_set_schedule", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("kf_renderer_set_schedule", C.c_int, [C.c_void_p] + [C.c_uint32] * 6),
     ("kf_renderer_read", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_write", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_read_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),

@@ -235,7 +235,7 @@ struct Renderer {
     std::vector<uint8_t> graph_key;
     uint32_t graph_ext = 0, graph_sh = 0;      // traversal launches in the graph
     // Schedule picked by kf_renderer_autotune (0: built-in default).
-    uint32_t tune_spb = 0, tune_refill = 0, tune_grab = 0, tune_tail = 0, tune_grab0 = 0;
+    uint32_t tune_spb = 0, tune_refill = 0, tune_grab = 0, tune_tail = 0, tune_grab0 = 0, tune_grabs = 0;
     uint32_t n_env_last = 1;                   // environments of the last frame (the denoiser's batch)
     ~Renderer();
     void ensure(size_t paths, uint32_t nslots, uint32_t spp);

@@ -190,13 +190,17 @@ def test_autotune_keeps_images(gpu):
     print(t)
     assert t["samples_per_batch"] in (1, 2, 4) and t["refill"] in (8, 16, 32, 48)
     assert t["grab"] in (64, 128, 256, 512, 1024) and t["grab0"] in (0, 256, 512, 1024)
-    assert 0 < t["frame_ms"] <= 1.25 * t["default_ms"] and t["trials"] >= 46
+    assert t["grab_shadow"] in (0, 64, 128)
+    assert 0 < t["frame_ms"] <= 1.25 * t["default_ms"] and t["trials"] >= 52
     rd.render(gs, w.cam, pc, seed=2)
     assert same(rd.read(0), before)
     rd.set_schedule(1, 48, 1024)  # an explicit schedule: the same image
     rd.render(gs, w.cam, pc, seed=2)
     assert same(rd.read(0), before)
     rd.set_schedule(2, 16, 64, 0, 2048)  # camera rays with their own reservation
+    rd.render(gs, w.cam, pc, seed=2)
+    assert same(rd.read(0), before)
+    rd.set_schedule(4, 8, 512, 3, 0, 16)  # shadow rays with their own (clamped to 16)
     rd.render(gs, w.cam, pc, seed=2)
     assert same(rd.read(0), before)
 
