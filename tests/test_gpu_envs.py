@@ -139,7 +139,7 @@ def test_frame_graph_replays_match_eager_frames(gpu):
     """A captured frame graph (three batches of one sample, two streams)
     replayed over frames with new seeds, a moved camera and the running
     mean, interleaved with eager work, equals eager rendering bit for bit;
-    a changed sample count captures a new graph."""
+    a changed sample count or reservation schedule captures a new graph."""
     w = scenes.config2(width=48, height=27, spp=3, path_depth=4, n_inst=24, subdiv=2)
     gs = GpuScene(w.data)
     rg, re = Renderer(w.width, w.height), Renderer(w.width, w.height)
@@ -150,11 +150,13 @@ def test_frame_graph_replays_match_eager_frames(gpu):
     probe["org"] = (0.0, -30.0, 0.0)
     probe["tmax"] = 100.0
     pc = w.pc
-    for f in range(6):
+    for f in range(7):
         pc.frame_count = f
         cam = cam2 if f in (2, 3) else w.cam
         if f == 5:
             pc.spp = 2
+        if f == 6:  # camera- and shadow-ray reservations of their own
+            rg.set_schedule(0, 0, 0, 0, 1024, 32)
         sg = rg.render(gs, cam, pc, seed=10 + f, samples_per_batch=1, use_graph=True)
         se = re.render(gs, cam, pc, seed=10 + f, samples_per_batch=1)
         gs.trace_closest(probe)  # eager work between replays
