@@ -47,8 +47,8 @@ def cpu_baseline(workload, budget_s=15.0):
         y = 0 if y1 >= H else y1
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": "FP32 CPU oracle (restatement of the reference shaders), %d rows of the %dx%d frame in "
-                      "64-row bands from row %d, %d spp, %d rays in %.1f s" % (rows, W, H, (H // 2) // band * band,
-                                                                                workload.pc.spp, rays, dt)}
+                      "64-row bands from row %d (wrapping to row 0 after the last row), %d spp, %d rays in %.1f s"
+                      % (rows, W, H, (H // 2) // band * band, workload.pc.spp, rays, dt)}
 
 
 def relmse(x, y):
