@@ -413,7 +413,9 @@ int kf_denoiser_set_params(kf_denoiser* d, const float* in);
 /* The denoise stage of a frame: the renderer's rgba target is replaced by the
  * denoised image (context.cpp:650-670 writes the OptiX output back to rgba). */
 int kf_denoiser_apply(kf_denoiser* d, kf_renderer* r);  /* after kf_render_envs: per environment */
-/* Device pointers: rgba/albedo/normal W*H float4 in HBM -> out W*H float4. */
+/* Device pointers: rgba/albedo/normal W*H float4 in HBM -> out W*H float4.
+ * kernel_ms (may be NULL) receives the kernels' time; both calls return when
+ * the output is written. */
 int kf_denoiser_infer_device(kf_denoiser* d, const float* d_rgba, const float* d_albedo, const float* d_normal,
                              uint32_t width, uint32_t height, float* d_out, float* kernel_ms);
 /* Batched denoise (no counterpart in the reference): n_images images of

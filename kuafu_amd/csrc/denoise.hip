@@ -1268,10 +1268,12 @@ int kf_denoiser_infer_device_batch(kf_denoiser* d, const float* d_rgba, const fl
         int Hp = (int)((height + 15) / 16 * 16), Wp = (int)((width + 15) / 16 * 16);
         if ((uint64_t)N * Hp * Wp >= (1ull << 31)) throw Error("kf_denoiser_infer_device_batch: batch too large");
         d->act.ensure(N, Hp, Wp, false);
-        hipEvent_t e0, e1;
-        KF_CHECK(hipEventCreate(&e0));
-        KF_CHECK(hipEventCreate(&e1));
-        KF_CHECK(hipEventRecord(e0, st));
+        hipEvent_t e0 = nullptr, e1 = nullptr;  // timing events only when the caller asks for the time
+        if (kernel_ms) {
+            KF_CHECK(hipEventCreate(&e0));
+            KF_CHECK(hipEventCreate(&e1));
+            KF_CHECK(hipEventRecord(e0, st));
+        }
         // the first convolution reads the render targets itself
         PackSrc pk;
         pk.rgba = (const float4*)d_rgba;
@@ -1283,14 +1285,18 @@ int kf_denoiser_infer_device_batch(kf_denoiser* d, const float* d_rgba, const fl
         k_unpack<<<blocks((size_t)N * width * height, TB), TB, 0, st>>>(d->act.y, (const float4*)d_rgba, N,
                                                                         (int)height, (int)width, Hp, Wp,
                                                                         (float4*)d_out);
-        KF_CHECK(hipEventRecord(e1, st));
         KF_CHECK(hipGetLastError());
-        KF_CHECK(hipEventSynchronize(e1));
-        float ms = 0;
-        KF_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (kernel_ms) *kernel_ms = ms;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        if (kernel_ms) {
+            KF_CHECK(hipEventRecord(e1, st));
+            KF_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            KF_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            *kernel_ms = ms;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        } else {
+            KF_CHECK(hipStreamSynchronize(st));
+        }
         return 0;
     } catch (const std::exception& e) {
         set_error(e.what());

@@ -228,6 +228,7 @@ struct Renderer {
     hipStream_t st2 = nullptr;                 // any-hit traversal, beside the main stream
     std::vector<hipEvent_t> sync_ev;           // per bounce: shading done, shadow rays done
     void* d_fv = nullptr;                      // per-frame seed and count read by the kernels
+    unsigned long long* h_totals = nullptr;    // pinned host copy of the frame's global counters
     // Frame graph (kf_render_opts.use_graph): every batch of a frame captured
     // once and replayed while everything baked into its kernel arguments
     // (graph_key) is unchanged.

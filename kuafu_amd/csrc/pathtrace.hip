@@ -836,6 +836,7 @@ Renderer::~Renderer() {
     }
     for (auto e : ev) (void)hipEventDestroy(e);
     for (auto e : sync_ev) (void)hipEventDestroy(e);
+    if (h_totals) (void)hipHostFree(h_totals);
     void* ptrs[] = {d_rgba, d_albedo, d_normal, d_bgra, ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed,
                     queue_a, queue_b, queue_e, sh_o, sh_d, sh_root, nee_a, nee_b, vis, samples, counters, d_lights,
                     d_jump, d_acc, d_out, d_cams, d_fv};
@@ -1035,6 +1036,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         r->sync_ev.resize(2 * kMaxBounces);
         for (auto& e : r->sync_ev) KF_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         KF_CHECK(hipStreamCreateWithFlags(&r->st2, hipStreamNonBlocking));  // default priority
+        KF_CHECK(hipHostMalloc((void**)&r->h_totals, 10 * sizeof(unsigned long long), hipHostMallocDefault));
     }
     int dev = 0;
     KF_CHECK(hipGetDevice(&dev));
@@ -1144,6 +1146,13 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         KF_CHECK(hipGetLastError());
     };
 
+    // The frame's global counters follow its last kernel on the stream into
+    // pinned memory, so that one host wait covers both.
+    auto read_totals = [&]() {
+        KF_CHECK(hipMemcpyAsync(r->h_totals, r->counters + kCntGlobal, 10 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+    };
     float ms_total = 0.f, ms_ext = 0.f, ms_sh = 0.f;
     if (o.use_graph) {
         // Everything the captured kernel arguments hold: scene view, frame
@@ -1191,12 +1200,15 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         KF_CHECK(hipEventRecord(r->ev[0], st));
         KF_CHECK(hipGraphLaunch(r->graph, st));
         KF_CHECK(hipEventRecord(r->ev[1], st));
-        KF_CHECK(hipEventSynchronize(r->ev[1]));
+        read_totals();
         KF_CHECK(hipEventElapsedTime(&ms_total, r->ev[0], r->ev[1]));
     } else {
         for (uint32_t s0 = 0; s0 < pc->spp; s0 += spb) {
             enqueue_batch(s0, true);
-            KF_CHECK(hipEventSynchronize(r->ev[1]));
+            if (s0 + spb >= pc->spp)
+                read_totals();
+            else
+                KF_CHECK(hipEventSynchronize(r->ev[1]));
             float ms = 0.f;
             KF_CHECK(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
             ms_total += ms;
@@ -1209,9 +1221,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             }
         }
     }
-    unsigned long long totals[10];
-    KF_CHECK(hipMemcpyAsync(totals, r->counters + kCntGlobal, sizeof(totals), hipMemcpyDeviceToHost, st));
-    KF_CHECK(hipStreamSynchronize(st));
+    const unsigned long long* totals = r->h_totals;
     if (totals[5]) throw Error("kf_render: traversal stack overflow (BVH deeper than the stack)");
     if (stats) {
         stats->primary_rays = totals[0];
