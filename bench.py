@@ -166,6 +166,34 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     return res
 
 
+def denoised_frames(w, gs, rd, frames, warmup, schedule=""):
+    """The denoised point of the time-to-relMSE sweep on its own: 1-spp frames
+    of the 1080p scene, each followed by the denoise, with the schedule tuned
+    for 1-spp frames (or the given one).  Host wall time per frame."""
+    import numpy as np
+    from kuafu_amd.denoiser import Denoiser
+    pc = w.pc
+    pc.spp = 1
+    pc.frame_count = 0
+    if schedule:
+        v = [int(x) for x in schedule.split(",")] + [0] * 6
+        rd.set_schedule(*v[:6])
+    else:
+        rd.autotune(gs, w.cam, pc)
+    den = Denoiser()
+    wall, trace = [], []
+    for k in range(warmup + frames):
+        t = time.perf_counter()
+        st = rd.render(gs, w.cam, pc, seed=37 + k)
+        den.apply(rd)
+        if k >= warmup:
+            wall.append((time.perf_counter() - t) * 1e3)
+            trace.append(st.trace_ms)
+    return {"workload": "config2_icospheres256_1080p_1spp_denoised", "frames": frames,
+            "frame_ms_p50": round(float(np.median(wall)), 3), "frame_ms_min": round(min(wall), 3),
+            "trace_ms_p50": round(float(np.median(trace)), 3)}
+
+
 def batched_environments(frames=8, schedule=None):
     """128 environments of 512x512 (8 spp, depth 6, 5k-400k triangles each):
     autotuned schedule, every frame one replay of a captured HIP graph, then
@@ -258,6 +286,8 @@ def main():
                     help="samples_per_batch,refill,grab[,tail_bounce[,grab0[,grab_shadow]]] instead of autotuning")
     ap.add_argument("--envs-only", action="store_true",
                     help="only the 128-environment workload (frames = --steps), e.g. for profiling")
+    ap.add_argument("--denoised-only", action="store_true",
+                    help="only 1-spp frames + denoise of the 1080p scene (frames = --steps), e.g. for profiling")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     args = ap.parse_args()
@@ -282,6 +312,10 @@ def main():
     rd = Renderer(w.width, w.height)
     pc = w.pc
     pc.frame_count = 0
+
+    if args.denoised_only:
+        print(json.dumps(denoised_frames(w, gs, rd, args.steps, args.warmup, args.schedule)))
+        return
 
     # schedule (batch size, refill threshold, reservation per wave) picked by
     # timing frames of this scene, before anything is measured
