@@ -36,6 +36,9 @@ def lib():
             "ko_render": (i32, [vp, vp, vp, u32, u32, u32, u32, u32, i32, i32, vp, vp, vp, vp, vp]),
             "ko_relmse": (C.c_double, [vp, vp, u32, u32, u32]),
             "ko_to_bgra8_srgb": (None, [vp, u32, vp]),
+            "ko_shading_term": (i32, [i32, i32, vp, u32, vp]),
+            "ko_sample": (i32, [i32, i32, u32, vp, u32, vp, vp]),
+            "ko_direct": (i32, [i32, vp, vp, vp]),
         }
         for k, (res, args) in sig.items():
             f = getattr(L, k)
@@ -148,3 +151,43 @@ def to_bgra8(rgba):
     out = np.zeros(rgba.shape[:-1] + (4,), np.uint8)
     lib().ko_to_bgra8_srgb(_p(rgba), rgba.size // 4, _p(out))
     return out
+
+
+# Shading terms and samplers (oracle.h: KO_* ids), for the closed-form pins.
+SCHLICK, GGX_D, G1, GEOM_SHADOW, SCHLICK_FRESNEL, REFRACT, REFLECT, LOCAL_TO_WORLD, PERPENDICULAR = range(9)
+_TERM_IO = [(2, 1), (2, 1), (2, 1), (3, 1), (2, 1), (7, 3), (6, 3), (6, 3), (3, 3)]
+SAMPLE_COSINE, SAMPLE_GGX, SAMPLE_SPHERE, SAMPLE_DISK = range(4)
+_SAMPLE_OUT = [3, 3, 3, 2]
+
+
+def shading_term(term, inputs, precision=64):
+    """Evaluate one shading term on rows of inputs (n, k) -> (n, m) float64."""
+    k, m = _TERM_IO[term]
+    a = np.ascontiguousarray(np.asarray(inputs, np.float64).reshape(-1, k))
+    out = np.zeros((len(a), m), np.float64)
+    if lib().ko_shading_term(term, precision, _p(a), len(a), _p(out)) != 0:
+        raise ValueError("ko_shading_term failed")
+    return out[:, 0] if m == 1 else out
+
+
+def sample(sampler, n, seed=0, param=None, precision=64):
+    """n successive samples from one rnd stream; returns (samples, draws)."""
+    p = None if param is None else np.ascontiguousarray(np.asarray(param, np.float64).ravel())
+    out = np.zeros((n, _SAMPLE_OUT[sampler]), np.float64)
+    draws = C.c_uint64()
+    if lib().ko_sample(sampler, precision, seed, _p(p), n, _p(out), C.byref(draws)) != 0:
+        raise ValueError("ko_sample failed")
+    return out, draws.value
+
+
+def direct(N, V, L, f, a2, diffuse, specular, transmission, inside, emission, seed, precision=64):
+    """calcDirectContribution for one shading point; returns (rgb, new seed)."""
+    a = np.ascontiguousarray(np.concatenate([np.ravel(N), np.ravel(V), np.ravel(L), [f, a2], np.ravel(diffuse),
+                                             np.ravel(specular), np.ravel(transmission), [float(inside)],
+                                             np.ravel(emission)]).astype(np.float64))
+    assert a.size == 24
+    s = C.c_uint32(seed)
+    out = np.zeros(3, np.float64)
+    if lib().ko_direct(precision, _p(a), C.byref(s), _p(out)) != 0:
+        raise ValueError("ko_direct failed")
+    return out, s.value

@@ -958,6 +958,67 @@ void render_pixel(const ko_scene& s, const kf_camera_ubo& cam, const kf_push_con
 
 }  // namespace
 
+namespace {
+template <class T> V3<T> v3_at(const double* p) { return V3<T>(T(p[0]), T(p[1]), T(p[2])); }
+template <class T> void put3(double* o, V3<T> v) { o[0] = (double)v.x; o[1] = (double)v.y; o[2] = (double)v.z; }
+
+template <class T> int shading_term(int term, const double* in, uint32_t n, double* out) {
+    static const int ins[] = {2, 2, 2, 3, 2, 7, 6, 6, 3};
+    static const int outs[] = {1, 1, 1, 1, 1, 3, 3, 3, 3};
+    if (term < 0 || term > KO_PERPENDICULAR) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* a = in + (size_t)i * ins[term];
+        double* o = out + (size_t)i * outs[term];
+        switch (term) {
+        case KO_SCHLICK: o[0] = (double)schlick(T(a[0]), T(a[1])); break;
+        case KO_GGX_D: o[0] = (double)ggx_d(T(a[0]), T(a[1])); break;
+        case KO_G1: o[0] = (double)g1(T(a[0]), T(a[1])); break;
+        case KO_GEOM_SHADOW: o[0] = (double)geom_shadow(T(a[0]), T(a[1]), T(a[2])); break;
+        case KO_SCHLICK_FRESNEL: o[0] = (double)schlick_fresnel(V3<T>(T(a[0])), T(a[1])).x; break;
+        case KO_REFRACT: put3(o, refract(v3_at<T>(a), v3_at<T>(a + 3), T(a[6]))); break;
+        case KO_REFLECT: put3(o, reflect(v3_at<T>(a), v3_at<T>(a + 3))); break;
+        case KO_LOCAL_TO_WORLD: put3(o, local_to_world(v3_at<T>(a), v3_at<T>(a + 3))); break;
+        case KO_PERPENDICULAR: put3(o, perpendicular(v3_at<T>(a))); break;
+        }
+    }
+    return 0;
+}
+
+template <class T> int sample_n(int sampler, uint32_t& seed, const double* param, uint32_t n, double* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (sampler) {
+        case KO_SAMPLE_COSINE: put3(out + 3 * (size_t)i, cosine_hemisphere(seed, v3_at<T>(param))); break;
+        case KO_SAMPLE_GGX: put3(out + 3 * (size_t)i, sample_ggx(seed, T(param[0]), v3_at<T>(param + 1))); break;
+        case KO_SAMPLE_SPHERE: put3(out + 3 * (size_t)i, uniform_sphere<T>(seed)); break;
+        case KO_SAMPLE_DISK: {
+            T x, y;
+            disk(seed, x, y);
+            out[2 * (size_t)i] = (double)x;
+            out[2 * (size_t)i + 1] = (double)y;
+            break;
+        }
+        default: return -1;
+        }
+    }
+    return 0;
+}
+
+template <class T> void direct_one(const double* a, uint32_t& seed, double* out) {
+    Shade<T> sh;
+    sh.N = v3_at<T>(a);
+    sh.V = v3_at<T>(a + 3);
+    V3<T> L = v3_at<T>(a + 6);
+    sh.f = T(a[9]);
+    sh.a2 = T(a[10]);
+    sh.diffuseColor = v3_at<T>(a + 11);
+    sh.specularColor = v3_at<T>(a + 14);
+    sh.transmissionColor = v3_at<T>(a + 17);
+    sh.inside = a[20] != 0.0;
+    sh.worldPos = V3<T>(T(0));
+    put3(out, direct(sh, L, v3_at<T>(a + 21), seed));
+}
+}  // namespace
+
 // ======================================================================== C ABI
 extern "C" {
 
@@ -1133,6 +1194,35 @@ double ko_relmse(const float* x, const float* y, uint32_t n, uint32_t sx, uint32
             acc += (a - b) * (a - b) / (b * b + 1e-2);
         }
     return acc / (3.0 * n);
+}
+
+int ko_shading_term(int term, int precision, const double* in, uint32_t n, double* out) {
+    if (precision == 32) return shading_term<float>(term, in, n, out);
+    if (precision == 64) return shading_term<double>(term, in, n, out);
+    return -1;
+}
+
+int ko_sample(int sampler, int precision, uint32_t seed, const double* param, uint32_t n, double* out,
+              uint64_t* draws) {
+    const uint32_t start = seed;
+    int rc = precision == 32 ? sample_n<float>(sampler, seed, param, n, out)
+                             : precision == 64 ? sample_n<double>(sampler, seed, param, n, out) : -1;
+    if (rc == 0 && draws) {
+        // Count LCG steps from the start state to the final one (the LCG has
+        // period 2^32, so the count is exact below that).
+        uint64_t k = 0;
+        for (uint32_t s = start; s != seed; ++k) lcg(s);
+        *draws = k;
+    }
+    return rc;
+}
+
+int ko_direct(int precision, const double* shade, uint32_t* seed, double* out) {
+    if (!shade || !seed || !out) return -1;
+    if (precision == 32) direct_one<float>(shade, *seed, out);
+    else if (precision == 64) direct_one<double>(shade, *seed, out);
+    else return -1;
+    return 0;
 }
 
 void ko_to_bgra8_srgb(const float* rgba, uint32_t n, uint8_t* out) {

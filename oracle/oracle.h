@@ -61,6 +61,41 @@ double ko_relmse(const float* x, const float* y, uint32_t n_pixels, uint32_t str
 /* RGBA32F -> B8G8R8A8_SRGB bytes (clamp, IEC 61966-2-1 encode, round). */
 void ko_to_bgra8_srgb(const float* rgba, uint32_t n_pixels, uint8_t* out);
 
+/* Unit access to the shading terms, so that tests can pin each against the
+ * closed forms the mathematics fixes.  precision 32 or 64; n rows of inputs
+ * (row-major, doubles, converted to the precision) -> n rows of outputs.
+ *   KO_SCHLICK          (cos, ior)              -> 1   Random.glsl:68-73
+ *   KO_GGX_D            (NdotH, a2)             -> 1   Random.glsl:75-79
+ *   KO_G1               (x, a2)                 -> 1   Random.glsl:117-121
+ *   KO_GEOM_SHADOW      (NdotV, NdotL, a2)      -> 1   Random.glsl:123-127
+ *   KO_SCHLICK_FRESNEL  (f0, lDotH)             -> 1   Random.glsl:133-136 (scalar f0)
+ *   KO_REFRACT          (I xyz, N xyz, eta)     -> 3   GLSL refract
+ *   KO_REFLECT          (I xyz, N xyz)          -> 3   GLSL reflect
+ *   KO_LOCAL_TO_WORLD   (d xyz, n xyz)          -> 3   Sampling.glsl:6-23
+ *   KO_PERPENDICULAR    (u xyz)                 -> 3   Random.glsl:58-65
+ * Returns 0, or -1 for an unknown term or precision. */
+enum {
+    KO_SCHLICK = 0, KO_GGX_D = 1, KO_G1 = 2, KO_GEOM_SHADOW = 3, KO_SCHLICK_FRESNEL = 4,
+    KO_REFRACT = 5, KO_REFLECT = 6, KO_LOCAL_TO_WORLD = 7, KO_PERPENDICULAR = 8
+};
+int ko_shading_term(int term, int precision, const double* in, uint32_t n, double* out);
+
+/* n successive draws of a sampler from one rnd stream starting at seed:
+ *   KO_SAMPLE_COSINE  param N xyz         -> 3   Sampling.glsl:25-35
+ *   KO_SAMPLE_GGX     param a2, N xyz     -> 3   Random.glsl:83-101 (half vector)
+ *   KO_SAMPLE_SPHERE  no param            -> 3   Sampling.glsl:72-83 (point in the unit ball)
+ *   KO_SAMPLE_DISK    no param            -> 2   Sampling.glsl:124-134
+ * *draws receives the number of rnd() calls the n samples consumed. */
+enum { KO_SAMPLE_COSINE = 0, KO_SAMPLE_GGX = 1, KO_SAMPLE_SPHERE = 2, KO_SAMPLE_DISK = 3 };
+int ko_sample(int sampler, int precision, uint32_t seed, const double* param, uint32_t n, double* out,
+              uint64_t* draws);
+
+/* calcDirectContribution (PathTrace.rchit:103-171) for one shading point:
+ * shade = N xyz, V xyz, L xyz, f, a2, diffuseColor rgb, specularColor rgb,
+ * transmissionColor rgb, inside (0/1), lightEmission rgb (24 doubles).
+ * out = the returned rgb; *seed is the ray seed, advanced by the draws. */
+int ko_direct(int precision, const double* shade, uint32_t* seed, double* out);
+
 #ifdef __cplusplus
 }
 #endif
