@@ -102,6 +102,35 @@ typedef struct kf_push_constants {
     uint32_t nee_min_bounces;
 } kf_push_constants;
 
+/* The layouts above are the contract with the reference's std430 blocks and
+ * host structs: pinned at compile time for every client of this header. */
+#ifdef __cplusplus
+#define KF_LAYOUT_ASSERT(cond, msg) static_assert(cond, msg)
+#else
+#define KF_LAYOUT_ASSERT(cond, msg) _Static_assert(cond, msg)
+#endif
+KF_LAYOUT_ASSERT(sizeof(kf_vertex) == 48, "Vertex is 48 B (vertex.hpp:28-34)");
+KF_LAYOUT_ASSERT(offsetof(kf_vertex, normal) == 12 && offsetof(kf_vertex, color) == 24 &&
+                     offsetof(kf_vertex, uv) == 36, "Vertex field offsets (Geometry.glsl unpack)");
+KF_LAYOUT_ASSERT(sizeof(kf_material) == 80, "NiceMaterialSSBO is 80 B (geometry.hpp:70-87)");
+KF_LAYOUT_ASSERT(offsetof(kf_material, emission) == 16 && offsetof(kf_material, alpha) == 32 &&
+                     offsetof(kf_material, ior) == 48 && offsetof(kf_material, diffuse_tex) == 56 &&
+                     offsetof(kf_material, transmission_tex) == 68, "NiceMaterialSSBO field offsets");
+KF_LAYOUT_ASSERT(sizeof(kf_instance) == 80 && offsetof(kf_instance, geometry_index) == 64,
+                 "GeometryInstanceSSBO is 80 B (geometry.hpp:91-99)");
+KF_LAYOUT_ASSERT(sizeof(kf_dir_light) == 32, "DirectionalLightUBO is 32 B (light.hpp:41-44)");
+KF_LAYOUT_ASSERT(sizeof(kf_point_lights) == 1024 && offsetof(kf_point_lights, rgbs) == 512,
+                 "PointLightsUBO is 1024 B (light.hpp:46-49)");
+KF_LAYOUT_ASSERT(sizeof(kf_active_lights) == 1536 && offsetof(kf_active_lights, proj) == 512 &&
+                     offsetof(kf_active_lights, front) == 1024 && offsetof(kf_active_lights, sftp) == 1408,
+                 "ActiveLightsUBO is 1536 B (light.hpp:51-58)");
+KF_LAYOUT_ASSERT(sizeof(kf_camera_ubo) == 320 && offsetof(kf_camera_ubo, proj_inverse) == 192 &&
+                     offsetof(kf_camera_ubo, position) == 256 && offsetof(kf_camera_ubo, front) == 272,
+                 "CameraUBO is 320 B (camera.hpp:181-192)");
+KF_LAYOUT_ASSERT(sizeof(kf_push_constants) == 48 && offsetof(kf_push_constants, frame_count) == 16 &&
+                     offsetof(kf_push_constants, nee_min_bounces) == 44,
+                 "RtPushConstants is 48 B (rt.hpp:31-42)");
+
 /* RGBA8 image, row-major, 4 bytes per texel, sRGB-encoded (vkCore.hpp:585). */
 typedef struct kf_texture_desc {
     uint32_t width, height;
