@@ -34,6 +34,8 @@ def lib():
             "ko_trace_closest": (None, [vp, vp, u32, vp, i32, i32]),
             "ko_trace_any": (None, [vp, vp, u32, vp, i32, i32]),
             "ko_render": (i32, [vp, vp, vp, u32, u32, u32, u32, u32, i32, i32, vp, vp, vp, vp, vp]),
+            "ko_render_region": (i32, [vp, vp, vp, u32, u32, u32, u32, u32, u32, u32, i32, i32, vp, vp, vp, vp,
+                                       vp]),
             "ko_relmse": (C.c_double, [vp, vp, u32, u32, u32]),
             "ko_to_bgra8_srgb": (None, [vp, u32, vp]),
             "ko_shading_term": (i32, [i32, i32, vp, u32, vp]),
@@ -125,16 +127,18 @@ class Scene:
         return occ
 
     def render(self, cam, pc, width, height, seed=0, rows=None, precision=32, threads=0, rgba=None,
-               keep_samples=False):
-        """One frame; returns dict(rgba, albedo, normal, samples, counts)."""
+               keep_samples=False, cols=None):
+        """One frame (or the rows / cols ranges of it); returns dict(rgba,
+        albedo, normal, samples, counts) of full W x H size."""
         y0, y1 = (0, height) if rows is None else rows
+        x0, x1 = (0, width) if cols is None else cols
         rgba = np.zeros((height, width, 4), np.float32) if rgba is None else np.ascontiguousarray(rgba, np.float32)
         alb = np.zeros((height, width, 4), np.float32)
         nrm = np.zeros((height, width, 4), np.float32)
         smp = np.zeros((pc.spp, height, width, 3), np.float32) if keep_samples else None
         cnt = np.zeros(3, np.uint64)
-        rc = lib().ko_render(self.h, C.addressof(cam), C.addressof(pc), seed, width, height, y0, y1, precision,
-                             threads, _p(rgba), _p(alb), _p(nrm), _p(smp), _p(cnt))
+        rc = lib().ko_render_region(self.h, C.addressof(cam), C.addressof(pc), seed, width, height, x0, x1, y0, y1,
+                                    precision, threads, _p(rgba), _p(alb), _p(nrm), _p(smp), _p(cnt))
         if rc != 0:
             raise RuntimeError("ko_render failed")
         return dict(rgba=rgba, albedo=alb, normal=nrm, samples=smp, counts=cnt)

@@ -1166,15 +1166,25 @@ void ko_trace_any(const ko_scene* s, const kf_ray* rays, uint32_t n, uint8_t* oc
 int ko_render(const ko_scene* s, const kf_camera_ubo* cam, const kf_push_constants* pc, uint32_t seed,
               uint32_t width, uint32_t height, uint32_t y0, uint32_t y1, int precision, int threads, float* rgba,
               float* albedo, float* normal, float* samples, uint64_t* counts) {
-    if (!s || !cam || !pc || !rgba || !albedo || !normal || y1 > height || y0 > y1) return -1;
+    return ko_render_region(s, cam, pc, seed, width, height, 0, width, y0, y1, precision, threads, rgba, albedo,
+                            normal, samples, counts);
+}
+
+int ko_render_region(const ko_scene* s, const kf_camera_ubo* cam, const kf_push_constants* pc, uint32_t seed,
+                     uint32_t width, uint32_t height, uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1,
+                     int precision, int threads, float* rgba, float* albedo, float* normal, float* samples,
+                     uint64_t* counts) {
+    if (!s || !cam || !pc || !rgba || !albedo || !normal || y1 > height || y0 > y1 || x1 > width || x0 > x1)
+        return -1;
     if (precision != 32 && precision != 64) return -1;
     set_threads(threads);
     uint32_t S0 = tea(seed, (uint32_t)pc->frame_count);
     uint64_t c0 = 0, c1 = 0, c2 = 0;
-    long npx = (long)(y1 - y0) * width;
+    const uint32_t rw = x1 - x0;
+    long npx = (long)(y1 - y0) * rw;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : c0, c1, c2)
     for (long k = 0; k < npx; ++k) {
-        uint32_t y = y0 + (uint32_t)(k / width), x = (uint32_t)(k % width);
+        uint32_t y = y0 + (uint32_t)(k / rw), x = x0 + (uint32_t)(k % rw);
         Counts c;
         if (precision == 32)
             render_pixel<float>(*s, *cam, *pc, S0, x, y, width, height, rgba, albedo, normal, samples, c);

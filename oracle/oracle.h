@@ -55,6 +55,13 @@ int ko_render(const ko_scene* s, const kf_camera_ubo* cam, const kf_push_constan
               uint32_t width, uint32_t height, uint32_t y0, uint32_t y1, int precision, int threads,
               float* rgba, float* albedo, float* normal, float* samples, uint64_t* counts);
 
+/* The same over the pixel rectangle [x0, x1) x [y0, y1) of the W x H frame
+ * (buffers are still W x H). */
+int ko_render_region(const ko_scene* s, const kf_camera_ubo* cam, const kf_push_constants* pc, uint32_t seed,
+                     uint32_t width, uint32_t height, uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1,
+                     int precision, int threads, float* rgba, float* albedo, float* normal, float* samples,
+                     uint64_t* counts);
+
 /* relMSE = mean over pixels and RGB of (x - y)^2 / (y^2 + 1e-2) (FP64 sum). */
 double ko_relmse(const float* x, const float* y, uint32_t n_pixels, uint32_t stride_x, uint32_t stride_y);
 
