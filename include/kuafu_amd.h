@@ -411,6 +411,11 @@ float* kf_renderer_device_target(kf_renderer* r, int target);
  * config.hpp:144, vkCore.hpp:3008-3111); W*H*4 bytes, order B,G,R,A. */
 int kf_renderer_download_bgra8(kf_renderer* r, uint8_t* out);
 
+/* Single-channel FP32 image of the rgba target, (r + g + b) / 3 per pixel in
+ * FP32 (W*H floats): the IR-camera output of active-light sensing.  No
+ * counterpart in the reference, which exposes only the BGRA8 download. */
+int kf_renderer_download_gray32f(kf_renderer* r, float* out);
+
 /* Measurement utility (no reference counterpart): streaming copy of `bytes`
  * between two HBM buffers, `iters` times; reports the achieved GB/s (read +
  * write bytes / kernel time).  The achievable-bandwidth denominator of the

@@ -157,6 +157,7 @@ EXPORTS = [
     ("kf_renderer_read_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     ("kf_renderer_device_target", C.c_void_p, [C.c_void_p, C.c_int]),
     ("kf_renderer_download_bgra8", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("kf_renderer_download_gray32f", C.c_int, [C.c_void_p, C.c_void_p]),
     ("kf_bench_copy", C.c_int, [C.c_uint64, C.c_int, C.c_void_p]),
     ("kf_denoiser_create", C.c_int, [C.c_char_p, C.c_void_p]),
     ("kf_denoiser_destroy", None, [C.c_void_p]),

@@ -432,5 +432,11 @@ class Renderer:
         check(lib().kf_renderer_download_bgra8(self.h, ptr(out)))
         return out
 
+    def download_gray32f(self):
+        """Single-channel FP32 image (r + g + b) / 3 of the rgba target."""
+        out = np.zeros((self.h_, self.w), np.float32)
+        check(lib().kf_renderer_download_gray32f(self.h, ptr(out)))
+        return out
+
     def device_target(self, target):
         return lib().kf_renderer_device_target(self.h, target)

@@ -57,8 +57,17 @@ template <class T> static T* upload(const T* h, size_t n) {
     KF_CHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream()));
     return d;
 }
+// Frees on destruction paths, which cannot throw: a failure is reported by
+// kf_last_error, and the error hipFree leaves is consumed here so that it does
+// not surface in an unrelated later call.
 template <class T> static void release(T*& p) {
-    if (p) (void)hipFree((void*)p);
+    if (p) {
+        const hipError_t e = hipFree((void*)p);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error(std::string("hipFree: ") + hipGetErrorString(e));
+        }
+    }
     p = nullptr;
 }
 
@@ -108,7 +117,8 @@ Scene::~Scene() {
     if (vstage_ev) (void)hipEventDestroy(vstage_ev);
     if (h_stage) (void)hipHostFree(h_stage);
     if (stage_ev) (void)hipEventDestroy(stage_ev);
-    release(d_inst_src); release(d_tex); release(d_tex_info); release(d_env_info); release(d_srgb);
+    d_env_info = nullptr;  // points into d_tex_info, not an allocation of its own
+    release(d_inst_src); release(d_tex); release(d_tex_info); release(d_srgb);
 }
 
 // Environments of a batched scene: instance ranges, BVH2 / BVH4 node ranges
@@ -395,8 +405,17 @@ int kf_scene_create(const kf_scene_desc* d, kf_scene** out) {
         s->d_toff = upload(s->h_toff.data(), s->h_toff.size());
         s->d_mats = upload(d->materials, (size_t)s->n_mat);
         // textures (RGBA8 sRGB, vkCore.hpp:585) and the decode table
+        // cube-map faces are appended to the texture pool; env_info points
+        // at their records inside d_tex_info
         s->n_tex = d->num_textures;
-        upload_textures(*s, d->textures, d->num_textures, &s->d_tex, &s->d_tex_info);
+        std::vector<kf_texture_desc> all;
+        if (d->num_textures) all.assign(d->textures, d->textures + d->num_textures);
+        if (d->env_kind == KF_ENV_CUBEMAP) {
+            if (!d->env_faces) throw Error("kf_scene_create: cubemap without faces");
+            for (int f = 0; f < 6; ++f) all.push_back(d->env_faces[f]);
+        }
+        upload_textures(*s, all.data(), (uint32_t)all.size(), &s->d_tex, &s->d_tex_info);
+        if (d->env_kind == KF_ENV_CUBEMAP) s->d_env_info = s->d_tex_info + d->num_textures;
         std::vector<float> lut(256);
         for (int i = 0; i < 256; ++i) {
             double c = i / 255.0;
@@ -405,16 +424,6 @@ int kf_scene_create(const kf_scene_desc* d, kf_scene** out) {
         s->d_srgb = upload(lut.data(), 256);
         s->env_kind = d->env_kind;
         for (int i = 0; i < 8; ++i) s->env_grad[i] = d->env_gradient[i];
-        if (d->env_kind == KF_ENV_CUBEMAP) {
-            if (!d->env_faces) throw Error("kf_scene_create: cubemap without faces");
-            // faces are appended to the texture pool; env_info indexes into it
-            std::vector<kf_texture_desc> all(d->textures, d->textures + d->num_textures);
-            for (int f = 0; f < 6; ++f) all.push_back(d->env_faces[f]);
-            release(s->d_tex);
-            release(s->d_tex_info);
-            upload_textures(*s, all.data(), (uint32_t)all.size(), &s->d_tex, &s->d_tex_info);
-            s->d_env_info = s->d_tex_info + d->num_textures;
-        }
         if (d->dir_light) s->dl = *d->dir_light;
         if (d->point_lights) s->pl = *d->point_lights;
         if (d->active_lights) s->al = *d->active_lights;

@@ -1407,6 +1407,32 @@ extern "C" int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_bat
     return 0;
 }
 
+// Single-channel FP32 image of the rgba target: (r + g + b) / 3 per pixel.
+__global__ void k_to_gray32f(uint32_t n, const float4* __restrict__ rgba, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = rgba[i];
+    out[i] = (c.x + c.y + c.z) / 3.0f;
+}
+
+extern "C" int kf_renderer_download_gray32f(kf_renderer* r, float* out) {
+    try {
+        if (!r || !out) throw Error("kf_renderer_download_gray32f: null argument");
+        hipStream_t st = stream();
+        uint32_t n = r->W * r->H;
+        // staging shared with the BGRA8 download: 4 bytes per pixel either way
+        if (!r->d_bgra) KF_CHECK(hipMalloc((void**)&r->d_bgra, 4 * (size_t)n));
+        k_to_gray32f<<<(n + TB - 1) / TB, TB, 0, st>>>(n, r->d_rgba, (float*)r->d_bgra);
+        KF_CHECK(hipGetLastError());
+        KF_CHECK(hipMemcpyAsync(out, r->d_bgra, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
 extern "C" int kf_renderer_download_bgra8(kf_renderer* r, uint8_t* out) {
     try {
         if (!r || !out) throw Error("kf_renderer_download_bgra8: null argument");

@@ -83,7 +83,13 @@ def test_network_gradients_match_reference(gpu):
     d = Denoiser(random_seed=0)
     p = he_params(1)
     d.set_params(p)
-    inp, tgt = batch(2)
+    # One 128 x 128 image: 256 pixels at level 3 and 64 at level 4.  Targets
+    # above every output make the loss gradient one-signed, so the weight
+    # gradients of the deep layers add up over pixels instead of cancelling,
+    # and the BF16 rounding flips between the GPU's FP32 accumulation and the
+    # FP64 reference stay small against them.
+    inp, tgt = batch(2, n=1, h=128, w=128)
+    tgt = (tgt + 2.0).astype(np.float32)
     loss, g = d.gradients(inp, tgt)
     net = U.Net(p, bf16=True)
     s = net.forward(inp)
@@ -93,11 +99,9 @@ def test_network_gradients_match_reference(gpu):
     sl, _ = U.param_slices()
     # Tolerance: activations are stored in BF16; a value within one FP32
     # rounding of a BF16 rounding boundary rounds one way on the GPU (FP32
-    # MFMA accumulation) and the other in the FP64 reference.  The deep levels
-    # see few pixels (32 at level 3 of this 2 x 32 x 32 batch), so such flips
-    # leave up to ~6 % relative error in their weight gradients (measured on
-    # seeds 1-3 with two different conv kernels); a wrong tap, channel or
-    # mask gives errors of order 1.  The median over tensors stays ~1 %.
+    # MFMA accumulation) and the other in the FP64 reference.  On this batch
+    # such flips leave at most ~1 % relative error in any tensor (median
+    # 0.4 %); a wrong tap, channel or mask gives errors of order 1.
     errs = []
     for k, (w, b, ci, co) in enumerate(sl):
         for a, z in ((w[0], w[1]), (b[0], b[1])):
@@ -107,9 +111,9 @@ def test_network_gradients_match_reference(gpu):
                 continue
             err = np.linalg.norm(g[a:z] - ref) / np.linalg.norm(ref)
             errs.append(err)
-            assert err < 0.1, (k, a, err)
+            assert err < 2e-2, (k, a, err)
     print("per-tensor relative gradient error: worst %.2e median %.2e" % (max(errs), np.median(errs)))
-    assert np.median(errs) < 0.03
+    assert np.median(errs) < 1e-2
     # forward output against the reference
     out = d.forward(inp)
     ro = net.output(s)
@@ -161,3 +165,43 @@ def test_denoise_stage_on_render_targets(gpu, width, height):
     rec = np.concatenate([before[..., :3], r.read(1)[..., :3], r.read(2)[..., :3]], -1)[None, :hc, :wc]
     ref = d.forward(np.ascontiguousarray(rec))
     assert np.abs(ref[0] - after[:hc, :wc, :3]).max() <= 1e-3 * max(1.0, np.abs(ref).max())
+
+
+def test_trained_network_gain(gpu):
+    # The shipped weights (kuafu_amd/weights/unet.kfw, trained on seeds
+    # 0-1999 of the training generator) on held-out scenes of that generator
+    # and on the config-1 frame against the FP64 oracle's 1024-spp image: the
+    # denoised frame's relMSE must beat the noisy input's by a wide margin.
+    import os
+
+    import oracle
+    from kuafu_amd import GpuScene, Renderer, scenes
+    from kuafu_amd.denoiser import WEIGHTS
+    assert os.path.exists(WEIGHTS)
+    d = Denoiser(weights=WEIGHTS)
+    cases = []
+    for seed in (3001, 3002, 3003):
+        w = scenes.training_scene(seed)
+        gs = GpuScene(w.data)
+        rd = Renderer(w.width, w.height)
+        pc = w.pc
+        pc.spp = 2048
+        rd.render(gs, w.cam, pc, seed=999, samples_per_batch=64)
+        cases.append(("heldout%d" % seed, w, gs, rd, rd.read(0)[..., :3].copy()))
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fp64_config1_1024spp.npy")
+    w = scenes.config1()
+    gs = GpuScene(w.data)
+    cases.append(("config1_fp64", w, gs, Renderer(w.width, w.height), np.load(golden).astype(np.float32)))
+    for name, w, gs, rd, ref in cases:
+        for spp, factor in ((1, 10.0), (4, 5.0)):
+            pc = w.pc
+            pc.spp = spp
+            pc.frame_count = 0
+            rd.render(gs, w.cam, pc, seed=1)
+            noisy = rd.read(0)[..., :3].copy()
+            d.apply(rd)
+            den = rd.read(0)[..., :3]
+            en, ed = oracle.relmse(noisy, ref), oracle.relmse(den, ref)
+            print("%s %d spp: relMSE noisy %.4f -> denoised %.4f (%.1fx)" % (name, spp, en, ed, en / ed))
+            assert np.isfinite(den).all()
+            assert ed * factor < en, (name, spp, en, ed)
