@@ -4,14 +4,13 @@
 // X axis, the capsule along X), written as parametric generators.  Every
 // triangle is wound so that cross(v1 - v0, v2 - v0) points out of the surface.
 #include <cctype>
+#include <filesystem>
 #include <cmath>
 #include <cstring>
 #include <fstream>
 #include <iterator>
-#include <map>
 #include <sstream>
 #include <stdexcept>
-#include <tuple>
 
 #include "../../include/core/geometry.hpp"
 
@@ -187,59 +186,83 @@ std::shared_ptr<Geometry> createCapsule(float halfLength, float radius, bool dyn
     return finish(g, dynamic, std::move(mat));
 }
 
+// Index of a material equal to m in the process-global list, appended when
+// there is none (the duplicate lookup of geometry.cpp:155-175).
+static uint32_t intern_material(const NiceMaterial& m) {
+    for (size_t j = 0; j < global::materials.size(); ++j)
+        if (global::materials[j] == m) return (uint32_t)j;
+    global::materials.push_back(m);
+    return global::materialIndex++;
+}
+
+// One imported mesh: a Geometry of its own vertices and triangles with one
+// material, as loadScene builds it (geometry.cpp:181-229).
+static std::shared_ptr<Geometry> imported(std::vector<Vertex> v, std::vector<uint32_t> idx, uint32_t mat,
+                                          const std::string& path, bool dynamic) {
+    auto g = std::make_shared<Geometry>();
+    g->path = path;
+    g->dynamic = dynamic;
+    g->vertices = std::move(v);
+    g->indices = std::move(idx);
+    g->initialized = false;
+    g->matIndex.assign(g->indices.size() / 3, mat);
+    g->isOpaque = global::materials[mat].alpha >= 1.0F;
+    return g;
+}
+
 // STL, binary (80-byte header, u32 facet count, 50 bytes per facet: normal,
 // three vertices, u16 attribute) or ASCII (solid / facet normal / outer loop /
 // vertex x3 / endloop / endfacet / endsolid).  As assimp imports it for the
-// reference's loadScene (geometry.cpp:46-232): one mesh per solid, three
-// vertices of its own per facet carrying the facet normal (the import flags do
-// not join identical vertices, and GenNormals keeps normals a file has), no
-// UVs, and one default material whose only property is a diffuse colour of
-// 0.6 grey; the reference turns it into diffuse 0.6, specular 0.5, IOR 1.4,
-// roughness 0 (no shininess and not an OBJ file), opaque.  Colours that some
-// binary exporters pack into the attribute bytes are not read.
+// reference's loadScene (geometry.cpp:46-232): three vertices of its own per
+// facet carrying the facet normal (the import flags do not join identical
+// vertices, and GenNormals keeps normals a file has), no UVs, and one default
+// material whose only property is a diffuse colour of 0.6 grey; the
+// reference turns it into diffuse 0.6, specular 0.5, IOR 1.4, roughness 0
+// (no shininess and not an OBJ file), opaque.  Every solid uses that one
+// material, so aiProcess_PreTransformVertices, which merges the meshes of a
+// material, returns one mesh for the file.  Colours that some binary
+// exporters pack into the attribute bytes are not read.
 static std::vector<std::shared_ptr<Geometry>> loadStl(const std::string& path, bool dynamic) {
     std::ifstream in(path, std::ios::binary);
     if (!in) throw std::runtime_error("Failed to load scene: cannot open " + path);
     std::string data((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
-    std::vector<std::vector<glm::vec3>> solids;  // per solid: normal, v0, v1, v2 per facet
+    std::vector<glm::vec3> facets;  // normal, v0, v1, v2 per facet, solids one after another
     uint32_t n = 0;
     if (data.size() >= 84) std::memcpy(&n, data.data() + 80, 4);
     if (data.size() >= 84 && data.size() == 84 + 50ull * n) {
-        solids.emplace_back();
         for (uint32_t f = 0; f < n; ++f) {
             float v[12];
             std::memcpy(v, data.data() + 84 + 50ull * f, sizeof(v));
-            for (int k = 0; k < 4; ++k) solids.back().push_back(glm::vec3(v[3 * k], v[3 * k + 1], v[3 * k + 2]));
+            for (int k = 0; k < 4; ++k) facets.push_back(glm::vec3(v[3 * k], v[3 * k + 1], v[3 * k + 2]));
         }
     } else {
         std::istringstream ss(data);
         std::string tok;
         if (!(ss >> tok) || tok != "solid") throw std::runtime_error("Failed to load scene: not an STL file, " + path);
-        solids.emplace_back();
         glm::vec3 nrm(0.0f);
-        int nv = 0;
+        int nv = -1;  // vertices of the open facet; -1: none open
         while (ss >> tok) {
-            if (tok == "solid") {
-                solids.emplace_back();
-            } else if (tok == "facet") {
+            if (tok == "facet") {
                 ss >> tok;  // "normal"
                 if (tok != "normal" || !(ss >> nrm.x >> nrm.y >> nrm.z))
                     throw std::runtime_error("Failed to load scene: bad facet in " + path);
-                solids.back().push_back(nrm);
+                facets.push_back(nrm);
                 nv = 0;
             } else if (tok == "vertex") {
                 glm::vec3 p;
-                if (solids.back().empty() || !(ss >> p.x >> p.y >> p.z) || nv >= 3)
+                if (nv < 0 || !(ss >> p.x >> p.y >> p.z) || nv >= 3)
                     throw std::runtime_error("Failed to load scene: bad vertex in " + path);
-                solids.back().push_back(p);
+                facets.push_back(p);
                 ++nv;
             } else if (tok == "endfacet") {
                 if (nv != 3) throw std::runtime_error("Failed to load scene: facet without 3 vertices in " + path);
+                nv = -1;
             }
         }
-        for (auto& sd : solids)  // a facet cut off by the end of the file
-            if (sd.size() % 4) throw std::runtime_error("Failed to load scene: truncated facet in " + path);
+        if (facets.size() % 4)  // a facet cut off by the end of the file
+            throw std::runtime_error("Failed to load scene: truncated facet in " + path);
     }
+    if (facets.empty()) throw std::runtime_error("A mesh in the file has no triangles: " + path);
     NiceMaterial m;
     m.diffuseColor = glm::vec3(0.6f);
     m.specular = 0.5f;
@@ -250,57 +273,144 @@ static std::vector<std::shared_ptr<Geometry>> loadStl(const std::string& path, b
     m.alpha = 1.0f;
     m.emission = glm::vec3(0.0f);
     m.emissionStrength = 1.0f;
-    std::vector<std::shared_ptr<Geometry>> out;
-    uint32_t mat = ~0u;
-    for (auto& sd : solids) {
-        if (sd.size() < 4) continue;  // a solid without facets has no mesh
-        auto g = std::make_shared<Geometry>();
-        for (size_t f = 0; f + 4 <= sd.size(); f += 4)
-            for (int k = 1; k <= 3; ++k) {
-                Vertex vx{};
-                vx.pos = sd[f + k];
-                vx.normal = sd[f];
-                g->indices.push_back((uint32_t)g->vertices.size());
-                g->vertices.push_back(vx);
-            }
-        g->initialized = false;
-        g->dynamic = dynamic;
-        g->isOpaque = true;
-        if (mat == ~0u) {  // one material for the file, shared by its solids
-            mat = global::materialIndex++;
-            global::materials.push_back(m);
+    std::vector<Vertex> verts;
+    std::vector<uint32_t> idx;
+    for (size_t f = 0; f + 4 <= facets.size(); f += 4)
+        for (int k = 1; k <= 3; ++k) {
+            Vertex vx{};
+            vx.pos = facets[f + k];
+            vx.normal = facets[f];
+            idx.push_back((uint32_t)verts.size());
+            verts.push_back(vx);
         }
-        g->matIndex.assign(g->indices.size() / 3, mat);
-        g->path = path;
-        out.push_back(std::move(g));
-    }
-    if (out.empty()) throw std::runtime_error("A mesh in the file has no triangles: " + path);
-    return out;
+    return {imported(std::move(verts), std::move(idx), intern_material(m), path, dynamic)};
 }
 
-// Wavefront OBJ: v / vt / vn / f (polygons fanned), UVs flipped vertically and
-// missing normals generated, like the reference's assimp flags
-// (Triangulate | GenNormals | FlipUVs, geometry.cpp:52-54).  STL: loadStl.
-std::vector<std::shared_ptr<Geometry>> loadScene(std::string_view fname, bool dynamic) {
-    std::string path(fname);
-    std::string ext = path.size() >= 4 ? path.substr(path.size() - 4) : std::string();
-    for (auto& ch : ext) ch = (char)std::tolower((unsigned char)ch);
-    if (ext == ".stl") return loadStl(path, dynamic);
-    if (ext != ".obj")
-        throw std::runtime_error("Failed to load scene: only .obj and .stl are supported in this build, " + path);
+// ------------------------------------------------------------------ OBJ + MTL
+// Wavefront OBJ with its MTL library, as assimp (5.0.1, the version the
+// reference's CMakeLists asks for) imports them for loadScene with
+// Triangulate | GenNormals | FlipUVs | PreTransformVertices
+// (geometry.cpp:50-54), then converted by geometry.cpp:78-150:
+//   * materials: assimp's list starts with its default material, then the
+//     MTL's newmtl entries in file order, then names that usemtl asks for but
+//     the library lacks (with default values).  A material's defaults are
+//     Kd 0.6 grey, Ke 0, d 1, Ns 0, Ni 1;  Kd, Ke, d (opacity), Tr (alpha =
+//     1 - Tr), Ns (shininess), Ni (IOR) and map_Kd are read.  The reference
+//     makes diffuseColor = Kd, emission = Ke with strength 1, alpha = opacity,
+//     ior = Ni, specular 0.5, metallic 0, transmission 0, roughness from the
+//     shininess (Ns <= 5: 1, else 1 - 0.025 sqrt(Ns - 5), at least 0.001; no
+//     Ns: 1 for an OBJ file) and the diffuse texture path relative to the
+//     OBJ's directory; equal materials are shared through the global list.
+//   * meshes: faces before any usemtl use the default material; every face
+//     corner is a vertex of its own (the OBJ importer does not share them);
+//     polygons are fanned; v of the UVs flipped; a face without normals gets
+//     its flat face normal normalize(cross(v1 - v0, v2 - v0)) (GenNormals on
+//     unshared vertices).  PreTransformVertices merges the meshes of each
+//     material: one Geometry per material that has faces, in material order.
+namespace {
+struct ObjMaterial {
+    std::string name;
+    glm::vec3 kd = glm::vec3(0.6f), ke = glm::vec3(0.0f);
+    float d = 1.0f, ns = 0.0f, ni = 1.0f;
+    std::string map_kd;
+};
+
+std::string trim(const std::string& t) {
+    size_t a = t.find_first_not_of(" \t\r\n"), b = t.find_last_not_of(" \t\r\n");
+    return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
+}
+
+// The file of a map_* statement: its last token (options such as -s u v or
+// -clamp on come first).
+std::string map_file(std::istringstream& ss) {
+    std::string tok, last;
+    while (ss >> tok) last = tok;
+    return last;
+}
+
+void read_mtl(const std::filesystem::path& file, std::vector<ObjMaterial>& mats) {
+    std::ifstream in(file);
+    if (!in) {
+        KF_WARN("OBJ material library " + file.string() + " not found, using default materials");
+        return;
+    }
+    std::string line;
+    ObjMaterial* cur = nullptr;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        std::string tag;
+        if (!(ss >> tag) || tag[0] == '#') continue;
+        if (tag == "newmtl") {
+            std::string rest;
+            std::getline(ss, rest);
+            mats.push_back(ObjMaterial());
+            mats.back().name = trim(rest);
+            cur = &mats.back();
+        } else if (!cur) {
+            continue;
+        } else if (tag == "Kd") {
+            ss >> cur->kd.x >> cur->kd.y >> cur->kd.z;
+        } else if (tag == "Ke") {
+            ss >> cur->ke.x >> cur->ke.y >> cur->ke.z;
+        } else if (tag == "d") {
+            ss >> cur->d;
+        } else if (tag == "Tr") {
+            float tr = 0.0f;
+            if (ss >> tr) cur->d = 1.0f - tr;
+        } else if (tag == "Ns") {
+            ss >> cur->ns;
+        } else if (tag == "Ni") {
+            ss >> cur->ni;
+        } else if (tag == "map_Kd") {
+            cur->map_kd = map_file(ss);
+        }
+    }
+}
+
+NiceMaterial convert(const ObjMaterial& om, const std::filesystem::path& dir) {
+    NiceMaterial m;
+    m.diffuseColor = om.kd;
+    m.emission = om.ke;
+    m.emissionStrength = 1.0f;
+    m.alpha = om.d;
+    m.ior = om.ni;
+    m.specular = 0.5f;
+    m.metallic = 0.0f;
+    m.transmission = 0.0f;
+    if (om.ns > 0.0f) {
+        m.roughness = om.ns <= 5.0f ? 1.0f : 1.0f - std::sqrt(om.ns - 5.0f) * 0.025f;
+        if (m.roughness < 0.001f) m.roughness = 0.001f;
+    } else {
+        m.roughness = 1.0f;
+    }
+    if (!om.map_kd.empty()) {
+        std::filesystem::path t(om.map_kd);
+        m.diffuseTexPath = t.is_absolute() ? t.string() : (dir / t).string();
+    }
+    return m;
+}
+}  // namespace
+
+static std::vector<std::shared_ptr<Geometry>> loadObjFile(const std::string& path, bool dynamic) {
     std::ifstream in(path);
     if (!in) throw std::runtime_error("Failed to load scene: cannot open " + path);
+    const std::filesystem::path dir = std::filesystem::absolute(path).parent_path();
+    std::vector<ObjMaterial> mats(1);  // [0]: assimp's default material
+    mats[0].name = "DefaultMaterial";
     std::vector<glm::vec3> P, N;
     std::vector<glm::vec2> T;
-    auto g = std::make_shared<Geometry>();
-    std::map<std::tuple<int, int, int>, uint32_t> cache;
-    bool haveNormals = true;
+    struct Corner {
+        int v, t, n;
+    };
+    std::vector<std::vector<Corner>> faces;  // triangles
+    std::vector<uint32_t> face_mat;
+    uint32_t cur = 0;
     std::string line;
     auto fix = [](int i, size_t n) { return i < 0 ? (int)n + i : i - 1; };
     while (std::getline(in, line)) {
         std::istringstream ss(line);
         std::string tag;
-        ss >> tag;
+        if (!(ss >> tag) || tag[0] == '#') continue;
         if (tag == "v") {
             glm::vec3 p;
             ss >> p.x >> p.y >> p.z;
@@ -313,8 +423,25 @@ std::vector<std::shared_ptr<Geometry>> loadScene(std::string_view fname, bool dy
             glm::vec2 t;
             ss >> t.x >> t.y;
             T.push_back(glm::vec2(t.x, 1.0f - t.y));
+        } else if (tag == "mtllib") {
+            std::string rest;
+            std::getline(ss, rest);
+            std::filesystem::path lib(trim(rest));
+            read_mtl(lib.is_absolute() ? lib : dir / lib, mats);
+        } else if (tag == "usemtl") {
+            std::string rest;
+            std::getline(ss, rest);
+            const std::string name = trim(rest);
+            uint32_t k = 0;
+            while (k < mats.size() && mats[k].name != name) ++k;
+            if (k == mats.size()) {
+                KF_WARN("OBJ material " + name + " not found for " + path + ", using default values");
+                mats.push_back(ObjMaterial());
+                mats.back().name = name;
+            }
+            cur = k;
         } else if (tag == "f") {
-            std::vector<uint32_t> poly;
+            std::vector<Corner> poly;
             std::string tok;
             while (ss >> tok) {
                 int vi = 0, ti = 0, ni = 0;
@@ -326,32 +453,53 @@ std::vector<std::shared_ptr<Geometry>> loadScene(std::string_view fname, bool dy
                     if (!ts.empty()) ti = std::stoi(ts);
                     if (b != std::string::npos) ni = std::stoi(tok.substr(b + 1));
                 }
-                int v = fix(vi, P.size()), t = ti ? fix(ti, T.size()) : -1, n = ni ? fix(ni, N.size()) : -1;
-                if (v < 0 || v >= (int)P.size()) throw std::runtime_error("Failed to load scene: bad face in " + path);
-                if (n < 0) haveNormals = false;
-                auto key = std::make_tuple(v, t, n);
-                auto it = cache.find(key);
-                if (it == cache.end()) {
-                    Vertex vx{};
-                    vx.pos = P[v];
-                    if (t >= 0 && t < (int)T.size()) vx.texCoord = T[t];
-                    if (n >= 0 && n < (int)N.size()) vx.normal = N[n];
-                    it = cache.emplace(key, (uint32_t)g->vertices.size()).first;
-                    g->vertices.push_back(vx);
-                }
-                poly.push_back(it->second);
+                Corner c{fix(vi, P.size()), ti ? fix(ti, T.size()) : -1, ni ? fix(ni, N.size()) : -1};
+                if (c.v < 0 || c.v >= (int)P.size()) throw std::runtime_error("Failed to load scene: bad face in " + path);
+                if (c.t >= (int)T.size()) c.t = -1;
+                if (c.n >= (int)N.size()) c.n = -1;
+                poly.push_back(c);
             }
-            for (size_t k = 1; k + 1 < poly.size(); ++k) g->indices.insert(g->indices.end(), {poly[0], poly[k], poly[k + 1]});
+            for (size_t k = 1; k + 1 < poly.size(); ++k) {
+                faces.push_back({poly[0], poly[k], poly[k + 1]});
+                face_mat.push_back(cur);
+            }
         }
     }
-    if (g->indices.empty()) throw std::runtime_error("A mesh in the file has no triangles: " + path);
-    if (!haveNormals) g->recalculateNormals();
-    g->path = path;
-    NiceMaterial m;
-    m.roughness = 1.0f;  // OBJ without shininess -> roughness 1 (geometry.cpp:118-121)
-    finish(g, dynamic, m);
-    g->path = path;
-    return {g};
+    std::vector<std::shared_ptr<Geometry>> out;
+    for (uint32_t m = 0; m < mats.size(); ++m) {
+        std::vector<Vertex> verts;
+        std::vector<uint32_t> idx;
+        for (size_t f = 0; f < faces.size(); ++f) {
+            if (face_mat[f] != m) continue;
+            const auto& c = faces[f];
+            const glm::vec3 flat = glm::normalize(glm::cross(P[c[1].v] - P[c[0].v], P[c[2].v] - P[c[0].v]));
+            const bool has_n = c[0].n >= 0 && c[1].n >= 0 && c[2].n >= 0;
+            for (int k = 0; k < 3; ++k) {
+                Vertex vx{};
+                vx.pos = P[c[k].v];
+                if (c[k].t >= 0) vx.texCoord = T[c[k].t];
+                vx.normal = has_n ? N[c[k].n] : flat;
+                idx.push_back((uint32_t)verts.size());
+                verts.push_back(vx);
+            }
+        }
+        if (idx.empty()) continue;
+        out.push_back(imported(std::move(verts), std::move(idx), intern_material(convert(mats[m], dir)), path,
+                               dynamic));
+    }
+    if (out.empty()) throw std::runtime_error("A mesh in the file has no triangles: " + path);
+    return out;
+}
+
+// loadScene (geometry.cpp:46-232) for Wavefront OBJ (+ MTL) and STL files.
+std::vector<std::shared_ptr<Geometry>> loadScene(std::string_view fname, bool dynamic) {
+    std::string path(fname);
+    std::string ext = path.size() >= 4 ? path.substr(path.size() - 4) : std::string();
+    for (auto& ch : ext) ch = (char)std::tolower((unsigned char)ch);
+    if (ext == ".stl") return loadStl(path, dynamic);
+    if (ext != ".obj")
+        throw std::runtime_error("Failed to load scene: only .obj and .stl are supported in this build, " + path);
+    return loadObjFile(path, dynamic);
 }
 
 std::shared_ptr<Geometry> loadObj(std::string_view path, bool dynamic) {

@@ -105,17 +105,54 @@ static int cpu_checks() {
     CHECK(throws([&] { cam->downloadLatestFrame(); }));  // nothing rendered yet
     CHECK(throws([&] { k.setWindow(100, 100); }));
 
-    // OBJ import (triangulated, normals generated when absent)
+    // OBJ import (triangulated, a vertex per face corner, flat normals
+    // generated when absent, assimp's default material)
     {
         std::ofstream f("/tmp/kf_facade_test.obj");
         f << "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nf 1/1 2/2 3/3 4/4\n";
     }
     auto obj = loadObj("/tmp/kf_facade_test.obj");
-    CHECK(obj->indices.size() == 6 && obj->vertices.size() == 4);
+    CHECK(obj->indices.size() == 6 && obj->vertices.size() == 6);
     CHECK(std::fabs(obj->vertices[0].normal.z - 1.0f) < 1e-6f && obj->vertices[2].texCoord.y == 0.0f);
+    CHECK(obj->vertices[5].pos == glm::vec3(0, 1, 0) && obj->vertices[3].pos == glm::vec3(0, 0, 0));
+    {
+        const NiceMaterial& m = global::materials[obj->matIndex[0]];
+        CHECK(m.diffuseColor == glm::vec3(0.6f) && m.ior == 1.0f && m.roughness == 1.0f && m.specular == 0.5f &&
+              m.emission == glm::vec3(0.0f) && m.emissionStrength == 1.0f && m.alpha == 1.0f && obj->isOpaque);
+    }
     CHECK(throws([&] { loadObj("/tmp/does_not_exist.obj"); }));
 
-    // STL import: ASCII (two solids -> two meshes, one shared material) and
+    // OBJ + MTL: one mesh per material in the library's order, materials read
+    // from Kd / Ns / d / Ni / map_Kd, shared between loads of the same file
+    {
+        std::ofstream m("/tmp/kf_facade_two.mtl");
+        m << "# two materials and an unused one\nnewmtl red\nKd 0.8 0.1 0.1\nNs 105\n\n"
+             "newmtl glass\nKd 1 1 1\nd 0.5\nNi 1.5\nKe 0.5 0.25 0\nmap_Kd -s 1 1 1 tex/glass.png\n"
+             "newmtl unused\nKd 0 0 1\nTr 0.25\n";
+        std::ofstream f("/tmp/kf_facade_two.obj");
+        f << "mtllib kf_facade_two.mtl\nv 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvn 0 0 1\n"
+             "usemtl glass\nf 1//1 3//1 4//1\nusemtl red\nf 1//1 2//1 3//1\nusemtl glass\nf 2 3 4\n";
+    }
+    size_t nmat = global::materials.size();
+    auto two = loadScene("/tmp/kf_facade_two.obj", false);
+    CHECK(two.size() == 2 && global::materials.size() == nmat + 2);
+    CHECK(two[0]->indices.size() == 3 && two[1]->indices.size() == 6 && two[1]->vertices.size() == 6);
+    {
+        const NiceMaterial& red = global::materials[two[0]->matIndex[0]];
+        const NiceMaterial& glass = global::materials[two[1]->matIndex[0]];
+        CHECK(red.diffuseColor == glm::vec3(0.8f, 0.1f, 0.1f) && std::fabs(red.roughness - 0.75f) < 1e-6f &&
+              red.ior == 1.0f && red.alpha == 1.0f && red.diffuseTexPath.empty() && two[0]->isOpaque);
+        CHECK(glass.alpha == 0.5f && glass.ior == 1.5f && glass.roughness == 1.0f && !two[1]->isOpaque &&
+              glass.emission == glm::vec3(0.5f, 0.25f, 0.0f) && glass.emissionStrength == 1.0f);
+        CHECK(glass.diffuseTexPath == "/tmp/tex/glass.png");
+    }
+    CHECK(two[1]->vertices[0].normal == glm::vec3(0, 0, 1) && two[1]->vertices[5].normal == glm::vec3(0, 0, 1));
+    auto again = loadScene("/tmp/kf_facade_two.obj", false);
+    CHECK(global::materials.size() == nmat + 2 && again[0]->matIndex[0] == two[0]->matIndex[0] &&
+          again[1]->matIndex[0] == two[1]->matIndex[0]);
+    CHECK(throws([&] { loadObj("/tmp/kf_facade_two.obj"); }));  // complex scene
+
+    // STL import: ASCII (two solids -> one mesh of their shared material) and
     // binary (80-byte header, facet count, 50-byte facets); three vertices of
     // its own per facet carrying the facet normal; assimp's default material
     {
@@ -125,11 +162,14 @@ static int cpu_checks() {
              "   vertex 0 1 0\n  endloop\n endfacet\nendsolid a\nsolid b\n facet normal 1 0 0\n  outer loop\n"
              "   vertex 2 0 0\n   vertex 2 1 0\n   vertex 2 0 1\n  endloop\n endfacet\nendsolid b\n";
     }
+    nmat = global::materials.size();
     auto stl = loadScene("/tmp/kf_facade_test.stl", false);
-    CHECK(stl.size() == 2 && stl[0]->vertices.size() == 6 && stl[0]->indices.size() == 6);
-    CHECK(stl[1]->vertices.size() == 3 && stl[1]->vertices[2].pos.z == 1.0f && stl[1]->vertices[0].normal.x == 1.0f);
+    CHECK(stl.size() == 1 && stl[0]->vertices.size() == 9 && stl[0]->indices.size() == 9);
+    CHECK(stl[0]->vertices[8].pos.z == 1.0f && stl[0]->vertices[6].normal.x == 1.0f);
     CHECK(stl[0]->vertices[4].pos.x == 1.0f && stl[0]->vertices[4].pos.y == 1.0f && stl[0]->vertices[5].normal.z == 1.0f);
-    CHECK(stl[0]->matIndex.size() == 2 && stl[0]->matIndex[0] == stl[1]->matIndex[0] && stl[0]->isOpaque);
+    CHECK(stl[0]->matIndex.size() == 3 && stl[0]->matIndex[2] == stl[0]->matIndex[0] && stl[0]->isOpaque);
+    CHECK(loadScene("/tmp/kf_facade_test.stl", false)[0]->matIndex[0] == stl[0]->matIndex[0]);
+    CHECK(global::materials.size() <= nmat + 1);
     {
         const NiceMaterial& m = global::materials[stl[0]->matIndex[0]];
         CHECK(m.diffuseColor == glm::vec3(0.6f) && m.roughness == 0.0f && m.specular == 0.5f && m.ior == 1.4f &&
