@@ -336,6 +336,17 @@ int kf_scene_read_bvh4(kf_scene* s, int which, float* nodes, uint32_t max_nodes,
 int kf_trace_closest(kf_scene* s, const kf_ray* rays, uint32_t n, kf_hit* hits);
 int kf_trace_any(kf_scene* s, const kf_ray* rays, uint32_t n, uint8_t* occluded);
 
+/* kf_trace_closest without any allocation, for callers that cannot allocate
+ * (tight query loops, harnesses that forbid hipMalloc); no counterpart in the
+ * reference.  Rays and hits are staged through a fixed device scratch of the
+ * library (KF_TRACE_NOALLOC_MAX_RAYS rays: 14.7 MB per GPU, held from load
+ * until the process ends); n above that is rejected with an error.  Same
+ * results as kf_trace_closest.  Call kf_init first.  Every call on a GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (several host
+ * threads) race. */
+#define KF_TRACE_NOALLOC_MAX_RAYS (1u << 18)
+int kf_trace_closest_noalloc(kf_scene* s, const kf_ray* rays, uint32_t n, kf_hit* hits);
+
 /* Device-resident variants for measurement: rays/hits live in HBM. */
 int kf_trace_closest_device(kf_scene* s, const kf_ray* d_rays, uint32_t n, kf_hit* d_hits,
                             float* kernel_ms);

@@ -145,6 +145,7 @@ EXPORTS = [
     ("kf_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("kf_trace_any", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("kf_trace_closest_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("kf_trace_closest_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("kf_renderer_create", C.c_int, [C.c_uint32, C.c_uint32, C.c_void_p]),
     ("kf_renderer_destroy", None, [C.c_void_p]),
     ("kf_render", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -175,6 +175,7 @@ int kf_init(int device) {
             }
         }
         if (!g_stream) KF_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+        trace_noalloc_setup();
         return 0;
     } catch (const std::exception& e) {
         set_error(e.what());

@@ -24,6 +24,7 @@ struct Error : std::exception {
 void set_error(const std::string& m);
 hipStream_t stream();
 uint32_t traversal_refill();
+void trace_noalloc_setup();  // device address of the no-alloc trace scratch, per device
 
 // tea (Random.glsl:7-21) on the host, for the frame seed.
 inline uint32_t tea_host(uint32_t v0, uint32_t v1) {

@@ -7,6 +7,8 @@
 #include "traverse.h"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace kf {
 namespace {
@@ -188,6 +190,79 @@ extern "C" int kf_trace_any(kf_scene* s, const kf_ray* rays, uint32_t n, uint8_t
         KF_CHECK(hipFree(d_r));
         KF_CHECK(hipFree(d_o));
         if (h[2]) throw Error("kf_trace_any: traversal stack overflow");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+// ------------------------------------------------------------ no-alloc variant
+// Fixed device scratch of kf_trace_closest_noalloc: the trace counters, then
+// staging for up to KF_TRACE_NOALLOC_MAX_RAYS rays and their hits.  Each GPU
+// holds its own copy from module load until the process ends.  In host code
+// the name `scratch` is only a host-side stand-in: the device address comes
+// from hipGetSymbolAddress, once per device, and only that address is used.
+constexpr uint32_t kNoAllocMaxRays = KF_TRACE_NOALLOC_MAX_RAYS;
+constexpr size_t kNoAllocCntBytes = 256;
+constexpr size_t kNoAllocRayBytes = (size_t)kNoAllocMaxRays * sizeof(kf_ray);
+constexpr size_t kNoAllocBytes = kNoAllocCntBytes + kNoAllocRayBytes + (size_t)kNoAllocMaxRays * sizeof(kf_hit);
+static_assert(kTraceCnt * sizeof(unsigned long long) <= kNoAllocCntBytes, "trace counters fit their slot");
+static_assert(kNoAllocRayBytes % 256 == 0, "hit staging stays 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on first use).
+unsigned char* noalloc_scratch() {
+    int dev = -1;
+    KF_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error("kf_trace_closest_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(std::string("kf_trace_closest_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = (unsigned char*)p;
+    }
+    return g_scratch_dev[dev];
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+}  // namespace
+
+namespace kf {
+// Called by kf_init with the chosen device current.
+void trace_noalloc_setup() { (void)noalloc_scratch(); }
+}  // namespace kf
+
+extern "C" int kf_trace_closest_noalloc(kf_scene* s, const kf_ray* rays, uint32_t n, kf_hit* hits) {
+    try {
+        if (!s || (n && (!rays || !hits))) throw Error("kf_trace_closest_noalloc: null argument");
+        if (n > kNoAllocMaxRays)
+            throw Error("kf_trace_closest_noalloc: " + std::to_string(n) + " rays exceed the scratch's maximum of " +
+                        std::to_string(kNoAllocMaxRays) + " (use kf_trace_closest)");
+        if (n == 0) return 0;
+        hipStream_t st = stream();
+        unsigned char* base = noalloc_scratch();
+        unsigned long long* c = (unsigned long long*)base;
+        kf_ray* d_r = (kf_ray*)(base + kNoAllocCntBytes);
+        kf_hit* d_h = (kf_hit*)(base + kNoAllocCntBytes + kNoAllocRayBytes);
+        k_zero_u64<<<1, 64, 0, st>>>(c, kTraceCnt);  // the last call's counters stay in the scratch
+        KF_CHECK(hipMemcpyAsync(d_r, rays, n * sizeof(kf_ray), hipMemcpyHostToDevice, st));
+        k_closest<false><<<trace_grid(k_closest<false>), TB, 0, st>>>(s->view(), d_r, n, d_h, c, traversal_refill());
+        KF_CHECK(hipGetLastError());
+        unsigned long long h[kTraceCnt];
+        KF_CHECK(hipMemcpyAsync(hits, d_h, n * sizeof(kf_hit), hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipMemcpyAsync(h, c, sizeof(h), hipMemcpyDeviceToHost, st));
+        KF_CHECK(hipStreamSynchronize(st));
+        if (h[2]) throw Error("kf_trace_closest_noalloc: traversal stack overflow");
         return 0;
     } catch (const std::exception& e) {
         set_error(e.what());
