@@ -26,12 +26,21 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s at 1080p; time-to-relMSE 1e-2, denoised vs plain spp, 1 GPU"
 
 
+def host_threads():
+    """Threads the CPU leg may use: the process's CPU share (OMP_NUM_THREADS
+    where the host sets it, else the CPUs this process may run on)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return (int(env) if env.isdigit() and int(env) > 0 else aff), aff
+
+
 def cpu_baseline(workload, budget_s=15.0):
     """Time the FP32 oracle on bands of rows of the same frame (bounded CPU
     sample): 64-row bands from the middle of the frame, wrapping around, until
-    about budget_s seconds of rendering have run."""
+    about budget_s seconds of rendering have run, on every thread of the
+    process's CPU share."""
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
+    threads, aff = host_threads()
     sc = oracle.Scene(workload.data)
     W, H = workload.width, workload.height
     band = min(64, H)
@@ -46,6 +55,7 @@ def cpu_baseline(workload, budget_s=15.0):
         rows += y1 - y
         y = 0 if y1 >= H else y1
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "host_cpu_count": os.cpu_count(), "affinity_cpus": aff,
             "sample": "FP32 CPU oracle (restatement of the reference shaders), %d rows of the %dx%d frame in "
                       "64-row bands from row %d (wrapping to row 0 after the last row), %d spp, %d rays in %.1f s"
                       % (rows, W, H, (H // 2) // band * band, workload.pc.spp, rays, dt)}
@@ -87,20 +97,38 @@ def unet_rate(den, rd, width, height, n_images, reps=5):
     return {"kernel_ms": round(k, 3), "tflops": round(tf, 1), "frac_of_bf16_peak": round(tf / MFMA_BF16_PEAK_TFLOPS, 3)}
 
 
+FP64_ROWS = list(range(15, 1080, 30))
+FP64_REF = os.path.join(ROOT, "tests", "golden", "fp64_config2_rows_4096spp.npy")
+
+
 def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
-    """Wall time to reach relMSE <= target against a high-spp reference of the
-    same frame: plain tracer (spp doubling) vs tracer + denoiser; each point
-    the median of 3 frames (the same seed: the last frame is the one scored)."""
+    """Wall time to reach relMSE <= target against the FP64 CPU oracle: plain
+    tracer (spp doubling) vs tracer + denoiser; each point the median of 3
+    frames (the same seed: the last frame is the one scored).  The reference
+    is the FP64 oracle's 4096-spp render of every 30th row of the 1080p frame
+    (36 rows x 1920 px, seed 424242, written by tests/golden/make_fp64_refs.py);
+    frames are scored on those rows.  The GPU's own 4096-spp frame is scored
+    against it too (the converged-image relMSE)."""
     import numpy as np
     from kuafu_amd.denoiser import Denoiser
     pc = w.pc
+    fp64 = os.path.exists(FP64_REF) and (w.width, w.height) == (1920, 1080)
     chunk = 128
     pc.spp = chunk
     for f in range(ref_spp // chunk):
         pc.frame_count = f
         rd.render(gs, w.cam, pc, seed=900000 + f)
-    ref = rd.read(0)[..., :3].astype(np.float64).copy()
+    gpu_ref = rd.read(0)[..., :3].astype(np.float64).copy()
     pc.frame_count = 0
+    if fp64:
+        ref = np.load(FP64_REF).astype(np.float64)
+        rows = FP64_ROWS
+        score = lambda img: relmse(img[rows], ref)  # noqa: E731
+        ref_name = ("FP64 CPU oracle, %d spp, rows 15, 45, ..., 1065 of the frame (36 x 1920 px), seed 424242 "
+                    "(tests/golden/fp64_config2_rows_4096spp.npy)" % ref_spp)
+    else:
+        score = lambda img: relmse(img, gpu_ref)  # noqa: E731
+        ref_name = "GPU FP32 tracer, %d spp, same frame (no FP64 rows for this frame size)" % ref_spp
 
     def timed(fn, reps=3):  # median wall time of reps calls (the last call's result stays)
         ts = []
@@ -115,7 +143,7 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     while spp <= 1024:
         pc.spp = spp
         dt = timed(lambda: rd.render(gs, w.cam, pc, seed=31 * spp + 1))
-        e = relmse(rd.read(0)[..., :3], ref)
+        e = score(rd.read(0)[..., :3])
         plain.append({"spp": spp, "ms": round(dt * 1e3, 2), "relmse": round(e, 5)})
         if e <= target:
             break
@@ -138,7 +166,7 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
             den.apply(rd)
 
         dt = timed(frame)
-        e = relmse(rd.read(0)[..., :3], ref)
+        e = score(rd.read(0)[..., :3])
         dn.append({"spp": spp, "ms": round(dt * 1e3, 2), "relmse": round(e, 5)})
         if e <= target:
             break
@@ -148,16 +176,17 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     for spp in (1, 2, 4, 8):
         pc.spp = spp
         rd.render(gs, w.cam, pc, seed=41 * spp + 7)
-        noisy = relmse(rd.read(0)[..., :3], ref)
+        noisy = score(rd.read(0)[..., :3])
         t = time.perf_counter()
         den.apply(rd)
         dt = time.perf_counter() - t
-        table.append({"spp": spp, "noisy_relmse": round(noisy, 5), "denoised_relmse": round(relmse(rd.read(0)[..., :3], ref), 5),
+        table.append({"spp": spp, "noisy_relmse": round(noisy, 5), "denoised_relmse": round(score(rd.read(0)[..., :3]), 5),
                       "denoise_ms": round(dt * 1e3, 2)})
     pc.spp = 16
     hit_p = [p for p in plain if p["relmse"] <= target]
     hit_d = [d for d in dn if d["relmse"] <= target]
-    res = {"target_relmse": target, "reference": "GPU FP32 tracer, %d spp, same frame" % ref_spp,
+    res = {"target_relmse": target, "reference": ref_name,
+           "gpu_%dspp_relmse" % ref_spp: round(score(gpu_ref), 6) if fp64 else None,
            "plain": plain, "denoised": dn, "denoiser_by_spp": table,
            "unet": unet_rate(den, rd, w.width, w.height, 1),
            "denoised_schedule": {k: tune1[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow")}}
@@ -250,11 +279,59 @@ def batched_environments(frames=8, schedule=None):
     return res
 
 
+def active_light_frame(steps=3):
+    """The active-light sensing frame (1280x720, 32 spp, depth 6): left and
+    right IR cameras (55 mm baseline) and the RGB camera rendered one after
+    another, as setCamera + run + download per camera; the IR images come back
+    as FP32 single-channel images, the RGB image as BGRA8 (downloadLatestFrame).
+    Host wall time per three-camera frame, downloads included."""
+    import numpy as np
+    from kuafu_amd import GpuScene, Renderer, scenes
+    ws = [scenes.config3(camera=c) for c in ("left", "right", "rgb")]
+    w = ws[0]
+    gs = GpuScene(w.data)
+    rd = Renderer(w.width, w.height)
+    pc = w.pc
+    pc.frame_count = 0
+    tune = rd.autotune(gs, w.cam, pc)
+
+    def frame(seed):
+        rays, kms = 0, 0.0
+        for k, cw in enumerate(ws):
+            st = rd.render(gs, cw.cam, pc, seed=seed + k)
+            rays += st.rays()
+            kms += st.trace_ms
+            if k < 2:
+                rd.download_gray32f()
+            else:
+                rd.download_bgra8()
+        return rays, kms
+
+    frame(1)
+    lat, rays, kms = [], 0, 0.0
+    for k in range(steps):
+        t = time.perf_counter()
+        r, m = frame(10 + 3 * k)
+        lat.append((time.perf_counter() - t) * 1e3)
+        rays += r
+        kms += m
+    gs.close()
+    rd.close()
+    return {"workload": "config3_active_light_720p_32spp_3cameras", "cameras": ["ir_left", "ir_right", "rgb"],
+            "outputs": ["f32 single channel", "f32 single channel", "bgra8 srgb"],
+            "width": w.width, "height": w.height, "spp": pc.spp, "path_depth": pc.path_depth,
+            "frame_ms_p50": round(float(np.median(lat)), 2), "rays_per_frame": rays // steps,
+            "mrays_s": round(rays / (sum(lat) * 1e-3) / 1e6, 1),
+            "kernel_mrays_s": round(rays / (kms * 1e-3) / 1e6, 1),
+            "schedule": {k: tune[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0",
+                                               "grab_shadow")}}
+
+
 def other_workloads(steps=3):
     """Mrays/s of the other scene recipes on the same build (not the headline)."""
     from kuafu_amd import GpuScene, Renderer, scenes
     out = {}
-    for w in (scenes.config3(camera="left"), scenes.config1()):
+    for w in (scenes.config1(),):
         gs = GpuScene(w.data)
         rd = Renderer(w.width, w.height)
         pc = w.pc
@@ -391,6 +468,7 @@ def main():
         out["time_to_relmse"] = time_to_relmse(w, gs, rd)
         print("bench: other scenes", file=sys.stderr, flush=True)
         out["other_workloads"] = other_workloads()
+        out["other_workloads"]["config3_active_light_720p_32spp_3cameras"] = active_light_frame()
         print("bench: batched environments", file=sys.stderr, flush=True)
         out["other_workloads"]["config5_envs128_512x512_8spp"] = batched_environments()
     if not args.no_cpu_baseline:
