@@ -433,18 +433,28 @@ def main():
     tris_per_ray = cst.tri_tests / max(closest, 1)
     bytes_per_ray = 32 + 20 + 128 * nodes_per_ray + 64 * insts_per_ray + 48 * tris_per_ray
     achieved = bytes_per_ray * ext_rays / (ext_ms * 1e-3) / 1e9
-    traffic = None
+    # HBM traffic per launch is not measurable from inside the process: it
+    # comes from the committed rocprofv3 counter profile of this kernel on
+    # this scene and schedule (FETCH_SIZE and WRITE_SIZE, each scaled by what
+    # the counter reads for a copy of known size), named with its build.
+    traffic, tsrc = None, None
     tpath = os.path.join(ROOT, "profiles", "extend_traffic_config2.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_launch")
+        tsrc = ("profiles/extend_traffic_config2.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of k_extend<false>, "
+                "schedule %s, build %s, %s)" % (tj.get("schedule", "?"), tj.get("build", "?"), tj.get("date", "?")))
+    avg_ms = ext_ms / max(launches, 1)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
+            "dram_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+            "dram_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
             "kernel": "k_extend (closest-hit traversal)",
             "bytes_per_ray": round(bytes_per_ray, 1), "node_visits_per_ray": round(nodes_per_ray, 2),
             "instance_visits_per_ray": round(insts_per_ray, 2), "tri_tests_per_ray": round(tris_per_ray, 2),
             "tlas_node_visits_per_ray": round(cst.tlas_node_visits / max(closest, 1), 2),
-            "avg_launch_ms": round(ext_ms / launches, 4)}
+            "avg_launch_ms": round(avg_ms, 4)}
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,

@@ -216,6 +216,7 @@ typedef struct kf_render_stats {
                                   then ended (not in extension_rays) */
     uint64_t tlas_node_visits; /* of node_visits, the TLAS ones */
     uint64_t tail_rays;        /* closest-hit rays traced by the tail kernel (in primary/extension_rays) */
+    uint64_t graph_captures;   /* frame graphs this renderer has captured so far (kf_render_opts.use_graph) */
 } kf_render_stats;
 
 /* Render options that the reference takes from Config / push constants. */

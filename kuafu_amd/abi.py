@@ -84,7 +84,7 @@ class RenderStats(C.Structure):
                 ("extend_ms", C.c_float), ("shadow_ms", C.c_float), ("other_ms", C.c_float),
                 ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
                 ("instance_visits", C.c_uint64), ("speculative_rays", C.c_uint64),
-                ("tlas_node_visits", C.c_uint64), ("tail_rays", C.c_uint64)]
+                ("tlas_node_visits", C.c_uint64), ("tail_rays", C.c_uint64), ("graph_captures", C.c_uint64)]
 
     def rays(self):
         return int(self.primary_rays + self.extension_rays + self.shadow_rays)

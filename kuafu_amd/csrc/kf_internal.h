@@ -236,6 +236,7 @@ struct Renderer {
     hipGraphExec_t graph = nullptr;
     std::vector<uint8_t> graph_key;
     uint32_t graph_ext = 0, graph_sh = 0;      // traversal launches in the graph
+    uint64_t graph_captures = 0;               // frame graphs captured by this renderer so far
     // Schedule picked by kf_renderer_autotune (0: built-in default).
     uint32_t tune_spb = 0, tune_refill = 0, tune_grab = 0, tune_tail = 0, tune_grab0 = 0, tune_grabs = 0;
     uint32_t n_env_last = 1;                   // environments of the last frame (the denoiser's batch)

@@ -1192,6 +1192,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
                 throw Error(std::string("kf_render: graph instantiation failed: ") + hipGetErrorString(e));
             }
             r->graph_key = key;
+            ++r->graph_captures;
             r->graph_ext = n_ext;
             r->graph_sh = n_sh;
         }
@@ -1233,6 +1234,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         stats->speculative_rays = totals[7];
         stats->tlas_node_visits = totals[8];
         stats->tail_rays = totals[9];
+        stats->graph_captures = r->graph_captures;
         stats->trace_ms = ms_total;
         stats->extend_ms = ms_ext;
         stats->shadow_ms = ms_sh;

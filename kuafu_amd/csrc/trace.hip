@@ -78,18 +78,26 @@ unsigned trace_grid(Kernel kernel) {
     return (unsigned)std::max(nb, 1) * (unsigned)cus;
 }
 
-// Streaming copy, four independent 16-byte loads in flight per thread.
-__global__ __launch_bounds__(TB) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
-        uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-        dst[i] = a;
-        dst[i + stride] = b;
-        dst[i + 2 * stride] = c;
-        dst[i + 3 * stride] = d;
+// Streaming copy for the achievable-bandwidth figure: each block copies one
+// contiguous 64 KiB chunk, each thread four 16-byte vectors per round with all
+// loads issued before the stores, through nontemporal loads and stores (the
+// data is touched once).  Measured on MI355X against grid-stride variants and
+// hipMemcpy device-to-device: 5.4-5.7 TB/s (read + write) on 1-4 GiB, where
+// grid-stride copies reach 4.0-5.2 and hipMemcpy 4.6-5.1.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr size_t kCopyChunk = 4096;  // 16-byte vectors per block
+__global__ __launch_bounds__(TB) void k_copy(const u32x4* __restrict__ src, u32x4* __restrict__ dst, size_t n) {
+    const size_t b0 = (size_t)blockIdx.x * kCopyChunk;
+    const size_t b1 = b0 + kCopyChunk < n ? b0 + kCopyChunk : n;
+    size_t i = b0 + threadIdx.x;
+    for (; i + 3 * TB < b1; i += 4 * TB) {
+        u32x4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(&src[i + u * TB]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(v[u], &dst[i + u * TB]);
     }
-    for (; i < n; i += stride) dst[i] = src[i];
+    for (; i < b1; i += TB) dst[i] = src[i];
 }
 
 }  // namespace
@@ -102,14 +110,11 @@ extern "C" int kf_bench_copy(uint64_t bytes, int iters, float* gbps) {
         if (!gbps || bytes < 16 || iters < 1) throw Error("kf_bench_copy: bad argument");
         hipStream_t st = stream();
         size_t n = bytes / 16;
-        uint4 *a = nullptr, *b = nullptr;
+        u32x4 *a = nullptr, *b = nullptr;
         KF_CHECK(hipMalloc((void**)&a, n * 16));
         KF_CHECK(hipMalloc((void**)&b, n * 16));
         KF_CHECK(hipMemsetAsync(a, 1, n * 16, st));
-        int dev = 0, cus = 256;
-        KF_CHECK(hipGetDevice(&dev));
-        KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        unsigned grid = (unsigned)cus * 16;
+        const unsigned grid = (unsigned)((n + kCopyChunk - 1) / kCopyChunk);
         k_copy<<<grid, TB, 0, st>>>(a, b, n);  // warm-up
         hipEvent_t e0, e1;
         KF_CHECK(hipEventCreate(&e0));

@@ -164,6 +164,10 @@ def test_frame_graph_replays_match_eager_frames(gpu):
         assert (sg.primary_rays, sg.extension_rays, sg.shadow_rays) == (se.primary_rays, se.extension_rays,
                                                                         se.shadow_rays)
         assert sg.extend_launches == se.extend_launches and sg.trace_ms > 0
+        # captured at frame 0, again when the sample count changes (frame 5)
+        # and when the reservations change (frame 6); replayed otherwise
+        assert sg.graph_captures == 1 + (f >= 5) + (f >= 6), (f, sg.graph_captures)
+        assert se.graph_captures == 0
 
 
 def test_frame_graph_batched_environments(gpu):
