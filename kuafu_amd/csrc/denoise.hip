@@ -116,13 +116,18 @@ struct ConvShape {
 
 // FUSE bits: kFuseUp upsampled input channels, kFusePool pooled output,
 // kFusePack the 32-channel network input computed from the render targets,
-// kOutF32 FP32 output (Yf) instead of BF16 (Y), kMask output masked by M > 0.
-constexpr int kFuseUp = 1, kFusePool = 2, kFusePack = 4, kOutF32 = 8, kMask = 16;
+// kOutF32 FP32 output (Yf) instead of BF16 (Y), kMask output masked by M > 0,
+// kFixW one set of weights for the whole launch (one 32-channel input chunk
+// and at most BN outputs), loaded once before the loop so that their staging
+// registers are free inside it: with them the first layer fits 3 waves per
+// SIMD (1.45 -> 1.30 ms on 128 x 512x512).  The output layer at 4 waves with
+// fixed weights spills and was not faster.
+constexpr int kFuseUp = 1, kFusePool = 2, kFusePack = 4, kOutF32 = 8, kMask = 16, kFixW = 32;
 
 // Waves per SIMD the register budget is set for: 3 (168 registers) where that
-// needs no spills, else 2 (64 output channels per item, first layer, masked
-// data gradient).
-constexpr int conv_waves(int bn, int fuse) { return (bn > 32 || (fuse & (kFusePack | kMask))) ? 2 : 3; }
+// needs no spills, else 2 (64 output channels per item, masked data
+// gradient).
+constexpr int conv_waves(int bn, int fuse) { return (bn > 32 || (fuse & kMask)) ? 2 : 3; }
 
 template <int BN, int FUSE>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(BN, FUSE), conv_waves(BN, FUSE)))) void k_conv(const uint16_t* __restrict__ X, int ldx, int Cin,
@@ -273,7 +278,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(B
         }
         const bool has_next = ni < n_items;
         const Item nxt = has_next ? decode(ni) : cur;
-        const bool next_w = has_next && (nxt.co0 != cur.co0 || nc != c);
+        const bool next_w = (FUSE & kFixW) == 0 && has_next && (nxt.co0 != cur.co0 || nc != c);
         if (c == 0) {
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
@@ -699,7 +704,7 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
     const int fuse = (fz.X2 ? kFuseUp : 0) | (fz.P ? kFusePool : 0) | (fz.pk.rgba ? kFusePack : 0);
     if (fuse & (fuse - 1)) throw Error("conv: one fusion at a time");
     if (fuse && (M || Yf || cstore <= 16)) throw Error("conv: fusions write plain BF16 outputs of > 16 channels");
-    if (fz.pk.rgba && (Cin != CBK || fz.pk.h > H || fz.pk.w > W || fz.pk.h < 1 || fz.pk.w < 1))
+    if (fz.pk.rgba && (Cin != CBK || Cout > 32 || fz.pk.h > H || fz.pk.w > W || fz.pk.h < 1 || fz.pk.w < 1))
         throw Error("conv: bad packed input");
     const int flags = fuse | (Yf ? kOutF32 : 0) | (M ? kMask : 0);
     auto go = [&](auto bn, auto fl) {
@@ -730,7 +735,7 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
     } else if (fuse == kFusePool) {
         if (wide) go(I64(), std::integral_constant<int, kFusePool>()); else go(I32(), std::integral_constant<int, kFusePool>());
     } else if (fuse == kFusePack) {
-        go(I32(), std::integral_constant<int, kFusePack>());
+        go(I32(), std::integral_constant<int, kFusePack | kFixW>());  // 32 inputs, 32 outputs: one weight set
     } else {
         if (wide) go(I64(), std::integral_constant<int, 0>()); else outs(I32());
     }
