@@ -8,9 +8,9 @@ HBM before timing starts.
 value = rays traced (primary + extension + shadow, counted on the device) /
 wall time of the K timed frames / 1e6  [Mrays/s].
 roofline = the closest-hit traversal kernel (k_extend): algorithmic bytes per
-ray = 32 (ray) + 20 (hit) + 128 * BVH4 node visits + 64 * instance records +
-48 * triangle tests, counted by the instrumented traversal on the same frame,
-/ the kernel's HIP-event time.
+ray = 32 (ray) + 20 (hit) + 128 * BVH4 node visits + 192 * instance records
+(each with its BLAS root node) + 48 * triangle tests, counted by the
+instrumented traversal on the same frame, / the kernel's HIP-event time.
 cpu_baseline = the FP32 CPU oracle on a bounded band of rows of the same frame.
 """
 import argparse
@@ -431,7 +431,7 @@ def main():
     nodes_per_ray = cst.node_visits / max(closest, 1)
     insts_per_ray = cst.instance_visits / max(closest, 1)
     tris_per_ray = cst.tri_tests / max(closest, 1)
-    bytes_per_ray = 32 + 20 + 128 * nodes_per_ray + 64 * insts_per_ray + 48 * tris_per_ray
+    bytes_per_ray = 32 + 20 + 128 * nodes_per_ray + 192 * insts_per_ray + 48 * tris_per_ray
     achieved = bytes_per_ray * ext_rays / (ext_ms * 1e-3) / 1e9
     # HBM traffic per launch is not measurable from inside the process: it
     # comes from the committed rocprofv3 counter profile of this kernel on

@@ -382,8 +382,11 @@ __global__ void k_blas_box(uint32_t n_geom, const int* __restrict__ root, const 
 }
 
 // Instance records + padded world AABBs (corners transformed in double).
+// root: BVH4 root ref of each geometry (node index, ~triangle slot for a
+// BLAS of one triangle, kEmptyRef); blas4: the BVH4 nodes it indexes.
 __global__ void k_instances(uint32_t n, const kf_instance* __restrict__ src, const float* __restrict__ w2o,
                             uint32_t n_geom, const uint32_t* __restrict__ gflags, const int* __restrict__ root,
+                            const Node4* __restrict__ blas4, uint32_t n_blas4,
                             const float4* __restrict__ glo, const float4* __restrict__ ghi,
                             const uint32_t* __restrict__ inst_env, Inst* __restrict__ out,
                             float4* __restrict__ ilo, float4* __restrict__ ihi, uint32_t* __restrict__ iseg) {
@@ -394,9 +397,24 @@ __global__ void k_instances(uint32_t n, const kf_instance* __restrict__ src, con
     Inst rec;
     for (int r = 0; r < 3; ++r) rec.w2o[r] = make_float4(w2o[12 * i + 4 * r], w2o[12 * i + 4 * r + 1],
                                                          w2o[12 * i + 4 * r + 2], w2o[12 * i + 4 * r + 3]);
-    bool valid = g < n_geom && !(gflags[g] & KF_GEOM_HIDDEN) && root[g] != kEmptyRef;
+    const int r4 = g < n_geom ? root[g] : kEmptyRef;
+    bool valid = g < n_geom && !(gflags[g] & KF_GEOM_HIDDEN) && r4 != kEmptyRef && (r4 < 0 || (uint32_t)r4 < n_blas4);
     int flags = (valid ? kInstValid : 0) | ((g < n_geom && (gflags[g] & KF_GEOM_OPAQUE)) ? kInstOpaque : 0);
-    rec.info = make_int4(valid ? root[g] : kEmptyRef, (int)g, flags, 0);
+    rec.info = make_int4(valid ? r4 : kEmptyRef, (int)g, flags, 0);
+    rec.pad = make_int4(0, 0, 0, 0);
+    // the BLAS root node, as the traversal's instance step tests it
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    rec.lox = rec.hix = rec.loy = rec.hiy = rec.loz = rec.hiz = z;
+    rec.ref = make_int4((int)kEmptyItem, (int)kEmptyItem, (int)kEmptyItem, (int)kEmptyItem);
+    if (valid && r4 >= 0) {
+        const Node4 nd = blas4[r4];
+        rec.lox = nd.lox; rec.hix = nd.hix; rec.loy = nd.loy; rec.hiy = nd.hiy; rec.loz = nd.loz; rec.hiz = nd.hiz;
+        rec.ref = nd.ref;
+    } else if (valid) {  // one triangle: its box (the geometry's) as the only child
+        const float4 a = glo[g], b = ghi[g];
+        rec.lox.x = a.x; rec.hix.x = b.x; rec.loy.x = a.y; rec.hiy.x = b.y; rec.loz.x = a.z; rec.hiz.x = b.z;
+        rec.ref.x = (int)((3u << 30) | (uint32_t)~r4);  // triangle item
+    }
     out[i] = rec;
     float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0);
     if (valid) {
@@ -955,8 +973,9 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     float4* ihi = dalloc<float4>(std::max<uint32_t>(n, 1));
     uint32_t* iseg = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
     if (n)
-        k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, s.d_gflags, s.d_blas_root4, glo,
-                                                  ghi, multi ? s.d_inst_env : nullptr, s.d_inst, ilo, ihi, iseg);
+        k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, s.d_gflags, s.d_blas_root4,
+                                                  s.d_blas4, s.n_blas4, glo, ghi,
+                                                  multi ? s.d_inst_env : nullptr, s.d_inst, ilo, ihi, iseg);
     const uint32_t n_seg = multi ? s.n_env : 1u;
     const uint32_t* seg_off = multi ? s.d_env_off : s.d_tlas_aux;
     const uint32_t* node_off = multi ? s.d_env_node_off : s.d_tlas_aux + 2;

@@ -65,12 +65,22 @@ struct alignas(16) Node4 {
     int4 pad;
 };
 
-// Instance record read by the traversal, 64 B: world->object rows and
-// (root_ref, geometry, flags, 0).  flags bit0 = opaque, bit1 = valid.
+// Instance record read by the traversal, 192 B.  Lines 0-6 repeat the root
+// node of the instance's BLAS in the Node4 layout (child boxes in object
+// space, child items), so the traversal step that reaches an instance also
+// tests its BLAS root's children, with the object-space ray it has just
+// computed; a BLAS of one triangle gets a node with that triangle as its
+// only child, an invalid instance a node without children.  Lines 7-9 are
+// the world->object rows, line 10 (root_ref, geometry, flags, 0) with flags
+// bit0 = opaque, bit1 = valid; line 11 is padding.
 struct alignas(16) Inst {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 ref;
     float4 w2o[3];
     int4 info;
+    int4 pad;
 };
+static_assert(sizeof(Inst) == 192, "instance record layout");
 
 enum : int { kInstOpaque = 1, kInstValid = 2 };
 constexpr int kEmptyRef = 0x7fffffff;  // root ref of a BLAS without triangles

@@ -191,8 +191,14 @@ __global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2
     }
 }
 
+// Register budgets of the two traversal kernels that run side by side
+// (k_extend of bounce b+1 beside k_shadow of bounce b): 4 waves of k_extend
+// at 104 VGPRs and one wave of k_shadow at 96 fill a SIMD's 512, so the
+// shadow rays are traced while the closest-hit launch is still resident.
+// amdgpu_num_vgpr counts in units of two registers on gfx950 (the unified
+// VGPR/AGPR file), hence 52 and 48.
 template <bool COUNT>
-__global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t cap, const uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_num_vgpr(52))) void k_extend(SceneView S, int bounce, uint32_t cap, const uint32_t* __restrict__ queue,
                                                const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
                                                const uint32_t* __restrict__ seed, float4* __restrict__ hit,
                                                uint32_t* __restrict__ hit_prim, unsigned long long* __restrict__ cnt,
@@ -218,7 +224,7 @@ __global__ __launch_bounds__(TB) void k_extend(SceneView S, int bounce, uint32_t
         hit[p] = make_float4(b.t, b.b1, b.b2, __uint_as_float(b.inst));
         hit_prim[p] = b.prim | (b.back << 31);
     };
-    trace_persistent<false, COUNT>(S, n, cnt + bounce * kCntPerBounce + kCntExtendChunks, refill, grab, true, load,
+    trace_persistent<false, COUNT>(S, n, cnt + bounce * kCntPerBounce + kCntExtendChunks, refill, grab, S.any_alpha != 0u, load,
                                    store, ovf, tot);
     if (COUNT) {
         wave_add(&cnt[kCntGlobal + 3], tot.nodes);
@@ -598,7 +604,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
     if (NEE_INLINE && ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
 
-__global__ __launch_bounds__(TB) void k_shadow(SceneView S, int bounce, uint32_t cap, const float4* __restrict__ sh_o,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_num_vgpr(48))) void k_shadow(SceneView S, int bounce, uint32_t cap, const float4* __restrict__ sh_o,
                                                const float4* __restrict__ sh_d, const uint32_t* __restrict__ sh_root,
                                                uint8_t* __restrict__ vis,
                                                unsigned long long* __restrict__ cnt, uint32_t refill, uint32_t grab) {
@@ -740,7 +746,7 @@ __global__ __launch_bounds__(TB) void k_tail(SceneView S, Frame F, const DevLigh
                 S.any_alpha ? tea(tea((p % F.WH) % F.WHe, S0 + 1u + F.s0 + p / F.WH), (uint32_t)b) : 0u;
             THit h;
             TCount c{0, 0, 0, 0};
-            traverse<false, false>(S, __float_as_uint(d.w), xyz(o), xyz(d), 0.001f, 10000.0f, true, aseed, h, c,
+            traverse<false, false>(S, __float_as_uint(d.w), xyz(o), xyz(d), 0.001f, 10000.0f, S.any_alpha != 0u, aseed, h, c,
                                    ovf);
             ++n_traced;
             hit[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.inst));

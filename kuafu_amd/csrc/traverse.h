@@ -28,13 +28,16 @@ constexpr float kUnit = 5.96046448e-8f;  // 2^-24
 constexpr float kGamma5 = 5.0f * kUnit / (1.0f - 5.0f * kUnit);
 constexpr float kSlabScale = 1.0f + 2.0f * kGamma5;
 
+// The shear axes kx, ky, kz are kept as lane masks (k0: axis 0, k1: axis
+// 1, neither: axis 2), which the compiler holds in scalar registers.
 struct Pre {
     v3 o, invd;  // invd: culling only (hardware reciprocal)
-    int kx, ky, kz;
+    bool x0, x1, y0, y1, z0, z1;
     float Sx, Sy, Sz;
 };
 
 KF_DEV float comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+KF_DEV float comp2(v3 a, bool k0, bool k1) { return k0 ? a.x : (k1 ? a.y : a.z); }
 // Component-wise choice.  (c ? a : b on structure lvalues compiles to a copy
 // from a chosen address, which forces the traversal state into memory.)
 KF_DEV v3 sel3(bool c, v3 a, v3 b) { return mk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
@@ -52,12 +55,16 @@ KF_DEV void precompute(v3 o, v3 d, Pre& p) {
     int ky = kx == 2 ? 0 : kx + 1;
     const float dz = comp(d, kz);
     const bool swap = dz < 0.0f;
-    p.kx = swap ? ky : kx;
-    p.ky = swap ? kx : ky;
-    p.kz = kz;
+    const int pkx = swap ? ky : kx, pky = swap ? kx : ky;
+    p.x0 = pkx == 0;
+    p.x1 = pkx == 1;
+    p.y0 = pky == 0;
+    p.y1 = pky == 1;
+    p.z0 = kz == 0;
+    p.z1 = kz == 1;
     p.Sz = 1.0f / dz;
-    p.Sx = comp(d, p.kx) * p.Sz;
-    p.Sy = comp(d, p.ky) * p.Sz;
+    p.Sx = comp(d, pkx) * p.Sz;
+    p.Sy = comp(d, pky) * p.Sz;
 }
 
 struct TriHit {
@@ -72,13 +79,13 @@ KF_DEV bool intersect(const Pre& p, float4 v0, float4 v1, float4 v2, float tmin,
     v3 A = mk(v0.x, v0.y, v0.z) - p.o;
     v3 B = mk(v1.x, v1.y, v1.z) - p.o;
     v3 C = mk(v2.x, v2.y, v2.z) - p.o;
-    float Akz = comp(A, p.kz), Bkz = comp(B, p.kz), Ckz = comp(C, p.kz);
-    float Ax = comp(A, p.kx) - p.Sx * Akz;
-    float Ay = comp(A, p.ky) - p.Sy * Akz;
-    float Bx = comp(B, p.kx) - p.Sx * Bkz;
-    float By = comp(B, p.ky) - p.Sy * Bkz;
-    float Cx = comp(C, p.kx) - p.Sx * Ckz;
-    float Cy = comp(C, p.ky) - p.Sy * Ckz;
+    float Akz = comp2(A, p.z0, p.z1), Bkz = comp2(B, p.z0, p.z1), Ckz = comp2(C, p.z0, p.z1);
+    float Ax = comp2(A, p.x0, p.x1) - p.Sx * Akz;
+    float Ay = comp2(A, p.y0, p.y1) - p.Sy * Akz;
+    float Bx = comp2(B, p.x0, p.x1) - p.Sx * Bkz;
+    float By = comp2(B, p.y0, p.y1) - p.Sy * Bkz;
+    float Cx = comp2(C, p.x0, p.x1) - p.Sx * Ckz;
+    float Cy = comp2(C, p.y0, p.y1) - p.Sy * Ckz;
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
@@ -131,13 +138,16 @@ KF_DEV v3 xform_dir(const Inst& I, v3 v) {
 // ---------------------------------------------------------------- traversal
 // The traversal is a state machine over 32-bit items: bits 31:30 the kind,
 // bits 29:0 an index into that kind's array.  Every step fetches exactly one
-// record (64 B for nodes and instances, 48 B for triangles) and processes it,
-// so all lanes of a wave wait on one memory round trip per step whatever kind
-// of record each lane is on.  The stack lives in private memory with its top
-// cached in a register: a pop takes the register and refills it from memory,
-// and that load overlaps the next record fetch.
+// record (BVH4 node, instance record, triangle) and processes it, so all
+// lanes of a wave wait on one memory round trip per step whatever kind of
+// record each lane is on.  An instance record carries its BLAS root node:
+// its step transforms the ray and tests the root's children at once, so
+// entering an instance costs no step of its own.  The stack keeps its top
+// in a register; the entry below it is read with every record fetch, so a
+// pop costs no extra round trip.
 enum : uint32_t { kTlasNode = 0u, kBlasNode = 1u, kInstance = 2u, kTriangle = 3u };
 constexpr uint32_t kIdxMask = 0x3fffffffu;  // scenes are limited to 2^30 records of each kind
+constexpr uint32_t kNoPrim = 0x7fffffffu;   // best.prim before the first hit
 
 KF_DEV uint32_t make_item(uint32_t kind, uint32_t idx) { return (kind << 30) | (idx & kIdxMask); }
 
@@ -154,7 +164,7 @@ KF_DEV T opaque_sgpr(T v) {
     return v;
 }
 
-__device__ const float4 kZeroRecord[8] = {};
+__device__ const float4 kZeroRecord[8] = {};  // at least the lines a step reads of any record
 
 struct TravScene {
     const float4 *tlas, *blas, *inst, *tri;
@@ -218,6 +228,8 @@ struct RingStack {
     }
 };
 
+static_assert(RingStack::kCap < 256 && PrivStack::kCap < 256, "stack depth in the low byte of Trav::spx");
+
 // One query: ray, t range, the TLAS root item it starts from (its
 // environment's; kEmptyItem: nothing to hit), alpha coin seed and an opaque
 // payload for the caller (where to write the result).
@@ -248,12 +260,13 @@ struct Trav {
     Pre obj;         // object-space ray of the current instance
     float tmin;
     THit best;
-    uint32_t cur_inst, cur_flags, cur_geom;
+    uint32_t cur_inst;  // instance whose BLAS the object-space ray belongs to
     uint32_t cur;    // item processed by the next step
     uint32_t top;    // stack top, cached in a register (valid when has_top)
-    int sp;          // entries in the private array below the cached top
+    uint32_t spx;    // bits 7:0 the entries in the stack below the cached
+                     // top, bits 31:8 the steps taken (malformed-tree guard)
     bool has_top, found, use_alpha;
-    uint32_t alpha_seed, steps;
+    uint32_t alpha_seed;
     TCount cnt;
 
     // Benign state for a lane without a query (it then reads only the zero
@@ -261,20 +274,20 @@ struct Trav {
     KF_DEV void reset() {
         o = d = mk(0.0f, 0.0f, 0.0f);
         obj.o = obj.invd = o;
-        obj.kx = 0;
-        obj.ky = 1;
-        obj.kz = 2;
+        obj.x0 = obj.y1 = true;  // kx = 0, ky = 1, kz = 2
+        obj.x1 = obj.y0 = obj.z0 = obj.z1 = false;
         obj.Sx = obj.Sy = obj.Sz = 0.0f;
-        tmin = 0.0f;
+        tmin = 0.001f;  // the path tracer's constant, so that its kernels can fold it
         best.t = best.b1 = best.b2 = 0.0f;
-        best.inst = best.prim = ~0u;
+        best.inst = ~0u;
+        best.prim = kNoPrim;  // closest hit: bit 31 holds the hit kind while tracing
         best.back = 0u;
-        cur_inst = cur_flags = cur_geom = 0u;
+        cur_inst = 0u;
         cur = top = 0u;
-        sp = 0;
+        spx = 0u;
         has_top = found = false;
         use_alpha = false;
-        alpha_seed = steps = 0u;
+        alpha_seed = 0u;
         cnt.nodes = cnt.tris = cnt.insts = cnt.tlas = 0u;
     }
 
@@ -287,21 +300,30 @@ struct Trav {
         tmin = f ? r.tmin : tmin;
         best.t = f ? r.tmax : best.t;
         best.inst = f ? ~0u : best.inst;
-        best.prim = f ? ~0u : best.prim;
+        best.prim = f ? kNoPrim : best.prim;
         best.b1 = f ? 0.0f : best.b1;
         best.b2 = f ? 0.0f : best.b2;
         best.back = f ? 0u : best.back;
         cur = f ? root : cur;
-        sp = f ? 0 : sp;
+        spx = f ? 0u : spx;
         has_top = f ? false : has_top;
         found = f ? false : found;
         use_alpha = use_alpha_;
         alpha_seed = f ? r.alpha_seed : alpha_seed;
-        steps = f ? 0u : steps;
         cnt.nodes = f ? 0u : cnt.nodes;
         cnt.tlas = f ? 0u : cnt.tlas;
         cnt.tris = f ? 0u : cnt.tris;
         cnt.insts = f ? 0u : cnt.insts;
+    }
+
+    // The query's result: closest hits keep the hit kind in bit 31 of
+    // best.prim while tracing.
+    KF_DEV THit result() const {
+        THit r = best;
+        const bool hit = !ANY && found;
+        r.prim = hit ? (best.prim & 0x7fffffffu) : ~0u;
+        r.back = hit ? (best.prim >> 31) : 0u;
+        return r;
     }
 
     // One record; returns false when the query has finished.  stk: kStack
@@ -329,33 +351,68 @@ struct Trav {
         // zero record instead and finish the query with the overflow flag.
         // Kept free of branches: an early exit here makes the compiler copy
         // the whole traversal state on every step.
-        const bool bad = live && (idx >= lim || ++steps > kMaxSteps);
+        int sp = (int)(spx & 0xffu);
+        const uint32_t steps = (spx & ~0xffu) + 0x100u;  // this step counted, in bits 31:8
+        const bool bad = live && (idx >= lim || steps > (kMaxSteps << 8));
         const bool on = live && !bad;
         const float4* base = kind == kTlasNode ? p_tlas : kind == kBlasNode ? p_blas : kind == kInstance ? p_inst : p_tri;
         const bool node_kind = kind <= kBlasNode;
-        const float4* P = on ? base + (node_kind ? 8u : (kind == kTriangle ? 3u : 4u)) * (size_t)idx : p_zero;
-        // Seven 16-byte global loads issued together.  BVH4 nodes are 128 B;
-        // instances (64 B) and triangles (48 B) repeat loads of their own
-        // first lines instead of reading past the record.  The empty asm
-        // needs all results, which keeps the compiler from sinking parts of
-        // the fetch into the kind-specific code below (each sunk load would
-        // cost the wave another memory round trip).
+        const float4* P = on ? base + (node_kind ? 8u : (kind == kTriangle ? 3u : 12u)) * (size_t)idx : p_zero;
+        // Ten 16-byte global loads issued together: lines 0-6 of a BVH4 node
+        // (128 B) or of an instance record's root node, lines 7-9 of an
+        // instance record (its transform; other lanes read the zero record
+        // there).  Triangles (48 B) repeat loads of their own first
+        // lines instead of reading past the record.  The empty asm needs all
+        // results, which keeps the compiler from sinking parts of the fetch
+        // into the kind-specific code below (each sunk load would cost the
+        // wave another memory round trip).
         const gf4* G = (const gf4*)P;
-        const bool wide = node_kind && on;
+        const bool wide = kind != kTriangle && on;
+        const bool irec = kind == kInstance && on;
+        const gf4* GI = irec ? G + 7 : (const gf4*)p_zero;
         const vf4 r0 = G[0], r1 = G[1], r2 = G[2], r3 = G[(kind == kTriangle && on) ? 2 : 3];
-        const vf4 r4 = G[wide ? 4 : 0], r5 = G[wide ? 5 : 1], r6 = G[wide ? 6 : 2];  // line 7 is padding
+        const vf4 r4 = G[wide ? 4 : 0], r5 = G[wide ? 5 : 1], r6 = G[wide ? 6 : 2];  // line 7 of a node is padding
+        const vf4 r7 = GI[0], r8 = GI[1], r9 = GI[2];
         const uint32_t below = stk.peek(sp);
-        asm volatile("" ::"v"(r0), "v"(r1), "v"(r2), "v"(r3), "v"(r4), "v"(r5), "v"(r6), "v"(below));
+        asm volatile("" ::"v"(r0), "v"(r1), "v"(r2), "v"(r3), "v"(r4), "v"(r5), "v"(r6), "v"(r7), "v"(r8), "v"(r9),
+                     "v"(below));
         const float4 q0 = make_float4(r0.x, r0.y, r0.z, r0.w), q1 = make_float4(r1.x, r1.y, r1.z, r1.w);
-        const float4 q2 = make_float4(r2.x, r2.y, r2.z, r2.w), q3 = make_float4(r3.x, r3.y, r3.z, r3.w);
+        const float4 q2 = make_float4(r2.x, r2.y, r2.z, r2.w);
 
         // Every lane evaluates the record as a node, an instance and a
         // triangle and keeps the result of its kind: in a wave64 nearly every
         // step has lanes of all three kinds, so branching would run all three
-        // anyway and would copy the traversal state at each join.
-        const bool is_node = kind <= kBlasNode && on;
-        const bool is_inst = kind == kInstance && on;
+        // anyway and would copy the traversal state at each join.  An
+        // instance record is a node as well (its BLAS root's children).
+        const bool is_node = kind != kTriangle && on;
+        const bool is_inst = irec;
         const bool is_tri = kind == kTriangle && on;
+
+        // instance: its BLAS is entered with the object-space ray, which this
+        // step's node test already uses for the BLAS root's children (an
+        // invalid instance's record has no children, so entering it changes
+        // nothing the traversal reads later).  Other lanes keep their ray.
+        {
+            const bool enter = is_inst;
+            Inst I;
+            I.w2o[0] = make_float4(r7.x, r7.y, r7.z, r7.w);
+            I.w2o[1] = make_float4(r8.x, r8.y, r8.z, r8.w);
+            I.w2o[2] = make_float4(r9.x, r9.y, r9.z, r9.w);
+            Pre nobj;
+            precompute(xform_point(I, o), xform_dir(I, d), nobj);
+            obj.o = sel3(enter, nobj.o, obj.o);
+            obj.invd = sel3(enter, nobj.invd, obj.invd);
+            obj.x0 = enter ? nobj.x0 : obj.x0;
+            obj.x1 = enter ? nobj.x1 : obj.x1;
+            obj.y0 = enter ? nobj.y0 : obj.y0;
+            obj.y1 = enter ? nobj.y1 : obj.y1;
+            obj.z0 = enter ? nobj.z0 : obj.z0;
+            obj.z1 = enter ? nobj.z1 : obj.z1;
+            obj.Sx = enter ? nobj.Sx : obj.Sx;
+            obj.Sy = enter ? nobj.Sy : obj.Sy;
+            obj.Sz = enter ? nobj.Sz : obj.Sz;
+            cur_inst = enter ? idx : cur_inst;
+        }
 
         // triangle, against the current instance's object-space ray
         TriHit h;
@@ -363,25 +420,32 @@ struct Trav {
         const uint32_t prim = __float_as_uint(q0.w);
         bool take = tri_hit;
         if (!ANY) {
-            take = take && (h.t < best.t || cur_inst < best.inst || (cur_inst == best.inst && prim < best.prim));
-            if (take && use_alpha && !(cur_flags & kInstOpaque)) {
-                const float a = S.mats[S.matidx[S.toff[cur_geom] + prim]].alpha;
-                take = !(a == 0.0f || alpha_coin(alpha_seed, cur_inst, prim) > a);
+            take = take && (h.t < best.t || cur_inst < best.inst ||
+                            (cur_inst == best.inst && prim < (best.prim & 0x7fffffffu)));
+            // scenes with non-opaque geometry only: the instance's flags and
+            // geometry are read here, for candidate hits
+            if (take && use_alpha) {
+                const float4 info = S.inst[12u * (size_t)cur_inst + 10u];  // Inst::info
+                if (!(__float_as_int(info.z) & kInstOpaque)) {
+                    const float a = S.mats[S.matidx[S.toff[__float_as_uint(info.y)] + prim]].alpha;
+                    take = !(a == 0.0f || alpha_coin(alpha_seed, cur_inst, prim) > a);
+                }
             }
         }
         best.t = take ? h.t : best.t;
         if (!ANY) {
             best.b1 = take ? h.b1 : best.b1;
             best.b2 = take ? h.b2 : best.b2;
-            best.back = take ? h.back : best.back;
             best.inst = take ? cur_inst : best.inst;
-            best.prim = take ? prim : best.prim;
+            best.prim = take ? (prim | (h.back << 31)) : best.prim;
         }
         found = found || take;
 
-        // node: four child boxes (Node4 layout: lox, hix, loy, hiy, loz, hiz, ref)
-        const bool blas = kind == kBlasNode;
-        const v3 ro = sel3(blas, obj.o, o), ri = sel3(blas, obj.invd, rcp3(d));  // rcp: culling only
+        // node: four child boxes (Node4 layout: lox, hix, loy, hiy, loz, hiz,
+        // ref), against the world ray (TLAS) or the object-space ray (BLAS
+        // nodes, and instance records, whose ray was set above)
+        const bool world = kind == kTlasNode;
+        const v3 ro = sel3(world, o, obj.o), ri = sel3(world, rcp3(d), obj.invd);  // rcp: culling only
         const float lox[4] = {r0.x, r0.y, r0.z, r0.w}, hix[4] = {r1.x, r1.y, r1.z, r1.w};
         const float loy[4] = {r2.x, r2.y, r2.z, r2.w}, hiy[4] = {r3.x, r3.y, r3.z, r3.w};
         const float loz[4] = {r4.x, r4.y, r4.z, r4.w}, hiz[4] = {r5.x, r5.y, r5.z, r5.w};
@@ -418,38 +482,15 @@ struct Trav {
         cx(1, 3);
         cx(1, 2);
 
-        // instance: enter its BLAS with the object-space ray
-        const int root = __float_as_int(q3.x);
-        const uint32_t iflags = __float_as_uint(q3.z);
-        const bool enter = is_inst && (iflags & kInstValid) && root != kEmptyRef;
-        Inst I;
-        I.w2o[0] = q0;
-        I.w2o[1] = q1;
-        I.w2o[2] = q2;
-        Pre nobj;
-        precompute(xform_point(I, o), xform_dir(I, d), nobj);
-        obj.o = sel3(enter, nobj.o, obj.o);
-        obj.invd = sel3(enter, nobj.invd, obj.invd);
-        obj.kx = enter ? nobj.kx : obj.kx;
-        obj.ky = enter ? nobj.ky : obj.ky;
-        obj.kz = enter ? nobj.kz : obj.kz;
-        obj.Sx = enter ? nobj.Sx : obj.Sx;
-        obj.Sy = enter ? nobj.Sy : obj.Sy;
-        obj.Sz = enter ? nobj.Sz : obj.Sz;
-        cur_inst = enter ? idx : cur_inst;
-        cur_flags = enter ? iflags : cur_flags;
-        cur_geom = enter ? __float_as_uint(q3.y) : cur_geom;
-        const uint32_t inst_root = root >= 0 ? make_item(kBlasNode, (uint32_t)root) : make_item(kTriangle, (uint32_t)~root);
-
         if (COUNT) {
-            cnt.nodes += is_node ? 1u : 0u;
-            cnt.tlas += (is_node && kind == kTlasNode) ? 1u : 0u;
+            cnt.nodes += (node_kind && on) ? 1u : 0u;
+            cnt.tlas += (on && kind == kTlasNode) ? 1u : 0u;
             cnt.tris += is_tri ? 1u : 0u;
             cnt.insts += is_inst ? 1u : 0u;
         }
         const int node_act = nhit >= 2 ? kPush : (nhit == 1 ? kNext : kPop);
-        int act = is_node ? node_act : (enter ? kNext : ((ANY && take) ? kDone : kPop));
-        const uint32_t next = is_node ? citem[0] : inst_root;
+        int act = is_node ? node_act : ((ANY && take) ? kDone : kPop);
+        const uint32_t next = citem[0];
         if (bad) {
             overflow = 1;
             act = kDone;
@@ -476,6 +517,7 @@ struct Trav {
         top = popping ? below : top;
         has_top = popping ? sp_in > 0 : has_top;
         if (popping && sp_in > 0) stk.popped(sp_in - 1);
+        spx = steps | (uint32_t)sp;
         return alive;
     }
 };
@@ -500,7 +542,7 @@ KF_DEV bool traverse(const SceneView& S, uint32_t root, v3 o, v3 d, float tmin, 
     if (root != kEmptyItem)
         while (T.step(TS, stk, overflow, true)) {
         }
-    best = T.best;
+    best = T.result();
     cnt.nodes += T.cnt.nodes;
     cnt.tlas += T.cnt.tlas;
     cnt.tris += T.cnt.tris;
@@ -551,7 +593,8 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
             bool fresh = false;
             TRay r;  // the ray this lane fetches, if any
             r.o = r.d = mk(0.0f, 0.0f, 0.0f);
-            r.tmin = r.tmax = 0.0f;
+            r.tmin = 0.001f;
+            r.tmax = 0.0f;
             r.root = kEmptyItem;
             r.alpha_seed = r.payload = 0u;
             const uint32_t rank = (uint32_t)__popcll(idle & below);
@@ -592,7 +635,7 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
                 // without instances has nothing to trace: finish it at once
                 // as a miss.
                 const bool empty = fresh && (!(r.tmin <= r.tmax) || r.root == kEmptyItem);
-                if (empty) store(r.payload, T.best, false);
+                if (empty) store(r.payload, T.result(), false);
                 live = live || (fresh && !empty);
             }
         }
@@ -604,7 +647,7 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
         const bool alive = T.step(TS, stk, overflow, live);
         if (live && !alive) {
             live = false;
-            store(payload, T.best, T.found);
+            store(payload, T.result(), T.found);
             if (COUNT) {
                 total.nodes += T.cnt.nodes;
                 total.tlas += T.cnt.tlas;
