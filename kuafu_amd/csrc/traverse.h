@@ -581,8 +581,6 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
     stk.spill = spill;
     bool live = false;
     uint32_t payload = 0;
-    const uint32_t lane = threadIdx.x & 63u;
-    const unsigned long long below = (1ull << lane) - 1ull;
     uint32_t chunk = blockIdx.x & 7u;  // wave-uniform from here on
     uint32_t moved = 0;                // chunks found used up
     uint32_t res_q = 0, res_n = 0;     // reserved queries [res_q, res_q + res_n)
@@ -597,14 +595,16 @@ KF_DEV void trace_persistent(const SceneView& S, uint32_t n, unsigned long long*
             r.tmax = 0.0f;
             r.root = kEmptyItem;
             r.alpha_seed = r.payload = 0u;
-            const uint32_t rank = (uint32_t)__popcll(idle & below);
+            // idle lanes below this one (mbcnt: no per-lane mask to keep)
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
             const bool wants = !live;
             uint32_t got = 0;
             while (got < nidle && (res_n > 0u || moved < 8u)) {
                 if (res_n == 0u) {
                     const uint64_t c0 = chunk_begin(n, chunk), len = chunk_begin(n, chunk + 1u) - c0;
                     unsigned long long base = 0;
-                    if (lane == 0) {
+                    if (__builtin_amdgcn_mbcnt_lo(~0u, 0u) == 0u) {  // lane 0
                         base = __hip_atomic_load(&ctr[chunk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         if (base < len) base = atomicAdd(&ctr[chunk], (unsigned long long)grab);
                     }
