@@ -41,6 +41,8 @@ static_assert(TB == (int)kRingThreads, "traversal kernels index their LDS stack 
 // traced by k_tail (part of the extension rays).
 constexpr int kMaxBounces = 64;
 constexpr int kCntPerBounce = 20;
+// throughput and depth of a path before its first bounce
+__device__ constexpr float4 kPathStart = {1.0f, 1.0f, 1.0f, 0.0f};  // .w: depth 0 as uint bits
 constexpr int kCntExtendChunks = 3;
 constexpr int kCntShadowChunks = 11;
 constexpr int kCntExtQueue = 19;
@@ -116,9 +118,7 @@ __global__ void k_zero(unsigned long long* c, int n) {
 }
 
 __global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2* __restrict__ jump,
-                                               float4* __restrict__ ray_o,
-                                               float4* __restrict__ ray_d, float4* __restrict__ thr,
-                                               float4* __restrict__ col, float4* __restrict__ rw,
+                                               float4* __restrict__ ray_o, float4* __restrict__ ray_d,
                                                uint32_t* __restrict__ seed, uint32_t* __restrict__ queue,
                                                unsigned long long* __restrict__ cnt) {
     uint32_t n = F.WH * F.nsamp;
@@ -170,9 +170,10 @@ __global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2
         // ray_d.w carries the path's TLAS root item through every bounce
         ray_o[p] = make_float4(org[0], org[1], org[2], 0.0f);
         ray_d[p] = make_float4(dir4[0], dir4[1], dir4[2], __uint_as_float(env_root_item(S, env)));
-        thr[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
-        col[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        rw[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // throughput (1, depth 0), colour (0) and BSDF weight (0) of a new
+        // path are implicit at bounce 0 (kPathStart): k_shade and k_connect
+        // use them there instead of reading them back, which saves 48 B of
+        // writes and as many of reads per path
         seed[p] = rs;
         // Camera rays are traced in 8-row bands, column by column, so a
         // wave's 64 rays cover an 8x8 pixel tile (paths stay stored by
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
         uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool active = q < n;
         uint32_t p = active ? queue[q] : 0;
-        const float4 t4 = active ? thr[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 t4 = active ? (bounce == 0 ? kPathStart : thr[p]) : make_float4(0.f, 0.f, 0.f, 0.f);
         uint32_t depth = __float_as_uint(t4.w);
         // The path is traced at the next bounce unless k_connect ends it for
         // a reason already known here (miss, emitter, zero throughput, depth):
@@ -641,10 +642,11 @@ KF_DEV bool connect_path(const Frame& F, int bounce, uint32_t p, float4* __restr
     const uint32_t K = F.slots;
     const uint32_t maxDepth = F.pc.path_depth;
     bool cont = false;
-    float4 t4 = thr[p];
+    const bool first = bounce == 0;  // the path's state is kPathStart, colour 0, weight 0
+    float4 t4 = first ? kPathStart : thr[p];
     v3 weight = xyz(t4);
     uint32_t depth = __float_as_uint(t4.w);
-    v3 color = xyz(col[p]);
+    v3 color = first ? splat(0.0f) : xyz(col[p]);
     float4 e4 = em[p];
     bool term = __float_as_uint(e4.w) != 0u;
     uint32_t rs = seed[p];
@@ -670,7 +672,9 @@ KF_DEV bool connect_path(const Frame& F, int bounce, uint32_t p, float4* __restr
     }
     // rgen:109-138
     color = color + xyz(e4) * weight;
-    weight = weight * xyz(rw[p]);
+    // k_shade wrote rw for every path it continues; a path it ended at
+    // bounce 0 still has the initial weight 0
+    weight = weight * ((first && term) ? splat(0.0f) : xyz(rw[p]));
     color = color + shadow * weight;
     uint32_t d2 = term ? maxDepth + 1 : depth;
     cont = true;
@@ -1084,7 +1088,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         k_zero<<<(nz + TB - 1) / TB, TB, 0, st>>>(r->counters, nz);
         if (timed) KF_CHECK(hipEventRecord(r->ev[0], st));
         k_raygen<<<std::min<uint32_t>((B + TB - 1) / TB, 65535u), TB, 0, st>>>(
-            S, F, d_jump, r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->seed, r->queue_a, r->counters);
+            S, F, d_jump, r->ray_o, r->ray_d, r->seed, r->queue_a, r->counters);
         uint32_t* qa = r->queue_a;
         uint32_t* qb = r->queue_b;
         // Per bounce b:  A: shade(b) -> B: [shadow(b)], connect(b) ; A:
