@@ -129,6 +129,109 @@ constexpr int kFuseUp = 1, kFusePool = 2, kFusePack = 4, kOutF32 = 8, kMask = 16
 // gradient).
 constexpr int conv_waves(int bn, int fuse) { return (bn > 32 || (fuse & kMask)) ? 2 : 3; }
 
+// Epilogue of a work item, straight from the accumulators (bias from Bsm,
+// ReLU, optional mask, BF16 or FP32 store, optional 2x2 average pool).
+template <int BN, int FUSE>
+__device__ __attribute__((always_inline)) inline void conv_store(const f32x4 (&acc)[4][BN / 16], const float* Bsm,
+                                                                int n, int y0, int x0, int co0, int wv, int lane,
+                                                                int H, int W, uint16_t* __restrict__ Y, int ldy,
+                                                                float* __restrict__ Yf, int ldyf, int cstore,
+                                                                const uint16_t* __restrict__ M, int ldm, int relu,
+                                                                uint16_t* __restrict__ P) {
+    const int fr = lane & 15;
+    const struct {
+        int n, y0, x0, co0;
+    } cur = {n, y0, x0, co0};
+    // Epilogue straight from the accumulators: lane holds channels
+    // co0 + nb * 16 + 4 (lane / 16) .. + 3 of pixel (lane % 16) of row mt.
+    const int xo = cur.x0 + (wv & 1) * 16 + fr;
+    // per-tile base (uniform, 64-bit) + per-lane 32-bit offsets
+    const size_t pbase = (size_t)(cur.n * H + cur.y0) * W + cur.x0;
+    const int xl = (wv & 1) * 16 + fr;
+#pragma unroll
+    for (int nb = 0; nb < BN / 16; ++nb) {
+        const int ch0 = cur.co0 + nb * 16 + (lane >> 4) * 4;
+        const float4 b4 = *(const float4*)&Bsm[nb * 16 + (lane >> 4) * 4];
+        const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
+        uint32_t pool[4][2] = {};
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const int yo = cur.y0 + 4 * (wv >> 1) + mt;
+            if (yo >= H || xo >= W || ch0 >= cstore) continue;
+            const int loc = (4 * (wv >> 1) + mt) * W + xl;  // pixel - pbase
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[e] = acc[mt][nb][e] + bv[e];
+                if (relu) v[e] = fmaxf(v[e], 0.0f);
+                if constexpr ((FUSE & kMask) != 0)
+                    if (ch0 + e < cstore && !(bf2f(M[pbase * ldm + (unsigned)(loc * ldm + ch0 + e)]) > 0.0f))
+                        v[e] = 0.0f;
+            }
+            if constexpr ((FUSE & kOutF32) != 0) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (ch0 + e < cstore) Yf[pbase * ldyf + (unsigned)(loc * ldyf + ch0 + e)] = v[e];
+            } else {
+                // v_cvt_pk_bf16_f32: round to nearest even, as f2bf
+                typedef float f2 __attribute__((ext_vector_type(2)));
+                typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+                const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[0], v[1]}, b2));
+                const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[2], v[3]}, b2));
+                uint16_t* yp = Y + pbase * ldy + (unsigned)(loc * ldy + ch0);
+                if (ch0 + 4 <= cstore) {
+                    *(uint2*)yp = make_uint2(lo, hi);
+                    if constexpr ((FUSE & kFusePool) != 0) {
+                        // 2x2 average pool of the rounded outputs: rows mt, mt + 1
+                        // here, columns xo, xo + 1 in this lane and the next
+                        pool[mt][0] = lo;
+                        pool[mt][1] = hi;
+                    }
+                } else {
+                    const uint32_t h4[2] = {lo, hi};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (ch0 + e < cstore) yp[e] = (uint16_t)(h4[e >> 1] >> (16 * (e & 1)));
+                }
+            }
+        }
+        if constexpr ((FUSE & kFusePool) != 0) {
+            // pooled pixel (yo / 2, xo / 2) from rows 2 mp, 2 mp + 1 of the lanes
+            // of columns xo (even) and xo + 1 (the next lane); summed in the
+            // order (2y, 2x), (2y, 2x + 1), (2y + 1, 2x), (2y + 1, 2x + 1)
+#pragma unroll
+            for (int mp = 0; mp < 2; ++mp) {
+                const int yo = cur.y0 + 4 * (wv >> 1) + 2 * mp;
+                uint32_t q4[4][2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    q4[0][h] = pool[2 * mp][h];
+                    q4[1][h] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pool[2 * mp][h], 0xB1, 0xF, 0xF, false);
+                    q4[2][h] = pool[2 * mp + 1][h];
+                    q4[3][h] =
+                        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pool[2 * mp + 1][h], 0xB1, 0xF, 0xF, false);
+                }
+                if ((fr & 1) || yo >= H || xo >= W || ch0 + 4 > cstore) continue;
+                float m[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float sum = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) sum += bf2f((uint16_t)(q4[k][e >> 1] >> (16 * (e & 1))));
+                    m[e] = 0.25f * sum;
+                }
+                typedef float f2 __attribute__((ext_vector_type(2)));
+                typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+                const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{m[0], m[1]}, b2));
+                const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{m[2], m[3]}, b2));
+                const size_t qbase = (size_t)(cur.n * (H >> 1) + (cur.y0 >> 1)) * (W >> 1) + (cur.x0 >> 1);
+                const int qloc = (2 * (wv >> 1) + mp) * (W >> 1) + (xl >> 1);
+                *(uint2*)(P + qbase * cstore + (unsigned)(qloc * cstore + ch0)) = make_uint2(lo, hi);
+            }
+        }
+    }
+}
+
 template <int BN, int FUSE>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(BN, FUSE), conv_waves(BN, FUSE)))) void k_conv(const uint16_t* __restrict__ X, int ldx, int Cin,
                                              const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
@@ -318,101 +421,170 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(B
             }
             if (dx == 1 && next_w) load_w(nxt.co0, nc * CBK);
         }
-        if (c == nchunk - 1) {
-            // Epilogue straight from the accumulators: lane holds channels
-            // co0 + nb * 16 + 4 (lane / 16) .. + 3 of pixel (lane % 16) of row mt.
-            const int xo = cur.x0 + (wv & 1) * 16 + fr;
-            // per-tile base (uniform, 64-bit) + per-lane 32-bit offsets
-            const size_t pbase = (size_t)(cur.n * H + cur.y0) * W + cur.x0;
-            const int xl = (wv & 1) * 16 + fr;
-#pragma unroll
-            for (int nb = 0; nb < BN / 16; ++nb) {
-                const int ch0 = cur.co0 + nb * 16 + (lane >> 4) * 4;
-                const float4 b4 = *(const float4*)&Bsm[nb * 16 + (lane >> 4) * 4];
-                const float bv[4] = {b4.x, b4.y, b4.z, b4.w};
-                uint32_t pool[4][2] = {};
-#pragma unroll
-                for (int mt = 0; mt < 4; ++mt) {
-                    const int yo = cur.y0 + 4 * (wv >> 1) + mt;
-                    if (yo >= H || xo >= W || ch0 >= cstore) continue;
-                    const int loc = (4 * (wv >> 1) + mt) * W + xl;  // pixel - pbase
-                    float v[4];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        v[e] = acc[mt][nb][e] + bv[e];
-                        if (relu) v[e] = fmaxf(v[e], 0.0f);
-                        if constexpr ((FUSE & kMask) != 0)
-                            if (ch0 + e < cstore && !(bf2f(M[pbase * ldm + (unsigned)(loc * ldm + ch0 + e)]) > 0.0f))
-                                v[e] = 0.0f;
-                    }
-                    if constexpr ((FUSE & kOutF32) != 0) {
-#pragma unroll
-                        for (int e = 0; e < 4; ++e)
-                            if (ch0 + e < cstore) Yf[pbase * ldyf + (unsigned)(loc * ldyf + ch0 + e)] = v[e];
-                    } else {
-                        // v_cvt_pk_bf16_f32: round to nearest even, as f2bf
-                        typedef float f2 __attribute__((ext_vector_type(2)));
-                        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
-                        const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[0], v[1]}, b2));
-                        const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[2], v[3]}, b2));
-                        uint16_t* yp = Y + pbase * ldy + (unsigned)(loc * ldy + ch0);
-                        if (ch0 + 4 <= cstore) {
-                            *(uint2*)yp = make_uint2(lo, hi);
-                            if constexpr ((FUSE & kFusePool) != 0) {
-                                // 2x2 average pool of the rounded outputs: rows mt, mt + 1
-                                // here, columns xo, xo + 1 in this lane and the next
-                                pool[mt][0] = lo;
-                                pool[mt][1] = hi;
-                            }
-                        } else {
-                            const uint32_t h4[2] = {lo, hi};
-#pragma unroll
-                            for (int e = 0; e < 4; ++e)
-                                if (ch0 + e < cstore) yp[e] = (uint16_t)(h4[e >> 1] >> (16 * (e & 1)));
-                        }
-                    }
-                }
-                if constexpr ((FUSE & kFusePool) != 0) {
-                    // pooled pixel (yo / 2, xo / 2) from rows 2 mp, 2 mp + 1 of the lanes
-                    // of columns xo (even) and xo + 1 (the next lane); summed in the
-                    // order (2y, 2x), (2y, 2x + 1), (2y + 1, 2x), (2y + 1, 2x + 1)
-#pragma unroll
-                    for (int mp = 0; mp < 2; ++mp) {
-                        const int yo = cur.y0 + 4 * (wv >> 1) + 2 * mp;
-                        uint32_t q4[4][2];
-#pragma unroll
-                        for (int h = 0; h < 2; ++h) {
-                            q4[0][h] = pool[2 * mp][h];
-                            q4[1][h] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pool[2 * mp][h], 0xB1, 0xF, 0xF, false);
-                            q4[2][h] = pool[2 * mp + 1][h];
-                            q4[3][h] =
-                                (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pool[2 * mp + 1][h], 0xB1, 0xF, 0xF, false);
-                        }
-                        if ((fr & 1) || yo >= H || xo >= W || ch0 + 4 > cstore) continue;
-                        float m[4];
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            float sum = 0.0f;
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) sum += bf2f((uint16_t)(q4[k][e >> 1] >> (16 * (e & 1))));
-                            m[e] = 0.25f * sum;
-                        }
-                        typedef float f2 __attribute__((ext_vector_type(2)));
-                        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
-                        const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{m[0], m[1]}, b2));
-                        const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{m[2], m[3]}, b2));
-                        const size_t qbase = (size_t)(cur.n * (H >> 1) + (cur.y0 >> 1)) * (W >> 1) + (cur.x0 >> 1);
-                        const int qloc = (2 * (wv >> 1) + mp) * (W >> 1) + (xl >> 1);
-                        *(uint2*)(P + qbase * cstore + (unsigned)(qloc * cstore + ch0)) = make_uint2(lo, hi);
-                    }
-                }
-            }
-        }
+        if (c == nchunk - 1)
+            conv_store<BN, FUSE>(acc, Bsm, cur.n, cur.y0, cur.x0, cur.co0, wv, lane, H, W, Y, ldy, Yf, ldyf, cstore, M, ldm,
+                                 relu, P);
         if (!has_next) break;
         i = ni;
         c = nc;
         cur = nxt;
         need_w = next_w;
+    }
+}
+
+// ---------------------------------------------------------- convolution, 8 waves
+// The forward convolutions without packed input, mask or FP32 output (plain,
+// upsampled input, pooled output), for 32 or 64 output channels per item:
+// blocks of 8 waves on 32 x 16 pixel tiles (wave w owns the same 16 x 4
+// pixel block as in k_conv, rows 4 (w / 2) ..), with two LDS buffers, each
+// holding one step's 34 x 18 halo and weights.  Both go from memory straight
+// into LDS (buffer loads with the LDS bit: lane l's 16 bytes land at the
+// instruction's LDS base + 16 l, and an offset past the resource writes
+// zeros), so the loads of step s + 1 run into one buffer while the waves
+// run the MFMAs of step s on the other, without staging registers.  The
+// MFMAs, their order and the epilogue are those of k_conv, so the outputs are
+// the same bits.
+constexpr int TB2 = 512, TY2 = 16, HYY2 = TY2 + 2, NHALO2 = HXX * HYY2;
+constexpr int kHaloUnits2 = NHALO2 * 4;                      // 16-byte units of the halo
+constexpr int kHaloLoads2 = (kHaloUnits2 + 63) / 64;         // wave loads that cover it
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int BN>
+struct Conv2Shape {
+    static constexpr int kWUnits = 9 * BN * 4;               // 16-byte units of one chunk's weights
+    static constexpr int kWLoads = kWUnits / 64;
+    static constexpr int kHalo = kHaloLoads2 * 64 * 8;       // halves; the last load's spill-over included
+    static constexpr int kBuf = kHalo + kWUnits * 8 + 2 * 64;  // + 64 FP32 biases
+    static_assert(kWUnits % 64 == 0, "whole wave loads of weights");
+};
+
+template <int BN, int FUSE>
+__global__ __launch_bounds__(TB2) void k_conv2(const uint16_t* __restrict__ X, int ldx, int Cin,
+                                              const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
+                                              int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
+                                              int cstore, int relu, int n_items, const uint16_t* __restrict__ X2,
+                                              int ldx2, int csplit, uint16_t* __restrict__ P) {
+    static_assert((FUSE & ~(kFuseUp | kFusePool)) == 0 && (BN == 32 || BN == 64), "k_conv2 variants");
+    using SH = Conv2Shape<BN>;
+    // two separate arrays, so that the compiler tells the buffers apart and
+    // the MFMA reads of one need not wait for the loads into the other
+    __shared__ __attribute__((aligned(16))) uint16_t buf0[SH::kBuf];
+    __shared__ __attribute__((aligned(16))) uint16_t buf1[SH::kBuf];
+    const int tiles_x = (W + TX - 1) / TX, tiles_y = (H + TY2 - 1) / TY2;
+    const int n_tiles = N * tiles_x * tiles_y;
+    const int nchunk = Cin / CBK;
+    const int t = threadIdx.x, lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int fr = lane & 15;
+    struct Item {
+        int n, y0, x0, co0;
+    };
+    auto decode = [&](int i) {  // i is uniform: the fields live in scalar registers
+        Item it;
+        const int tile = i % n_tiles;
+        it.co0 = __builtin_amdgcn_readfirstlane((i / n_tiles) * BN);
+        it.x0 = __builtin_amdgcn_readfirstlane((tile % tiles_x) * TX);
+        const int r = tile / tiles_x;
+        it.y0 = __builtin_amdgcn_readfirstlane((r % tiles_y) * TY2);
+        it.n = __builtin_amdgcn_readfirstlane(r / tiles_y);
+        return it;
+    };
+    // The loads of one step into buffer B: halo unit u = load j, lane l is
+    // LDS row u / 4 (a halo pixel), 16-byte slot u % 4, which holds channel
+    // group (u % 4) ^ ((row / 2) % 4) of that pixel (the swizzle of cofs).
+    auto issue = [&](uint16_t* B, const Item& it, int ci0, bool with_w) __attribute__((always_inline)) {
+        const bool up = (FUSE & kFuseUp) != 0 && ci0 >= csplit;  // the upsampled half-resolution tensor
+        const int sh = up ? 1 : 0, Wl = W >> sh, ld = up ? ldx2 : ldx, c0 = up ? ci0 - csplit : ci0;
+        const uint32_t img = (uint32_t)(H >> sh) * (uint32_t)Wl * (uint32_t)ld;
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc((up ? X2 : X) + (size_t)it.n * img, 2 * img);
+        for (int j = wv; j < kHaloLoads2; j += TB2 / 64) {
+            const int u = j * 64 + lane, r = u >> 2;
+            const int q = (u & 3) ^ ((r >> 1) & 3);
+            const int hy = r / HXX, hx = r - hy * HXX;
+            const int yy = it.y0 - 1 + hy, xx = it.x0 - 1 + hx;
+            const bool ok = u < kHaloUnits2 && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W;
+            const uint32_t off = ok ? 2u * (uint32_t)(((yy >> sh) * Wl + (xx >> sh)) * ld + c0 + q * 8) : kOOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void*)(B + j * 512), 16, off, 0, 0, 0);
+        }
+        if (with_w) {
+            const __amdgpu_buffer_rsrc_t rw = buffer_rsrc(Wt, 18u * Cout * Cin);
+            uint16_t* Ws = B + SH::kHalo;
+            for (int j = wv; j < SH::kWLoads; j += TB2 / 64) {
+                const int u = j * 64 + lane, r = u >> 2;
+                const int q = (u & 3) ^ ((r >> 1) & 3);
+                const int tap = r / BN, co = it.co0 + (r - tap * BN);
+                const uint32_t off = co < Cout ? 2u * (uint32_t)((tap * Cout + co) * Cin + ci0 + q * 8) : kOOB;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rw, (lds_void*)(Ws + j * 512), 16, off, 0, 0, 0);
+            }
+            if (wv == TB2 / 64 - 1) {  // the biases of the item's channels, 4 bytes per lane
+                const __amdgpu_buffer_rsrc_t rb = buffer_rsrc(bias, bias ? 4u * Cout : 0u);
+                const uint32_t off = it.co0 + lane < Cout ? 4u * (uint32_t)(it.co0 + lane) : kOOB;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void*)(Ws + SH::kWUnits * 8), 4, off, 0, 0, 0);
+            }
+        }
+    };
+    int i = blockIdx.x, c = 0;
+    if (i >= n_items) return;
+    Item cur = decode(i);
+    // weights (output block, chunk) held by each buffer: a step whose weights
+    // a buffer already holds loads only the halo
+    int wkey0 = cur.co0 * 1024, wkey1 = -1;
+    issue(buf0, cur, 0, true);
+    f32x4 acc[4][BN / 16];
+    // One step: MFMAs of (cur, c) from buffer A while the next step's loads
+    // go into buffer B.  Returns false after the block's last step.
+    auto step = [&](uint16_t* A, uint16_t* B, int& wkeyB) __attribute__((always_inline)) -> bool {
+        int ni = i, nc = c + 1;
+        if (nc == nchunk) {
+            nc = 0;
+            ni = i + gridDim.x;
+        }
+        const bool has_next = ni < n_items;
+        const Item nxt = has_next ? decode(ni) : cur;
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's loads into A have landed
+        __syncthreads();                     // every wave's; and B is no longer read
+        if (has_next) {
+            const int k = nxt.co0 * 1024 + nc;
+            issue(B, nxt, nc * CBK, k != wkeyB);
+            wkeyB = k;
+        }
+        const uint16_t* Hs = A;
+        const uint16_t* Ws = A + SH::kHalo;
+        if (c == 0) {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int nb = 0; nb < BN / 16; ++nb) acc[mt][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            bf16x8 a[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                a[j] = __builtin_bit_cast(
+                    bf16x8, *(const uint4*)&Hs[cofs((4 * (wv >> 1) + j) * HXX + (wv & 1) * 16 + fr + dx, lane >> 4)]);
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const int tap = dy * 3 + dx;
+#pragma unroll
+                for (int nb = 0; nb < BN / 16; ++nb) {
+                    const bf16x8 b =
+                        __builtin_bit_cast(bf16x8, *(const uint4*)&Ws[cofs(tap * BN + nb * 16 + fr, lane >> 4)]);
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt)
+                        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[mt + dy], acc[mt][nb], 0, 0, 0);
+                }
+            }
+        }
+        if (c == nchunk - 1)
+            conv_store<BN, FUSE>(acc, (const float*)(Ws + SH::kWUnits * 8), cur.n, cur.y0, cur.x0, cur.co0, wv, lane, H,
+                                 W, Y, ldy, nullptr, 0, cstore, nullptr, 0, relu, P);
+        if (!has_next) return false;
+        i = ni;
+        c = nc;
+        cur = nxt;
+        return true;
+    };
+    while (step(buf0, buf1, wkey1) && step(buf1, buf0, wkey0)) {
     }
 }
 
@@ -676,6 +848,19 @@ unsigned conv_grid(unsigned n_items) {
     return std::min(n_items, resident);
 }
 
+template <int BN, int FUSE>
+unsigned conv2_grid(unsigned n_items) {
+    static unsigned resident = 0;
+    if (!resident) {
+        int dev = 0, cus = 256, nb = 0;
+        KF_CHECK(hipGetDevice(&dev));
+        KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_conv2<BN, FUSE>, TB2, 0));
+        resident = (unsigned)std::max(nb, 1) * (unsigned)cus;
+    }
+    return std::min(n_items, resident);
+}
+
 // Optional fusions: input channels csplit.. come from X2 (ldx2), an H/2 x W/2
 // tensor upsampled by nearest neighbour; P receives the 2x2 average pool of
 // the bf16 outputs (H/2 x W/2 x cstore).
@@ -698,7 +883,7 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
     if ((uint64_t)tiles * blocks(cstore, 16) >= (1u << 31)) throw Error("conv: too many tiles");
     // the kernel indexes pixels, weights and rows of a tile with 32-bit integers
     const uint64_t ld_max = std::max<uint64_t>(std::max(ldy, ldyf), std::max(ldm, ldx));
-    if ((uint64_t)N * H * W >= (1u << 31) || 18ull * Cout * Cin >= (1u << 31) || (uint64_t)TY * W * ld_max >= (1u << 31) ||
+    if ((uint64_t)N * H * W >= (1u << 31) || 18ull * Cout * Cin >= (1u << 31) || (uint64_t)TY2 * W * ld_max >= (1u << 31) ||
         2ull * H * W * ldx >= (1u << 31) || (fz.X2 && 2ull * (H / 2) * (W / 2) * fz.ldx2 >= (1u << 31)))
         throw Error("conv: tensor too large for 32-bit indexing (2 GiB per image)");
     const int fuse = (fz.X2 ? kFuseUp : 0) | (fz.P ? kFusePool : 0) | (fz.pk.rgba ? kFusePack : 0);
@@ -717,9 +902,6 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
     using I16 = std::integral_constant<int, 16>;
     using I32 = std::integral_constant<int, 32>;
     using I64 = std::integral_constant<int, 64>;
-    // 64 output channels per item where they divide the output: twice the
-    // MFMAs per staged halo (measured 6 % less U-Net time on 128 x 512x512)
-    const bool wide = cstore % 64 == 0 && !Yf && !M;
     auto outs = [&](auto bn) {  // the output kinds of an unfused convolution
         switch (flags) {
             case 0: go(bn, std::integral_constant<int, 0>()); break;
@@ -728,16 +910,34 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
             default: go(bn, std::integral_constant<int, kOutF32 | kMask>()); break;
         }
     };
-    if (cstore <= 16) {
+    auto go2 = [&](auto bn, auto fl) {
+        constexpr int BN = decltype(bn)::value, F = decltype(fl)::value;
+        const unsigned n = (unsigned)N * blocks(H, TY2) * blocks(W, TX) * blocks(cstore, BN);
+        k_conv2<BN, F><<<conv2_grid<BN, F>(n), TB2, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, cstore,
+                                                            relu, (int)n, fz.X2, fz.ldx2, fz.csplit, fz.P);
+    };
+    // k_conv2 takes the plain, upsampled-input and pooled-output layers of
+    // more than 16 output channels, 64 per item where they divide the output
+    // (twice the MFMAs per staged halo); k_conv the packed first layer, the
+    // FP32 output layer and the masked data gradients.  (Keeping the
+    // 32-channel full-resolution layers on k_conv, which had looked faster
+    // for two of them under the profiler, made the whole denoise of
+    // 128 x 512x512 slower: 14.16-14.18 against 13.96-14.07 ms.)
+    if (!Yf && !M && !fz.pk.rgba && cstore > 16) {
+        const bool wide = cstore % 64 == 0;
+        auto pick = [&](auto fl) {
+            if (wide) go2(I64(), fl);
+            else go2(I32(), fl);
+        };
+        if (fuse == kFuseUp) pick(std::integral_constant<int, kFuseUp>());
+        else if (fuse == kFusePool) pick(std::integral_constant<int, kFusePool>());
+        else pick(std::integral_constant<int, 0>());
+    } else if (cstore <= 16) {
         outs(I16());
-    } else if (fuse == kFuseUp) {
-        if (wide) go(I64(), std::integral_constant<int, kFuseUp>()); else go(I32(), std::integral_constant<int, kFuseUp>());
-    } else if (fuse == kFusePool) {
-        if (wide) go(I64(), std::integral_constant<int, kFusePool>()); else go(I32(), std::integral_constant<int, kFusePool>());
     } else if (fuse == kFusePack) {
         go(I32(), std::integral_constant<int, kFusePack | kFixW>());  // 32 inputs, 32 outputs: one weight set
     } else {
-        if (wide) go(I64(), std::integral_constant<int, 0>()); else outs(I32());
+        outs(I32());  // masked data gradient
     }
     KF_CHECK(hipGetLastError());
 }
