@@ -555,25 +555,30 @@ __global__ __launch_bounds__(TB2) void k_conv2(const uint16_t* __restrict__ X, i
 #pragma unroll
                 for (int nb = 0; nb < BN / 16; ++nb) acc[mt][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
+        // per horizontal offset dx: the 6 halo rows and the 3 x BN / 16 weight
+        // fragments of its taps are read up front, so that the reads are in
+        // flight together and the MFMAs (in k_conv's order) wait on them once
 #pragma unroll
         for (int dx = 0; dx < 3; ++dx) {
-            bf16x8 a[6];
+            bf16x8 a[6], bw[3][BN / 16];
 #pragma unroll
             for (int j = 0; j < 6; ++j)
                 a[j] = __builtin_bit_cast(
                     bf16x8, *(const uint4*)&Hs[cofs((4 * (wv >> 1) + j) * HXX + (wv & 1) * 16 + fr + dx, lane >> 4)]);
 #pragma unroll
-            for (int dy = 0; dy < 3; ++dy) {
-                const int tap = dy * 3 + dx;
+            for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-                for (int nb = 0; nb < BN / 16; ++nb) {
-                    const bf16x8 b =
-                        __builtin_bit_cast(bf16x8, *(const uint4*)&Ws[cofs(tap * BN + nb * 16 + fr, lane >> 4)]);
+                for (int nb = 0; nb < BN / 16; ++nb)
+                    bw[dy][nb] = __builtin_bit_cast(
+                        bf16x8, *(const uint4*)&Ws[cofs((dy * 3 + dx) * BN + nb * 16 + fr, lane >> 4)]);
+            __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the MFMAs
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int nb = 0; nb < BN / 16; ++nb)
 #pragma unroll
                     for (int mt = 0; mt < 4; ++mt)
-                        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[mt + dy], acc[mt][nb], 0, 0, 0);
-                }
-            }
+                        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[dy][nb], a[mt + dy], acc[mt][nb], 0, 0, 0);
         }
         if (c == nchunk - 1)
             conv_store<BN, FUSE>(acc, (const float*)(Ws + SH::kWUnits * 8), cur.n, cur.y0, cur.x0, cur.co0, wv, lane, H,
