@@ -16,6 +16,11 @@ frames leave out (marked gpu).
   config-1 scene (tests/golden/fp64_config1_1024spp.npy): its relMSE must be
   the noise floor of the two estimates, measured from the per-pixel sample
   variance, with no excess a bias would add.
+* The 128-environment frame at its real size (128 x 512x512, 8 spp, depth
+  6, 11.9 M instanced triangles, 67 M paths in 4 batches): every camera ray
+  counted, finite pixels, the rows of three environments equal bit for bit
+  to renders of those environments alone and close to the oracle, and the
+  batched denoise of all 128 images equal to denoising one of them alone.
 """
 import os
 
@@ -168,3 +173,47 @@ def test_converged_frame_against_fp64_oracle(gpu):
     print("relMSE(GPU 4096 spp, FP64 1024 spp) = %.3e; noise floor %.3e" % (e, floor))
     assert e < 1.5 * floor + 2e-4, (e, floor)
     assert e < 2e-3
+
+
+def test_config5_full_scale(gpu):
+    from kuafu_amd.denoiser import Denoiser, WEIGHTS
+    print("config5 128 environments of 512x512, 8 spp, depth 6", flush=True)
+    w = scenes.config5()
+    d, pc, size, n_env = w.data, w.pc, w.env_height, w.n_env
+    pc.frame_count = 0
+    gs = GpuScene(d)
+    rd = Renderer(size, size * n_env)
+    st = rd.render_envs(gs, w.cam, pc, seed=11)
+    assert st.primary_rays == n_env * size * size * pc.spp
+    assert st.extension_rays > 0 and st.shadow_rays > 0
+    tg = [rd.read(k) for k in range(3)]
+    fin = np.isfinite(tg[0][..., :3]).all(-1).mean()
+    print("finite pixels %.6f; rays %d primary, %d extension, %d shadow"
+          % (fin, st.primary_rays, st.extension_rays, st.shadow_rays), flush=True)
+    assert fin > 0.9999
+    for e in (0, 77, n_env - 1):
+        sub = d.environment(e)
+        r1 = Renderer(size, size)
+        r1.render(GpuScene(sub), w.cam[e], pc, seed=11)
+        rows = slice(e * size, (e + 1) * size)
+        for k in range(3):
+            got, alone = tg[k][rows], r1.read(k)
+            assert ((got == alone) | (np.isnan(got) & np.isnan(alone))).all(), "environment %d target %d" % (e, k)
+        # two rows against the oracle on the environment's own scene
+        o = oracle.Scene(sub)
+        ref = o.render(w.cam[e], pc, size, size, seed=11, rows=(255, 257), threads=16)
+        got = tg[0][rows][255:257]
+        ok = np.abs(got - ref["rgba"][255:257]) <= 1e-4 * np.maximum(1.0, np.abs(ref["rgba"][255:257]))
+        print("environment %d rows 255-256 within 1e-4 of the oracle: %.5f" % (e, ok.mean()), flush=True)
+        assert ok.mean() > 0.999
+    # batched denoise of the 128 stacked images: environment 77 as if alone
+    den = Denoiser(weights=WEIGHTS)
+    den.apply(rd)
+    out = rd.read(0)
+    r1 = Renderer(size, size)
+    rows = slice(77 * size, 78 * size)
+    for k in range(3):
+        r1.write(k, tg[k][rows])
+    den.apply(r1)
+    assert (out[rows] == r1.read(0)).all()
+    assert np.isfinite(out[..., :3]).all()
