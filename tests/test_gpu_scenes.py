@@ -11,7 +11,9 @@ frames leave out (marked gpu).
   finite radiance, and the same image and ray counts whether its 33 M paths
   run as one wavefront batch or as four.
 * The config-3 RGB camera (the IR cameras are in test_gpu_parity.py) and the
-  IR cameras' FP32 single-channel output.
+  IR cameras' FP32 single-channel output; the full active-light frame (720p,
+  32 spp, depth 6, both IR cameras and the RGB camera) on row bands against
+  the oracle, with the IR images' single-channel download.
 * A converged GPU frame against the FP64 oracle's 1024-spp frame of the
   config-1 scene (tests/golden/fp64_config1_1024spp.npy): its relMSE must be
   the noise floor of the two estimates, measured from the per-pixel sample
@@ -217,3 +219,25 @@ def test_config5_full_scale(gpu):
     den.apply(r1)
     assert (out[rows] == r1.read(0)).all()
     assert np.isfinite(out[..., :3]).all()
+
+
+def test_config3_full_frame(gpu):
+    print("config3 1280x720, 32 spp, depth 6: IR left, IR right, RGB", flush=True)
+    for cam in ("left", "right", "rgb"):
+        w = scenes.config3(camera=cam)
+        gs = GpuScene(w.data)
+        rd = Renderer(w.width, w.height)
+        st = rd.render(gs, w.cam, w.pc, seed=4)
+        assert st.primary_rays == w.width * w.height * w.pc.spp
+        rgba = rd.read(0)
+        assert np.isfinite(rgba[..., :3]).all()
+        o = oracle.Scene(w.data)
+        for y0, y1 in ((0, 1), (359, 361), (718, 720)):
+            ref = o.render(w.cam, w.pc, w.width, w.height, seed=4, rows=(y0, y1), threads=16)["rgba"][y0:y1]
+            ok = np.abs(rgba[y0:y1] - ref) <= 1e-4 * np.maximum(1.0, np.abs(ref))
+            print("%s rows %d-%d within 1e-4 of the oracle: %.5f" % (cam, y0, y1, ok.mean()), flush=True)
+            assert ok.mean() > 0.999
+        if cam != "rgb":
+            gray = rd.download_gray32f()
+            assert gray.shape == (w.height, w.width)
+            assert (gray == (rgba[..., 0] + rgba[..., 1] + rgba[..., 2]) / np.float32(3.0)).all()
