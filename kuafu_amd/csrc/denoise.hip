@@ -9,8 +9,13 @@
 //   final 3x3 conv to RGB, residual on log1p(noisy RGB).
 // Tensors are NHWC BF16 with FP32 accumulation.  Convolutions are implicit
 // GEMMs on v_mfma_f32_16x16x32_bf16:
-//   k_conv   forward (bias + ReLU fused) and data gradient (the same kernel on
-//            flipped, transposed weights, ReLU mask of the forward input fused)
+//   k_conv2  the plain, pooled-output and upsampled-input convolutions
+//            (forward, and the unmasked data gradient): 8 waves, 32 x 16
+//            tiles, double-buffered LDS filled by loads straight into LDS
+//   k_conv   the first layer (packs the float4 render targets itself), the
+//            FP32 output layer and the masked data gradients (the same
+//            kernel on flipped, transposed weights, ReLU mask of the forward
+//            input fused); bias + ReLU fused in both
 //   k_wgrad  weight gradient, pixels as the K dimension, split-K with FP32
 //            atomics
 // Concatenations cost no copies: producers write into channel slices of the
