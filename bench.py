@@ -223,7 +223,7 @@ def denoised_frames(w, gs, rd, frames, warmup, schedule=""):
             "trace_ms_p50": round(float(np.median(trace)), 3)}
 
 
-def batched_environments(frames=8, schedule=None):
+def batched_environments(frames=8, schedule=None, full_tlas_camera=False):
     """128 environments of 512x512 (8 spp, depth 6, 5k-400k triangles each):
     autotuned schedule, every frame one replay of a captured HIP graph, then
     one batched denoise of the 128 images.  Frame latency = render + denoise,
@@ -244,14 +244,14 @@ def batched_environments(frames=8, schedule=None):
     else:
         tune = rd.autotune(gs, w.cam, pc)
     den = Denoiser()
-    rd.render_envs(gs, w.cam, pc, seed=1, use_graph=True)  # capture
+    rd.render_envs(gs, w.cam, pc, seed=1, use_graph=True, full_tlas_camera=full_tlas_camera)  # capture
     den.apply(rd)
     lat, rend, dms = [], [], []
     rays = 0
     kms = 0.0
     for k in range(frames):
         t0 = time.perf_counter()
-        st = rd.render_envs(gs, w.cam, pc, seed=2 + k, use_graph=True)
+        st = rd.render_envs(gs, w.cam, pc, seed=2 + k, use_graph=True, full_tlas_camera=full_tlas_camera)
         t1 = time.perf_counter()
         den.apply(rd)
         t2 = time.perf_counter()
@@ -365,6 +365,8 @@ def main():
                     help="only the 128-environment workload (frames = --steps), e.g. for profiling")
     ap.add_argument("--denoised-only", action="store_true",
                     help="only 1-spp frames + denoise of the 1080p scene (frames = --steps), e.g. for profiling")
+    ap.add_argument("--full-tlas-camera", action="store_true",
+                    help="camera rays start from the whole TLAS instead of their pixel tile's root (comparison)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     args = ap.parse_args()
@@ -380,7 +382,8 @@ def main():
     check(lib().kf_init(0))
     if args.envs_only:
         sched = [int(x) for x in args.schedule.split(",")] + [0] * 6 if args.schedule else None
-        print(json.dumps(batched_environments(frames=args.steps, schedule=sched[:6] if sched else None)))
+        print(json.dumps(batched_environments(frames=args.steps, schedule=sched[:6] if sched else None,
+                                              full_tlas_camera=args.full_tlas_camera)))
         return
     w = scenes.config2(width=args.width, height=args.height)
     print("bench: building scene (%d triangles)" % w.data.n_triangles(), file=sys.stderr, flush=True)
@@ -405,9 +408,11 @@ def main():
         print("bench: autotune", file=sys.stderr, flush=True)
         tune = rd.autotune(gs, w.cam, pc)
     # instrumented frame (untimed): node visits / triangle tests of k_extend
-    cst = rd.render(gs, w.cam, pc, seed=1, samples_per_batch=args.samples_per_batch, count_traversal=True)
+    ft = args.full_tlas_camera
+    cst = rd.render(gs, w.cam, pc, seed=1, samples_per_batch=args.samples_per_batch, count_traversal=True,
+                    full_tlas_camera=ft)
     for _ in range(args.warmup):
-        rd.render(gs, w.cam, pc, seed=0, samples_per_batch=args.samples_per_batch)
+        rd.render(gs, w.cam, pc, seed=0, samples_per_batch=args.samples_per_batch, full_tlas_camera=ft)
     print("bench: warmup done", file=sys.stderr, flush=True)
     rays = 0
     ext_rays = 0
@@ -416,7 +421,7 @@ def main():
     launches = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
-        st = rd.render(gs, w.cam, pc, seed=100 + k, samples_per_batch=args.samples_per_batch)
+        st = rd.render(gs, w.cam, pc, seed=100 + k, samples_per_batch=args.samples_per_batch, full_tlas_camera=ft)
         rays += st.rays()
         # rays k_extend traced (the tail kernel's are not in extend_ms)
         ext_rays += st.primary_rays + st.extension_rays + st.speculative_rays - st.tail_rays

@@ -229,6 +229,9 @@ typedef struct kf_render_opts {
                                  the scene, target, options and schedule stay the same (seed, frame
                                  count, cameras and lights may change); stats then carry the frame's
                                  kernel time only (extend_ms / shadow_ms are 0) */
+    uint32_t full_tlas_camera; /* camera rays start from their environment's TLAS root instead of
+                                 a root over the instances their 8x8 pixel tile can see (same
+                                 images, more TLAS steps; for comparison) */
 } kf_render_opts;
 
 /* ------------------------------------------------------------------------ */

@@ -98,7 +98,7 @@ class TuneResult(C.Structure):
 
 class RenderOpts(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("samples_per_batch", C.c_uint32), ("keep_samples", C.c_uint32),
-                ("count_traversal", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("count_traversal", C.c_uint32), ("use_graph", C.c_uint32), ("full_tlas_camera", C.c_uint32)]
 
 
 class BvhInfo(C.Structure):

@@ -380,18 +380,20 @@ class Renderer:
             pass
 
     def render(self, scene, cam, pc, seed=0, samples_per_batch=0, keep_samples=False, count_traversal=False,
-               use_graph=False):
-        o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal), int(use_graph))
+               use_graph=False, full_tlas_camera=False):
+        o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal), int(use_graph),
+                           int(full_tlas_camera))
         st = abi.RenderStats()
         check(lib().kf_render(self.h, scene.h, C.byref(cam), C.byref(pc), C.byref(o), C.byref(st)))
         return st
 
     def render_envs(self, scene, cams, pc, seed=0, samples_per_batch=0, keep_samples=False, count_traversal=False,
-                    use_graph=False):
+                    use_graph=False, full_tlas_camera=False):
         """One frame of every environment of a batched scene: cams[e] views
         environment e, whose image is rows [e*H/n, (e+1)*H/n) of the targets."""
         arr = (abi.CameraUBO * len(cams))(*cams)
-        o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal), int(use_graph))
+        o = abi.RenderOpts(seed, samples_per_batch, int(keep_samples), int(count_traversal), int(use_graph),
+                           int(full_tlas_camera))
         st = abi.RenderStats()
         check(lib().kf_render_envs(self.h, scene.h, len(cams), arr, C.byref(pc), C.byref(o), C.byref(st)))
         return st

@@ -932,8 +932,10 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
         throw Error("kf_scene_update_instances: a scene with environments keeps its instance count");
     if (h_inst != s.h_inst.data()) s.h_inst.assign(h_inst, h_inst + n);
     if (n != s.n_inst || !s.d_inst) {
-        dfree(s.d_inst); dfree(s.d_tlas); dfree(s.d_inst_src);
+        dfree(s.d_inst); dfree(s.d_tlas); dfree(s.d_inst_src); dfree(s.d_ilo); dfree(s.d_ihi);
         s.d_inst = dalloc<Inst>(std::max<uint32_t>(n, 1));
+        s.d_ilo = dalloc<float4>(std::max<uint32_t>(n, 1));
+        s.d_ihi = dalloc<float4>(std::max<uint32_t>(n, 1));
         s.d_tlas = dalloc<Node>(std::max<uint32_t>(n, 1));
         s.d_inst_src = dalloc<kf_instance>(std::max<uint32_t>(n, 1));
         s.n_inst = n;
@@ -969,8 +971,8 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     float4* glo = dalloc<float4>(std::max<uint32_t>(s.n_geom, 1));
     float4* ghi = dalloc<float4>(std::max<uint32_t>(s.n_geom, 1));
     if (s.n_geom) k_blas_box<<<blocks(s.n_geom, TB), TB, 0, st>>>(s.n_geom, s.d_blas_root, s.d_blas, s.d_tri, glo, ghi);
-    float4* ilo = dalloc<float4>(std::max<uint32_t>(n, 1));
-    float4* ihi = dalloc<float4>(std::max<uint32_t>(n, 1));
+    float4* ilo = s.d_ilo;  // kept: the renderer culls instances per pixel tile with them
+    float4* ihi = s.d_ihi;
     uint32_t* iseg = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
     if (n)
         k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, s.d_gflags, s.d_blas_root4,
@@ -1010,7 +1012,7 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     KF_CHECK(hipGetLastError());
     s.tlas_root4 = roots[0];  // BVH4 and BVH2 roots coincide by construction
     s.n_tlas4 = n_tlas2 ? s.tlas4_cap : 0;  // index bound; the node count stays on the device
-    dfree(d_w2o); dfree(glo); dfree(ghi); dfree(ilo); dfree(ihi); dfree(iseg);
+    dfree(d_w2o); dfree(glo); dfree(ghi); dfree(iseg);
     dfree(order); dfree(d_root); dfree(qa); dfree(qb);
 }
 

@@ -118,7 +118,7 @@ Scene::~Scene() {
     if (h_stage) (void)hipHostFree(h_stage);
     if (stage_ev) (void)hipEventDestroy(stage_ev);
     d_env_info = nullptr;  // points into d_tex_info, not an allocation of its own
-    release(d_inst_src); release(d_tex); release(d_tex_info); release(d_srgb);
+    release(d_inst_src); release(d_ilo); release(d_ihi); release(d_tex); release(d_tex_info); release(d_srgb);
 }
 
 // Environments of a batched scene: instance ranges, BVH2 / BVH4 node ranges

@@ -155,6 +155,7 @@ struct Scene {
     int* d_tlas_err = nullptr;
     Inst* d_inst = nullptr;
     kf_instance* d_inst_src = nullptr;
+    float4 *d_ilo = nullptr, *d_ihi = nullptr;  // instance world boxes (the TLAS leaf boxes)
     int tlas_root = kEmptyRef;
     // Batched environments: instances [h_env_off[e], h_env_off[e+1]) form
     // environment e, with its BVH2 nodes (and its BVH4 nodes, root first)
@@ -222,6 +223,12 @@ struct Renderer {
     float4 *sh_o = nullptr, *sh_d = nullptr;   // shadow queue (o.xyz, slot bits), (d.xyz, tmax)
     uint32_t* sh_root = nullptr;               // shadow queue: TLAS root item (scenes with environments)
     kf_camera_ubo* d_cams = nullptr;           // one camera per environment
+    // Camera-ray TLAS: the scene's TLAS nodes followed by kTileNodes nodes
+    // per 8x8 pixel tile, and each tile's root item (k_tile_roots)
+    Node4* d_ftlas = nullptr;
+    size_t ftlas_cap = 0;
+    uint32_t* d_tile_root = nullptr;
+    size_t tile_cap = 0;
     uint32_t cams_cap = 0;
     float4 *nee_a = nullptr, *nee_b = nullptr; // per slot (c_a.xyz, theta), (c_b.xyz, mode|group bits)
     uint8_t* vis = nullptr;

@@ -117,7 +117,229 @@ __global__ void k_zero(unsigned long long* c, int n) {
     if (i < n) c[i] = 0;
 }
 
+// ------------------------------------------------------------ camera-ray TLAS
+// The camera rays of an 8x8 pixel tile leave one point (the camera position,
+// pinhole cameras) in directions inside the tile's pyramid, so an instance
+// whose world box lies wholly outside that pyramid is hit by none of them.
+// k_tile_roots lists, per tile, the instances of its environment whose boxes
+// meet the pyramid and gives the tile a root item over just those: none ->
+// the empty item, one -> the instance, 2-4 -> one node, 5-16 -> a node over
+// up to four groups in depth order; more, or a camera with an aperture (rays
+// from a disc), -> the environment's TLAS root.  The closest hit does not
+// depend on the order in which records are visited (smallest t, ties by
+// instance and primitive), each instance keeps its TLAS leaf box, and a node
+// box here is the exact union of its children's, so camera rays find the hits
+// they find through the full TLAS.  Later bounces and shadow rays start from
+// the environment's root.
+constexpr uint32_t kTile = 8, kTileNodes = 5, kTileMax = 16;
+// The pyramid's planes pass half a pixel outside the tile, and each of the
+// tile's rays must lie at least this angle (radians) inside every plane, far
+// more than the float rounding of a ray's direction (about 1e-6); tiles of
+// narrower pixels use the environment's root.
+constexpr double kTileMinAngle = 2e-5;
+// A box is outside a plane when its nearest corner lies beyond it by this
+// fraction of the corner's distance from the camera.
+constexpr double kTileBoxEps = 1e-5;
+
+struct DV {
+    double x, y, z;
+};
+KF_DEV DV dv_sub(DV a, DV b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+KF_DEV double dv_dot(DV a, DV b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+KF_DEV DV dv_cross(DV a, DV b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+KF_DEV double dv_len(DV a) { return sqrt(dv_dot(a, a)); }
+KF_DEV DV dv_scale(DV a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+
+// World direction (not normalised) of the camera ray through raster point
+// (px, py) of a W x He image, as k_raygen maps it, in double.
+KF_DEV DV tile_dir(const kf_camera_ubo& cam, double px, double py, double W, double He) {
+    const double dx = px / W * 2.0 - 1.0, dy = py / He * 2.0 - 1.0;
+    const float* P = cam.proj_inverse;
+    const float* V = cam.view_inverse;
+    double t[3];
+    for (int r = 0; r < 3; ++r) t[r] = (double)P[r] * dx + (double)P[4 + r] * dy + (double)P[8 + r] + (double)P[12 + r];
+    DV d;
+    d.x = (double)V[0] * t[0] + (double)V[4] * t[1] + (double)V[8] * t[2];
+    d.y = (double)V[1] * t[0] + (double)V[5] * t[1] + (double)V[9] * t[2];
+    d.z = (double)V[2] * t[0] + (double)V[6] * t[1] + (double)V[10] * t[2];
+    return d;
+}
+
+// Node of up to four children (instance or node items) with their boxes;
+// unused slots are empty items with zero boxes.
+KF_DEV void tile_node(Node4* __restrict__ out, const uint32_t* items, const float4* lo, const float4* hi, int n) {
+    float b[6][4];
+    int ref[4];
+    for (int c = 0; c < 4; ++c) {
+        const bool on = c < n;
+        b[0][c] = on ? lo[c].x : 0.0f; b[1][c] = on ? hi[c].x : 0.0f;
+        b[2][c] = on ? lo[c].y : 0.0f; b[3][c] = on ? hi[c].y : 0.0f;
+        b[4][c] = on ? lo[c].z : 0.0f; b[5][c] = on ? hi[c].z : 0.0f;
+        ref[c] = on ? (int)items[c] : (int)kEmptyItem;
+    }
+    Node4 nd;
+    nd.lox = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
+    nd.hix = make_float4(b[1][0], b[1][1], b[1][2], b[1][3]);
+    nd.loy = make_float4(b[2][0], b[2][1], b[2][2], b[2][3]);
+    nd.hiy = make_float4(b[3][0], b[3][1], b[3][2], b[3][3]);
+    nd.loz = make_float4(b[4][0], b[4][1], b[4][2], b[4][3]);
+    nd.hiz = make_float4(b[5][0], b[5][1], b[5][2], b[5][3]);
+    nd.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+    nd.pad = make_int4(kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef);
+    *out = nd;
+}
+
+// One wave per tile (n_env * tx * ty tiles, tx x ty per environment): the
+// lanes test the environment's instances 64 at a time, lane 0 builds the
+// tile's nodes at out[n_tlas + kTileNodes * tile ...].  The scene's n_tlas
+// TLAS nodes are copied to out[0, n_tlas) first, so that the environments'
+// root items index the same nodes there.
+__global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const Node4* __restrict__ tlas,
+                                                   uint32_t n_tlas, const float4* __restrict__ ilo,
+                                                   const float4* __restrict__ ihi,
+                                                   const uint32_t* __restrict__ env_off, uint32_t tx, uint32_t ty,
+                                                   Node4* __restrict__ out, uint32_t* __restrict__ tile_root) {
+    __shared__ uint32_t s_cand[TB / 64][kTileMax];
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+    {
+        const float4* src = (const float4*)tlas;
+        float4* dst = (float4*)out;
+        for (size_t i = tid; i < (size_t)n_tlas * 8u; i += nth) dst[i] = src[i];
+    }
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t per_env = tx * ty, n_tiles = per_env * F.n_env;
+    for (uint32_t t = tid >> 6; t < n_tiles; t += nth >> 6) {  // wave-uniform
+        const uint32_t e = t / per_env, r = t - e * per_env, by = r / tx, bx = r - by * tx;
+        const uint32_t env_item = env_root_item(S, e);
+        uint32_t root = env_item;
+        const kf_camera_ubo& cam = F.cams[e];
+        // camera position as k_raygen computes it (the rays' origin, bit for bit)
+        const float o4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+        float org[4];
+        mat4_mul(cam.view_inverse, o4, org);
+        const DV o = {org[0], org[1], org[2]};
+        const double W = F.W, He = F.He;
+        const double xa = (double)(bx * kTile), xb = (double)min((bx + 1u) * kTile, F.W);
+        const double ya = (double)(by * kTile), yb = (double)min((by + 1u) * kTile, F.He);
+        bool ok = env_item != kEmptyItem && !(cam.position[3] > 0.0f) && isfinite(org[0]) && isfinite(org[1]) &&
+                  isfinite(org[2]);
+        DV n[4];
+        if (ok) {
+            const DV c[4] = {tile_dir(cam, xa - 0.5, ya - 0.5, W, He), tile_dir(cam, xb + 0.5, ya - 0.5, W, He),
+                             tile_dir(cam, xb + 0.5, yb + 0.5, W, He), tile_dir(cam, xa - 0.5, yb + 0.5, W, He)};
+            const DV q[4] = {tile_dir(cam, xa, ya, W, He), tile_dir(cam, xb, ya, W, He), tile_dir(cam, xb, yb, W, He),
+                             tile_dir(cam, xa, yb, W, He)};
+            const DV ctr = tile_dir(cam, 0.5 * (xa + xb), 0.5 * (ya + yb), W, He);
+            double sgn = 0.0;
+            for (int i = 0; i < 4; ++i) {
+                n[i] = dv_cross(c[i], c[(i + 1) & 3]);
+                const double s = dv_dot(n[i], ctr);
+                if (i == 0) sgn = s > 0.0 ? 1.0 : -1.0;
+                const double len = dv_len(n[i]);
+                ok = ok && s * sgn > 0.0 && len > 0.0 && isfinite(len);
+                n[i] = dv_scale(n[i], ok ? sgn / len : 0.0);  // unit normal pointing into the pyramid
+            }
+            for (int i = 0; i < 4 && ok; ++i)
+                for (int j = 0; j < 4; ++j) {
+                    const double ql = dv_len(q[j]);
+                    ok = ok && ql > 0.0 && dv_dot(n[i], q[j]) >= kTileMinAngle * ql;
+                }
+        }
+        if (ok) {
+            const uint32_t i0 = env_off ? env_off[e] : 0u, i1 = env_off ? env_off[e + 1] : S.n_inst;
+            uint32_t k = 0;
+            for (uint32_t base = i0; base < i1 && k <= kTileMax; base += 64u) {
+                const uint32_t i = base + lane;
+                bool in = i < i1;
+                if (in) {
+                    const float4 lo = ilo[i], hi = ihi[i];
+                    // lo > hi: an instance without triangles to hit
+                    in = lo.x <= hi.x && lo.y <= hi.y && lo.z <= hi.z;
+                    for (int pl = 0; pl < 4 && in; ++pl) {
+                        const DV pv = {n[pl].x > 0.0 ? hi.x : lo.x, n[pl].y > 0.0 ? hi.y : lo.y,
+                                       n[pl].z > 0.0 ? hi.z : lo.z};
+                        const DV v = dv_sub(pv, o);
+                        in = !(dv_dot(n[pl], v) < -kTileBoxEps * dv_len(v));
+                    }
+                }
+                const unsigned long long m = __ballot(in);
+                const uint32_t at = k + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (in && at < kTileMax) s_cand[w][at] = i;
+                k += (uint32_t)__popcll(m);
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (k == 0) {
+                root = kEmptyItem;
+            } else if (k == 1) {
+                root = make_item(kInstance, s_cand[w][0]);
+            } else if (k <= kTileMax && lane == 0) {
+                const uint32_t node0 = n_tlas + kTileNodes * t;
+                uint32_t c[kTileMax];
+                float4 lo[kTileMax], hi[kTileMax];
+                double key[kTileMax];
+                for (uint32_t j = 0; j < k; ++j) {
+                    c[j] = s_cand[w][j];
+                    lo[j] = ilo[c[j]];
+                    hi[j] = ihi[c[j]];
+                }
+                if (k <= 4) {
+                    uint32_t items[4];
+                    for (uint32_t j = 0; j < k; ++j) items[j] = make_item(kInstance, c[j]);
+                    tile_node(out + node0, items, lo, hi, (int)k);
+                } else {
+                    // groups of consecutive instances in order of their box
+                    // centres' depth along the tile's central ray
+                    const DV ctr = tile_dir(cam, 0.5 * (xa + xb), 0.5 * (ya + yb), W, He);
+                    for (uint32_t j = 0; j < k; ++j) {
+                        const DV m = {0.5 * ((double)lo[j].x + hi[j].x), 0.5 * ((double)lo[j].y + hi[j].y),
+                                      0.5 * ((double)lo[j].z + hi[j].z)};
+                        key[j] = dv_dot(dv_sub(m, o), ctr);
+                    }
+                    for (uint32_t j = 1; j < k; ++j)  // insertion sort, stable
+                        for (uint32_t a = j; a > 0 && key[a] < key[a - 1]; --a) {
+                            const double tk = key[a]; key[a] = key[a - 1]; key[a - 1] = tk;
+                            const uint32_t tc = c[a]; c[a] = c[a - 1]; c[a - 1] = tc;
+                            const float4 tl = lo[a]; lo[a] = lo[a - 1]; lo[a - 1] = tl;
+                            const float4 th = hi[a]; hi[a] = hi[a - 1]; hi[a - 1] = th;
+                        }
+                    const uint32_t g = (k + 3u) / 4u;
+                    uint32_t items[4];
+                    float4 glo[4], ghi[4];
+                    uint32_t j = 0;
+                    for (uint32_t gi = 0; gi < g; ++gi) {
+                        const uint32_t sz = k / g + (gi < k % g ? 1u : 0u);
+                        uint32_t gitems[4];
+                        float4 l = lo[j], h = hi[j];
+                        for (uint32_t a = 0; a < sz; ++a) {
+                            gitems[a] = make_item(kInstance, c[j + a]);
+                            l = make_float4(fminf(l.x, lo[j + a].x), fminf(l.y, lo[j + a].y), fminf(l.z, lo[j + a].z), 0.0f);
+                            h = make_float4(fmaxf(h.x, hi[j + a].x), fmaxf(h.y, hi[j + a].y), fmaxf(h.z, hi[j + a].z), 0.0f);
+                        }
+                        if (sz == 1) {
+                            items[gi] = gitems[0];
+                        } else {
+                            tile_node(out + node0 + 1u + gi, gitems, lo + j, hi + j, (int)sz);
+                            items[gi] = make_item(kTlasNode, node0 + 1u + gi);
+                        }
+                        glo[gi] = l;
+                        ghi[gi] = h;
+                        j += sz;
+                    }
+                    tile_node(out + node0, items, glo, ghi, (int)g);
+                }
+                root = make_item(kTlasNode, node0);
+            }
+        }
+        if (lane == 0) tile_root[t] = root;
+        __builtin_amdgcn_wave_barrier();  // s_cand is reused by the wave's next tile
+    }
+}
+
+// tile_root: each 8x8 tile's root item (k_tile_roots, tx x ty tiles per
+// environment), or null: the environment's TLAS root.
 __global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2* __restrict__ jump,
+                                               const uint32_t* __restrict__ tile_root, uint32_t tx, uint32_t ty,
                                                float4* __restrict__ ray_o, float4* __restrict__ ray_d,
                                                uint32_t* __restrict__ seed, uint32_t* __restrict__ queue,
                                                unsigned long long* __restrict__ cnt) {
@@ -167,9 +389,12 @@ __global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2
             float d4[4] = {nd.x, nd.y, nd.z, 0.0f};
             mat4_mul(cam.view_inverse, d4, dir4);
         }
-        // ray_d.w carries the path's TLAS root item through every bounce
+        // ray_d.w carries the ray's TLAS root item: the pixel tile's for the
+        // camera ray, the environment's from the next bounce on
+        const uint32_t root =
+            tile_root ? tile_root[(env * ty + y / kTile) * tx + x / kTile] : env_root_item(S, env);
         ray_o[p] = make_float4(org[0], org[1], org[2], 0.0f);
-        ray_d[p] = make_float4(dir4[0], dir4[1], dir4[2], __uint_as_float(env_root_item(S, env)));
+        ray_d[p] = make_float4(dir4[0], dir4[1], dir4[2], __uint_as_float(root));
         // throughput (1, depth 0), colour (0) and BSDF weight (0) of a new
         // path are implicit at bounce 0 (kPathStart): k_shade and k_connect
         // use them there instead of reading them back, which saves 48 B of
@@ -377,6 +602,8 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
     float4 h = hit[p];
     float4 ro = ray_o[p], rd = ray_d[p];
     v3 rdir = xyz(rd);
+    // root item of the rays from here on (a camera ray carries its tile's)
+    const uint32_t root = bounce == 0 ? env_root_item(S, (p % F.WH) / F.WHe) : __float_as_uint(rd.w);
     uint32_t rs = seed[p];
     uint32_t inst = __float_as_uint(h.w);
     for (uint32_t j = 0; j < K; ++j) vis[(size_t)p * K + j] = 0;
@@ -518,7 +745,7 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
             v3 sum_d = splat(0.0f), sum_p = splat(0.0f), sum_a = splat(0.0f);
             if (!NEE_INLINE) {
                 nc.sh = sh;
-                nc.root = __float_as_uint(rd.w);
+                nc.root = root;
                 nc.on = true;
             }
             const int nl = NEE_INLINE ? LT->n_slots : 0;
@@ -533,7 +760,7 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
                 THit hh;
                 TCount cc{0, 0, 0, 0};
                 n_shadow++;
-                bool occ = traverse<true, false>(S, __float_as_uint(rd.w), worldPos, Ld, 0.001f, maxd, false, 0u, hh,
+                bool occ = traverse<true, false>(S, root, worldPos, Ld, 0.001f, maxd, false, 0u, hh,
                                                  cc, ovf);
                 if (!occ) {
                     v3 ca, cb;
@@ -548,7 +775,7 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
             }
             if (NEE_INLINE) nee_a[(size_t)p * K] = f4((sum_d + sum_p) + sum_a, 0.0f);
             ray_o[p] = f4(worldPos, 0.0f);
-            ray_d[p] = f4(L, rd.w);
+            ray_d[p] = f4(L, __uint_as_float(root));
             em[p] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
             rw[p] = f4(weight, 0.0f);
             ext = !all_zero(xyz(t4) * weight) && depth + 1u <= F.pc.path_depth;  // as in k_connect
@@ -849,7 +1076,7 @@ Renderer::~Renderer() {
     if (h_totals) (void)hipHostFree(h_totals);
     void* ptrs[] = {d_rgba, d_albedo, d_normal, d_bgra, ray_o, ray_d, thr, col, rw, em, hit, hit_prim, seed,
                     queue_a, queue_b, queue_e, sh_o, sh_d, sh_root, nee_a, nee_b, vis, samples, counters, d_lights,
-                    d_jump, d_acc, d_out, d_cams, d_fv};
+                    d_jump, d_acc, d_out, d_cams, d_fv, d_ftlas, d_tile_root};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1023,6 +1250,34 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
     KF_CHECK(hipMemcpyAsync(r->d_fv, &fv, sizeof(fv), hipMemcpyHostToDevice, st));
 
     SceneView S = s->view();
+    // Camera-ray roots per 8x8 pixel tile, over a copy of the TLAS followed
+    // by the tiles' nodes (pinhole cameras; k_tile_roots falls back to the
+    // environment's root per tile where that does not apply)
+    const uint32_t He = r->H / n_env;
+    const uint32_t tx = (r->W + kTile - 1) / kTile, ty = (He + kTile - 1) / kTile;
+    const size_t n_tiles = (size_t)tx * ty * n_env;
+    const size_t n_fnodes = (size_t)S.n_tlas_nodes + kTileNodes * n_tiles;
+    const bool tiles = !o.full_tlas_camera && s->n_inst > 0 && n_fnodes < (size_t)kIdxMask;
+    if (tiles) {
+        if (n_fnodes > r->ftlas_cap) {
+            if (r->d_ftlas) KF_CHECK(hipFree(r->d_ftlas));
+            r->d_ftlas = nullptr;
+            KF_CHECK(hipMalloc((void**)&r->d_ftlas, n_fnodes * sizeof(Node4)));
+            r->ftlas_cap = n_fnodes;
+        }
+        if (n_tiles > r->tile_cap) {
+            if (r->d_tile_root) KF_CHECK(hipFree(r->d_tile_root));
+            r->d_tile_root = nullptr;
+            KF_CHECK(hipMalloc((void**)&r->d_tile_root, n_tiles * sizeof(uint32_t)));
+            r->tile_cap = n_tiles;
+        }
+    }
+    const Node4* scene_tlas = S.tlas;
+    const uint32_t n_scene_tlas = S.n_tlas_nodes;
+    if (tiles) {
+        S.tlas = r->d_ftlas;
+        S.n_tlas_nodes = (uint32_t)n_fnodes;
+    }
     Frame F;
     memset(&F, 0, sizeof(F));  // F is compared bytewise as part of the graph key
     F.cams = r->d_cams;
@@ -1087,8 +1342,13 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         const int nz = s0 == 0 ? kCntSize : kCntGlobal;
         k_zero<<<(nz + TB - 1) / TB, TB, 0, st>>>(r->counters, nz);
         if (timed) KF_CHECK(hipEventRecord(r->ev[0], st));
+        if (tiles && s0 == 0)
+            k_tile_roots<<<(uint32_t)std::min<size_t>((n_tiles + TB / 64 - 1) / (TB / 64), 65535u), TB, 0, st>>>(
+                S, F, scene_tlas, n_scene_tlas, s->d_ilo, s->d_ihi, n_env > 1 ? s->d_env_off : nullptr, tx, ty,
+                r->d_ftlas, r->d_tile_root);
         k_raygen<<<std::min<uint32_t>((B + TB - 1) / TB, 65535u), TB, 0, st>>>(
-            S, F, d_jump, r->ray_o, r->ray_d, r->seed, r->queue_a, r->counters);
+            S, F, d_jump, tiles ? r->d_tile_root : nullptr, tx, ty, r->ray_o, r->ray_d, r->seed, r->queue_a,
+            r->counters);
         uint32_t* qa = r->queue_a;
         uint32_t* qb = r->queue_b;
         // Per bounce b:  A: shade(b) -> B: [shadow(b)], connect(b) ; A:
@@ -1177,7 +1437,8 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         const void* bufs[] = {r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->em, r->hit, r->hit_prim, r->seed,
                               r->queue_a, r->queue_b, r->queue_e, r->sh_o, r->sh_d, r->sh_root, r->nee_a,
                               r->nee_b, r->vis, r->samples, r->counters, d_lights, d_jump, d_acc, d_out,
-                              r->d_rgba, r->d_albedo, r->d_normal, (const void*)r->st2};
+                              r->d_rgba, r->d_albedo, r->d_normal, (const void*)r->st2,
+                              tiles ? r->d_tile_root : nullptr, scene_tlas, s->d_ilo, s->d_ihi};
         put(&S, sizeof(S));
         put(&F, sizeof(F));
         put(sched, sizeof(sched));
