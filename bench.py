@@ -470,6 +470,8 @@ def main():
                    "schedule": None if tune is None else {k: tune[k] for k in ("samples_per_batch", "refill", "grab",
                                                                                  "tail_bounce", "grab0",
                                                                                  "grab_shadow")},
+                   "autotune_ms": None if tune is None or "frame_ms" not in tune else
+                   [round(tune["default_ms"], 2), round(tune["frame_ms"], 2)],
                    "kernel_mrays_s": round(rays / (trace_ms * 1e-3) / 1e6, 2)},
         "roofline": roof,
     }
