@@ -165,33 +165,25 @@ KF_DEV DV tile_dir(const kf_camera_ubo& cam, double px, double py, double W, dou
     return d;
 }
 
-// Node of up to four children (instance or node items) with their boxes;
-// unused slots are empty items with zero boxes.
-KF_DEV void tile_node(Node4* __restrict__ out, const uint32_t* items, const float4* lo, const float4* hi, int n) {
-    float b[6][4];
-    int ref[4];
-    for (int c = 0; c < 4; ++c) {
-        const bool on = c < n;
-        b[0][c] = on ? lo[c].x : 0.0f; b[1][c] = on ? hi[c].x : 0.0f;
-        b[2][c] = on ? lo[c].y : 0.0f; b[3][c] = on ? hi[c].y : 0.0f;
-        b[4][c] = on ? lo[c].z : 0.0f; b[5][c] = on ? hi[c].z : 0.0f;
-        ref[c] = on ? (int)items[c] : (int)kEmptyItem;
-    }
-    Node4 nd;
-    nd.lox = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
-    nd.hix = make_float4(b[1][0], b[1][1], b[1][2], b[1][3]);
-    nd.loy = make_float4(b[2][0], b[2][1], b[2][2], b[2][3]);
-    nd.hiy = make_float4(b[3][0], b[3][1], b[3][2], b[3][3]);
-    nd.loz = make_float4(b[4][0], b[4][1], b[4][2], b[4][3]);
-    nd.hiz = make_float4(b[5][0], b[5][1], b[5][2], b[5][3]);
-    nd.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
-    nd.pad = make_int4(kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef);
-    *out = nd;
+// Child slot c of a node: box (lo, hi) and item; an unused slot is the
+// empty item with a zero box.
+KF_DEV void tile_child(Node4* __restrict__ nd, uint32_t c, float4 lo, float4 hi, uint32_t item) {
+    float* f = reinterpret_cast<float*>(nd);
+    f[c] = lo.x;
+    f[4 + c] = hi.x;
+    f[8 + c] = lo.y;
+    f[12 + c] = hi.y;
+    f[16 + c] = lo.z;
+    f[20 + c] = hi.z;
+    uint32_t* u = reinterpret_cast<uint32_t*>(nd);
+    u[24 + c] = item;
+    u[28 + c] = (uint32_t)kEmptyRef;
 }
 
 // One wave per tile (n_env * tx * ty tiles, tx x ty per environment): the
-// lanes test the environment's instances 64 at a time, lane 0 builds the
-// tile's nodes at out[n_tlas + kTileNodes * tile ...].  The scene's n_tlas
+// lanes test the environment's instances 64 at a time, then write the tile's
+// nodes at out[n_tlas + kTileNodes * tile ...] one child slot per lane (the
+// root at the first, a group of 2-4 instances at 1 + group).  The scene's n_tlas
 // TLAS nodes are copied to out[0, n_tlas) first, so that the environments'
 // root items index the same nodes there.
 __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const Node4* __restrict__ tlas,
@@ -200,6 +192,7 @@ __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const N
                                                    const uint32_t* __restrict__ env_off, uint32_t tx, uint32_t ty,
                                                    Node4* __restrict__ out, uint32_t* __restrict__ tile_root) {
     __shared__ uint32_t s_cand[TB / 64][kTileMax];
+    __shared__ uint32_t s_sorted[TB / 64][kTileMax];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
     {
         const float4* src = (const float4*)tlas;
@@ -231,6 +224,7 @@ __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const N
                              tile_dir(cam, xa, yb, W, He)};
             const DV ctr = tile_dir(cam, 0.5 * (xa + xb), 0.5 * (ya + yb), W, He);
             double sgn = 0.0;
+#pragma unroll
             for (int i = 0; i < 4; ++i) {
                 n[i] = dv_cross(c[i], c[(i + 1) & 3]);
                 const double s = dv_dot(n[i], ctr);
@@ -239,7 +233,9 @@ __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const N
                 ok = ok && s * sgn > 0.0 && len > 0.0 && isfinite(len);
                 n[i] = dv_scale(n[i], ok ? sgn / len : 0.0);  // unit normal pointing into the pyramid
             }
-            for (int i = 0; i < 4 && ok; ++i)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const double ql = dv_len(q[j]);
                     ok = ok && ql > 0.0 && dv_dot(n[i], q[j]) >= kTileMinAngle * ql;
@@ -255,11 +251,12 @@ __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const N
                     const float4 lo = ilo[i], hi = ihi[i];
                     // lo > hi: an instance without triangles to hit
                     in = lo.x <= hi.x && lo.y <= hi.y && lo.z <= hi.z;
-                    for (int pl = 0; pl < 4 && in; ++pl) {
+#pragma unroll
+                    for (int pl = 0; pl < 4; ++pl) {
                         const DV pv = {n[pl].x > 0.0 ? hi.x : lo.x, n[pl].y > 0.0 ? hi.y : lo.y,
                                        n[pl].z > 0.0 ? hi.z : lo.z};
                         const DV v = dv_sub(pv, o);
-                        in = !(dv_dot(n[pl], v) < -kTileBoxEps * dv_len(v));
+                        in = in && !(dv_dot(n[pl], v) < -kTileBoxEps * dv_len(v));
                     }
                 }
                 const unsigned long long m = __ballot(in);
@@ -273,60 +270,70 @@ __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const N
                 root = kEmptyItem;
             } else if (k == 1) {
                 root = make_item(kInstance, s_cand[w][0]);
-            } else if (k <= kTileMax && lane == 0) {
+            } else if (k <= kTileMax) {
+                // lane j < k holds candidate j; with more than four, lanes
+                // rank them by their box centres' depth along the tile's
+                // central ray (ties by list position) and consecutive ranks
+                // form up to four groups of 2-4 (or single instances)
                 const uint32_t node0 = n_tlas + kTileNodes * t;
-                uint32_t c[kTileMax];
-                float4 lo[kTileMax], hi[kTileMax];
-                double key[kTileMax];
-                for (uint32_t j = 0; j < k; ++j) {
-                    c[j] = s_cand[w][j];
-                    lo[j] = ilo[c[j]];
-                    hi[j] = ihi[c[j]];
-                }
-                if (k <= 4) {
-                    uint32_t items[4];
-                    for (uint32_t j = 0; j < k; ++j) items[j] = make_item(kInstance, c[j]);
-                    tile_node(out + node0, items, lo, hi, (int)k);
-                } else {
-                    // groups of consecutive instances in order of their box
-                    // centres' depth along the tile's central ray
+                const bool mine = lane < k;
+                const uint32_t ci = s_cand[w][mine ? lane : 0u];
+                uint32_t rank = lane;
+                if (k > 4) {
+                    const float4 lo = ilo[ci], hi = ihi[ci];
                     const DV ctr = tile_dir(cam, 0.5 * (xa + xb), 0.5 * (ya + yb), W, He);
+                    const DV m = {0.5 * ((double)lo.x + hi.x), 0.5 * ((double)lo.y + hi.y),
+                                  0.5 * ((double)lo.z + hi.z)};
+                    const double key = dv_dot(dv_sub(m, o), ctr);
+                    rank = 0;
                     for (uint32_t j = 0; j < k; ++j) {
-                        const DV m = {0.5 * ((double)lo[j].x + hi[j].x), 0.5 * ((double)lo[j].y + hi[j].y),
-                                      0.5 * ((double)lo[j].z + hi[j].z)};
-                        key[j] = dv_dot(dv_sub(m, o), ctr);
+                        const double kj = __shfl(key, (int)j);
+                        rank += (kj < key || (kj == key && j < lane)) ? 1u : 0u;
                     }
-                    for (uint32_t j = 1; j < k; ++j)  // insertion sort, stable
-                        for (uint32_t a = j; a > 0 && key[a] < key[a - 1]; --a) {
-                            const double tk = key[a]; key[a] = key[a - 1]; key[a - 1] = tk;
-                            const uint32_t tc = c[a]; c[a] = c[a - 1]; c[a - 1] = tc;
-                            const float4 tl = lo[a]; lo[a] = lo[a - 1]; lo[a - 1] = tl;
-                            const float4 th = hi[a]; hi[a] = hi[a - 1]; hi[a - 1] = th;
+                }
+                if (mine) s_sorted[w][rank] = ci;
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (k <= 4) {  // one node over the candidates
+                    if (lane < 4) {
+                        const uint32_t c = s_sorted[w][lane < k ? lane : 0u];
+                        if (lane < k) tile_child(out + node0, lane, ilo[c], ihi[c], make_item(kInstance, c));
+                        else tile_child(out + node0, lane, z, z, kEmptyItem);
+                    }
+                } else {
+                    const uint32_t g = (k + 3u) / 4u, base = k / g, rem = k % g;
+                    auto gsize = [&](uint32_t gi) { return base + (gi < rem ? 1u : 0u); };
+                    auto gstart = [&](uint32_t gi) { return gi * base + min(gi, rem); };
+                    if (lane < 16) {  // (group, slot): the groups' nodes
+                        const uint32_t gi = lane >> 2, slot = lane & 3u;
+                        if (gi < g && gsize(gi) >= 2) {
+                            Node4* nd = out + node0 + 1u + gi;
+                            if (slot < gsize(gi)) {
+                                const uint32_t c = s_sorted[w][gstart(gi) + slot];
+                                tile_child(nd, slot, ilo[c], ihi[c], make_item(kInstance, c));
+                            } else {
+                                tile_child(nd, slot, z, z, kEmptyItem);
+                            }
                         }
-                    const uint32_t g = (k + 3u) / 4u;
-                    uint32_t items[4];
-                    float4 glo[4], ghi[4];
-                    uint32_t j = 0;
-                    for (uint32_t gi = 0; gi < g; ++gi) {
-                        const uint32_t sz = k / g + (gi < k % g ? 1u : 0u);
-                        uint32_t gitems[4];
-                        float4 l = lo[j], h = hi[j];
-                        for (uint32_t a = 0; a < sz; ++a) {
-                            gitems[a] = make_item(kInstance, c[j + a]);
-                            l = make_float4(fminf(l.x, lo[j + a].x), fminf(l.y, lo[j + a].y), fminf(l.z, lo[j + a].z), 0.0f);
-                            h = make_float4(fmaxf(h.x, hi[j + a].x), fmaxf(h.y, hi[j + a].y), fmaxf(h.z, hi[j + a].z), 0.0f);
-                        }
-                        if (sz == 1) {
-                            items[gi] = gitems[0];
+                    } else if (lane < 20) {  // the root's children: groups, or single instances
+                        const uint32_t gi = lane - 16u;
+                        if (gi < g) {
+                            const uint32_t s0 = gstart(gi), n = gsize(gi);
+                            float4 l = ilo[s_sorted[w][s0]], h = ihi[s_sorted[w][s0]];
+                            for (uint32_t a = 1; a < n; ++a) {
+                                const uint32_t c = s_sorted[w][s0 + a];
+                                const float4 la = ilo[c], ha = ihi[c];
+                                l = make_float4(fminf(l.x, la.x), fminf(l.y, la.y), fminf(l.z, la.z), 0.0f);
+                                h = make_float4(fmaxf(h.x, ha.x), fmaxf(h.y, ha.y), fmaxf(h.z, ha.z), 0.0f);
+                            }
+                            const uint32_t item = n == 1 ? make_item(kInstance, s_sorted[w][s0])
+                                                         : make_item(kTlasNode, node0 + 1u + gi);
+                            tile_child(out + node0, gi, l, h, item);
                         } else {
-                            tile_node(out + node0 + 1u + gi, gitems, lo + j, hi + j, (int)sz);
-                            items[gi] = make_item(kTlasNode, node0 + 1u + gi);
+                            tile_child(out + node0, gi, z, z, kEmptyItem);
                         }
-                        glo[gi] = l;
-                        ghi[gi] = h;
-                        j += sz;
                     }
-                    tile_node(out + node0, items, glo, ghi, (int)g);
                 }
                 root = make_item(kTlasNode, node0);
             }
