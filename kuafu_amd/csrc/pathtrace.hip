@@ -150,19 +150,31 @@ KF_DEV DV dv_cross(DV a, DV b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x 
 KF_DEV double dv_len(DV a) { return sqrt(dv_dot(a, a)); }
 KF_DEV DV dv_scale(DV a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 
-// World direction (not normalised) of the camera ray through raster point
-// (px, py) of a W x He image, as k_raygen maps it, in double.
-KF_DEV DV tile_dir(const kf_camera_ubo& cam, double px, double py, double W, double He) {
-    const double dx = px / W * 2.0 - 1.0, dy = py / He * 2.0 - 1.0;
+// k_raygen's raster -> world direction map (not normalised) is affine in the
+// raster point: dir(px, py) = a + px bx + py by for a W x He image, where
+// target = proj_inverse (2 px / W - 1, 2 py / He - 1, 1, 1) and dir =
+// view_inverse (target.xyz, 0).  In double.
+struct DirMap {
+    DV a, bx, by;
+};
+KF_DEV DirMap dir_map(const kf_camera_ubo& cam, double W, double He) {
     const float* P = cam.proj_inverse;
     const float* V = cam.view_inverse;
-    double t[3];
-    for (int r = 0; r < 3; ++r) t[r] = (double)P[r] * dx + (double)P[4 + r] * dy + (double)P[8 + r] + (double)P[12 + r];
-    DV d;
-    d.x = (double)V[0] * t[0] + (double)V[4] * t[1] + (double)V[8] * t[2];
-    d.y = (double)V[1] * t[0] + (double)V[5] * t[1] + (double)V[9] * t[2];
-    d.z = (double)V[2] * t[0] + (double)V[6] * t[1] + (double)V[10] * t[2];
-    return d;
+    auto rot = [&](double t0, double t1, double t2) -> DV {
+        return {(double)V[0] * t0 + (double)V[4] * t1 + (double)V[8] * t2,
+                (double)V[1] * t0 + (double)V[5] * t1 + (double)V[9] * t2,
+                (double)V[2] * t0 + (double)V[6] * t1 + (double)V[10] * t2};
+    };
+    auto p = [&](int i) { return (double)P[i]; };
+    const double sx = 2.0 / W, sy = 2.0 / He;
+    DirMap m;
+    m.a = rot(p(8) + p(12) - p(0) - p(4), p(9) + p(13) - p(1) - p(5), p(10) + p(14) - p(2) - p(6));
+    m.bx = rot(p(0) * sx, p(1) * sx, p(2) * sx);
+    m.by = rot(p(4) * sy, p(5) * sy, p(6) * sy);
+    return m;
+}
+KF_DEV DV dir_at(const DirMap& m, double px, double py) {
+    return {m.a.x + px * m.bx.x + py * m.by.x, m.a.y + px * m.bx.y + py * m.by.y, m.a.z + px * m.bx.z + py * m.by.z};
 }
 
 // Child slot c of a node: box (lo, hi) and item; an unused slot is the
@@ -216,13 +228,13 @@ __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const N
         const double ya = (double)(by * kTile), yb = (double)min((by + 1u) * kTile, F.He);
         bool ok = env_item != kEmptyItem && !(cam.position[3] > 0.0f) && isfinite(org[0]) && isfinite(org[1]) &&
                   isfinite(org[2]);
+        const DirMap dm = dir_map(cam, W, He);
+        const DV ctr = dir_at(dm, 0.5 * (xa + xb), 0.5 * (ya + yb));  // the tile's central ray
         DV n[4];
         if (ok) {
-            const DV c[4] = {tile_dir(cam, xa - 0.5, ya - 0.5, W, He), tile_dir(cam, xb + 0.5, ya - 0.5, W, He),
-                             tile_dir(cam, xb + 0.5, yb + 0.5, W, He), tile_dir(cam, xa - 0.5, yb + 0.5, W, He)};
-            const DV q[4] = {tile_dir(cam, xa, ya, W, He), tile_dir(cam, xb, ya, W, He), tile_dir(cam, xb, yb, W, He),
-                             tile_dir(cam, xa, yb, W, He)};
-            const DV ctr = tile_dir(cam, 0.5 * (xa + xb), 0.5 * (ya + yb), W, He);
+            const DV c[4] = {dir_at(dm, xa - 0.5, ya - 0.5), dir_at(dm, xb + 0.5, ya - 0.5),
+                             dir_at(dm, xb + 0.5, yb + 0.5), dir_at(dm, xa - 0.5, yb + 0.5)};
+            const DV q[4] = {dir_at(dm, xa, ya), dir_at(dm, xb, ya), dir_at(dm, xb, yb), dir_at(dm, xa, yb)};
             double sgn = 0.0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -234,12 +246,11 @@ __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const N
                 n[i] = dv_scale(n[i], ok ? sgn / len : 0.0);  // unit normal pointing into the pyramid
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                const double ql = dv_len(q[j]);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const double ql = dv_len(q[j]);
-                    ok = ok && ql > 0.0 && dv_dot(n[i], q[j]) >= kTileMinAngle * ql;
-                }
+                for (int i = 0; i < 4; ++i) ok = ok && ql > 0.0 && dv_dot(n[i], q[j]) >= kTileMinAngle * ql;
+            }
         }
         if (ok) {
             const uint32_t i0 = env_off ? env_off[e] : 0u, i1 = env_off ? env_off[e + 1] : S.n_inst;
@@ -256,7 +267,8 @@ __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const N
                         const DV pv = {n[pl].x > 0.0 ? hi.x : lo.x, n[pl].y > 0.0 ? hi.y : lo.y,
                                        n[pl].z > 0.0 ? hi.z : lo.z};
                         const DV v = dv_sub(pv, o);
-                        in = in && !(dv_dot(n[pl], v) < -kTileBoxEps * dv_len(v));
+                        const double dd = dv_dot(n[pl], v);  // outside: dd < -kTileBoxEps |v|
+                        in = in && !(dd < 0.0 && dd * dd > kTileBoxEps * kTileBoxEps * dv_dot(v, v));
                     }
                 }
                 const unsigned long long m = __ballot(in);
@@ -281,7 +293,6 @@ __global__ __launch_bounds__(TB) void k_tile_roots(SceneView S, Frame F, const N
                 uint32_t rank = lane;
                 if (k > 4) {
                     const float4 lo = ilo[ci], hi = ihi[ci];
-                    const DV ctr = tile_dir(cam, 0.5 * (xa + xb), 0.5 * (ya + yb), W, He);
                     const DV m = {0.5 * ((double)lo.x + hi.x), 0.5 * ((double)lo.y + hi.y),
                                   0.5 * ((double)lo.z + hi.z)};
                     const double key = dv_dot(dv_sub(m, o), ctr);
