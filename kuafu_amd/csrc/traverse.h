@@ -45,8 +45,10 @@ KF_DEV v3 rcp3(v3 d) { return mk(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rc
 
 // Shear constants of the watertight test: kz = dominant axis, (kx, ky)
 // swapped when d[kz] < 0 to keep the winding; Sz = 1 / d[kz] correctly
-// rounded, Sx = d[kx] * Sz, Sy = d[ky] * Sz.
-KF_DEV void precompute(v3 o, v3 d, Pre& p) {
+// rounded, Sx = d[kx] * Sz, Sy = d[ky] * Sz.  In two parts so that a caller
+// can share the division: precompute_axes returns d[kz], precompute_scale
+// takes its correctly rounded reciprocal.
+KF_DEV float precompute_axes(v3 o, v3 d, Pre& p) {
     p.o = o;
     p.invd = rcp3(d);
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
@@ -62,9 +64,12 @@ KF_DEV void precompute(v3 o, v3 d, Pre& p) {
     p.y1 = pky == 1;
     p.z0 = kz == 0;
     p.z1 = kz == 1;
-    p.Sz = 1.0f / dz;
-    p.Sx = comp(d, pkx) * p.Sz;
-    p.Sy = comp(d, pky) * p.Sz;
+    return dz;
+}
+KF_DEV void precompute_scale(v3 d, float rcp_dz, Pre& p) {
+    p.Sz = rcp_dz;
+    p.Sx = comp2(d, p.x0, p.x1) * p.Sz;
+    p.Sy = comp2(d, p.y0, p.y1) * p.Sz;
 }
 
 struct TriHit {
@@ -73,9 +78,16 @@ struct TriHit {
 };
 
 // Watertight ray/triangle test without early exits (the caller runs it for
-// every lane of a wave).  en gates the double-precision fallback so that
-// lanes holding other records never take it.
-KF_DEV bool intersect(const Pre& p, float4 v0, float4 v1, float4 v2, float tmin, float tmax, bool en, TriHit& h) {
+// every lane of a wave), in two parts around its one division 1 / det, so
+// that a caller can share that division: intersect_begin returns the edge
+// functions, det and the scaled distance; intersect_end takes the correctly
+// rounded 1 / det.  en gates the double-precision fallback so that lanes
+// holding other records never take it.
+struct TriEdge {
+    float U, V, W, det, Tn;
+    bool outside;
+};
+KF_DEV TriEdge intersect_begin(const Pre& p, float4 v0, float4 v1, float4 v2, bool en) {
     v3 A = mk(v0.x, v0.y, v0.z) - p.o;
     v3 B = mk(v1.x, v1.y, v1.z) - p.o;
     v3 C = mk(v2.x, v2.y, v2.z) - p.o;
@@ -94,19 +106,25 @@ KF_DEV bool intersect(const Pre& p, float4 v0, float4 v1, float4 v2, float tmin,
         V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
         W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
     }
-    const bool outside = (U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f);
-    const float det = U + V + W;
+    TriEdge e;
+    e.outside = (U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f);
+    e.det = U + V + W;
     const float Az = p.Sz * Akz;
     const float Bz = p.Sz * Bkz;
     const float Cz = p.Sz * Ckz;
-    const float Tn = U * Az + V * Bz + W * Cz;
-    const float rcp = 1.0f / det;
-    const float t = Tn * rcp;
+    e.Tn = U * Az + V * Bz + W * Cz;
+    e.U = U;
+    e.V = V;
+    e.W = W;
+    return e;
+}
+KF_DEV bool intersect_end(const TriEdge& e, float rcp_det, float tmin, float tmax, TriHit& h) {
+    const float t = e.Tn * rcp_det;
     h.t = t;
-    h.b1 = V * rcp;
-    h.b2 = W * rcp;
-    h.back = det < 0.0f ? 1u : 0u;
-    return !outside && det != 0.0f && t >= tmin && t <= tmax;
+    h.b1 = e.V * rcp_det;
+    h.b2 = e.W * rcp_det;
+    h.back = e.det < 0.0f ? 1u : 0u;
+    return !e.outside && e.det != 0.0f && t >= tmin && t <= tmax;
 }
 
 KF_DEV float alpha_coin(uint32_t seed, uint32_t inst, uint32_t prim) {
@@ -392,31 +410,37 @@ struct Trav {
         // step's node test already uses for the BLAS root's children (an
         // invalid instance's record has no children, so entering it changes
         // nothing the traversal reads later).  Other lanes keep their ray.
-        {
-            const bool enter = is_inst;
-            Inst I;
-            I.w2o[0] = make_float4(r7.x, r7.y, r7.z, r7.w);
-            I.w2o[1] = make_float4(r8.x, r8.y, r8.z, r8.w);
-            I.w2o[2] = make_float4(r9.x, r9.y, r9.z, r9.w);
-            Pre nobj;
-            precompute(xform_point(I, o), xform_dir(I, d), nobj);
-            obj.o = sel3(enter, nobj.o, obj.o);
-            obj.invd = sel3(enter, nobj.invd, obj.invd);
-            obj.x0 = enter ? nobj.x0 : obj.x0;
-            obj.x1 = enter ? nobj.x1 : obj.x1;
-            obj.y0 = enter ? nobj.y0 : obj.y0;
-            obj.y1 = enter ? nobj.y1 : obj.y1;
-            obj.z0 = enter ? nobj.z0 : obj.z0;
-            obj.z1 = enter ? nobj.z1 : obj.z1;
-            obj.Sx = enter ? nobj.Sx : obj.Sx;
-            obj.Sy = enter ? nobj.Sy : obj.Sy;
-            obj.Sz = enter ? nobj.Sz : obj.Sz;
-            cur_inst = enter ? idx : cur_inst;
-        }
+        // A lane enters an instance or tests a triangle, never both: the two
+        // correctly rounded reciprocals (1 / d[kz] of the new shear, 1 / det
+        // of the triangle) share one division.
+        const bool enter = is_inst;
+        Inst I;
+        I.w2o[0] = make_float4(r7.x, r7.y, r7.z, r7.w);
+        I.w2o[1] = make_float4(r8.x, r8.y, r8.z, r8.w);
+        I.w2o[2] = make_float4(r9.x, r9.y, r9.z, r9.w);
+        Pre nobj;
+        const v3 nd = xform_dir(I, d);
+        const float dz = precompute_axes(xform_point(I, o), nd, nobj);
+        obj.o = sel3(enter, nobj.o, obj.o);
+        obj.invd = sel3(enter, nobj.invd, obj.invd);
+        obj.x0 = enter ? nobj.x0 : obj.x0;
+        obj.x1 = enter ? nobj.x1 : obj.x1;
+        obj.y0 = enter ? nobj.y0 : obj.y0;
+        obj.y1 = enter ? nobj.y1 : obj.y1;
+        obj.z0 = enter ? nobj.z0 : obj.z0;
+        obj.z1 = enter ? nobj.z1 : obj.z1;
+        cur_inst = enter ? idx : cur_inst;
 
-        // triangle, against the current instance's object-space ray
+        // triangle, against the current instance's object-space ray (lanes
+        // entering an instance discard it)
+        const TriEdge te = intersect_begin(obj, q0, q1, q2, is_tri);
+        const float q = 1.0f / (enter ? dz : te.det);
         TriHit h;
-        const bool tri_hit = intersect(obj, q0, q1, q2, tmin, best.t, is_tri, h) && is_tri;
+        const bool tri_hit = intersect_end(te, q, tmin, best.t, h) && is_tri;
+        precompute_scale(nd, q, nobj);
+        obj.Sx = enter ? nobj.Sx : obj.Sx;
+        obj.Sy = enter ? nobj.Sy : obj.Sy;
+        obj.Sz = enter ? nobj.Sz : obj.Sz;
         const uint32_t prim = __float_as_uint(q0.w);
         bool take = tri_hit;
         if (!ANY) {
