@@ -340,7 +340,25 @@ __global__ void k_tri_bounds(uint32_t n_tri, const float4* __restrict__ verts, c
                         fmaxf(p[0].z, fmaxf(p[1].z, p[2].z)), 0.f);
 }
 
-// Triangle slots in BVH leaf order: (v0, local prim id bits), v1, v2.
+// Triangle slot s, by axis: tri[3s + a] = (v0[a], v1[a], v2[a], w), w the
+// local prim id bits on line 0 and 0 on lines 1-2.  Line a minus the ray
+// origin's axis a is what a node's low planes subtract too, so the traversal
+// step shares those differences between its node and triangle tests.
+KF_DEV_INLINE void put_tri_slot(float4* __restrict__ tri, size_t s, const float3 p[3], uint32_t prim) {
+    tri[3 * s + 0] = make_float4(p[0].x, p[1].x, p[2].x, __uint_as_float(prim));
+    tri[3 * s + 1] = make_float4(p[0].y, p[1].y, p[2].y, 0.f);
+    tri[3 * s + 2] = make_float4(p[0].z, p[1].z, p[2].z, 0.f);
+}
+// Box of triangle slot s.
+KF_DEV_INLINE void tri_slot_box(const float4* __restrict__ tri, size_t s, float lo[3], float hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const float4 l = tri[3 * s + a];
+        lo[a] = fminf(l.x, fminf(l.y, l.z));
+        hi[a] = fmaxf(l.x, fmaxf(l.y, l.z));
+    }
+}
+
+// Triangle slots in BVH leaf order.
 __global__ void k_tri_reorder(uint32_t n_tri, const uint32_t* __restrict__ order, const float4* __restrict__ verts,
                               const uint32_t* __restrict__ idx, const uint32_t* __restrict__ voff,
                               const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tri_geom,
@@ -350,10 +368,12 @@ __global__ void k_tri_reorder(uint32_t n_tri, const uint32_t* __restrict__ order
     uint32_t i = order[s];
     uint32_t g = tri_geom[i];
     uint32_t vb = voff[g];
+    float3 p[3];
     for (int k = 0; k < 3; ++k) {
-        float4 v = verts[3 * (size_t)(vb + idx[3 * (size_t)i + k])];
-        tri[3 * (size_t)s + k] = make_float4(v.x, v.y, v.z, k == 0 ? __uint_as_float(i - toff[g]) : 0.f);
+        const float4 v = verts[3 * (size_t)(vb + idx[3 * (size_t)i + k])];
+        p[k] = make_float3(v.x, v.y, v.z);
     }
+    put_tri_slot(tri, s, p, i - toff[g]);
 }
 
 // Object-space bounds of each BLAS (union of the root's children, or the
@@ -366,12 +386,10 @@ __global__ void k_blas_box(uint32_t n_geom, const int* __restrict__ root, const 
     float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0);
     if (r == kEmptyRef) {
     } else if (r < 0) {
-        size_t s = (size_t)(~r);
-        for (int k = 0; k < 3; ++k) {
-            float4 v = tri[3 * s + k];
-            lo.x = fminf(lo.x, v.x); lo.y = fminf(lo.y, v.y); lo.z = fminf(lo.z, v.z);
-            hi.x = fmaxf(hi.x, v.x); hi.y = fmaxf(hi.y, v.y); hi.z = fmaxf(hi.z, v.z);
-        }
+        float l[3], h[3];
+        tri_slot_box(tri, (size_t)(~r), l, h);
+        lo = make_float4(l[0], l[1], l[2], 0);
+        hi = make_float4(h[0], h[1], h[2], 0);
     } else {
         Node n = nodes[r];
         lo = make_float4(fminf(n.a.x, n.b.x), fminf(n.a.z, n.b.z), fminf(n.c.x, n.c.z), 0);
@@ -797,8 +815,8 @@ __global__ void k_geom_slots(uint32_t g, const float4* __restrict__ verts, const
     for (int k = 0; k < 3; ++k) {
         const float4 v = verts[3 * (size_t)(voff[g] + idx[3 * i + k])];
         p[k] = make_float3(v.x, v.y, v.z);
-        tri[3 * slot + k] = make_float4(v.x, v.y, v.z, k == 0 ? __uint_as_float(prim) : 0.f);
     }
+    put_tri_slot(tri, slot, p, prim);
     lo[j] = make_float4(fminf(p[0].x, fminf(p[1].x, p[2].x)), fminf(p[0].y, fminf(p[1].y, p[2].y)),
                         fminf(p[0].z, fminf(p[1].z, p[2].z)), 0.f);
     hi[j] = make_float4(fmaxf(p[0].x, fmaxf(p[1].x, p[2].x)), fmaxf(p[0].y, fmaxf(p[1].y, p[2].y)),
@@ -876,14 +894,7 @@ __global__ void k_geom_refit4(uint32_t g, const uint32_t* __restrict__ toff, con
                 lo[2] = fminf(m.c.x, m.c.z); hi[2] = fmaxf(m.c.y, m.c.w);
             }
         } else if ((uint32_t)~r >= t0 && (uint32_t)~r < t0 + n) {
-            const size_t sl = (size_t)~r;
-            for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
-            for (int k = 0; k < 3; ++k) {
-                const float4 v = tri[3 * sl + k];
-                lo[0] = fminf(lo[0], v.x); hi[0] = fmaxf(hi[0], v.x);
-                lo[1] = fminf(lo[1], v.y); hi[1] = fmaxf(hi[1], v.y);
-                lo[2] = fminf(lo[2], v.z); hi[2] = fmaxf(hi[2], v.z);
-            }
+            tri_slot_box(tri, (size_t)~r, lo, hi);
         }
         for (int a = 0; a < 3; ++a) {
             blo[a][c] = lo[a];

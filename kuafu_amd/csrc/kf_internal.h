@@ -88,7 +88,7 @@ constexpr int kEmptyRef = 0x7fffffff;  // root ref of a BLAS without triangles
 // Everything the traversal and shading kernels read, by value.
 struct SceneView {
     const Node4* blas;        // BVH4 of all geometries
-    const float4* tri;        // 3 float4 per triangle slot; tri[3s].w = local prim id bits
+    const float4* tri;        // 3 float4 per triangle slot, by axis: (v0[a], v1[a], v2[a], -); tri[3s].w = local prim id bits
     const Node4* tlas;        // BVH4 over the instances (one tree per environment)
     int tlas_root;            // BVH4 root ref of environment 0's TLAS
     const uint32_t* env_root; // n_env > 1: root item of each environment's TLAS (kEmptyItem: no instance)

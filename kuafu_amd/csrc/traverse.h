@@ -47,10 +47,9 @@ KF_DEV v3 rcp3(v3 d) { return mk(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rc
 // swapped when d[kz] < 0 to keep the winding; Sz = 1 / d[kz] correctly
 // rounded, Sx = d[kx] * Sz, Sy = d[ky] * Sz.  In two parts so that a caller
 // can share the division: precompute_axes returns d[kz], precompute_scale
-// takes its correctly rounded reciprocal.
+// takes its correctly rounded reciprocal.  The caller sets invd.
 KF_DEV float precompute_axes(v3 o, v3 d, Pre& p) {
     p.o = o;
-    p.invd = rcp3(d);
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;
@@ -78,8 +77,9 @@ struct TriHit {
 };
 
 // Watertight ray/triangle test without early exits (the caller runs it for
-// every lane of a wave), in two parts around its one division 1 / det, so
-// that a caller can share that division: intersect_begin returns the edge
+// every lane of a wave) on the vertices relative to the ray origin, in two
+// parts around its one division 1 / det, so that a caller can share that
+// division: intersect_begin returns the edge
 // functions, det and the scaled distance; intersect_end takes the correctly
 // rounded 1 / det.  en gates the double-precision fallback so that lanes
 // holding other records never take it.
@@ -87,10 +87,7 @@ struct TriEdge {
     float U, V, W, det, Tn;
     bool outside;
 };
-KF_DEV TriEdge intersect_begin(const Pre& p, float4 v0, float4 v1, float4 v2, bool en) {
-    v3 A = mk(v0.x, v0.y, v0.z) - p.o;
-    v3 B = mk(v1.x, v1.y, v1.z) - p.o;
-    v3 C = mk(v2.x, v2.y, v2.z) - p.o;
+KF_DEV TriEdge intersect_begin(const Pre& p, v3 A, v3 B, v3 C, bool en) {  // A, B, C: vertices - p.o
     float Akz = comp2(A, p.z0, p.z1), Bkz = comp2(B, p.z0, p.z1), Ckz = comp2(C, p.z0, p.z1);
     float Ax = comp2(A, p.x0, p.x1) - p.Sx * Akz;
     float Ay = comp2(A, p.y0, p.y1) - p.Sy * Akz;
@@ -379,8 +376,9 @@ struct Trav {
         // Ten 16-byte global loads issued together: lines 0-6 of a BVH4 node
         // (128 B) or of an instance record's root node, lines 7-9 of an
         // instance record (its transform; other lanes read the zero record
-        // there).  Triangles (48 B) repeat loads of their own first
-        // lines instead of reading past the record.  The empty asm needs all
+        // there).  A triangle's x, y and z lines (48 B) go where a node's
+        // low x, y and z planes go (r0, r2, r4), and its other loads repeat
+        // its own lines instead of reading past the record.  The empty asm needs all
         // results, which keeps the compiler from sinking parts of the fetch
         // into the kind-specific code below (each sunk load would cost the
         // wave another memory round trip).
@@ -388,14 +386,13 @@ struct Trav {
         const bool wide = kind != kTriangle && on;
         const bool irec = kind == kInstance && on;
         const gf4* GI = irec ? G + 7 : (const gf4*)p_zero;
-        const vf4 r0 = G[0], r1 = G[1], r2 = G[2], r3 = G[(kind == kTriangle && on) ? 2 : 3];
-        const vf4 r4 = G[wide ? 4 : 0], r5 = G[wide ? 5 : 1], r6 = G[wide ? 6 : 2];  // line 7 of a node is padding
+        const bool trec = kind == kTriangle && on;
+        const vf4 r0 = G[0], r1 = G[1], r2 = G[trec ? 1 : 2], r3 = G[trec ? 2 : 3];
+        const vf4 r4 = G[wide ? 4 : 2], r5 = G[wide ? 5 : 1], r6 = G[wide ? 6 : 0];  // line 7 of a node is padding
         const vf4 r7 = GI[0], r8 = GI[1], r9 = GI[2];
         const uint32_t below = stk.peek(sp);
         asm volatile("" ::"v"(r0), "v"(r1), "v"(r2), "v"(r3), "v"(r4), "v"(r5), "v"(r6), "v"(r7), "v"(r8), "v"(r9),
                      "v"(below));
-        const float4 q0 = make_float4(r0.x, r0.y, r0.z, r0.w), q1 = make_float4(r1.x, r1.y, r1.z, r1.w);
-        const float4 q2 = make_float4(r2.x, r2.y, r2.z, r2.w);
 
         // Every lane evaluates the record as a node, an instance and a
         // triangle and keeps the result of its kind: in a wave64 nearly every
@@ -422,7 +419,12 @@ struct Trav {
         const v3 nd = xform_dir(I, d);
         const float dz = precompute_axes(xform_point(I, o), nd, nobj);
         obj.o = sel3(enter, nobj.o, obj.o);
-        obj.invd = sel3(enter, nobj.invd, obj.invd);
+        // one set of hardware reciprocals (culling only) for the ray this
+        // step's node test uses fresh: the world ray of a TLAS node, the new
+        // object-space ray of an instance (BLAS nodes reuse obj.invd)
+        const bool world = kind == kTlasNode;
+        const v3 rinv = rcp3(sel3(world, d, nd));
+        obj.invd = sel3(enter, rinv, obj.invd);
         obj.x0 = enter ? nobj.x0 : obj.x0;
         obj.x1 = enter ? nobj.x1 : obj.x1;
         obj.y0 = enter ? nobj.y0 : obj.y0;
@@ -431,9 +433,19 @@ struct Trav {
         obj.z1 = enter ? nobj.z1 : obj.z1;
         cur_inst = enter ? idx : cur_inst;
 
+        // The ray this step's node test uses: the world ray for a TLAS node,
+        // the object-space ray (new for an instance) otherwise.  Its low-plane
+        // differences are, on a triangle's lines, the triangle's vertices
+        // minus the object-space origin.
+        const v3 ro = sel3(world, o, obj.o), ri = sel3(world, rinv, obj.invd);
+        const float dlx[4] = {r0.x - ro.x, r0.y - ro.x, r0.z - ro.x, r0.w - ro.x};
+        const float dly[4] = {r2.x - ro.y, r2.y - ro.y, r2.z - ro.y, r2.w - ro.y};
+        const float dlz[4] = {r4.x - ro.z, r4.y - ro.z, r4.z - ro.z, r4.w - ro.z};
+
         // triangle, against the current instance's object-space ray (lanes
         // entering an instance discard it)
-        const TriEdge te = intersect_begin(obj, q0, q1, q2, is_tri);
+        const TriEdge te = intersect_begin(obj, mk(dlx[0], dly[0], dlz[0]), mk(dlx[1], dly[1], dlz[1]),
+                                           mk(dlx[2], dly[2], dlz[2]), is_tri);
         const float q = 1.0f / (enter ? dz : te.det);
         TriHit h;
         const bool tri_hit = intersect_end(te, q, tmin, best.t, h) && is_tri;
@@ -441,7 +453,7 @@ struct Trav {
         obj.Sx = enter ? nobj.Sx : obj.Sx;
         obj.Sy = enter ? nobj.Sy : obj.Sy;
         obj.Sz = enter ? nobj.Sz : obj.Sz;
-        const uint32_t prim = __float_as_uint(q0.w);
+        const uint32_t prim = __float_as_uint(r0.w);
         bool take = tri_hit;
         if (!ANY) {
             take = take && (h.t < best.t || cur_inst < best.inst ||
@@ -468,19 +480,17 @@ struct Trav {
         // node: four child boxes (Node4 layout: lox, hix, loy, hiy, loz, hiz,
         // ref), against the world ray (TLAS) or the object-space ray (BLAS
         // nodes, and instance records, whose ray was set above)
-        const bool world = kind == kTlasNode;
-        const v3 ro = sel3(world, o, obj.o), ri = sel3(world, rcp3(d), obj.invd);  // rcp: culling only
-        const float lox[4] = {r0.x, r0.y, r0.z, r0.w}, hix[4] = {r1.x, r1.y, r1.z, r1.w};
-        const float loy[4] = {r2.x, r2.y, r2.z, r2.w}, hiy[4] = {r3.x, r3.y, r3.z, r3.w};
-        const float loz[4] = {r4.x, r4.y, r4.z, r4.w}, hiz[4] = {r5.x, r5.y, r5.z, r5.w};
+        const float hix[4] = {r1.x, r1.y, r1.z, r1.w};
+        const float hiy[4] = {r3.x, r3.y, r3.z, r3.w};
+        const float hiz[4] = {r5.x, r5.y, r5.z, r5.w};
         uint32_t citem[4] = {__float_as_uint(r6.x), __float_as_uint(r6.y), __float_as_uint(r6.z), __float_as_uint(r6.w)};
         float key[4];
         int nhit = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const float l0x = (lox[c] - ro.x) * ri.x, l1x = (hix[c] - ro.x) * ri.x;
-            const float l0y = (loy[c] - ro.y) * ri.y, l1y = (hiy[c] - ro.y) * ri.y;
-            const float l0z = (loz[c] - ro.z) * ri.z, l1z = (hiz[c] - ro.z) * ri.z;
+            const float l0x = dlx[c] * ri.x, l1x = (hix[c] - ro.x) * ri.x;
+            const float l0y = dly[c] * ri.y, l1y = (hiy[c] - ro.y) * ri.y;
+            const float l0z = dlz[c] * ri.z, l1z = (hiz[c] - ro.z) * ri.z;
             const float tn = fmaxf(fmaxf(fminf(l0x, l1x), fminf(l0y, l1y)), fmaxf(fminf(l0z, l1z), tmin));
             const float tf = fminf(fminf(fmaxf(l0x, l1x), fmaxf(l0y, l1y)), fminf(fmaxf(l0z, l1z), best.t));
             const bool h = tn <= tf * kSlabScale && citem[c] != kEmptyItem;
