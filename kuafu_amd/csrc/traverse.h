@@ -272,7 +272,13 @@ KF_DEV uint32_t env_root_item(const SceneView& S, uint32_t e) {
 template <bool ANY, bool COUNT>
 struct Trav {
     v3 o, d;         // world-space ray
-    Pre obj;         // object-space ray of the current instance
+    Pre obj;         // object-space ray of the current instance (its axis
+                     // flags are kept in the masks below instead)
+    // Shear axes of the object-space ray as wave-wide lane masks (bit l:
+    // lane l's flag), updated with scalar instructions when lanes enter an
+    // instance and read back as per-lane conditions at no cost; as per-lane
+    // flags their update took about 30 vector instructions per step.
+    uint64_t mx0, mx1, my0, my1, mz0, mz1;
     float tmin;
     THit best;
     uint32_t cur_inst;  // instance whose BLAS the object-space ray belongs to
@@ -291,6 +297,8 @@ struct Trav {
         obj.o = obj.invd = o;
         obj.x0 = obj.y1 = true;  // kx = 0, ky = 1, kz = 2
         obj.x1 = obj.y0 = obj.z0 = obj.z1 = false;
+        mx0 = my1 = ~0ull;
+        mx1 = my0 = mz0 = mz1 = 0ull;
         obj.Sx = obj.Sy = obj.Sz = 0.0f;
         tmin = 0.001f;  // the path tracer's constant, so that its kernels can fold it
         best.t = best.b1 = best.b2 = 0.0f;
@@ -431,12 +439,14 @@ struct Trav {
             nd = xform_dir(I, d);
             dz = precompute_axes(xform_point(I, o), nd, nobj);
             obj.o = sel3(enter, nobj.o, obj.o);
-            obj.x0 = enter ? nobj.x0 : obj.x0;
-            obj.x1 = enter ? nobj.x1 : obj.x1;
-            obj.y0 = enter ? nobj.y0 : obj.y0;
-            obj.y1 = enter ? nobj.y1 : obj.y1;
-            obj.z0 = enter ? nobj.z0 : obj.z0;
-            obj.z1 = enter ? nobj.z1 : obj.z1;
+            // lanes outside the exec mask are not in E and keep their bits
+            const uint64_t E = __builtin_amdgcn_ballot_w64(enter);
+            mx0 = (__builtin_amdgcn_ballot_w64(nobj.x0) & E) | (mx0 & ~E);
+            mx1 = (__builtin_amdgcn_ballot_w64(nobj.x1) & E) | (mx1 & ~E);
+            my0 = (__builtin_amdgcn_ballot_w64(nobj.y0) & E) | (my0 & ~E);
+            my1 = (__builtin_amdgcn_ballot_w64(nobj.y1) & E) | (my1 & ~E);
+            mz0 = (__builtin_amdgcn_ballot_w64(nobj.z0) & E) | (mz0 & ~E);
+            mz1 = (__builtin_amdgcn_ballot_w64(nobj.z1) & E) | (mz1 & ~E);
             cur_inst = enter ? idx : cur_inst;
         }
         // one set of hardware reciprocals (culling only) for the ray this
@@ -459,9 +469,17 @@ struct Trav {
         // entering an instance discard it)
         TriEdge te;
         te.det = 1.0f;
-        if (any_tri)
-            te = intersect_begin(obj, mk(dlx[0], dly[0], dlz[0]), mk(dlx[1], dly[1], dlz[1]),
+        if (any_tri) {
+            Pre pt = obj;
+            pt.x0 = __builtin_amdgcn_inverse_ballot_w64(mx0);
+            pt.x1 = __builtin_amdgcn_inverse_ballot_w64(mx1);
+            pt.y0 = __builtin_amdgcn_inverse_ballot_w64(my0);
+            pt.y1 = __builtin_amdgcn_inverse_ballot_w64(my1);
+            pt.z0 = __builtin_amdgcn_inverse_ballot_w64(mz0);
+            pt.z1 = __builtin_amdgcn_inverse_ballot_w64(mz1);
+            te = intersect_begin(pt, mk(dlx[0], dly[0], dlz[0]), mk(dlx[1], dly[1], dlz[1]),
                                  mk(dlx[2], dly[2], dlz[2]), is_tri);
+        }
         TriHit h;
         bool tri_hit = false;
         if (any_tri || any_inst) {
