@@ -180,6 +180,12 @@ typedef struct kf_scene_desc {
      * environment holding every instance (env_instance_offset unused). */
     uint32_t num_envs;
     const uint32_t* env_instance_offset; /* num_envs + 1, from 0 to num_instances */
+    /* 0: geometries whose triangles equal an earlier geometry's (the same
+     * vertex positions at every corner, bit for bit) traverse that
+     * geometry's BVH (no counterpart in the reference, which builds one BLAS
+     * per geometry; results are the same, the traversal's working set is
+     * smaller); 1: every geometry traverses its own BVH. */
+    uint32_t separate_blas;
 } kf_scene_desc;
 
 /* A ray of a fixed batch: origin, tmin, direction, tmax (32 B). */

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 
 namespace kf {
 
@@ -793,6 +794,69 @@ void build_blas(Scene& s) {
     KF_CHECK(hipStreamSynchronize(st));
     dfree(err);
     if (h_err) throw Error("BLAS build: malformed BVH2");
+    upload_trav_roots(s);
+}
+
+// d_trav_root4[g] = the BVH4 root of the geometry whose BVH g traverses.
+void upload_trav_roots(Scene& s) {
+    if (s.h_canon.size() != s.n_geom) {
+        s.h_canon.resize(s.n_geom);
+        for (uint32_t g = 0; g < s.n_geom; ++g) s.h_canon[g] = g;
+    }
+    std::vector<int> roots(std::max<uint32_t>(s.n_geom, 1), kEmptyRef);
+    for (uint32_t g = 0; g < s.n_geom; ++g) roots[g] = s.h_blas_root4[s.h_canon[g]];
+    if (!s.d_trav_root4) s.d_trav_root4 = dalloc<int>(std::max<uint32_t>(s.n_geom, 1));
+    KF_CHECK(hipMemcpy(s.d_trav_root4, roots.data(), roots.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void share_blas(Scene& s, const kf_vertex* verts, const uint32_t* idx, bool separate) {
+    s.h_canon.resize(s.n_geom);
+    for (uint32_t g = 0; g < s.n_geom; ++g) s.h_canon[g] = g;
+    if (separate) return;
+    // FNV-1a over the triangle count and the corner positions' bits, then an
+    // exact comparison with the earlier geometries of the same hash
+    auto corner = [&](uint32_t g, size_t t, int k) -> const float* {
+        return verts[s.h_voff[g] + idx[3 * t + k]].pos;
+    };
+    std::unordered_map<uint64_t, std::vector<uint32_t>> seen;
+    for (uint32_t g = 0; g < s.n_geom; ++g) {
+        const size_t t0 = s.h_toff[g], nt = s.h_toff[g + 1] - t0;
+        if (nt == 0) continue;
+        uint64_t h = 1469598103934665603ull ^ nt;
+        for (size_t t = 0; t < nt; ++t)
+            for (int k = 0; k < 3; ++k) {
+                const uint8_t* b = (const uint8_t*)corner(g, t0 + t, k);
+                for (int i = 0; i < 12; ++i) h = (h ^ b[i]) * 1099511628211ull;
+            }
+        auto& list = seen[h];
+        for (uint32_t c : list) {
+            if (s.h_toff[c + 1] - s.h_toff[c] != nt) continue;
+            bool same = true;
+            for (size_t t = 0; t < nt && same; ++t)
+                for (int k = 0; k < 3 && same; ++k)
+                    same = std::memcmp(corner(c, s.h_toff[c] + t, k), corner(g, t0 + t, k), 12) == 0;
+            if (same) {
+                s.h_canon[g] = c;
+                break;
+            }
+        }
+        if (s.h_canon[g] == g) list.push_back(g);
+    }
+}
+
+void unshare_blas(Scene& s, uint32_t g) {
+    if (s.h_canon.size() != s.n_geom || g >= s.n_geom) return;
+    const uint32_t c = s.h_canon[g];
+    if (c == g) {  // g led a group: the next member leads the rest
+        uint32_t m = UINT32_MAX;
+        for (uint32_t d = 0; d < s.n_geom; ++d)
+            if (d != g && s.h_canon[d] == g) {
+                if (m == UINT32_MAX) m = d;
+                s.h_canon[d] = m;
+            }
+    }
+    s.h_canon[g] = g;
+    upload_trav_roots(s);
 }
 
 // ---------------------------------------------------------------- refit
@@ -986,7 +1050,7 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     float4* ihi = s.d_ihi;
     uint32_t* iseg = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
     if (n)
-        k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, s.d_gflags, s.d_blas_root4,
+        k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, s.d_gflags, s.d_trav_root4,
                                                   s.d_blas4, s.n_blas4, glo, ghi,
                                                   multi ? s.d_inst_env : nullptr, s.d_inst, ilo, ihi, iseg);
     const uint32_t n_seg = multi ? s.n_env : 1u;

@@ -139,7 +139,13 @@ struct Scene {
     size_t vstage_cap = 0;
     hipEvent_t vstage_ev = nullptr;
     bool vstage_pending = false;
-    std::vector<int> h_blas_root4;
+    std::vector<int> h_blas_root4;           // each geometry's own BVH4 root
+    // Geometry whose BVH the traversal uses for geometry g: g itself, or the
+    // first geometry with the same triangles (the same vertex positions at
+    // every corner, bit for bit), whose BVH is then a BVH of g's triangles
+    // with the same local primitive ids.  Shading still reads g's own data.
+    std::vector<uint32_t> h_canon;
+    int* d_trav_root4 = nullptr;             // BVH4 root of h_canon[g], referenced by instance records
     uint32_t n_blas4 = 0;
     Node4* d_tlas4 = nullptr;
     uint32_t n_tlas4 = 0, tlas4_cap = 0;    // n_tlas4: bound for traversal index checks (the capacity)
@@ -201,10 +207,19 @@ uint32_t collapse_bvh4(const Node* d_b2, uint32_t n_b2, const std::vector<int>& 
 // and BVH2 node offsets node_off (known by construction, no device read).
 void env_roots(const std::vector<uint32_t>& env_off, const std::vector<uint32_t>& node_off, std::vector<int>& roots);
 void build_blas(Scene& s);
+void upload_trav_roots(Scene& s);
 // Refit geometry g's BLAS (BVH2 + BVH4 + triangle slots) after its
 // vertices changed in place; the TLAS must be rebuilt afterwards.
 void refit_blas(Scene& s, uint32_t g);
 void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n);
+// Geometries with the same triangles (exact positions of every corner in
+// order) as an earlier one traverse its BVH (Scene::h_canon); separate: each
+// keeps its own.
+void share_blas(Scene& s, const kf_vertex* verts, const uint32_t* idx, bool separate);
+// Geometry g's vertices changed: it traverses its own BVH again, and a group
+// it led moves to its next member; d_trav_root4 follows (the TLAS must be
+// rebuilt afterwards).
+void unshare_blas(Scene& s, uint32_t g);
 
 // ------------------------------------------------------------------ renderer
 struct Renderer {

@@ -1,0 +1,82 @@
+"""Geometries with the same triangles traverse one BVH (marked gpu).
+
+A geometry whose vertex positions at every triangle corner equal an earlier
+geometry's, bit for bit, traverses that geometry's BVH (its local primitive
+ids are the same); shading still reads its own materials, normals and UVs.
+The icosphere field gives every instance its own geometry (materials belong
+to geometries), so all of them share one BVH.  Frames must equal, bit for
+bit, the frames of the same scene with a BVH per geometry
+(kf_scene_desc.separate_blas), with the same ray and record counts; a vertex
+update of the geometry whose BVH the others traverse, and of another
+member, must give the frame of a scene created with the new vertices, and
+closest hits equal to the oracle's.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+import oracle
+from kuafu_amd import scenes
+from kuafu_amd.api import GpuScene, Renderer
+from test_gpu_envs import same
+from test_gpu_parity import assert_hits_equal, rays_toward
+
+pytestmark = pytest.mark.gpu
+
+
+def _frame(data, w, seed=3):
+    gs = GpuScene(data)
+    rd = Renderer(w.width, w.height)
+    pc = w.pc
+    pc.frame_count = 0
+    st = rd.render(gs, w.cam, pc, seed=seed, keep_samples=True, count_traversal=True)
+    return st, rd.read_samples(pc.spp), [rd.read(k) for k in range(3)]
+
+
+def test_shared_blas_same_frames(gpu):
+    w = scenes.config2(width=96, height=54, spp=2, path_depth=4, n_inst=40, subdiv=2)
+    w.data.separate_blas = True
+    st0, smp0, tg0 = _frame(w.data, w)
+    w.data.separate_blas = False
+    st1, smp1, tg1 = _frame(w.data, w)
+    print("records per frame: nodes %d / %d, instances %d / %d, triangles %d / %d"
+          % (st0.node_visits, st1.node_visits, st0.instance_visits, st1.instance_visits, st0.tri_tests,
+             st1.tri_tests))
+    assert same(smp1, smp0)
+    for k in range(3):
+        assert same(tg1[k], tg0[k]), "target %d" % k
+    assert (st1.primary_rays, st1.extension_rays, st1.shadow_rays) == \
+        (st0.primary_rays, st0.extension_rays, st0.shadow_rays)
+    # one BVH built from the same triangles: the same records are visited
+    assert (st1.node_visits, st1.instance_visits, st1.tri_tests) == \
+        (st0.node_visits, st0.instance_visits, st0.tri_tests)
+
+
+def test_shared_blas_update_of_leader(gpu):
+    """Geometry 0 (whose BVH the other 11 traverse) and geometry 5 are
+    deformed in place; the rest keep sharing geometry 1's BVH from then on."""
+    w = scenes.config2(width=48, height=27, spp=2, path_depth=4, n_inst=12, subdiv=2)
+    gs = GpuScene(w.data)
+    rd = Renderer(w.width, w.height)
+    pc = w.pc
+    pc.frame_count = 0
+    rd.render(gs, w.cam, pc, seed=3)
+    moved = copy.deepcopy(w.data)
+    rng = np.random.default_rng(23)
+    for g in (0, 5):
+        v, t, mi, fl = moved.geoms[g]
+        v = v.copy()
+        p = v["pos"].astype(np.float64) * rng.uniform(0.6, 1.8, 3)
+        v["pos"] = p.astype(np.float32)
+        moved.geoms[g] = (v, t, mi, fl)
+        gs.update_geometry(g, v)
+    rays = rays_toward(np.random.default_rng(5), 4000, np.zeros(3), 10.0, 25.0)
+    assert_hits_equal(gs.trace_closest(rays), oracle.Scene(moved).trace_closest(rays))
+    rd.render(gs, w.cam, pc, seed=11)
+    a = rd.read(0).copy()
+    for separate in (False, True):
+        moved.separate_blas = separate
+        rd2 = Renderer(w.width, w.height)
+        rd2.render(GpuScene(moved), w.cam, pc, seed=11)
+        assert same(a, rd2.read(0)), "fresh scene, separate_blas=%s" % separate
