@@ -365,7 +365,7 @@ def main():
                     help="only the 128-environment workload (frames = --steps), e.g. for profiling")
     ap.add_argument("--denoised-only", action="store_true",
                     help="only 1-spp frames + denoise of the 1080p scene (frames = --steps), e.g. for profiling")
-    ap.add_argument("--separate-blas", action="store_true",
+    ap.add_argument("--separate-geometry", action="store_true",
                     help="every geometry traverses its own BVH, also where geometries have the same triangles "
                          "(comparison)")
     ap.add_argument("--full-tlas-camera", action="store_true",
@@ -389,7 +389,7 @@ def main():
                                               full_tlas_camera=args.full_tlas_camera)))
         return
     w = scenes.config2(width=args.width, height=args.height)
-    w.data.separate_blas = args.separate_blas
+    w.data.separate_geometry = args.separate_geometry
     print("bench: building scene (%d triangles)" % w.data.n_triangles(), file=sys.stderr, flush=True)
     gs = GpuScene(w.data)
     info = gs.bvh_info()

@@ -185,7 +185,7 @@ typedef struct kf_scene_desc {
      * geometry's BVH (no counterpart in the reference, which builds one BLAS
      * per geometry; results are the same, the traversal's working set is
      * smaller); 1: every geometry traverses its own BVH. */
-    uint32_t separate_blas;
+    uint32_t separate_geometry;
 } kf_scene_desc;
 
 /* A ray of a fixed batch: origin, tmin, direction, tmax (32 B). */

@@ -75,7 +75,7 @@ class SceneDesc(C.Structure):
                 ("num_textures", C.c_uint32), ("textures", C.c_void_p), ("dir_light", C.c_void_p),
                 ("point_lights", C.c_void_p), ("active_lights", C.c_void_p), ("env_kind", C.c_uint32),
                 ("env_faces", C.c_void_p), ("env_gradient", C.c_float * 8), ("num_envs", C.c_uint32),
-                ("env_instance_offset", C.c_void_p), ("separate_blas", C.c_uint32)]
+                ("env_instance_offset", C.c_void_p), ("separate_geometry", C.c_uint32)]
 
 
 class RenderStats(C.Structure):

@@ -120,7 +120,7 @@ class SceneData:
         self.env_gradient = np.zeros(8, np.float32)
         self.env_faces = None
         self.env_offsets = None  # batched environments: instance offsets, n_env + 1
-        self.separate_blas = False  # True: no BVH shared between geometries with the same triangles
+        self.separate_geometry = False  # True: no BVH shared between geometries with the same triangles
         self._n_point = 0
         self._n_active = 0
 
@@ -252,7 +252,7 @@ class SceneData:
             keep.append(eo)
             d.num_envs = len(eo) - 1
             d.env_instance_offset = ptr(eo)
-        d.separate_blas = int(bool(self.separate_blas))
+        d.separate_geometry = int(bool(self.separate_geometry))
         d._keep = keep
         return d
 

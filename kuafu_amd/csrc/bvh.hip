@@ -795,6 +795,7 @@ void build_blas(Scene& s) {
     dfree(err);
     if (h_err) throw Error("BLAS build: malformed BVH2");
     upload_trav_roots(s);
+    upload_mesh_sources(s);
 }
 
 // d_trav_root4[g] = the BVH4 root of the geometry whose BVH g traverses.
@@ -809,54 +810,106 @@ void upload_trav_roots(Scene& s) {
     KF_CHECK(hipMemcpy(s.d_trav_root4, roots.data(), roots.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
-void share_blas(Scene& s, const kf_vertex* verts, const uint32_t* idx, bool separate) {
-    s.h_canon.resize(s.n_geom);
-    for (uint32_t g = 0; g < s.n_geom; ++g) s.h_canon[g] = g;
-    if (separate) return;
-    // FNV-1a over the triangle count and the corner positions' bits, then an
-    // exact comparison with the earlier geometries of the same hash
-    auto corner = [&](uint32_t g, size_t t, int k) -> const float* {
-        return verts[s.h_voff[g] + idx[3 * t + k]].pos;
-    };
-    std::unordered_map<uint64_t, std::vector<uint32_t>> seen;
-    for (uint32_t g = 0; g < s.n_geom; ++g) {
-        const size_t t0 = s.h_toff[g], nt = s.h_toff[g + 1] - t0;
-        if (nt == 0) continue;
-        uint64_t h = 1469598103934665603ull ^ nt;
-        for (size_t t = 0; t < nt; ++t)
-            for (int k = 0; k < 3; ++k) {
-                const uint8_t* b = (const uint8_t*)corner(g, t0 + t, k);
-                for (int i = 0; i < 12; ++i) h = (h ^ b[i]) * 1099511628211ull;
-            }
-        auto& list = seen[h];
-        for (uint32_t c : list) {
-            if (s.h_toff[c + 1] - s.h_toff[c] != nt) continue;
-            bool same = true;
-            for (size_t t = 0; t < nt && same; ++t)
-                for (int k = 0; k < 3 && same; ++k)
-                    same = std::memcmp(corner(c, s.h_toff[c] + t, k), corner(g, t0 + t, k), 12) == 0;
-            if (same) {
-                s.h_canon[g] = c;
-                break;
-            }
-        }
-        if (s.h_canon[g] == g) list.push_back(g);
+// d_mesh[g] = (voff, toff) of the geometry whose arrays g's shading reads.
+void upload_mesh_sources(Scene& s) {
+    if (s.h_mesh.size() != s.n_geom) {
+        s.h_mesh.resize(s.n_geom);
+        for (uint32_t g = 0; g < s.n_geom; ++g) s.h_mesh[g] = g;
     }
+    std::vector<uint2> src(std::max<uint32_t>(s.n_geom, 1), make_uint2(0u, 0u));
+    for (uint32_t g = 0; g < s.n_geom; ++g) src[g] = make_uint2(s.h_voff[s.h_mesh[g]], s.h_toff[s.h_mesh[g]]);
+    if (!s.d_mesh) s.d_mesh = dalloc<uint2>(std::max<uint32_t>(s.n_geom, 1));
+    KF_CHECK(hipMemcpy(s.d_mesh, src.data(), src.size() * sizeof(uint2), hipMemcpyHostToDevice));
 }
 
-void unshare_blas(Scene& s, uint32_t g) {
-    if (s.h_canon.size() != s.n_geom || g >= s.n_geom) return;
-    const uint32_t c = s.h_canon[g];
-    if (c == g) {  // g led a group: the next member leads the rest
+namespace {
+// Groups of geometries equal under eq, each led by its first member, found
+// through a hash of the bytes key(g) feeds: lead[g] = the leader of g's group.
+template <class Key, class Eq>
+void group_equal(uint32_t n, std::vector<uint32_t>& lead, Key key, Eq eq) {
+    lead.resize(n);
+    std::unordered_map<uint64_t, std::vector<uint32_t>> seen;
+    for (uint32_t g = 0; g < n; ++g) {
+        lead[g] = g;
+        uint64_t h = 1469598103934665603ull;  // FNV-1a
+        key(g, [&](const void* p, size_t bytes) {
+            const uint8_t* b = (const uint8_t*)p;
+            for (size_t i = 0; i < bytes; ++i) h = (h ^ b[i]) * 1099511628211ull;
+        });
+        auto& list = seen[h];
+        for (uint32_t c : list)
+            if (eq(c, g)) {
+                lead[g] = c;
+                break;
+            }
+        if (lead[g] == g) list.push_back(g);
+    }
+}
+// g leaves its group: the next member leads the rest.
+void leave_group(std::vector<uint32_t>& lead, uint32_t g) {
+    if (lead[g] == g) {
         uint32_t m = UINT32_MAX;
-        for (uint32_t d = 0; d < s.n_geom; ++d)
-            if (d != g && s.h_canon[d] == g) {
+        for (uint32_t d = 0; d < lead.size(); ++d)
+            if (d != g && lead[d] == g) {
                 if (m == UINT32_MAX) m = d;
-                s.h_canon[d] = m;
+                lead[d] = m;
             }
     }
-    s.h_canon[g] = g;
+    lead[g] = g;
+}
+}  // namespace
+
+void share_geometry(Scene& s, const kf_vertex* verts, const uint32_t* idx, bool separate) {
+    const uint32_t n = s.n_geom;
+    s.h_canon.resize(n);
+    s.h_mesh.resize(n);
+    for (uint32_t g = 0; g < n; ++g) s.h_canon[g] = s.h_mesh[g] = g;
+    if (separate) return;
+    auto ntri = [&](uint32_t g) { return (size_t)(s.h_toff[g + 1] - s.h_toff[g]); };
+    auto nvert = [&](uint32_t g) { return (size_t)(s.h_voff[g + 1] - s.h_voff[g]); };
+    auto corner = [&](uint32_t g, size_t t, int k) -> const float* {
+        return verts[s.h_voff[g] + idx[3 * (s.h_toff[g] + t) + k]].pos;
+    };
+    // the BVH: the positions of every triangle corner, in order
+    group_equal(
+        n, s.h_canon,
+        [&](uint32_t g, auto feed) {
+            const size_t nt = ntri(g);
+            feed(&nt, sizeof(nt));
+            for (size_t t = 0; t < nt; ++t)
+                for (int k = 0; k < 3; ++k) feed(corner(g, t, k), 12);
+        },
+        [&](uint32_t c, uint32_t g) {
+            if (ntri(c) != ntri(g)) return false;
+            for (size_t t = 0; t < ntri(g); ++t)
+                for (int k = 0; k < 3; ++k)
+                    if (std::memcmp(corner(c, t, k), corner(g, t, k), 12)) return false;
+            return true;
+        });
+    // shading: the vertex records and the indices
+    group_equal(
+        n, s.h_mesh,
+        [&](uint32_t g, auto feed) {
+            const size_t nv = nvert(g), nt = ntri(g);
+            feed(&nv, sizeof(nv));
+            feed(&nt, sizeof(nt));
+            if (nv) feed(verts + s.h_voff[g], nv * sizeof(kf_vertex));
+            if (nt) feed(idx + 3 * (size_t)s.h_toff[g], 3 * nt * sizeof(uint32_t));
+        },
+        [&](uint32_t c, uint32_t g) {
+            return nvert(c) == nvert(g) && ntri(c) == ntri(g) &&
+                   !std::memcmp(verts + s.h_voff[c], verts + s.h_voff[g], nvert(g) * sizeof(kf_vertex)) &&
+                   !std::memcmp(idx + 3 * (size_t)s.h_toff[c], idx + 3 * (size_t)s.h_toff[g],
+                                3 * ntri(g) * sizeof(uint32_t));
+        });
+}
+
+void unshare_geometry(Scene& s, uint32_t g) {
+    if (g >= s.n_geom) return;
+    if (s.h_canon.size() == s.n_geom) leave_group(s.h_canon, g);
+    if (s.h_mesh.size() == s.n_geom) leave_group(s.h_mesh, g);
     upload_trav_roots(s);
+    upload_mesh_sources(s);
 }
 
 // ---------------------------------------------------------------- refit

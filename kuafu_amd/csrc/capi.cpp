@@ -93,6 +93,7 @@ SceneView Scene::view() const {
     v.matidx = d_matidx;
     v.voff = d_voff;
     v.toff = d_toff;
+    v.mesh = d_mesh;
     v.mats = d_mats;
     v.n_mats = n_mat;
     v.tex_data = d_tex;
@@ -109,7 +110,7 @@ Scene::~Scene() {
     if (g_stream) (void)hipStreamSynchronize(g_stream);
     release(d_verts); release(d_idx); release(d_matidx); release(d_voff); release(d_toff); release(d_tri_geom);
     release(d_mats); release(d_blas); release(d_tri); release(d_blas_root); release(d_tlas); release(d_inst);
-    release(d_blas4); release(d_blas_root4); release(d_trav_root4); release(d_tlas4); release(d_gflags); release(d_tlas_aux);
+    release(d_blas4); release(d_blas_root4); release(d_trav_root4); release(d_mesh); release(d_tlas4); release(d_gflags); release(d_tlas_aux);
     release(d_tlas_err);
     release(d_env_off); release(d_env_node_off); release(d_inst_env); release(d_env_root); release(d_env_cnt);
     release(d_node_off); release(d_blas_cnt);
@@ -433,7 +434,7 @@ int kf_scene_create(const kf_scene_desc* d, kf_scene** out) {
         KF_CHECK(hipEventCreate(&e0));
         KF_CHECK(hipEventCreate(&e1));
         KF_CHECK(hipEventRecord(e0, st));
-        share_blas(*s, d->vertices, d->indices, d->separate_blas != 0);
+        share_geometry(*s, d->vertices, d->indices, d->separate_geometry != 0);
         build_blas(*s);
         build_tlas(*s, d->instances, d->num_instances);
         KF_CHECK(hipEventRecord(e1, st));
@@ -491,7 +492,7 @@ int kf_scene_update_geometry(kf_scene* s, uint32_t g, const kf_vertex* vertices,
             s->vstage_pending = true;
         }
         refit_blas(*s, g);
-        unshare_blas(*s, g);
+        unshare_geometry(*s, g);
         // instance boxes follow the refit BLAS
         std::vector<kf_instance> inst = s->h_inst;
         build_tlas(*s, inst.data(), (uint32_t)inst.size());

@@ -103,6 +103,7 @@ struct SceneView {
     const uint32_t* matidx;   // per triangle
     const uint32_t* voff;     // per geometry
     const uint32_t* toff;     // per geometry
+    const uint2* mesh;        // per geometry: (voff, toff) of the arrays its vertices and indices are read from
     const kf_material* mats;
     uint32_t n_mats;
     const uint8_t* tex_data;  // all RGBA8 textures
@@ -146,6 +147,11 @@ struct Scene {
     // with the same local primitive ids.  Shading still reads g's own data.
     std::vector<uint32_t> h_canon;
     int* d_trav_root4 = nullptr;             // BVH4 root of h_canon[g], referenced by instance records
+    // Geometry whose vertex and index arrays shading reads for geometry g:
+    // g itself, or the first geometry with the same vertex records and
+    // indices, bit for bit (materials always come from g's own matIndex).
+    std::vector<uint32_t> h_mesh;
+    uint2* d_mesh = nullptr;                 // (voff, toff) of h_mesh[g]
     uint32_t n_blas4 = 0;
     Node4* d_tlas4 = nullptr;
     uint32_t n_tlas4 = 0, tlas4_cap = 0;    // n_tlas4: bound for traversal index checks (the capacity)
@@ -208,18 +214,20 @@ uint32_t collapse_bvh4(const Node* d_b2, uint32_t n_b2, const std::vector<int>& 
 void env_roots(const std::vector<uint32_t>& env_off, const std::vector<uint32_t>& node_off, std::vector<int>& roots);
 void build_blas(Scene& s);
 void upload_trav_roots(Scene& s);
+void upload_mesh_sources(Scene& s);
 // Refit geometry g's BLAS (BVH2 + BVH4 + triangle slots) after its
 // vertices changed in place; the TLAS must be rebuilt afterwards.
 void refit_blas(Scene& s, uint32_t g);
 void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n);
 // Geometries with the same triangles (exact positions of every corner in
-// order) as an earlier one traverse its BVH (Scene::h_canon); separate: each
-// keeps its own.
-void share_blas(Scene& s, const kf_vertex* verts, const uint32_t* idx, bool separate);
-// Geometry g's vertices changed: it traverses its own BVH again, and a group
-// it led moves to its next member; d_trav_root4 follows (the TLAS must be
-// rebuilt afterwards).
-void unshare_blas(Scene& s, uint32_t g);
+// order) as an earlier one traverse its BVH (Scene::h_canon), and those with
+// the same vertex records and indices shade from its arrays (Scene::h_mesh);
+// separate: each keeps its own.
+void share_geometry(Scene& s, const kf_vertex* verts, const uint32_t* idx, bool separate);
+// Geometry g's vertices changed: it traverses its own BVH and reads its own
+// arrays again, and groups it led move to their next member; d_trav_root4
+// and d_mesh follow (the TLAS must be rebuilt afterwards).
+void unshare_geometry(Scene& s, uint32_t g);
 
 // ------------------------------------------------------------------ renderer
 struct Renderer {

@@ -48,21 +48,22 @@ KF_DEV v3 rcp3(v3 d) { return mk(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rc
 // rounded, Sx = d[kx] * Sz, Sy = d[ky] * Sz.  In two parts so that a caller
 // can share the division: precompute_axes returns d[kz], precompute_scale
 // takes its correctly rounded reciprocal.  The caller sets invd.
+// kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2), kx = kz + 1
+// and ky = kz + 2 (mod 3), swapped when d[kz] < 0; the flags are written as
+// logic on the three comparisons (lane masks, no integer axes).
 KF_DEV float precompute_axes(v3 o, v3 d, Pre& p) {
     p.o = o;
-    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
-    int kx = kz == 2 ? 0 : kz + 1;
-    int ky = kx == 2 ? 0 : kx + 1;
-    const float dz = comp(d, kz);
-    const bool swap = dz < 0.0f;
-    const int pkx = swap ? ky : kx, pky = swap ? kx : ky;
-    p.x0 = pkx == 0;
-    p.x1 = pkx == 1;
-    p.y0 = pky == 0;
-    p.y1 = pky == 1;
-    p.z0 = kz == 0;
-    p.z1 = kz == 1;
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const bool xy = ax >= ay, xz = ax >= az, yz = ay >= az;
+    const bool z0 = xy && xz, z1 = !xy && yz, z2 = !z0 && !z1;
+    const float dz = z0 ? d.x : (z1 ? d.y : d.z);
+    const bool sw = dz < 0.0f;
+    p.z0 = z0;
+    p.z1 = z1;
+    p.x0 = sw ? z1 : z2;  // the (swapped) x axis is 0
+    p.x1 = sw ? z2 : z0;  // ... is 1
+    p.y0 = sw ? z2 : z1;  // the (swapped) y axis is 0
+    p.y1 = sw ? z0 : z2;  // ... is 1
     return dz;
 }
 KF_DEV void precompute_scale(v3 d, float rcp_dz, Pre& p) {

@@ -1,15 +1,16 @@
-"""Geometries with the same triangles traverse one BVH (marked gpu).
+"""Geometries with the same data share it (marked gpu).
 
 A geometry whose vertex positions at every triangle corner equal an earlier
 geometry's, bit for bit, traverses that geometry's BVH (its local primitive
-ids are the same); shading still reads its own materials, normals and UVs.
-The icosphere field gives every instance its own geometry (materials belong
-to geometries), so all of them share one BVH.  Frames must equal, bit for
-bit, the frames of the same scene with a BVH per geometry
-(kf_scene_desc.separate_blas), with the same ray and record counts; a vertex
-update of the geometry whose BVH the others traverse, and of another
-member, must give the frame of a scene created with the new vertices, and
-closest hits equal to the oracle's.
+ids are the same), and one whose vertex records and indices are equal reads
+them from that geometry's arrays when it is shaded; its materials always
+come from its own matIndex.  The icosphere field gives every instance its own
+geometry (materials belong to geometries), so all of them share one BVH and
+one vertex array.  Frames must equal, bit for bit, the frames of the same
+scene with nothing shared (kf_scene_desc.separate_geometry), with the same
+ray and record counts; a vertex update of the geometry the others read, and
+of another member, must give the frame of a scene created with the new
+vertices, and closest hits equal to the oracle's.
 """
 import copy
 
@@ -36,9 +37,9 @@ def _frame(data, w, seed=3):
 
 def test_shared_blas_same_frames(gpu):
     w = scenes.config2(width=96, height=54, spp=2, path_depth=4, n_inst=40, subdiv=2)
-    w.data.separate_blas = True
+    w.data.separate_geometry = True
     st0, smp0, tg0 = _frame(w.data, w)
-    w.data.separate_blas = False
+    w.data.separate_geometry = False
     st1, smp1, tg1 = _frame(w.data, w)
     print("records per frame: nodes %d / %d, instances %d / %d, triangles %d / %d"
           % (st0.node_visits, st1.node_visits, st0.instance_visits, st1.instance_visits, st0.tri_tests,
@@ -76,7 +77,7 @@ def test_shared_blas_update_of_leader(gpu):
     rd.render(gs, w.cam, pc, seed=11)
     a = rd.read(0).copy()
     for separate in (False, True):
-        moved.separate_blas = separate
+        moved.separate_geometry = separate
         rd2 = Renderer(w.width, w.height)
         rd2.render(GpuScene(moved), w.cam, pc, seed=11)
-        assert same(a, rd2.read(0)), "fresh scene, separate_blas=%s" % separate
+        assert same(a, rd2.read(0)), "fresh scene, separate_geometry=%s" % separate
