@@ -402,11 +402,14 @@ __global__ void k_blas_box(uint32_t n_geom, const int* __restrict__ root, const 
 
 // Instance records + padded world AABBs (corners transformed in double).
 // root: BVH4 root ref of each geometry (node index, ~triangle slot for a
-// BLAS of one triangle, kEmptyRef); blas4: the BVH4 nodes it indexes.
+// BLAS of one triangle, kEmptyRef); blas4: the BVH4 nodes it indexes; mesh:
+// (voff, toff) of the arrays each geometry's shading reads; toff: each
+// geometry's own triangle offset.
 __global__ void k_instances(uint32_t n, const kf_instance* __restrict__ src, const float* __restrict__ w2o,
                             uint32_t n_geom, const uint32_t* __restrict__ gflags, const int* __restrict__ root,
                             const Node4* __restrict__ blas4, uint32_t n_blas4,
                             const float4* __restrict__ glo, const float4* __restrict__ ghi,
+                            const uint2* __restrict__ mesh, const uint32_t* __restrict__ toff,
                             const uint32_t* __restrict__ inst_env, Inst* __restrict__ out,
                             float4* __restrict__ ilo, float4* __restrict__ ihi, uint32_t* __restrict__ iseg) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -420,7 +423,7 @@ __global__ void k_instances(uint32_t n, const kf_instance* __restrict__ src, con
     bool valid = g < n_geom && !(gflags[g] & KF_GEOM_HIDDEN) && r4 != kEmptyRef && (r4 < 0 || (uint32_t)r4 < n_blas4);
     int flags = (valid ? kInstValid : 0) | ((g < n_geom && (gflags[g] & KF_GEOM_OPAQUE)) ? kInstOpaque : 0);
     rec.info = make_int4(valid ? r4 : kEmptyRef, (int)g, flags, 0);
-    rec.pad = make_int4(0, 0, 0, 0);
+    rec.shade = g < n_geom ? make_int4((int)mesh[g].x, (int)mesh[g].y, (int)toff[g], 0) : make_int4(0, 0, 0, 0);
     // the BLAS root node, as the traversal's instance step tests it
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     rec.lox = rec.hix = rec.loy = rec.hiy = rec.loz = rec.hiz = z;
@@ -1104,7 +1107,7 @@ void build_tlas(Scene& s, const kf_instance* h_inst, uint32_t n) {
     uint32_t* iseg = dalloc<uint32_t>(std::max<uint32_t>(n, 1));
     if (n)
         k_instances<<<blocks(n, TB), TB, 0, st>>>(n, s.d_inst_src, d_w2o, s.n_geom, s.d_gflags, s.d_trav_root4,
-                                                  s.d_blas4, s.n_blas4, glo, ghi,
+                                                  s.d_blas4, s.n_blas4, glo, ghi, s.d_mesh, s.d_toff,
                                                   multi ? s.d_inst_env : nullptr, s.d_inst, ilo, ihi, iseg);
     const uint32_t n_seg = multi ? s.n_env : 1u;
     const uint32_t* seg_off = multi ? s.d_env_off : s.d_tlas_aux;

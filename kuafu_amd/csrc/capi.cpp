@@ -93,7 +93,6 @@ SceneView Scene::view() const {
     v.matidx = d_matidx;
     v.voff = d_voff;
     v.toff = d_toff;
-    v.mesh = d_mesh;
     v.mats = d_mats;
     v.n_mats = n_mat;
     v.tex_data = d_tex;

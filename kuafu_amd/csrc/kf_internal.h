@@ -72,13 +72,15 @@ struct alignas(16) Node4 {
 // computed; a BLAS of one triangle gets a node with that triangle as its
 // only child, an invalid instance a node without children.  Lines 7-9 are
 // the world->object rows, line 10 (root_ref, geometry, flags, 0) with flags
-// bit0 = opaque, bit1 = valid; line 11 is padding.
+// bit0 = opaque, bit1 = valid; line 11 (vertex offset, triangle offset) of
+// the arrays the geometry's shading reads and the geometry's own triangle
+// offset (its material indices), 0.
 struct alignas(16) Inst {
     float4 lox, hix, loy, hiy, loz, hiz;
     int4 ref;
     float4 w2o[3];
     int4 info;
-    int4 pad;
+    int4 shade;
 };
 static_assert(sizeof(Inst) == 192, "instance record layout");
 
@@ -103,7 +105,6 @@ struct SceneView {
     const uint32_t* matidx;   // per triangle
     const uint32_t* voff;     // per geometry
     const uint32_t* toff;     // per geometry
-    const uint2* mesh;        // per geometry: (voff, toff) of the arrays its vertices and indices are read from
     const kf_material* mats;
     uint32_t n_mats;
     const uint8_t* tex_data;  // all RGBA8 textures

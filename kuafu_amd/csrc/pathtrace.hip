@@ -639,11 +639,12 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
         uint32_t prim = hp & 0x7fffffffu;
         bool inside = (hp >> 31) != 0;
         Inst I = S.inst[inst];
-        uint32_t g = (uint32_t)I.info.y;
-        const size_t tri = (size_t)S.toff[g] + prim;  // its material index
-        const uint2 ms = S.mesh[g];                     // where its vertices and indices are read
-        const size_t mtri = (size_t)ms.y + prim;
-        const uint32_t vb = ms.x;
+        // the instance's geometry: where its vertices and indices are read,
+        // and its own triangle offset (material indices), on the record's
+        // last line, fetched with its world->object rows
+        const size_t tri = (size_t)(uint32_t)I.shade.z + prim;
+        const size_t mtri = (size_t)(uint32_t)I.shade.y + prim;
+        const uint32_t vb = (uint32_t)I.shade.x;
         uint32_t i0 = S.idx[3 * mtri + 0], i1 = S.idx[3 * mtri + 1], i2 = S.idx[3 * mtri + 2];
         const float4* V0 = S.verts + 3 * (size_t)(vb + i0);
         const float4* V1 = S.verts + 3 * (size_t)(vb + i1);
