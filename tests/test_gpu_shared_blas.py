@@ -81,3 +81,39 @@ def test_shared_blas_update_of_leader(gpu):
         rd2 = Renderer(w.width, w.height)
         rd2.render(GpuScene(moved), w.cam, pc, seed=11)
         assert same(a, rd2.read(0)), "fresh scene, separate_geometry=%s" % separate
+
+
+def test_shared_geometry_variants(gpu):
+    """The same positions with other normals (the BVH is shared, the vertex
+    arrays are not), an alpha-tested copy (its own material decides the alpha
+    test) and a one-triangle geometry twice (a BVH whose root is a triangle
+    slot): frames equal to the unshared scene's and close to the oracle's."""
+    from kuafu_amd.api import camera_ubo, push_constants, vertices
+    from test_gpu_parity import assert_frame_parity, render_both
+    data = scenes.tiny_scene(7, n_spheres=3)
+    sv, si = scenes.uv_sphere(12, 8, 0.5)
+    bent = sv.copy()
+    n = bent["normal"].astype(np.float64) + np.array([0.4, 0.0, 0.0])
+    bent["normal"] = (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
+    g_bent = data.add_geometry(bent, si, scenes.material(diffuse=(0.2, 0.8, 0.3)))
+    g_alpha = data.add_geometry(sv, si, scenes.material(diffuse=(0.9, 0.9, 0.2), alpha=0.4), opaque=False)
+    tri = vertices(np.array([(-0.6, 0.0, 0.0), (0.6, 0.0, 0.0), (0.0, 0.9, 0.0)], np.float32),
+                   normal=np.tile(np.float32([0, 0, 1]), (3, 1)))
+    g_t1 = data.add_geometry(tri, np.array([[0, 1, 2]], np.uint32), scenes.material(diffuse=(0.9, 0.2, 0.2)))
+    g_t2 = data.add_geometry(tri, np.array([[0, 1, 2]], np.uint32), scenes.material(diffuse=(0.2, 0.2, 0.9)))
+    for g, pos in ((g_bent, (-0.8, 0.6, 0.9)), (g_alpha, (0.7, 0.6, 1.3)), (g_t1, (-1.2, 0.2, 1.8)),
+                   (g_t2, (1.1, 0.2, 1.9))):
+        data.add_instance(g, scenes.mat_from_rows(np.eye(3), pos))
+    cam = camera_ubo((0.0, 2.0, 5.0), (0, -0.3, -1.0), (0, 1, 0), 48, 40, fx=50.0, fy=50.0)
+    w = scenes.Workload("variants", data, cam, push_constants(spp=4, path_depth=5), 48, 40)
+    data.separate_geometry = True
+    st0, smp0, tg0 = _frame(data, w)
+    data.separate_geometry = False
+    st1, smp1, tg1 = _frame(data, w)
+    assert same(smp1, smp0)
+    for k in range(3):
+        assert same(tg1[k], tg0[k]), "target %d" % k
+    assert (st1.node_visits, st1.instance_visits, st1.tri_tests) == \
+        (st0.node_visits, st0.instance_visits, st0.tri_tests)
+    (res,) = render_both(w)
+    assert_frame_parity(*res)
