@@ -985,8 +985,9 @@ __global__ __launch_bounds__(TB) void k_tail(SceneView S, Frame F, const DevLigh
                                              float4* __restrict__ em, uint32_t* __restrict__ seed,
                                              float4* __restrict__ nee_a, float4* __restrict__ nee_b,
                                              uint8_t* __restrict__ vis, float4* __restrict__ out,
-                                             unsigned long long* __restrict__ cnt) {
+                                             unsigned long long* __restrict__ cnt, uint32_t min_n) {
     const uint32_t n = (uint32_t)min(cnt[b0 * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
+    if (n <= min_n) return;  // k_tail2 takes these paths
     const uint32_t S0 = F.fv->S0;
     uint32_t n_shadow = 0, n_ext = 0, n_traced = 0, ovf = 0;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
@@ -1012,6 +1013,188 @@ __global__ __launch_bounds__(TB) void k_tail(SceneView S, Frame F, const DevLigh
     }
     wave_add(&cnt[kCntGlobal + 1], n_ext);
     wave_add(&cnt[kCntGlobal + 2], n_shadow);
+    wave_add(&cnt[kCntGlobal + 9], n_traced);
+    if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
+}
+
+// The deferred light loop of one path (rchit:457-460 order) with its shadow
+// rays written at the path's slots instead of appended to the shadow queue:
+// sh_o[slot] = (shading point, 1 if the slot has a shadow ray), sh_d[slot] =
+// (direction, length).  Returns whether any slot has one.
+KF_DEV bool nee_slots(const SceneView& S, const DevLights* __restrict__ LT, const NeeCtx& nc, uint32_t p, uint32_t K,
+                      float4* __restrict__ nee_a, float4* __restrict__ nee_b, float4* __restrict__ sh_o,
+                      float4* __restrict__ sh_d) {
+    bool any = false;
+    uint32_t unused_rs = 0u;
+    for (uint32_t j = 0; j < K; ++j) {
+        const size_t slot = (size_t)p * K + j;
+        bool emit = false;
+        v3 Ld = splat(0.0f);
+        float maxd = 0.0f;
+        if (nc.on && (int)j < LT->n_slots) {
+            const int kind = LT->kind[j], li = LT->index[j];
+            v3 lem;
+            const bool ok = light_eval<false>(S, LT, kind, li, nc.sh.worldPos, unused_rs, Ld, lem, maxd);
+            if (ok && dot(nc.sh.N, Ld) > 0.0f) {
+                v3 ca, cb;
+                float th;
+                int mode;
+                direct_split(nc.sh, Ld, lem, ca, cb, th, mode);
+                nee_a[slot] = f4(ca, th);
+                nee_b[slot] = f4(cb, __int_as_float(mode | (kind << 4)));
+                emit = true;
+            }
+        }
+        sh_o[slot] = f4(nc.sh.worldPos, emit ? 1.0f : 0.0f);
+        sh_d[slot] = f4(Ld, maxd);
+        any = any || emit;
+    }
+    return any;
+}
+
+// The last bounces, for frames whose shadow rays draw no random number before
+// they are traced (no soft light): two lanes per path, the first tracing the
+// closest-hit ray of bounce b while the second traces the shadow rays of
+// bounce b - 1 with the same traversal steps (Trav<MIXED>), as the wavefront
+// overlaps its two streams; then the first lane resolves bounce b - 1's
+// lights and updates the path, and shades bounce b (its shadow rays written
+// per slot).  The closest-hit ray of a path that Russian roulette then ends
+// is traced for nothing (counted apart, as in the wavefront).  Random
+// streams, light order and radiance are those of the wavefront and k_tail;
+// a path's chain of dependent traversals is about half as long, which pays
+// while few paths are left: the frame launches k_tail2 for at most max_n
+// paths (a quarter of its threads) and k_tail beyond (1080p, 16 spp: 33 k
+// paths at bounce 5; 1 spp: hundreds of thousands at bounce 3).
+__global__ __launch_bounds__(TB) void k_tail2(SceneView S, Frame F, const DevLights* __restrict__ LT, int b0,
+                                              const uint32_t* __restrict__ queue, float4* __restrict__ ray_o,
+                                              float4* __restrict__ ray_d, float4* __restrict__ thr,
+                                              float4* __restrict__ col, float4* __restrict__ hit,
+                                              uint32_t* __restrict__ hit_prim, float4* __restrict__ rw,
+                                              float4* __restrict__ em, uint32_t* __restrict__ seed,
+                                              float4* __restrict__ nee_a, float4* __restrict__ nee_b,
+                                              uint8_t* __restrict__ vis, float4* __restrict__ sh_o,
+                                              float4* __restrict__ sh_d, float4* __restrict__ out,
+                                              unsigned long long* __restrict__ cnt, uint32_t max_n) {
+    const uint32_t n = (uint32_t)min(cnt[b0 * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
+    if (n > max_n) return;  // k_tail takes these paths
+    const uint32_t S0 = F.fv->S0;
+    const uint32_t K = F.slots;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool second = (lane & 1u) != 0u;  // traces the shadow rays
+    const uint32_t npair = (gridDim.x * blockDim.x) >> 1;
+    uint32_t n_shadow = 0, n_ext = 0, n_traced = 0, n_spec = 0, ovf = 0, unused = 0;
+    const TravScene TS(S);
+    PrivStack stk;
+    Trav<false, false, true> T;
+    T.reset();
+    for (uint32_t q = (blockIdx.x * blockDim.x + threadIdx.x) >> 1; q < n; q += npair) {
+        const uint32_t p = queue[q];
+        const uint32_t root_env = env_root_item(S, (p % F.WH) / F.WHe);
+        int b = b0;
+        bool closest = true;   // a closest-hit ray of bounce b to trace
+        bool pending = false;  // shadow rays of bounce b - 1 to trace
+        for (;;) {
+            // traversal: the first lane's closest-hit query, the second
+            // lane's shadow queries one slot after another
+            uint32_t j = 0;
+            size_t slot = 0;
+            bool alive = false;
+            auto start_shadow = [&]() {
+                for (; j < K; ++j) {
+                    slot = (size_t)p * K + j;
+                    const float4 so = sh_o[slot];
+                    if (so.w == 0.0f) continue;
+                    const float4 sd = sh_d[slot];
+                    TRay r;
+                    r.o = xyz(so);
+                    r.d = xyz(sd);
+                    r.tmin = 0.001f;
+                    r.tmax = sd.w;
+                    r.root = root_env;
+                    r.alpha_seed = 0u;
+                    r.payload = 0u;
+                    T.start(true, r, r.root, false);
+                    T.any_q = true;
+                    ++n_shadow;
+                    if (!(r.tmin <= r.tmax) || r.root == kEmptyItem) {
+                        vis[slot] = 1;  // nothing can occlude it
+                        continue;
+                    }
+                    ++j;
+                    return true;
+                }
+                return false;
+            };
+            if (!second && closest) {
+                const float4 o = ray_o[p], d = ray_d[p];
+                TRay r;
+                r.o = xyz(o);
+                r.d = xyz(d);
+                r.tmin = 0.001f;
+                r.tmax = 10000.0f;
+                r.root = __float_as_uint(d.w);
+                // alpha coin seed as in k_extend
+                r.alpha_seed = S.any_alpha ? tea(tea((p % F.WH) % F.WHe, S0 + 1u + F.s0 + p / F.WH), (uint32_t)b) : 0u;
+                r.payload = 0u;
+                T.start(true, r, r.root, S.any_alpha != 0u);
+                T.any_q = false;
+                ++n_traced;
+                alive = r.root != kEmptyItem;
+                if (!alive) {
+                    const THit h = T.result();
+                    hit[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.inst));
+                    hit_prim[p] = h.prim | (h.back << 31);
+                }
+            }
+            if (second && pending) alive = start_shadow();
+            while (__ballot(alive) != 0ull) {
+                const bool more = T.template step<true>(TS, stk, ovf, alive);
+                if (alive && !more) {
+                    alive = false;
+                    if (!second) {
+                        const THit h = T.result();
+                        hit[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.inst));
+                        hit_prim[p] = h.prim | (h.back << 31);
+                    } else {
+                        vis[slot] = T.found ? 0 : 1;
+                        alive = start_shadow();
+                    }
+                }
+            }
+            // the second lane's visibility bytes before the first lane reads them
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            int flags = 0;  // bit 0: shade bounce b, bit 1: its ray continues, bit 2: it has shadow rays
+            if (!second) {
+                bool done = false;
+                if (b > b0) {  // bounce b - 1: lights in order, path update, Russian roulette
+                    if (connect_path<false>(F, b - 1, p, thr, col, rw, em, seed, nee_a, nee_b, vis, out)) {
+                        ++n_ext;
+                    } else {
+                        done = true;
+                        if (closest) ++n_spec;
+                    }
+                }
+                if (!done && closest) {
+                    NeeCtx nc;
+                    nc.on = false;
+                    const bool ext = shade_path<false>(S, F, LT, b, p, thr[p], false, ray_o, ray_d, hit, hit_prim, rw, em,
+                                                       seed, nee_a, nee_b, vis, nullptr, nullptr, nullptr, nullptr,
+                                                       nullptr, cnt, unused, ovf, nc);
+                    const bool sh = nee_slots(S, LT, nc, p, K, nee_a, nee_b, sh_o, sh_d);
+                    flags = 1 | (ext ? 2 : 0) | (sh ? 4 : 0);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            flags = __shfl(flags, (int)(lane & ~1u));
+            if (!(flags & 1)) break;
+            closest = (flags & 2) != 0;
+            pending = (flags & 4) != 0;
+            ++b;
+        }
+    }
+    wave_add(&cnt[kCntGlobal + 1], n_ext);
+    wave_add(&cnt[kCntGlobal + 2], n_shadow);
+    wave_add(&cnt[kCntGlobal + 7], n_spec);
     wave_add(&cnt[kCntGlobal + 9], n_traced);
     if (ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
 }
@@ -1394,9 +1577,17 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
         launch_extend(0, qa);
         for (uint32_t b = 0; b < nb; ++b) {
             if (b == b_tail) {
+                // the path count is on the device: k_tail2 (two lanes per
+                // path) runs for at most a quarter of the grid's threads,
+                // k_tail beyond, and the other launch returns at once
+                const uint32_t pair_max = inline_nee ? 0u : grid * TB / 4u;
+                if (!inline_nee)
+                    k_tail2<<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->col,
+                                                 r->hit, r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b,
+                                                 r->vis, r->sh_o, r->sh_d, d_out, r->counters, pair_max);
                 k_tail<<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->col, r->hit,
                                             r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b, r->vis, d_out,
-                                            r->counters);
+                                            r->counters, pair_max);
                 break;
             }
             if (inline_nee)

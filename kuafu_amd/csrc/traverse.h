@@ -269,8 +269,10 @@ KF_DEV uint32_t env_root_item(const SceneView& S, uint32_t e) {
 // ANY: shadow-ray query (opaque, stop at the first hit in [tmin, tmax]).
 // Closest-hit queries with use_alpha run the any-hit alpha test
 // (PathTrace.rahit:30-48) for non-opaque instances, with an order-independent
-// coin (alpha_coin).
-template <bool ANY, bool COUNT>
+// coin (alpha_coin).  MIXED (with ANY false): each lane's query is a shadow
+// query or a closest-hit one (any_q), so that lanes of one wave trace both
+// kinds with the same steps.
+template <bool ANY, bool COUNT, bool MIXED = false>
 struct Trav {
     v3 o, d;         // world-space ray
     Pre obj;         // object-space ray of the current instance (its axis
@@ -288,6 +290,7 @@ struct Trav {
     uint32_t spx;    // bits 7:0 the entries in the stack below the cached
                      // top, bits 31:8 the steps taken (malformed-tree guard)
     bool has_top, found, use_alpha;
+    bool any_q;      // MIXED: this lane's query is a shadow query
     uint32_t alpha_seed;
     TCount cnt;
 
@@ -311,6 +314,7 @@ struct Trav {
         spx = 0u;
         has_top = found = false;
         use_alpha = false;
+        any_q = false;
         alpha_seed = 0u;
         cnt.nodes = cnt.tris = cnt.insts = cnt.tlas = 0u;
     }
@@ -332,7 +336,7 @@ struct Trav {
         spx = f ? 0u : spx;
         has_top = f ? false : has_top;
         found = f ? false : found;
-        use_alpha = use_alpha_;
+        use_alpha = f ? use_alpha_ : use_alpha;
         alpha_seed = f ? r.alpha_seed : alpha_seed;
         cnt.nodes = f ? 0u : cnt.nodes;
         cnt.tlas = f ? 0u : cnt.tlas;
@@ -496,8 +500,9 @@ struct Trav {
         const uint32_t prim = __float_as_uint(r0.w);
         bool take = tri_hit;
         if (!ANY) {
-            take = take && (h.t < best.t || cur_inst < best.inst ||
-                            (cur_inst == best.inst && prim < (best.prim & 0x7fffffffu)));
+            const bool closer = h.t < best.t || cur_inst < best.inst ||
+                                (cur_inst == best.inst && prim < (best.prim & 0x7fffffffu));
+            take = take && ((MIXED && any_q) || closer);
             // scenes with non-opaque geometry only: the instance's flags and
             // geometry are read here, for candidate hits
             if (take && use_alpha) {
@@ -537,7 +542,7 @@ struct Trav {
             // Nearer child first; any-hit queries take leaf children
             // (triangles, instances: kinds 2 and 3) before inner ones, which
             // finds occluders sooner.
-            key[c] = h ? ((ANY && (citem[c] >> 31)) ? -INFINITY : tn) : INFINITY;
+            key[c] = h ? (((ANY || (MIXED && any_q)) && (citem[c] >> 31)) ? -INFINITY : tn) : INFINITY;
             nhit += h ? 1 : 0;
         }
         // sorting network on (key, item), ascending
@@ -563,7 +568,7 @@ struct Trav {
             cnt.insts += is_inst ? 1u : 0u;
         }
         const int node_act = nhit >= 2 ? kPush : (nhit == 1 ? kNext : kPop);
-        int act = is_node ? node_act : ((ANY && take) ? kDone : kPop);
+        int act = is_node ? node_act : (((ANY || (MIXED && any_q)) && take) ? kDone : kPop);
         const uint32_t next = citem[0];
         if (bad) {
             overflow = 1;
