@@ -104,7 +104,8 @@ FP64_REF = os.path.join(ROOT, "tests", "golden", "fp64_config2_rows_4096spp.npy"
 def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     """Wall time to reach relMSE <= target against the FP64 CPU oracle: plain
     tracer (spp doubling) vs tracer + denoiser; each point the median of 3
-    frames (the same seed: the last frame is the one scored).  The reference
+    frames (the same seed: the last frame is the one scored), each frame one
+    replay of its captured HIP graph (captured before the timed frames).  The reference
     is the FP64 oracle's 4096-spp render of every 30th row of the 1080p frame
     (36 rows x 1920 px, seed 424242, written by tests/golden/make_fp64_refs.py);
     frames are scored on those rows.  The GPU's own 4096-spp frame is scored
@@ -142,7 +143,8 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
     spp = 1
     while spp <= 1024:
         pc.spp = spp
-        dt = timed(lambda: rd.render(gs, w.cam, pc, seed=31 * spp + 1))
+        rd.render(gs, w.cam, pc, seed=31 * spp + 1, use_graph=True)  # capture, untimed
+        dt = timed(lambda: rd.render(gs, w.cam, pc, seed=31 * spp + 1, use_graph=True))
         e = score(rd.read(0)[..., :3])
         plain.append({"spp": spp, "ms": round(dt * 1e3, 2), "relmse": round(e, 5)})
         if e <= target:
@@ -162,9 +164,10 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
         pc.spp = spp
 
         def frame():
-            rd.render(gs, w.cam, pc, seed=37 * spp + 3)
+            rd.render(gs, w.cam, pc, seed=37 * spp + 3, use_graph=True)
             den.apply(rd)
 
+        frame()  # graph capture, untimed
         dt = timed(frame)
         e = score(rd.read(0)[..., :3])
         dn.append({"spp": spp, "ms": round(dt * 1e3, 2), "relmse": round(e, 5)})
@@ -189,7 +192,8 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
            "gpu_%dspp_relmse" % ref_spp: round(score(gpu_ref), 6) if fp64 else None,
            "plain": plain, "denoised": dn, "denoiser_by_spp": table,
            "unet": unet_rate(den, rd, w.width, w.height, 1),
-           "denoised_schedule": {k: tune1[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow")}}
+           "denoised_schedule": {k: tune1[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0",
+                                                       "grab_shadow", "tail_pairs")}}
     if hit_p and hit_d:
         res["speedup"] = round(hit_p[0]["ms"] / hit_d[0]["ms"], 2)
     return res
@@ -197,30 +201,35 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
 
 def denoised_frames(w, gs, rd, frames, warmup, schedule=""):
     """The denoised point of the time-to-relMSE sweep on its own: 1-spp frames
-    of the 1080p scene, each followed by the denoise, with the schedule tuned
-    for 1-spp frames (or the given one).  Host wall time per frame."""
+    of the 1080p scene (each a replay of the frame's HIP graph), each followed
+    by the denoise, with the schedule tuned for 1-spp frames (or the given
+    one).  Host wall time per frame."""
     import numpy as np
     from kuafu_amd.denoiser import Denoiser
     pc = w.pc
     pc.spp = 1
     pc.frame_count = 0
+    tune = None
     if schedule:
         v = [int(x) for x in schedule.split(",")] + [0] * 6
         rd.set_schedule(*v[:6])
     else:
-        rd.autotune(gs, w.cam, pc)
+        tune = rd.autotune(gs, w.cam, pc)
     den = Denoiser()
     wall, trace = [], []
     for k in range(warmup + frames):
         t = time.perf_counter()
-        st = rd.render(gs, w.cam, pc, seed=37 + k)
+        st = rd.render(gs, w.cam, pc, seed=37 + k, use_graph=True)
         den.apply(rd)
         if k >= warmup:
             wall.append((time.perf_counter() - t) * 1e3)
             trace.append(st.trace_ms)
     return {"workload": "config2_icospheres256_1080p_1spp_denoised", "frames": frames,
             "frame_ms_p50": round(float(np.median(wall)), 3), "frame_ms_min": round(min(wall), 3),
-            "trace_ms_p50": round(float(np.median(trace)), 3)}
+            "trace_ms_p50": round(float(np.median(trace)), 3),
+            "schedule": None if tune is None else {k: tune[k] for k in ("samples_per_batch", "refill", "grab",
+                                                                        "tail_bounce", "grab0", "grab_shadow",
+                                                                        "tail_pairs")}}
 
 
 def batched_environments(frames=8, schedule=None, full_tlas_camera=False):
@@ -471,9 +480,9 @@ def main():
         "config": {"workload": w.name, "width": w.width, "height": w.height, "spp": pc.spp,
                    "path_depth": pc.path_depth, "triangles": int(info.triangles), "instances": int(info.instances),
                    "rays_per_frame": rays // args.steps,
-                   "schedule": None if tune is None else {k: tune[k] for k in ("samples_per_batch", "refill", "grab",
-                                                                                 "tail_bounce", "grab0",
-                                                                                 "grab_shadow")},
+                   "schedule": None if tune is None else dict(
+                       {k: tune[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0",
+                                             "grab_shadow")}, tail_pairs=tune.get("tail_pairs", 1)),
                    "autotune_ms": None if tune is None or "frame_ms" not in tune else
                    [round(tune["default_ms"], 2), round(tune["frame_ms"], 2)],
                    "kernel_mrays_s": round(rays / (trace_ms * 1e-3) / 1e6, 2)},

@@ -408,6 +408,8 @@ typedef struct kf_tune_result {
     uint32_t tail_bounce; /* bounce from which one thread carries each path to its end (0: never) */
     uint32_t grab0;       /* queries reserved per wave for the camera rays (0: grab) */
     uint32_t grab_shadow; /* ... for the shadow rays (0: grab) */
+    uint32_t tail_pairs;  /* 1: two lanes per path carry the tail bounces while few paths remain (at most a
+                             quarter of the launch's threads; no soft light), 0: one lane per path */
 } kf_tune_result;
 int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_envs, const kf_camera_ubo* cams,
                          const kf_push_constants* pc, kf_tune_result* out);

@@ -93,7 +93,8 @@ class RenderStats(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("samples_per_batch", C.c_uint32), ("refill", C.c_uint32), ("grab", C.c_uint32),
                 ("frame_ms", C.c_float), ("default_ms", C.c_float), ("trials", C.c_uint32),
-                ("tail_bounce", C.c_uint32), ("grab0", C.c_uint32), ("grab_shadow", C.c_uint32)]
+                ("tail_bounce", C.c_uint32), ("grab0", C.c_uint32), ("grab_shadow", C.c_uint32),
+                ("tail_pairs", C.c_uint32)]
 
 
 class RenderOpts(C.Structure):

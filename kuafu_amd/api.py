@@ -409,8 +409,8 @@ class Renderer:
         t = abi.TuneResult()
         check(lib().kf_renderer_autotune(self.h, scene.h, len(cams), arr, C.byref(pc), C.byref(t)))
         return dict(samples_per_batch=t.samples_per_batch, refill=t.refill, grab=t.grab, tail_bounce=t.tail_bounce,
-                    grab0=t.grab0, grab_shadow=t.grab_shadow, frame_ms=t.frame_ms, default_ms=t.default_ms,
-                    trials=t.trials)
+                    grab0=t.grab0, grab_shadow=t.grab_shadow, tail_pairs=t.tail_pairs, frame_ms=t.frame_ms,
+                    default_ms=t.default_ms, trials=t.trials)
 
     def set_schedule(self, samples_per_batch=0, refill=0, grab=0, tail_bounce=0, grab0=0, grab_shadow=0):
         """grab0 / grab_shadow: reservation per wave for the camera / shadow rays (0: grab)."""

@@ -280,6 +280,7 @@ struct Renderer {
     uint64_t graph_captures = 0;               // frame graphs captured by this renderer so far
     // Schedule picked by kf_renderer_autotune (0: built-in default).
     uint32_t tune_spb = 0, tune_refill = 0, tune_grab = 0, tune_tail = 0, tune_grab0 = 0, tune_grabs = 0;
+    uint32_t tune_pairs = 1;  // 1: k_tail2 may carry the tail bounces of few paths, 0: k_tail only
     uint32_t n_env_last = 1;                   // environments of the last frame (the denoiser's batch)
     ~Renderer();
     void ensure(size_t paths, uint32_t nslots, uint32_t spp);

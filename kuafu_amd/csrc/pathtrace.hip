@@ -1580,7 +1580,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
                 // the path count is on the device: k_tail2 (two lanes per
                 // path) runs for at most a quarter of the grid's threads,
                 // k_tail beyond, and the other launch returns at once
-                const uint32_t pair_max = inline_nee ? 0u : grid * TB / 4u;
+                const uint32_t pair_max = (inline_nee || !r->tune_pairs) ? 0u : grid * TB / 4u;
                 if (!inline_nee)
                     k_tail2<<<grid, TB, 0, st>>>(S, F, d_lights, (int)b, qa, r->ray_o, r->ray_d, r->thr, r->col,
                                                  r->hit, r->hit_prim, r->rw, r->em, r->seed, r->nee_a, r->nee_b,
@@ -1763,7 +1763,7 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
         uint32_t trials = 0;
         // kernel time of one frame under a schedule (0 = default); the best
         // of three (of one for frames over 50 ms)
-        uint32_t b_tail = 0, b_grab0 = 0, b_grabs = 0;
+        uint32_t b_tail = 0, b_grab0 = 0, b_grabs = 0, b_pairs = 1;
         auto run = [&](uint32_t spb, uint32_t refill, uint32_t grab) {
             r->tune_spb = spb;
             r->tune_refill = refill;
@@ -1771,6 +1771,7 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
             r->tune_tail = b_tail;
             r->tune_grab0 = b_grab0;
             r->tune_grabs = b_grabs;
+            r->tune_pairs = b_pairs;
             float best = INFINITY;
             for (int k = 0; k < 3; ++k) {
                 kf_render_stats st{};
@@ -1800,6 +1801,7 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
                 if (which == 3) b_tail = v;
                 if (which == 4) b_grab0 = v;
                 if (which == 5) b_grabs = v;
+                if (which == 6) b_pairs = v;
                 const float m = which == 0 ? run(v, b_refill, b_grab)
                                            : (which == 1 ? run(b_spb, v, b_grab) : run(b_spb, b_refill,
                                                                                        which == 2 ? v : b_grab));
@@ -1851,6 +1853,13 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
         uint32_t best_tail = 0;
         sweep(tails, best_tail, 3);
         b_tail = best_tail;
+        // the tail kernel: two lanes per path while few paths remain, or one
+        // (the tail bounce is swept again with the kernel picked)
+        uint32_t best_pairs = 1;
+        sweep({1u, 0u}, best_pairs, 6);
+        b_pairs = best_pairs;
+        sweep(tails, best_tail, 3);
+        b_tail = best_tail;
         // the shadow rays' reservation (0: the same; at 16 spp of the 1080p
         // scene 64 against 256 saves 2 % of the frame, at 1 spp 4 %)
         const std::vector<uint32_t> grabss = {0u, 64u, 128u};
@@ -1863,6 +1872,8 @@ extern "C" int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_en
         r->tune_tail = b_tail;
         r->tune_grab0 = b_grab0;
         r->tune_grabs = b_grabs;
+        r->tune_pairs = b_pairs;
+        out->tail_pairs = b_pairs;
         out->tail_bounce = b_tail;
         out->grab0 = b_grab0;
         out->grab_shadow = b_grabs;
@@ -1891,6 +1902,7 @@ extern "C" int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_bat
     r->tune_tail = tail_bounce;
     r->tune_grab0 = grab0;
     r->tune_grabs = grab_shadow;
+    r->tune_pairs = 1;
     return 0;
 }
 
