@@ -24,6 +24,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s at 1080p; time-to-relMSE 1e-2, denoised vs plain spp, 1 GPU"
+SCHEDULE_KEYS = ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow", "tail_pairs")
+
+
+def parse_schedule(text):
+    """'spb,refill,grab[,tail_bounce[,grab0[,grab_shadow[,tail_pairs]]]]' -> the
+    7 arguments of Renderer.set_schedule (missing ones 0, tail_pairs 1)."""
+    v = [int(x) for x in text.split(",")]
+    if not 3 <= len(v) <= 7:
+        raise SystemExit("bench.py: --schedule takes 3 to 7 comma-separated integers, got %r" % text)
+    return (v + [0, 0, 0, 1][len(v) - 3:])[:7]
 
 
 def host_threads():
@@ -192,8 +202,7 @@ def time_to_relmse(w, gs, rd, target=1e-2, ref_spp=4096):
            "gpu_%dspp_relmse" % ref_spp: round(score(gpu_ref), 6) if fp64 else None,
            "plain": plain, "denoised": dn, "denoiser_by_spp": table,
            "unet": unet_rate(den, rd, w.width, w.height, 1),
-           "denoised_schedule": {k: tune1[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0",
-                                                       "grab_shadow", "tail_pairs")}}
+           "denoised_schedule": {k: tune1[k] for k in SCHEDULE_KEYS if k in tune1}}
     if hit_p and hit_d:
         res["speedup"] = round(hit_p[0]["ms"] / hit_d[0]["ms"], 2)
     return res
@@ -211,8 +220,7 @@ def denoised_frames(w, gs, rd, frames, warmup, schedule=""):
     pc.frame_count = 0
     tune = None
     if schedule:
-        v = [int(x) for x in schedule.split(",")] + [0] * 6
-        rd.set_schedule(*v[:6])
+        rd.set_schedule(*parse_schedule(schedule))
     else:
         tune = rd.autotune(gs, w.cam, pc)
     den = Denoiser()
@@ -227,9 +235,7 @@ def denoised_frames(w, gs, rd, frames, warmup, schedule=""):
     return {"workload": "config2_icospheres256_1080p_1spp_denoised", "frames": frames,
             "frame_ms_p50": round(float(np.median(wall)), 3), "frame_ms_min": round(min(wall), 3),
             "trace_ms_p50": round(float(np.median(trace)), 3),
-            "schedule": None if tune is None else {k: tune[k] for k in ("samples_per_batch", "refill", "grab",
-                                                                        "tail_bounce", "grab0", "grab_shadow",
-                                                                        "tail_pairs")}}
+            "schedule": None if tune is None else {k: tune[k] for k in SCHEDULE_KEYS if k in tune}}
 
 
 def batched_environments(frames=8, schedule=None, full_tlas_camera=False):
@@ -248,8 +254,7 @@ def batched_environments(frames=8, schedule=None, full_tlas_camera=False):
     pc.frame_count = 0
     if schedule:
         rd.set_schedule(*schedule)
-        tune = dict(zip(("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow"), schedule), default_ms=0.0,
-                    frame_ms=0.0)
+        tune = dict(zip(SCHEDULE_KEYS, schedule), default_ms=0.0, frame_ms=0.0)
     else:
         tune = rd.autotune(gs, w.cam, pc)
     den = Denoiser()
@@ -280,7 +285,7 @@ def batched_environments(frames=8, schedule=None, full_tlas_camera=False):
            "frame_ms_p99": round(float(np.percentile(lat, 99)), 2),
            "render_ms_p50": round(float(np.percentile(rend, 50)), 2),
            "denoise_ms_p50": round(float(np.percentile(dms, 50)), 2), "frames": frames,
-           "schedule": {k: tune[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow")},
+           "schedule": {k: tune[k] for k in SCHEDULE_KEYS if k in tune},
            "autotune_ms": [round(tune["default_ms"], 1), round(tune["frame_ms"], 1)],
            "unet": unet_rate(den, rd, w.width, w.env_height, w.n_env)}
     gs.close()
@@ -332,8 +337,7 @@ def active_light_frame(steps=3):
             "frame_ms_p50": round(float(np.median(lat)), 2), "rays_per_frame": rays // steps,
             "mrays_s": round(rays / (sum(lat) * 1e-3) / 1e6, 1),
             "kernel_mrays_s": round(rays / (kms * 1e-3) / 1e6, 1),
-            "schedule": {k: tune[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0",
-                                               "grab_shadow")}}
+            "schedule": {k: tune[k] for k in SCHEDULE_KEYS if k in tune}}
 
 
 def other_workloads(steps=3):
@@ -369,7 +373,8 @@ def main():
     ap.add_argument("--no-relmse", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--schedule", default="",
-                    help="samples_per_batch,refill,grab[,tail_bounce[,grab0[,grab_shadow]]] instead of autotuning")
+                    help="samples_per_batch,refill,grab[,tail_bounce[,grab0[,grab_shadow[,tail_pairs]]]] instead "
+                         "of autotuning")
     ap.add_argument("--envs-only", action="store_true",
                     help="only the 128-environment workload (frames = --steps), e.g. for profiling")
     ap.add_argument("--denoised-only", action="store_true",
@@ -393,8 +398,8 @@ def main():
 
     check(lib().kf_init(0))
     if args.envs_only:
-        sched = [int(x) for x in args.schedule.split(",")] + [0] * 6 if args.schedule else None
-        print(json.dumps(batched_environments(frames=args.steps, schedule=sched[:6] if sched else None,
+        sched = parse_schedule(args.schedule) if args.schedule else None
+        print(json.dumps(batched_environments(frames=args.steps, schedule=sched,
                                               full_tlas_camera=args.full_tlas_camera)))
         return
     w = scenes.config2(width=args.width, height=args.height)
@@ -414,9 +419,9 @@ def main():
     # timing frames of this scene, before anything is measured
     tune = None
     if args.schedule:
-        v = [int(x) for x in args.schedule.split(",")] + [0, 0, 0]
-        rd.set_schedule(*v[:6])
-        tune = dict(zip(("samples_per_batch", "refill", "grab", "tail_bounce", "grab0", "grab_shadow"), v[:6]))
+        v = parse_schedule(args.schedule)
+        rd.set_schedule(*v)
+        tune = dict(zip(SCHEDULE_KEYS, v))
     elif not args.no_autotune and not args.samples_per_batch:
         print("bench: autotune", file=sys.stderr, flush=True)
         tune = rd.autotune(gs, w.cam, pc)
@@ -480,9 +485,7 @@ def main():
         "config": {"workload": w.name, "width": w.width, "height": w.height, "spp": pc.spp,
                    "path_depth": pc.path_depth, "triangles": int(info.triangles), "instances": int(info.instances),
                    "rays_per_frame": rays // args.steps,
-                   "schedule": None if tune is None else dict(
-                       {k: tune[k] for k in ("samples_per_batch", "refill", "grab", "tail_bounce", "grab0",
-                                             "grab_shadow")}, tail_pairs=tune.get("tail_pairs", 1)),
+                   "schedule": None if tune is None else {k: tune[k] for k in SCHEDULE_KEYS if k in tune},
                    "autotune_ms": None if tune is None or "frame_ms" not in tune else
                    [round(tune["default_ms"], 2), round(tune["frame_ms"], 2)],
                    "kernel_mrays_s": round(rays / (trace_ms * 1e-3) / 1e6, 2)},

@@ -420,6 +420,12 @@ int kf_renderer_autotune(kf_renderer* r, kf_scene* s, uint32_t num_envs, const k
  * the reservations to [16, 4096]. */
 int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_batch, uint32_t refill, uint32_t grab,
                              uint32_t tail_bounce, uint32_t grab0, uint32_t grab_shadow);
+/* Choose the kernel that carries the bounces from tail_bounce on (cleared to
+ * 1 by kf_renderer_set_schedule): 1 lets two lanes per path take them while
+ * at most a quarter of the launch's threads' worth of paths is left and no
+ * light has a random perturbation (kf_tune_result.tail_pairs), 0 keeps one
+ * lane per path.  Images do not depend on it. */
+int kf_renderer_set_tail_pairs(kf_renderer* r, uint32_t tail_pairs);
 
 /* Read a render target: 0 = rgba, 1 = albedo, 2 = normal; W*H*4 floats. */
 int kf_renderer_read(kf_renderer* r, int target, float* out);

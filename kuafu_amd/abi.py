@@ -154,6 +154,7 @@ EXPORTS = [
                                  C.c_void_p]),
     ("kf_renderer_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("kf_renderer_set_schedule", C.c_int, [C.c_void_p] + [C.c_uint32] * 6),
+    ("kf_renderer_set_tail_pairs", C.c_int, [C.c_void_p, C.c_uint32]),
     ("kf_renderer_read", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_write", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("kf_renderer_read_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),

@@ -412,10 +412,13 @@ class Renderer:
                     grab0=t.grab0, grab_shadow=t.grab_shadow, tail_pairs=t.tail_pairs, frame_ms=t.frame_ms,
                     default_ms=t.default_ms, trials=t.trials)
 
-    def set_schedule(self, samples_per_batch=0, refill=0, grab=0, tail_bounce=0, grab0=0, grab_shadow=0):
-        """grab0 / grab_shadow: reservation per wave for the camera / shadow rays (0: grab)."""
+    def set_schedule(self, samples_per_batch=0, refill=0, grab=0, tail_bounce=0, grab0=0, grab_shadow=0,
+                     tail_pairs=1):
+        """grab0 / grab_shadow: reservation per wave for the camera / shadow rays (0: grab);
+        tail_pairs 1: two lanes per path may carry the tail bounces, 0: one lane."""
         check(lib().kf_renderer_set_schedule(self.h, samples_per_batch, refill, grab, tail_bounce, grab0,
                                              grab_shadow))
+        check(lib().kf_renderer_set_tail_pairs(self.h, tail_pairs))
 
     def read(self, target=0):
         out = np.zeros((self.h_, self.w, 4), np.float32)

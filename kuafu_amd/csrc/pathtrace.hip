@@ -1644,8 +1644,8 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             const uint8_t* b = (const uint8_t*)p;
             key.insert(key.end(), b, b + n);
         };
-        const uint32_t sched[11] = {spb, K, nb, refill, grab, grid, (uint32_t)o.count_traversal,
-                                    (uint32_t)o.keep_samples, b_tail, grab0, grabs};
+        const uint32_t sched[12] = {spb, K, nb, refill, grab, grid, (uint32_t)o.count_traversal,
+                                    (uint32_t)o.keep_samples, b_tail, grab0, grabs, r->tune_pairs};
         const void* bufs[] = {r->ray_o, r->ray_d, r->thr, r->col, r->rw, r->em, r->hit, r->hit_prim, r->seed,
                               r->queue_a, r->queue_b, r->queue_e, r->sh_o, r->sh_d, r->sh_root, r->nee_a,
                               r->nee_b, r->vis, r->samples, r->counters, d_lights, d_jump, d_acc, d_out,
@@ -1903,6 +1903,19 @@ extern "C" int kf_renderer_set_schedule(kf_renderer* r, uint32_t samples_per_bat
     r->tune_grab0 = grab0;
     r->tune_grabs = grab_shadow;
     r->tune_pairs = 1;
+    return 0;
+}
+
+extern "C" int kf_renderer_set_tail_pairs(kf_renderer* r, uint32_t tail_pairs) {
+    if (!r) {
+        set_error("kf_renderer_set_tail_pairs: null renderer");
+        return -1;
+    }
+    if (tail_pairs > 1) {
+        set_error("kf_renderer_set_tail_pairs: tail_pairs must be 0 or 1");
+        return -1;
+    }
+    r->tune_pairs = tail_pairs;
     return 0;
 }
 

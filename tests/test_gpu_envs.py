@@ -248,25 +248,42 @@ def _tail_workloads():
     return out
 
 
-def test_tail_kernel_equals_wavefront(gpu):
-    """Bounces finished by k_tail (one thread per remaining path, shadow rays
-    inline) give the wavefront's per-sample radiance, targets and ray counts
-    bit for bit, whichever bounce it takes over from."""
-    for w in _tail_workloads():
-        gs = GpuScene(w.data)
-        pc = w.pc
-        pc.frame_count = 0
-        ref = Renderer(w.width, w.height)
-        st0 = ref.render(gs, w.cam, pc, seed=7, keep_samples=True)
-        smp0 = ref.read_samples(pc.spp)
-        tg0 = [ref.read(k) for k in range(3)]
-        for tail in range(1, pc.path_depth + 1):
+def _assert_tail_equals_wavefront(w, tails, pairs=(1, 0)):
+    gs = GpuScene(w.data)
+    pc = w.pc
+    pc.frame_count = 0
+    ref = Renderer(w.width, w.height)
+    st0 = ref.render(gs, w.cam, pc, seed=7, keep_samples=True)
+    smp0 = ref.read_samples(pc.spp)
+    tg0 = [ref.read(k) for k in range(3)]
+    for tail in tails:
+        for tp in pairs:
+            name = "%s tail %d pairs %d" % (w.name, tail, tp)
             rd = Renderer(w.width, w.height)
-            rd.set_schedule(0, 0, 0, tail)
+            rd.set_schedule(0, 0, 0, tail, tail_pairs=tp)
             st = rd.render(gs, w.cam, pc, seed=7, keep_samples=True)
-            assert same(rd.read_samples(pc.spp), smp0), "%s tail %d" % (w.name, tail)
+            assert same(rd.read_samples(pc.spp), smp0), name
             for k in range(3):
-                assert same(rd.read(k), tg0[k]), "%s tail %d target %d" % (w.name, tail, k)
-            assert st.primary_rays == st0.primary_rays and st.shadow_rays == st0.shadow_rays
+                assert same(rd.read(k), tg0[k]), "%s target %d" % (name, k)
+            assert st.primary_rays == st0.primary_rays and st.shadow_rays == st0.shadow_rays, name
             # the wavefront also traces rays that Russian roulette then drops
-            assert st.extension_rays == st0.extension_rays
+            assert st.extension_rays == st0.extension_rays, name
+
+
+def test_tail_kernel_equals_wavefront(gpu):
+    """Bounces finished by a tail kernel give the wavefront's per-sample
+    radiance, targets and ray counts bit for bit, whichever bounce it takes
+    over from: k_tail2 (two lanes per path, the closest-hit ray of bounce b
+    beside the shadow rays of bounce b-1; tail_pairs 1, scenes without
+    random light perturbations) and k_tail (one thread per path, shadow rays
+    inline; tail_pairs 0, and the soft-shadow scene either way)."""
+    for w in _tail_workloads():
+        _assert_tail_equals_wavefront(w, range(1, w.pc.path_depth + 1))
+
+
+def test_tail_kernel_beyond_pair_capacity(gpu):
+    """At bounce 1 of a 480x360 frame at 3 spp (518400 paths, more than half
+    of them on the ground or a sphere) more paths are alive than k_tail2 has
+    lane pairs for (a quarter of a launch's threads: 131072 on 256 CUs), so
+    k_tail takes them although tail_pairs is 1."""
+    _assert_tail_equals_wavefront(scenes.config1(480, 360, spp=3), (1, 2, 3), pairs=(1,))
