@@ -120,14 +120,10 @@ struct ConvShape {
 };
 
 // FUSE bits: kFuseUp upsampled input channels, kFusePool pooled output,
-// kFusePack the 32-channel network input computed from the render targets,
-// kOutF32 FP32 output (Yf) instead of BF16 (Y), kMask output masked by M > 0,
-// kFixW one set of weights for the whole launch (one 32-channel input chunk
-// and at most BN outputs), loaded once before the loop so that their staging
-// registers are free inside it: with them the first layer fits 3 waves per
-// SIMD (1.45 -> 1.30 ms on 128 x 512x512).  The output layer at 4 waves with
-// fixed weights spills and was not faster.
-constexpr int kFuseUp = 1, kFusePool = 2, kFusePack = 4, kOutF32 = 8, kMask = 16, kFixW = 32;
+// kFusePack the 32-channel network input computed from the render targets
+// (k_conv2_pack), kOutF32 FP32 output (Yf) instead of BF16 (Y), kMask output
+// masked by M > 0.
+constexpr int kFuseUp = 1, kFusePool = 2, kFusePack = 4, kOutF32 = 8, kMask = 16;
 
 // Waves per SIMD the register budget is set for: 3 (168 registers) where that
 // needs no spills, else 2 (64 output channels per item, masked data
@@ -244,7 +240,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(B
                                              float* __restrict__ Yf, int ldyf, int cstore,
                                              const uint16_t* __restrict__ M, int ldm, int relu, int n_items,
                                              const uint16_t* __restrict__ X2, int ldx2, int csplit,
-                                             uint16_t* __restrict__ P, PackSrc pk) {
+                                             uint16_t* __restrict__ P) {
     using SH = ConvShape<BN>;
     __shared__ __attribute__((aligned(16))) uint16_t smem[SH::kLds];
     uint16_t* Hs = smem;
@@ -271,29 +267,6 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(B
     };
     uint4 hreg[SH::kHR], wreg[SH::kWR];
     auto load_halo = [&](const Item& it, int ci0, bool valid) {
-        if constexpr ((FUSE & kFusePack) != 0) {
-            static_assert(SH::kHR >= 6, "the packed input stages 6 registers");
-            // two halo pixels per thread, 3 float4 each (zero outside the image)
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int hp = t + r * TB;
-                float4 c = make_float4(0.f, 0.f, 0.f, 0.f), a = c, nm = c;
-                if (hp < NHALO) {
-                    const int hy = hp / HXX, hx = hp - hy * HXX;
-                    const int yy = it.y0 + hy - 1, xx = it.x0 + hx - 1;
-                    if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                        const size_t src = ((size_t)it.n * pk.h + min(yy, pk.h - 1)) * pk.w + min(xx, pk.w - 1);
-                        c = pk.rgba[src];
-                        a = pk.alb[src];
-                        nm = pk.nrm[src];
-                    }
-                }
-                hreg[3 * r] = __builtin_bit_cast(uint4, c);
-                hreg[3 * r + 1] = __builtin_bit_cast(uint4, a);
-                hreg[3 * r + 2] = __builtin_bit_cast(uint4, nm);
-            }
-            return;
-        }
         // Halo slot r of this thread: pixel hp = (t + r TB) / 4 of the (TX + 2) x
         // (TY + 2) halo, 16-byte channel group q, read through a buffer
         // resource over the item's image: offsets past its end (pixels off the
@@ -341,28 +314,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(B
     f32x4 acc[4][BN / 16];
     while (true) {
         __syncthreads();  // the previous step's LDS reads (and epilogue) are done
-        if constexpr ((FUSE & kFusePack) != 0) {
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int hp = t + r * TB;
-                if (hp >= NHALO) continue;
-                float v[9];
-                pack9(__builtin_bit_cast(float4, hreg[3 * r]), __builtin_bit_cast(float4, hreg[3 * r + 1]),
-                      __builtin_bit_cast(float4, hreg[3 * r + 2]), v);
-                // v_cvt_pk_bf16_f32 rounds to nearest even as f2bf (no NaN after sane)
-                typedef float f2 __attribute__((ext_vector_type(2)));
-                typedef __bf16 b2 __attribute__((ext_vector_type(2)));
-                uint32_t o[5];
-#pragma unroll
-                for (int j = 0; j < 5; ++j)
-                    o[j] = __builtin_bit_cast(uint32_t,
-                                              __builtin_convertvector(f2{v[2 * j], j < 4 ? v[2 * j + 1] : 0.f}, b2));
-                *(uint4*)&Hs[cofs(hp, 0)] = make_uint4(o[0], o[1], o[2], o[3]);
-                *(uint4*)&Hs[cofs(hp, 1)] = make_uint4(o[4], 0u, 0u, 0u);
-                *(uint4*)&Hs[cofs(hp, 2)] = make_uint4(0u, 0u, 0u, 0u);
-                *(uint4*)&Hs[cofs(hp, 3)] = make_uint4(0u, 0u, 0u, 0u);
-            }
-        } else {
+        {
 #pragma unroll
             for (int r = 0; r < SH::kHR; ++r) {
                 const int k = t + r * TB;
@@ -386,7 +338,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(B
         }
         const bool has_next = ni < n_items;
         const Item nxt = has_next ? decode(ni) : cur;
-        const bool next_w = (FUSE & kFixW) == 0 && has_next && (nxt.co0 != cur.co0 || nc != c);
+        const bool next_w = has_next && (nxt.co0 != cur.co0 || nc != c);
         if (c == 0) {
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
@@ -417,13 +369,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(conv_waves(B
                         acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a[mt + dy], acc[mt][nb], 0, 0, 0);
                 }
             }
-            if (dx == 0) {
-                if constexpr ((FUSE & kFusePack) != 0) {
-                    if (has_next) load_halo(nxt, nc * CBK, true);
-                } else {
-                    load_halo(nxt, nc * CBK, has_next);
-                }
-            }
+            if (dx == 0) load_halo(nxt, nc * CBK, has_next);
             if (dx == 1 && next_w) load_w(nxt.co0, nc * CBK);
         }
         if (c == nchunk - 1)
@@ -467,9 +413,11 @@ template <int BN, int FUSE>
 __global__ __launch_bounds__(TB2) void k_conv2(const uint16_t* __restrict__ X, int ldx, int Cin,
                                               const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
                                               int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
-                                              int cstore, int relu, int n_items, const uint16_t* __restrict__ X2,
-                                              int ldx2, int csplit, uint16_t* __restrict__ P) {
-    static_assert((FUSE & ~(kFuseUp | kFusePool)) == 0 && (BN == 32 || BN == 64), "k_conv2 variants");
+                                              float* __restrict__ Yf, int ldyf, int cstore, int relu, int n_items,
+                                              const uint16_t* __restrict__ X2, int ldx2, int csplit,
+                                              uint16_t* __restrict__ P) {
+    static_assert(((FUSE & ~(kFuseUp | kFusePool)) == 0 && (BN == 32 || BN == 64)) || (FUSE == kOutF32 && BN == 16),
+                  "k_conv2 variants");
     using SH = Conv2Shape<BN>;
     // two separate arrays, so that the compiler tells the buffers apart and
     // the MFMA reads of one need not wait for the loads into the other
@@ -587,7 +535,7 @@ __global__ __launch_bounds__(TB2) void k_conv2(const uint16_t* __restrict__ X, i
         }
         if (c == nchunk - 1)
             conv_store<BN, FUSE>(acc, (const float*)(Ws + SH::kWUnits * 8), cur.n, cur.y0, cur.x0, cur.co0, wv, lane, H,
-                                 W, Y, ldy, nullptr, 0, cstore, nullptr, 0, relu, P);
+                                 W, Y, ldy, Yf, ldyf, cstore, nullptr, 0, relu, P);
         if (!has_next) return false;
         i = ni;
         c = nc;
@@ -596,6 +544,158 @@ __global__ __launch_bounds__(TB2) void k_conv2(const uint16_t* __restrict__ X, i
     };
     while (step(buf0, buf1, wkey1) && step(buf1, buf0, wkey0)) {
     }
+}
+
+// ------------------------------------------- first layer, 8 waves, packed input
+// The network's first convolution (9 input channels packed from the float4
+// render targets, at most 32 outputs, one weight set) on k_conv2's 32 x 16
+// tiles.  The three targets' halo pixels go from memory straight into LDS
+// (raw, two buffers: the next tile's loads run while this one is packed and
+// convolved); the threads pack them into the BF16 halo (channels 0-8; the
+// zero channel groups 2 and 3 are written once), then the MFMAs and the
+// epilogue are k_conv's, so the outputs are the bits of k_conv<32, kFusePack>.
+constexpr int kRawLoads = (NHALO2 + 63) / 64;  // wave loads per target over the halo
+constexpr int kRawUnits = kRawLoads * 64;      // float4 per target and buffer (the last load's spill-over included)
+
+__global__ __launch_bounds__(TB2) void k_conv2_pack(const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
+                                                   int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
+                                                   int cstore, int relu, int n_items, PackSrc pk) {
+    constexpr int BN = 32;
+    using SH = Conv2Shape<BN>;
+    __shared__ __attribute__((aligned(16))) float4 raw0[3 * kRawUnits];
+    __shared__ __attribute__((aligned(16))) float4 raw1[3 * kRawUnits];
+    __shared__ __attribute__((aligned(16))) uint16_t Hs[NHALO2 * CCLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Ws[SH::kWUnits * 8 + 2 * 64];  // + 64 FP32 biases (one per lane)
+    const int tiles_x = (W + TX - 1) / TX, tiles_y = (H + TY2 - 1) / TY2;
+    const int n_tiles = N * tiles_x * tiles_y;
+    const int t = threadIdx.x, lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int fr = lane & 15;
+    struct Item {
+        int n, y0, x0;
+    };
+    auto decode = [&](int i) {  // i is uniform: the fields live in scalar registers
+        Item it;
+        const int tile = i % n_tiles;
+        it.x0 = __builtin_amdgcn_readfirstlane((tile % tiles_x) * TX);
+        const int r = tile / tiles_x;
+        it.y0 = __builtin_amdgcn_readfirstlane((r % tiles_y) * TY2);
+        it.n = __builtin_amdgcn_readfirstlane(r / tiles_y);
+        return it;
+    };
+    // halo pixel u of the item (row u / (TX + 2), column u % (TX + 2)) into
+    // unit u of each target's part of buffer B; pixels off the H x W input
+    // read zeros, those past the targets' h x w repeat the edge pixel
+    auto issue = [&](float4* B, const Item& it) __attribute__((always_inline)) {
+        const uint32_t img = (uint32_t)pk.h * (uint32_t)pk.w;
+        const float4* src[3] = {pk.rgba, pk.alb, pk.nrm};
+        for (int j = wv; j < kRawLoads; j += TB2 / 64) {
+            const int u = j * 64 + lane;
+            const int hy = u / HXX, hx = u - hy * HXX;
+            const int yy = it.y0 - 1 + hy, xx = it.x0 - 1 + hx;
+            const bool ok = u < NHALO2 && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W;
+            const uint32_t off = ok ? 16u * (uint32_t)(min(yy, pk.h - 1) * pk.w + min(xx, pk.w - 1)) : kOOB;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(src[a] + (size_t)it.n * img, 16u * img);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void*)(B + a * kRawUnits + j * 64), 16, off, 0, 0, 0);
+            }
+        }
+    };
+    int i = blockIdx.x;
+    if (i >= n_items) return;
+    Item cur = decode(i);
+    issue(raw0, cur);
+    {  // the one weight set and its biases (k_conv2's layout), and the zero channel groups
+        const __amdgpu_buffer_rsrc_t rw = buffer_rsrc(Wt, 18u * Cout * CBK);
+        for (int j = wv; j < SH::kWLoads; j += TB2 / 64) {
+            const int u = j * 64 + lane, r = u >> 2;
+            const int q = (u & 3) ^ ((r >> 1) & 3);
+            const int tap = r / BN, co = r - tap * BN;
+            const uint32_t off = co < Cout ? 2u * (uint32_t)((tap * Cout + co) * CBK + q * 8) : kOOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rw, (lds_void*)(Ws + j * 512), 16, off, 0, 0, 0);
+        }
+        if (wv == TB2 / 64 - 1) {
+            const __amdgpu_buffer_rsrc_t rb = buffer_rsrc(bias, bias ? 4u * Cout : 0u);
+            const uint32_t off = lane < Cout ? 4u * (uint32_t)lane : kOOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void*)(Ws + SH::kWUnits * 8), 4, off, 0, 0, 0);
+        }
+        for (int r = t; r < NHALO2; r += TB2) {
+            *(uint4*)&Hs[cofs(r, 2)] = make_uint4(0u, 0u, 0u, 0u);
+            *(uint4*)&Hs[cofs(r, 3)] = make_uint4(0u, 0u, 0u, 0u);
+        }
+    }
+    const float* Bsm = (const float*)(Ws + SH::kWUnits * 8);
+    // One step: the tile whose raw halo is in A, while the next tile's loads
+    // go into B (issued first, so that they run beside the pack, the MFMAs
+    // and the stores); the outputs of the previous tile are stored before
+    // this tile's MFMAs, so that the wait at the next step's start finds them
+    // long under way.  Returns false after the block's last tile.
+    f32x4 acc[4][BN / 16];
+    Item prev = cur;
+    bool have_prev = false;
+    auto step = [&](const float4* A, float4* B) __attribute__((always_inline)) -> bool {
+        const int ni = i + gridDim.x;
+        const bool has_next = ni < n_items;
+        const Item nxt = has_next ? decode(ni) : cur;
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's loads into A (and the weights) have landed
+        __syncthreads();                     // every wave's; B and Hs are no longer read
+        if (has_next) issue(B, nxt);
+        for (int r = t; r < NHALO2; r += TB2) {
+            float v[9];
+            pack9(A[r], A[kRawUnits + r], A[2 * kRawUnits + r], v);
+            // v_cvt_pk_bf16_f32 rounds to nearest even as f2bf (no NaN after sane)
+            typedef float f2 __attribute__((ext_vector_type(2)));
+            typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+            uint32_t o[5];
+#pragma unroll
+            for (int j = 0; j < 5; ++j)
+                o[j] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{v[2 * j], j < 4 ? v[2 * j + 1] : 0.f}, b2));
+            *(uint4*)&Hs[cofs(r, 0)] = make_uint4(o[0], o[1], o[2], o[3]);
+            *(uint4*)&Hs[cofs(r, 1)] = make_uint4(o[4], 0u, 0u, 0u);
+        }
+        // Hs is packed: every wave's LDS writes are done.  A barrier without
+        // __syncthreads' fence, which would wait for the loads into B too.
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        if (have_prev)
+            conv_store<BN, 0>(acc, Bsm, prev.n, prev.y0, prev.x0, 0, wv, lane, H, W, Y, ldy, nullptr, 0, cstore,
+                              nullptr, 0, relu, nullptr);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nb = 0; nb < BN / 16; ++nb) acc[mt][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            bf16x8 a[6], bw[3][BN / 16];
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                a[j] = __builtin_bit_cast(
+                    bf16x8, *(const uint4*)&Hs[cofs((4 * (wv >> 1) + j) * HXX + (wv & 1) * 16 + fr + dx, lane >> 4)]);
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int nb = 0; nb < BN / 16; ++nb)
+                    bw[dy][nb] = __builtin_bit_cast(
+                        bf16x8, *(const uint4*)&Ws[cofs((dy * 3 + dx) * BN + nb * 16 + fr, lane >> 4)]);
+            __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the MFMAs
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int nb = 0; nb < BN / 16; ++nb)
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt)
+                        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[dy][nb], a[mt + dy], acc[mt][nb], 0, 0, 0);
+        }
+        prev = cur;
+        have_prev = true;
+        if (!has_next) return false;
+        i = ni;
+        cur = nxt;
+        return true;
+    };
+    while (step(raw0, raw1) && step(raw1, raw0)) {
+    }
+    conv_store<BN, 0>(acc, Bsm, prev.n, prev.y0, prev.x0, 0, wv, lane, H, W, Y, ldy, nullptr, 0, cstore, nullptr, 0,
+                      relu, nullptr);
 }
 
 // dW[tap, co, ci] += sum_p dY[p, co] * X[p + off(tap), ci] over one pixel chunk.
@@ -871,6 +971,18 @@ unsigned conv2_grid(unsigned n_items) {
     return std::min(n_items, resident);
 }
 
+unsigned conv2_pack_grid(unsigned n_items) {
+    static unsigned resident = 0;
+    if (!resident) {
+        int dev = 0, cus = 256, nb = 0;
+        KF_CHECK(hipGetDevice(&dev));
+        KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_conv2_pack, TB2, 0));
+        resident = (unsigned)std::max(nb, 1) * (unsigned)cus;
+    }
+    return std::min(n_items, resident);
+}
+
 // Optional fusions: input channels csplit.. come from X2 (ldx2), an H/2 x W/2
 // tensor upsampled by nearest neighbour; P receives the 2x2 average pool of
 // the bf16 outputs (H/2 x W/2 x cstore).
@@ -907,7 +1019,7 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
         const unsigned n = tiles * blocks(cstore, BN);
         k_conv<BN, F><<<conv_grid<BN, F>(n), TB, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf,
                                                          cstore, M, ldm, relu, (int)n, fz.X2, fz.ldx2, fz.csplit,
-                                                         fz.P, fz.pk);
+                                                         fz.P);
     };
     using I16 = std::integral_constant<int, 16>;
     using I32 = std::integral_constant<int, 32>;
@@ -923,13 +1035,14 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
     auto go2 = [&](auto bn, auto fl) {
         constexpr int BN = decltype(bn)::value, F = decltype(fl)::value;
         const unsigned n = (unsigned)N * blocks(H, TY2) * blocks(W, TX) * blocks(cstore, BN);
-        k_conv2<BN, F><<<conv2_grid<BN, F>(n), TB2, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, cstore,
-                                                            relu, (int)n, fz.X2, fz.ldx2, fz.csplit, fz.P);
+        k_conv2<BN, F><<<conv2_grid<BN, F>(n), TB2, 0, st>>>(X, ldx, Cin, Wt, bias, Cout, N, H, W, Y, ldy, Yf, ldyf,
+                                                            cstore, relu, (int)n, fz.X2, fz.ldx2, fz.csplit, fz.P);
     };
     // k_conv2 takes the plain, upsampled-input and pooled-output layers of
     // more than 16 output channels, 64 per item where they divide the output
-    // (twice the MFMAs per staged halo); k_conv the packed first layer, the
-    // FP32 output layer and the masked data gradients.  (Keeping the
+    // (twice the MFMAs per staged halo), and the FP32 output layer (16 per
+    // item); k_conv2_pack the packed first layer; k_conv the masked data
+    // gradients and the BF16 layers of at most 16 outputs.  (Keeping the
     // 32-channel full-resolution layers on k_conv, which had looked faster
     // for two of them under the profiler, made the whole denoise of
     // 128 x 512x512 slower: 14.16-14.18 against 13.96-14.07 ms.)
@@ -942,10 +1055,13 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
         if (fuse == kFuseUp) pick(std::integral_constant<int, kFuseUp>());
         else if (fuse == kFusePool) pick(std::integral_constant<int, kFusePool>());
         else pick(std::integral_constant<int, 0>());
+    } else if (cstore <= 16 && flags == kOutF32) {
+        go2(I16(), std::integral_constant<int, kOutF32>());
     } else if (cstore <= 16) {
         outs(I16());
-    } else if (fuse == kFusePack) {
-        go(I32(), std::integral_constant<int, kFusePack | kFixW>());  // 32 inputs, 32 outputs: one weight set
+    } else if (fuse == kFusePack) {  // 32 inputs, at most 32 outputs: one weight set
+        const unsigned n = (unsigned)N * blocks(H, TY2) * blocks(W, TX);
+        k_conv2_pack<<<conv2_pack_grid(n), TB2, 0, st>>>(Wt, bias, Cout, N, H, W, Y, ldy, cstore, relu, (int)n, fz.pk);
     } else {
         outs(I32());  // masked data gradient
     }
