@@ -554,19 +554,30 @@ __global__ __launch_bounds__(TB2) void k_conv2(const uint16_t* __restrict__ X, i
 // convolved); the threads pack them into the BF16 halo (channels 0-8; the
 // zero channel groups 2 and 3 are written once), then the MFMAs and the
 // epilogue are k_conv's, so the outputs are the bits of k_conv<32, kFusePack>.
-constexpr int kRawLoads = (NHALO2 + 63) / 64;  // wave loads per target over the halo
-constexpr int kRawUnits = kRawLoads * 64;      // float4 per target and buffer (the last load's spill-over included)
+// Rows of the first layer's tiles.  16: one 8-wave block per CU (1.07 ms on
+// 128 x 512x512); 8 rows fit two 4-wave blocks per CU and were slower (1.31 ms).
+constexpr int kPackRows = 16;
+template <int TYP>  // tile rows: 4 of them per pair of waves
+struct PackShape {
+    static constexpr int kThreads = TYP * 32, kWaves = kThreads / 64;
+    static constexpr int kHalo = HXX * (TYP + 2);           // halo pixels
+    static constexpr int kRawLoads = (kHalo + 63) / 64;     // wave loads per target over the halo
+    static constexpr int kRawUnits = kRawLoads * 64;        // float4 per target and buffer (the last load's spill-over included)
+};
 
-__global__ __launch_bounds__(TB2) void k_conv2_pack(const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
+template <int TYP>
+__global__ __launch_bounds__(PackShape<TYP>::kThreads) void k_conv2_pack(const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
                                                    int Cout, int N, int H, int W, uint16_t* __restrict__ Y, int ldy,
                                                    int cstore, int relu, int n_items, PackSrc pk) {
     constexpr int BN = 32;
     using SH = Conv2Shape<BN>;
+    using PS = PackShape<TYP>;
+    constexpr int kRawUnits = PS::kRawUnits, kRawLoads = PS::kRawLoads, NH = PS::kHalo, NT = PS::kThreads;
     __shared__ __attribute__((aligned(16))) float4 raw0[3 * kRawUnits];
     __shared__ __attribute__((aligned(16))) float4 raw1[3 * kRawUnits];
-    __shared__ __attribute__((aligned(16))) uint16_t Hs[NHALO2 * CCLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Hs[NH * CCLD];
     __shared__ __attribute__((aligned(16))) uint16_t Ws[SH::kWUnits * 8 + 2 * 64];  // + 64 FP32 biases (one per lane)
-    const int tiles_x = (W + TX - 1) / TX, tiles_y = (H + TY2 - 1) / TY2;
+    const int tiles_x = (W + TX - 1) / TX, tiles_y = (H + TYP - 1) / TYP;
     const int n_tiles = N * tiles_x * tiles_y;
     const int t = threadIdx.x, lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const int fr = lane & 15;
@@ -578,7 +589,7 @@ __global__ __launch_bounds__(TB2) void k_conv2_pack(const uint16_t* __restrict__
         const int tile = i % n_tiles;
         it.x0 = __builtin_amdgcn_readfirstlane((tile % tiles_x) * TX);
         const int r = tile / tiles_x;
-        it.y0 = __builtin_amdgcn_readfirstlane((r % tiles_y) * TY2);
+        it.y0 = __builtin_amdgcn_readfirstlane((r % tiles_y) * TYP);
         it.n = __builtin_amdgcn_readfirstlane(r / tiles_y);
         return it;
     };
@@ -588,11 +599,11 @@ __global__ __launch_bounds__(TB2) void k_conv2_pack(const uint16_t* __restrict__
     auto issue = [&](float4* B, const Item& it) __attribute__((always_inline)) {
         const uint32_t img = (uint32_t)pk.h * (uint32_t)pk.w;
         const float4* src[3] = {pk.rgba, pk.alb, pk.nrm};
-        for (int j = wv; j < kRawLoads; j += TB2 / 64) {
+        for (int j = wv; j < kRawLoads; j += PS::kWaves) {
             const int u = j * 64 + lane;
             const int hy = u / HXX, hx = u - hy * HXX;
             const int yy = it.y0 - 1 + hy, xx = it.x0 - 1 + hx;
-            const bool ok = u < NHALO2 && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W;
+            const bool ok = u < NH && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W;
             const uint32_t off = ok ? 16u * (uint32_t)(min(yy, pk.h - 1) * pk.w + min(xx, pk.w - 1)) : kOOB;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -607,19 +618,19 @@ __global__ __launch_bounds__(TB2) void k_conv2_pack(const uint16_t* __restrict__
     issue(raw0, cur);
     {  // the one weight set and its biases (k_conv2's layout), and the zero channel groups
         const __amdgpu_buffer_rsrc_t rw = buffer_rsrc(Wt, 18u * Cout * CBK);
-        for (int j = wv; j < SH::kWLoads; j += TB2 / 64) {
+        for (int j = wv; j < SH::kWLoads; j += PS::kWaves) {
             const int u = j * 64 + lane, r = u >> 2;
             const int q = (u & 3) ^ ((r >> 1) & 3);
             const int tap = r / BN, co = r - tap * BN;
             const uint32_t off = co < Cout ? 2u * (uint32_t)((tap * Cout + co) * CBK + q * 8) : kOOB;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rw, (lds_void*)(Ws + j * 512), 16, off, 0, 0, 0);
         }
-        if (wv == TB2 / 64 - 1) {
+        if (wv == PS::kWaves - 1) {
             const __amdgpu_buffer_rsrc_t rb = buffer_rsrc(bias, bias ? 4u * Cout : 0u);
             const uint32_t off = lane < Cout ? 4u * (uint32_t)lane : kOOB;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void*)(Ws + SH::kWUnits * 8), 4, off, 0, 0, 0);
         }
-        for (int r = t; r < NHALO2; r += TB2) {
+        for (int r = t; r < NH; r += NT) {
             *(uint4*)&Hs[cofs(r, 2)] = make_uint4(0u, 0u, 0u, 0u);
             *(uint4*)&Hs[cofs(r, 3)] = make_uint4(0u, 0u, 0u, 0u);
         }
@@ -640,7 +651,7 @@ __global__ __launch_bounds__(TB2) void k_conv2_pack(const uint16_t* __restrict__
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's loads into A (and the weights) have landed
         __syncthreads();                     // every wave's; B and Hs are no longer read
         if (has_next) issue(B, nxt);
-        for (int r = t; r < NHALO2; r += TB2) {
+        for (int r = t; r < NH; r += NT) {
             float v[9];
             pack9(A[r], A[kRawUnits + r], A[2 * kRawUnits + r], v);
             // v_cvt_pk_bf16_f32 rounds to nearest even as f2bf (no NaN after sane)
@@ -971,13 +982,14 @@ unsigned conv2_grid(unsigned n_items) {
     return std::min(n_items, resident);
 }
 
+template <int TYP>
 unsigned conv2_pack_grid(unsigned n_items) {
     static unsigned resident = 0;
     if (!resident) {
         int dev = 0, cus = 256, nb = 0;
         KF_CHECK(hipGetDevice(&dev));
         KF_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_conv2_pack, TB2, 0));
+        KF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_conv2_pack<TYP>, PackShape<TYP>::kThreads, 0));
         resident = (unsigned)std::max(nb, 1) * (unsigned)cus;
     }
     return std::min(n_items, resident);
@@ -1060,8 +1072,10 @@ void conv_launch(const uint16_t* X, int ldx, int Cin, const uint16_t* Wt, const 
     } else if (cstore <= 16) {
         outs(I16());
     } else if (fuse == kFusePack) {  // 32 inputs, at most 32 outputs: one weight set
-        const unsigned n = (unsigned)N * blocks(H, TY2) * blocks(W, TX);
-        k_conv2_pack<<<conv2_pack_grid(n), TB2, 0, st>>>(Wt, bias, Cout, N, H, W, Y, ldy, cstore, relu, (int)n, fz.pk);
+        constexpr int TYP = kPackRows;
+        const unsigned n = (unsigned)N * blocks(H, TYP) * blocks(W, TX);
+        k_conv2_pack<TYP><<<conv2_pack_grid<TYP>(n), PackShape<TYP>::kThreads, 0, st>>>(Wt, bias, Cout, N, H, W, Y, ldy,
+                                                                                      cstore, relu, (int)n, fz.pk);
     } else {
         outs(I32());  // masked data gradient
     }
