@@ -361,71 +361,75 @@ __global__ __launch_bounds__(TB) void k_raygen(SceneView S, Frame F, const uint2
                                                float4* __restrict__ ray_o, float4* __restrict__ ray_d,
                                                uint32_t* __restrict__ seed, uint32_t* __restrict__ queue,
                                                unsigned long long* __restrict__ cnt) {
-    uint32_t n = F.WH * F.nsamp;
+    const uint32_t n = F.WH * F.nsamp;
     const uint32_t S0 = F.fv->S0;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t i = F.s0 + p / F.WH;
-        const uint32_t env = (p % F.WH) / F.WHe;
-        const uint32_t pixel = (p % F.WH) - env * F.WHe;  // within the environment's image
+    // one thread per pixel and all of the batch's samples of it: the jitter
+    // stream's seed tea(pixel, S0) is the same for every sample
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < F.WH; r += gridDim.x * blockDim.x) {
+        const uint32_t env = r / F.WHe;
+        const uint32_t pixel = r - env * F.WHe;  // within the environment's image
         const uint32_t x = pixel % F.W, y = pixel / F.W;
         const kf_camera_ubo& cam = F.cams[env];
-        // jitter stream: tea(pixel, S0) advanced by 2i draws (rgen:23, 35)
-        uint32_t js = tea(pixel, S0);
-        uint2 jm = jump[i];
-        js = jm.x * js + jm.y;
-        float r0 = rnd(js);
-        float r1 = rnd(js);
-        uint32_t rs = tea(pixel, S0 + 1u + i);
-        float posx = float(x) + r0, posy = float(y) + r1;
-        float npx = posx / float(F.W), npy = posy / float(F.He);
-        float dx = npx * 2.0f - 1.0f, dy = npy * 2.0f - 1.0f;
-        float aperture = cam.position[3];
-        float focus = cam.front[3];
-        float ox, oy;
-        do {  // diskSampling (Sampling.glsl:124-134), always drawn (rgen:42)
-            float a = rnd(rs), b = rnd(rs);
-            ox = 2.0f * a - 1.0f;
-            oy = 2.0f * b - 1.0f;
-        } while (ox * ox + oy * oy >= 1.0f);
-        float half = aperture / 2.0f;
-        ox = half * ox;
-        oy = half * oy;
-        float tv[4] = {dx, dy, 1.0f, 1.0f};
-        float target[4];
-        mat4_mul(cam.proj_inverse, tv, target);
-        float org[4], dir4[4];
-        if (aperture > 0.0f) {
-            float o4[4] = {ox, oy, 0.0f, 1.0f};
-            mat4_mul(cam.view_inverse, o4, org);
-            v3 nd = normalize(mk(target[0], target[1], target[2]) * focus - mk(ox, oy, 0.0f));
-            float d4[4] = {nd.x, nd.y, nd.z, 0.0f};
-            mat4_mul(cam.view_inverse, d4, dir4);
-        } else {
-            float o4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
-            mat4_mul(cam.view_inverse, o4, org);
-            v3 nd = normalize(mk(target[0], target[1], target[2]));
-            float d4[4] = {nd.x, nd.y, nd.z, 0.0f};
-            mat4_mul(cam.view_inverse, d4, dir4);
-        }
-        // ray_d.w carries the ray's TLAS root item: the pixel tile's for the
-        // camera ray, the environment's from the next bounce on
-        const uint32_t root =
-            tile_root ? tile_root[(env * ty + y / kTile) * tx + x / kTile] : env_root_item(S, env);
-        ray_o[p] = make_float4(org[0], org[1], org[2], 0.0f);
-        ray_d[p] = make_float4(dir4[0], dir4[1], dir4[2], __uint_as_float(root));
-        // throughput (1, depth 0), colour (0) and BSDF weight (0) of a new
-        // path are implicit at bounce 0 (kPathStart): k_shade and k_connect
-        // use them there instead of reading them back, which saves 48 B of
-        // writes and as many of reads per path
-        seed[p] = rs;
-        // Camera rays are traced in 8-row bands, column by column, so a
-        // wave's 64 rays cover an 8x8 pixel tile (paths stay stored by
-        // sample and pixel; only the order of the queue changes).
-        {
-            const uint32_t r = p % F.WH, band = r / (8u * F.W);
-            const uint32_t rows = min(8u, F.H - band * 8u), off = r - band * 8u * F.W;
-            const uint32_t x = off / rows, yy = off - x * rows;
-            queue[p] = (p - r) + (band * 8u + yy) * F.W + x;
+        const uint32_t js0 = tea(pixel, S0);
+        for (uint32_t p = r; p < n; p += F.WH) {
+            const uint32_t i = F.s0 + p / F.WH;
+            // jitter stream: tea(pixel, S0) advanced by 2i draws (rgen:23, 35)
+            uint2 jm = jump[i];
+            uint32_t js = jm.x * js0 + jm.y;
+            float r0 = rnd(js);
+            float r1 = rnd(js);
+            uint32_t rs = tea(pixel, S0 + 1u + i);
+            float posx = float(x) + r0, posy = float(y) + r1;
+            float npx = posx / float(F.W), npy = posy / float(F.He);
+            float dx = npx * 2.0f - 1.0f, dy = npy * 2.0f - 1.0f;
+            float aperture = cam.position[3];
+            float focus = cam.front[3];
+            float ox, oy;
+            do {  // diskSampling (Sampling.glsl:124-134), always drawn (rgen:42)
+                float a = rnd(rs), b = rnd(rs);
+                ox = 2.0f * a - 1.0f;
+                oy = 2.0f * b - 1.0f;
+            } while (ox * ox + oy * oy >= 1.0f);
+            float half = aperture / 2.0f;
+            ox = half * ox;
+            oy = half * oy;
+            float tv[4] = {dx, dy, 1.0f, 1.0f};
+            float target[4];
+            mat4_mul(cam.proj_inverse, tv, target);
+            float org[4], dir4[4];
+            if (aperture > 0.0f) {
+                float o4[4] = {ox, oy, 0.0f, 1.0f};
+                mat4_mul(cam.view_inverse, o4, org);
+                v3 nd = normalize(mk(target[0], target[1], target[2]) * focus - mk(ox, oy, 0.0f));
+                float d4[4] = {nd.x, nd.y, nd.z, 0.0f};
+                mat4_mul(cam.view_inverse, d4, dir4);
+            } else {
+                float o4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+                mat4_mul(cam.view_inverse, o4, org);
+                v3 nd = normalize(mk(target[0], target[1], target[2]));
+                float d4[4] = {nd.x, nd.y, nd.z, 0.0f};
+                mat4_mul(cam.view_inverse, d4, dir4);
+            }
+            // ray_d.w carries the ray's TLAS root item: the pixel tile's for the
+            // camera ray, the environment's from the next bounce on
+            const uint32_t root =
+                tile_root ? tile_root[(env * ty + y / kTile) * tx + x / kTile] : env_root_item(S, env);
+            ray_o[p] = make_float4(org[0], org[1], org[2], 0.0f);
+            ray_d[p] = make_float4(dir4[0], dir4[1], dir4[2], __uint_as_float(root));
+            // throughput (1, depth 0), colour (0) and BSDF weight (0) of a new
+            // path are implicit at bounce 0 (kPathStart): k_shade and k_connect
+            // use them there instead of reading them back, which saves 48 B of
+            // writes and as many of reads per path
+            seed[p] = rs;
+            // Camera rays are traced in 8-row bands, column by column, so a
+            // wave's 64 rays cover an 8x8 pixel tile (paths stay stored by
+            // sample and pixel; only the order of the queue changes).
+            {
+                const uint32_t band = r / (8u * F.W);
+                const uint32_t rows = min(8u, F.H - band * 8u), off = r - band * 8u * F.W;
+                const uint32_t x = off / rows, yy = off - x * rows;
+                queue[p] = (p - r) + (band * 8u + yy) * F.W + x;
+            }
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1550,7 +1554,7 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             k_tile_roots<<<(uint32_t)std::min<size_t>((n_tiles + TB / 64 - 1) / (TB / 64), 65535u), TB, 0, st>>>(
                 S, F, scene_tlas, n_scene_tlas, s->d_ilo, s->d_ihi, n_env > 1 ? s->d_env_off : nullptr, tx, ty,
                 r->d_ftlas, r->d_tile_root);
-        k_raygen<<<std::min<uint32_t>((B + TB - 1) / TB, 65535u), TB, 0, st>>>(
+        k_raygen<<<std::min<uint32_t>((WH + TB - 1) / TB, 65535u), TB, 0, st>>>(
             S, F, d_jump, tiles ? r->d_tile_root : nullptr, tx, ty, r->ray_o, r->ray_d, r->seed, r->queue_a,
             r->counters);
         uint32_t* qa = r->queue_a;
