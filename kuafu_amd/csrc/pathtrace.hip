@@ -604,29 +604,49 @@ KF_DEV void nee_deferred(const SceneView& S, const DevLights* __restrict__ LT, c
     }
 }
 
+// The per-path inputs of shading (throughput, hit, ray, random state),
+// loaded apart from the shading itself so that k_shade can fetch the next
+// path's while it shades this one.
+struct ShadeIn {
+    float4 t4, h, ro, rd;
+    uint32_t rs, hp;
+};
+KF_DEV ShadeIn shade_in(uint32_t p, float4 t4, const float4* __restrict__ hit, const uint32_t* __restrict__ hit_prim,
+                        const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
+                        const uint32_t* __restrict__ seed) {
+    ShadeIn in;
+    in.t4 = t4;
+    in.h = hit[p];
+    in.ro = ray_o[p];
+    in.rd = ray_d[p];
+    in.rs = seed[p];
+    in.hp = hit_prim[p];
+    return in;
+}
+
 // Closest hit / miss of path p at this bounce (rchit:322-469, rmiss:12-31):
 // emission, BSDF sample and next ray, NEE (shadow-ray records, or the sum
 // with the shadow rays traced here when NEE_INLINE), AOVs.  Returns whether
 // the path is traced at the next bounce unless Russian roulette ends it.
 template <bool NEE_INLINE>
 KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __restrict__ LT, int bounce, uint32_t p,
-                       float4 t4, bool aov, float4* __restrict__ ray_o, float4* __restrict__ ray_d,
-                       const float4* __restrict__ hit, const uint32_t* __restrict__ hit_prim, float4* __restrict__ rw,
-                       float4* __restrict__ em, uint32_t* __restrict__ seed, float4* __restrict__ nee_a,
+                       const ShadeIn& in, bool aov, float4* __restrict__ ray_o, float4* __restrict__ ray_d,
+                       float4* __restrict__ rw, float4* __restrict__ em, uint32_t* __restrict__ seed, float4* __restrict__ nee_a,
                        float4* __restrict__ nee_b, uint8_t* __restrict__ vis, float4* __restrict__ sh_o,
                        float4* __restrict__ sh_d, uint32_t* __restrict__ sh_root, float4* __restrict__ albedo_out,
                        float4* __restrict__ normal_out, unsigned long long* __restrict__ cnt, uint32_t& n_shadow,
                        uint32_t& ovf, NeeCtx& nc) {
     const uint32_t K = F.slots;
+    const float4 t4 = in.t4;
     const uint32_t depth = __float_as_uint(t4.w);
     bool ext = false;
     nc.on = false;
-    float4 h = hit[p];
-    float4 ro = ray_o[p], rd = ray_d[p];
+    const float4 h = in.h;
+    const float4 ro = in.ro, rd = in.rd;
     v3 rdir = xyz(rd);
     // root item of the rays from here on (a camera ray carries its tile's)
     const uint32_t root = bounce == 0 ? env_root_item(S, (p % F.WH) / F.WHe) : __float_as_uint(rd.w);
-    uint32_t rs = seed[p];
+    uint32_t rs = in.rs;
     uint32_t inst = __float_as_uint(h.w);
     for (uint32_t j = 0; j < K; ++j) vis[(size_t)p * K + j] = 0;
     if (inst == ~0u) {
@@ -639,7 +659,7 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
         }
         if (NEE_INLINE) nee_a[(size_t)p * K] = make_float4(0.f, 0.f, 0.f, 0.f);
     } else {
-        uint32_t hp = hit_prim[p];
+        uint32_t hp = in.hp;
         uint32_t prim = hp & 0x7fffffffu;
         bool inside = (hp >> 31) != 0;
         Inst I = S.inst[inst];
@@ -826,16 +846,27 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
                                               uint32_t* __restrict__ sh_root, float4* __restrict__ albedo_out,
                                               float4* __restrict__ normal_out,
                                               uint32_t* __restrict__ ext_queue, unsigned long long* __restrict__ cnt) {
-    uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
-    uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t iters = (n + stride - 1) / stride;
+    const uint32_t n = (uint32_t)min(cnt[bounce * kCntPerBounce + 0], (unsigned long long)(F.WH * F.nsamp));
+    const uint32_t stride = gridDim.x * blockDim.x, tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t iters = (n + stride - 1) / stride;
     uint32_t n_shadow = 0, ovf = 0;  // shadow rays of this thread
+    // A software pipeline over the thread's paths: the queue entry two paths
+    // ahead and the inputs of the next path are fetched before this path is
+    // shaded, so that their memory round trips overlap its shading.
+    auto fetch = [&](uint32_t p) __attribute__((always_inline)) {
+        return shade_in(p, bounce == 0 ? kPathStart : thr[p], hit, hit_prim, ray_o, ray_d, seed);
+    };
+    bool active = tid < n, active1 = iters > 1 && stride + tid < n;
+    uint32_t p = active ? queue[tid] : 0, p1 = active1 ? queue[stride + tid] : 0;
+    ShadeIn in{};
+    if (active) in = fetch(p);
     for (uint32_t it = 0; it < iters; ++it) {
-        uint32_t q = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
-        bool active = q < n;
-        uint32_t p = active ? queue[q] : 0;
-        const float4 t4 = active ? (bounce == 0 ? kPathStart : thr[p]) : make_float4(0.f, 0.f, 0.f, 0.f);
-        uint32_t depth = __float_as_uint(t4.w);
+        const uint32_t q2 = (it + 2) * stride + tid;
+        const bool active2 = it + 2 < iters && q2 < n;
+        const uint32_t p2 = active2 ? queue[q2] : 0;
+        ShadeIn in1{};
+        if (active1) in1 = fetch(p1);
+        const uint32_t depth = __float_as_uint(in.t4.w);
         // The path is traced at the next bounce unless k_connect ends it for
         // a reason already known here (miss, emitter, zero throughput, depth):
         // only Russian roulette, decided in k_connect, can still end it.
@@ -844,14 +875,18 @@ __global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLig
         NeeCtx nc;
         nc.on = false;
         if (active)
-            ext = shade_path<NEE_INLINE>(S, F, LT, bounce, p, t4, aov, ray_o, ray_d, hit, hit_prim, rw, em, seed, nee_a,
-                                         nee_b, vis, sh_o, sh_d, sh_root, albedo_out, normal_out, cnt, n_shadow, ovf,
-                                         nc);
+            ext = shade_path<NEE_INLINE>(S, F, LT, bounce, p, in, aov, ray_o, ray_d, rw, em, seed, nee_a, nee_b, vis,
+                                         sh_o, sh_d, sh_root, albedo_out, normal_out, cnt, n_shadow, ovf, nc);
         if (!NEE_INLINE)
             nee_deferred(S, LT, nc, p, F.slots, nee_a, nee_b, sh_o, sh_d, sh_root, &cnt[bounce * kCntPerBounce + 1],
                          n_shadow);
         const uint32_t at = block_append(&cnt[(bounce + 1) * kCntPerBounce + kCntExtQueue], ext);
         if (ext) ext_queue[at] = p;
+        active = active1;
+        p = p1;
+        in = in1;
+        active1 = active2;
+        p1 = p2;
     }
     wave_add(&cnt[kCntGlobal + 2], n_shadow);
     if (NEE_INLINE && ovf) atomicAdd(&cnt[kCntGlobal + 5], 1ull);
@@ -1009,8 +1044,9 @@ __global__ __launch_bounds__(TB) void k_tail(SceneView S, Frame F, const DevLigh
             hit[p] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.inst));
             hit_prim[p] = h.prim | (h.back << 31);
             NeeCtx nc;
-            shade_path<true>(S, F, LT, b, p, thr[p], false, ray_o, ray_d, hit, hit_prim, rw, em, seed, nee_a, nee_b,
-                             vis, nullptr, nullptr, nullptr, nullptr, nullptr, cnt, n_shadow, ovf, nc);
+            shade_path<true>(S, F, LT, b, p, shade_in(p, thr[p], hit, hit_prim, ray_o, ray_d, seed), false, ray_o,
+                             ray_d, rw, em, seed, nee_a, nee_b, vis, nullptr, nullptr, nullptr, nullptr, nullptr, cnt,
+                             n_shadow, ovf, nc);
             if (!connect_path<true>(F, b, p, thr, col, rw, em, seed, nee_a, nee_b, vis, out)) break;
             ++n_ext;  // traced at the next bounce
         }
@@ -1181,9 +1217,10 @@ __global__ __launch_bounds__(TB) void k_tail2(SceneView S, Frame F, const DevLig
                 if (!done && closest) {
                     NeeCtx nc;
                     nc.on = false;
-                    const bool ext = shade_path<false>(S, F, LT, b, p, thr[p], false, ray_o, ray_d, hit, hit_prim, rw, em,
-                                                       seed, nee_a, nee_b, vis, nullptr, nullptr, nullptr, nullptr,
-                                                       nullptr, cnt, unused, ovf, nc);
+                    const bool ext = shade_path<false>(S, F, LT, b, p,
+                                                       shade_in(p, thr[p], hit, hit_prim, ray_o, ray_d, seed), false,
+                                                       ray_o, ray_d, rw, em, seed, nee_a, nee_b, vis, nullptr,
+                                                       nullptr, nullptr, nullptr, nullptr, cnt, unused, ovf, nc);
                     const bool sh = nee_slots(S, LT, nc, p, K, nee_a, nee_b, sh_o, sh_d);
                     flags = 1 | (ext ? 2 : 0) | (sh ? 4 : 0);
                 }
