@@ -834,8 +834,13 @@ KF_DEV bool shade_path(const SceneView& S, const Frame& F, const DevLights* __re
     return ext;
 }
 
+// k_shade waits on memory most of its time (the hit, the instance record,
+// the vertices and the material, one after another), so it is held to 128
+// registers for 4 waves per SIMD: 12 spilled registers cost less than the
+// fourth wave gains (489 -> 464 us per launch on the 1080p frame).  With
+// inline shadow rays it keeps its registers (2 waves).
 template <bool NEE_INLINE>
-__global__ __launch_bounds__(TB) void k_shade(SceneView S, Frame F, const DevLights* __restrict__ LT, int bounce,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(NEE_INLINE ? 1 : 4))) void k_shade(SceneView S, Frame F, const DevLights* __restrict__ LT, int bounce,
                                               const uint32_t* __restrict__ queue, float4* __restrict__ ray_o,
                                               float4* __restrict__ ray_d, const float4* __restrict__ thr,
                                               const float4* __restrict__ hit, const uint32_t* __restrict__ hit_prim,
