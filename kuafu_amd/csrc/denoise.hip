@@ -553,7 +553,8 @@ __global__ __launch_bounds__(TB2) void k_conv2(const uint16_t* __restrict__ X, i
 // (raw, two buffers: the next tile's loads run while this one is packed and
 // convolved); the threads pack them into the BF16 halo (channels 0-8; the
 // zero channel groups 2 and 3 are written once), then the MFMAs and the
-// epilogue are k_conv's, so the outputs are the bits of k_conv<32, kFusePack>.
+// epilogue are k_conv's: each output pixel gets the same MFMA sequence, so
+// the outputs do not depend on the tiling.
 // Rows of the first layer's tiles.  16: one 8-wave block per CU (1.07 ms on
 // 128 x 512x512); 8 rows fit two 4-wave blocks per CU and were slower (1.31 ms).
 constexpr int kPackRows = 16;
