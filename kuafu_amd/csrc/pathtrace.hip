@@ -1380,6 +1380,8 @@ uint32_t resident_grid(Kernel kernel, int n_cu) {
     return (uint32_t)std::max(nb, 1) * (uint32_t)n_cu;
 }
 
+constexpr uint32_t kLastShadowGrab = 32;  // queries per wave reservation, last wavefront bounce's shadow rays
+
 // Default batch: up to 32M paths (16 spp at 1920x1080).  Each bounce's
 // traversal launch then holds tens of rays per lane, so the tail of the
 // slowest rays is a small part of it; the state costs about 200 B per path
@@ -1650,8 +1652,12 @@ void render_frame(Renderer* r, Scene* s, uint32_t n_env, const kf_camera_ubo* ca
             KF_CHECK(hipStreamWaitEvent(r->st2, r->sync_ev[2 * b], 0));
             if (!inline_nee) {
                 if (timed) KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 2], r->st2));
+                // the last wavefront bounce's shadow rays run with nothing
+                // beside them and the tail waiting: small reservations, so
+                // that few waves are left holding the longest rays
+                const uint32_t gsh = b + 1 < b_tail ? grabs : std::min<uint32_t>(grabs, kLastShadowGrab);
                 k_shadow<<<grid_sh, TB, 0, r->st2>>>(S, (int)b, B * K, r->sh_o, r->sh_d, r->sh_root, r->vis,
-                                                     r->counters, refill, grabs);
+                                                     r->counters, refill, gsh);
                 if (timed) KF_CHECK(hipEventRecord(r->ev[2 + 4 * b + 3], r->st2));
                 ++n_sh;
             }
