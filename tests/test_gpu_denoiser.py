@@ -167,6 +167,27 @@ def test_denoise_stage_on_render_targets(gpu, width, height):
     assert np.abs(ref[0] - after[:hc, :wc, :3]).max() <= 1e-3 * max(1.0, np.abs(ref).max())
 
 
+def test_denoise_stage_ragged_raster(gpu):
+    # a raster whose sides are not multiples of 16: the render-target path
+    # pads it to 80 x 112 by repeating edge pixels (zeros beyond); the record
+    # path on the same padding done here must give the same image
+    from kuafu_amd import GpuScene, Renderer, scenes
+    width, height = 100, 70
+    w = scenes.config1(width, height, spp=1)
+    r = Renderer(w.width, w.height)
+    r.render(GpuScene(w.data), w.cam, w.pc)
+    before = r.read(0)
+    d = Denoiser(random_seed=5)
+    d.apply(r)
+    after = r.read(0)
+    assert np.isfinite(after).all() and (after[..., 3] == 1).all()
+    rec = np.concatenate([before[..., :3], r.read(1)[..., :3], r.read(2)[..., :3]], -1)[None]
+    hp, wp = -(-height // 16) * 16, -(-width // 16) * 16
+    rec = np.pad(rec, ((0, 0), (0, hp - height), (0, wp - width), (0, 0)), mode="edge")
+    ref = d.forward(np.ascontiguousarray(rec, np.float32))[0, :height, :width]
+    assert np.abs(ref - after[..., :3]).max() <= 1e-3 * max(1.0, np.abs(ref).max())
+
+
 def test_trained_network_gain(gpu):
     # The shipped weights (kuafu_amd/weights/unet.kfw, trained on seeds
     # 0-1999 of the training generator) on held-out scenes of that generator
